@@ -1,0 +1,135 @@
+"""Benchmark: Yee Mcell-steps/s and HBM-roofline fraction on one MI355X.
+
+Workload (the roofline configuration): 512x512x512 cells at dl = 20 nm, fp32 fields, 12-layer
+CPML on all six faces, 32 seeded boxes (plain eps or 2-pole Lorentz ADE), a centred Ez point
+dipole and a field monitor on the central xy plane (8 frequencies).  One "step" is one full
+Yee time step of that scene (H sweep, ADE, E sweep, dipole, DFT), inputs resident in HBM.
+
+Prints one JSON line.  ``value`` = Mcell-steps/s over the timed steps (HIP events on the
+stepper's stream).  ``roofline`` is for the dominant kernel: algorithmic bytes per launch /
+its average device time from event pairs around each launch.  ``cpu_baseline`` is the float64
+NumPy oracle on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Yee Mcell-steps/s & %HBM peak; batched sims/s + p50/p99 latency vs serial"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(tabs) -> dict:
+    """Bytes each sweep must move per step (float32 fields, uint16 material class):
+
+    H sweep: read E (12 B) + read/write H (24 B) per cell, + 16 B per cell per CPML slab axis
+    (2 psi read+write); E sweep: read H (12 B) + read/write E (24 B) + class index (2 B) per
+    cell, + 16 B per cell per slab axis, + 4 B per dispersive component (history term).
+    """
+    nx, ny, nz = tabs.shape
+    n = nx * ny * nz
+    slab_cells = 0
+    for a, ax in enumerate(tabs.axes):
+        w = ax.pml_lo + ax.pml_hi
+        slab_cells += n // tabs.shape[a] * w
+    disp_comps = 0 if tabs.disp is None else tabs.disp.comp.size
+    h = 36 * n + 16 * slab_cells
+    e = 38 * n + 16 * slab_cells + 4 * disp_comps
+    return {"update_h": h, "update_e": e, "cells": n, "slab_cells": slab_cells}
+
+
+def cpu_baseline(n: int, steps: int = 2, nx_sample: int = 64) -> dict:
+    from openem_amd import scenes, tables
+    from oracle.yee64 import Oracle64
+    sc = scenes.box_soup_scene(shape=(nx_sample, n, n), num_steps=steps + 1)
+    tabs = tables.build(sc, steps + 1)
+    o = Oracle64(tabs)
+    o.step()                       # first step touches every array once
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    cells = nx_sample * n * n
+    return {"value": round(cells * steps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"float64 NumPy oracle, {steps} timed steps on a {nx_sample}x{n}x{n} "
+                      f"x-slab of the same box-soup workload (same seed, PML, boxes, dipole, "
+                      f"monitor), single process"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dft-batch", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(a scene is stepped on a single MI355X)", file=sys.stderr)
+        return 2
+
+    from openem_amd import scenes, tables
+    from openem_amd.solver import Stepper
+
+    n_total = args.warmup + args.steps + args.profile_steps + 8
+    t0 = time.time()
+    sc = scenes.box_soup_scene(n=args.n, num_steps=max(n_total, 4000))
+    tabs = tables.build(sc, n_total)
+    st = Stepper(tabs, dft_batch=args.dft_batch)
+    setup_s = time.time() - t0
+    print(f"[bench] setup {setup_s:.1f}s, {sc.shape}, classes {st.n_classes}, "
+          f"ADE entries {tabs.disp.comp.size if tabs.disp else 0}", file=sys.stderr, flush=True)
+    st.run(args.warmup)
+    t1 = time.perf_counter()
+    ms = st.timed_run(args.steps)
+    wall = time.perf_counter() - t1
+    prof = st.profile_run(args.profile_steps)
+    st.close()
+
+    cells = int(np.prod(sc.shape))
+    ms_step = ms / args.steps
+    value = cells / (ms_step * 1e-3) / 1e6
+    ab = algorithmic_bytes(tabs)
+    k_dom = max(("update_h", "update_e"), key=lambda k: prof[k][0] / max(prof[k][1], 1))
+    avg_ms = prof[k_dom][0] / prof[k_dom][1]
+    achieved = ab[k_dom] / (avg_ms * 1e-3) / 1e9
+    step_bytes = ab["update_h"] + ab["update_e"]
+    kern = {k: round(v[0] / v[1], 5) for k, v in prof.items() if v[1]}
+    print(f"[bench] per-kernel avg ms: {kern}", file=sys.stderr, flush=True)
+    print(f"[bench] timed {args.steps} steps: {ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall)",
+          file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline(args.n)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"box-soup {args.n}^3 (CPML x6, 32 boxes eps/2-pole Lorentz, "
+                               f"Ez dipole, 8-freq xy field monitor), one Yee step",
+                   "cells": cells, "dft_batch": args.dft_batch},
+        "roofline": {"bound": "hbm", "kernel": k_dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": ab[k_dom], "avg_launch_ms": round(avg_ms, 5)},
+        "step_hbm_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

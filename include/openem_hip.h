@@ -1,0 +1,161 @@
+/*
+ * openem_hip.h - C ABI of the MI355X (gfx950) FDTD time-stepper.
+ *
+ * The reference drives its CUDA kernels from Python through cupy (`openem/device.py:145-170`,
+ * "the only place that imports cupy ... changing to ctypes ... changes only this one file",
+ * `openem/device.py:3-10`) and builds every device table in `openem/solver.py:403-489`.
+ * This header is the replacement boundary: plain pointers and sizes, integer status codes,
+ * no exceptions, the caller owns host buffers, the library owns device buffers.
+ *
+ * Life cycle: oem_ctx_create -> oem_set_axis x3 -> oem_set_materials_* -> optional tables
+ * (dispersion, plane sources, dipoles, monitors) -> oem_prepare -> oem_run ... -> oem_read_*
+ * -> oem_ctx_destroy.  A context is used from one host thread at a time.
+ *
+ * Layout: fields are C order (nx, ny, nz) float32, z contiguous, flat cell (i*ny + j)*nz + k,
+ * the reference's flat index (`openem/grid.py:92-103`) with no pitch padding.
+ */
+#ifndef OPENEM_HIP_H
+#define OPENEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OEM_ABI_VERSION 1
+
+enum oem_status {
+    OEM_OK = 0,
+    OEM_EINVAL = 1,   /* bad argument or table mismatch   (reference: ValueError)           */
+    OEM_ENOMEM = 2,   /* device allocation failed                                            */
+    OEM_EHIP = 3,     /* HIP runtime error                                                   */
+    OEM_ESTATE = 4,   /* call out of order (e.g. oem_run before oem_prepare)                 */
+    OEM_ENOTSUP = 5   /* unsupported physics  (reference: NotImplementedError)               */
+};
+
+/* Kernel families reported by oem_profile_run (index into its output arrays). */
+enum oem_kernel_id {
+    OEM_K_UPDATE_H = 0, OEM_K_UPDATE_E = 1, OEM_K_DIPOLE_E = 2, OEM_K_DIPOLE_H = 3,
+    OEM_K_DISP_PRE = 4, OEM_K_DISP_POST = 5, OEM_K_DFT_SNAP = 6, OEM_K_DFT_FLUSH = 7,
+    OEM_K_STEP = 8, OEM_K_COUNT = 9
+};
+
+typedef struct oem_ctx oem_ctx;
+
+typedef struct oem_dims {
+    int32_t nx, ny, nz;        /* cells (PML layers included)                                 */
+    int32_t n_steps;           /* length of every per-step table (>= steps ever run)          */
+    int32_t pml_lo[3];         /* CPML slab thickness per axis at the low / high end;          */
+    int32_t pml_hi[3];         /*   psi is stored only inside these slabs                     */
+    int32_t complex_fields;    /* 1 = Bloch complex fields (not in ABI v1: OEM_ENOTSUP)       */
+    int32_t dft_batch;         /* DFT snapshot depth T (steps per fp64 flush), 1..64          */
+    float ch;                  /* dt / mu0                                                    */
+    float cw;                  /* eps0 / dt                                                   */
+} oem_dims;
+
+/* Library version (OEM_ABI_VERSION).  Replaces nothing: handshake for the ctypes binding. */
+int oem_version(void);
+
+/* Copy the calling thread's last error message (NUL-terminated, truncated to n). */
+int oem_last_error(char *buf, size_t n);
+
+/* Number of visible HIP devices (0 on a machine without a GPU; never an error). */
+int oem_device_count(int32_t *n);
+
+/* Allocate fields E, H (zeroed) and the CPML psi slabs on `device`.
+ * Replaces device_tables._alloc_fields (`openem/device_tables.py:42-86`). */
+int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out);
+int oem_ctx_destroy(oem_ctx *ctx);
+
+/* One axis (0=x,1=y,2=z): 1/dl primal and dual, neighbour tables nxt/prv (int32), masks
+ * mnx/mpv/pec, CPML (a, b, 1/kappa) at E and H positions, each of length n with the identity
+ * (0, 1, 1) outside the PML.  Replaces grid.Axis.index_tables (`openem/grid.py:219-304`),
+ * coeffs._pml_axis_arrays (`openem/coeffs.py:21-45`) and device_tables._axis_device_tables
+ * (`openem/device_tables.py:118-165`). */
+int oem_set_axis(oem_ctx *ctx, int32_t axis, const float *idl_p, const float *idl_d,
+                 const int32_t *nxt, const int32_t *prv, const float *mnx, const float *mpv,
+                 const float *pec, const float *a_e, const float *b_e, const float *k_e,
+                 const float *a_h, const float *b_h, const float *k_h);
+
+/* Material classes: one uint16 class per cell and a LUT of n_classes rows of 8 floats
+ * (ca_x, cb_x, ca_y, cb_y, ca_z, cb_z, flags as uint32 bits 0..2 = "component is dispersive",
+ * pad).  Replaces the (ca, cb) tables and the uint16 LUT of coeffs._e_coeffs / _cacb_lut
+ * (`openem/coeffs.py:98-136`, `:174-208`). */
+int oem_set_materials_lut(oem_ctx *ctx, const uint16_t *cls, int32_t n_classes, const float *lut);
+
+/* Dense fallback: ca, cb each 3*ncell floats (component-major).  Replaces
+ * device_tables._e_coeff_tables (`openem/device_tables.py:168-219`). */
+int oem_set_materials_dense(oem_ctx *ctx, const float *ca, const float *cb);
+
+/* Complex-pole ADE in CSR form: entry e -> component comp[e] at flat cell cell[e], poles
+ * ofs[e]..ofs[e+1]; per pole an index cidx into the (A-1, B) coefficient LUT; hist_box
+ * (x0,x1,y0,y1,z0,z1) bounds all entry cells.  Replaces device_tables._dispersion_tables
+ * (`openem/device_tables.py:235-285`) and the dispersion_pre/post launches
+ * (`openem/solver.py:920-1024`). */
+int oem_set_dispersion(oem_ctx *ctx, int32_t n_entry, const int32_t *comp, const int32_t *cell,
+                       const int32_t *ofs, int32_t n_pole, const int32_t *cidx, int32_t n_coef,
+                       const float *am1_re, const float *am1_im, const float *b_re,
+                       const float *b_im, const int32_t *hist_box);
+
+/* One-face TF/SF plane wave along z (pol 0: Ex/Hy, 1: Ey/Hx), at most 2 per context.
+ * ex_inc / hy_inc have n_steps+1 float64 samples.  Replaces sources_setup._plane_source_one
+ * (`openem/sources_setup.py:93-119`) and the inject_h / inject_e launches
+ * (`openem/kernels/source_dft.cu:40-73`). */
+int oem_add_plane_source(oem_ctx *ctx, int32_t pol, int32_t kh, int32_t ks, float coef_h,
+                         float coef_e, const double *ex_inc, const double *hy_inc);
+
+/* Point dipoles (electric: E -= cb*coef*amp[src][n]; magnetic: H -= ch*coef*amp[src][n]).
+ * amp is n_src rows of n_steps+1 float64 samples.  Points on the same (comp, cell) are
+ * applied by one thread in input order (no atomics, deterministic).  Replaces
+ * sources_setup._dipole_table (`openem/sources_setup.py:419-457`) and inject_dipole /
+ * inject_dipole_h (`openem/kernels/source_dft.cu:147-206`). */
+int oem_add_dipoles(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const int32_t *comp,
+                    const int32_t *cell, const float *coef, const int32_t *src_of,
+                    int32_t n_src, const double *amp);
+
+/* Plane DFT monitor (normal `axis`, E plane index km >= 1): accumulates
+ * (E_t1, E_t2, (H_t1[km]+H_t1[km-1])/2, (H_t2..)/2) for nf frequencies in float64.
+ * phase is n_steps*nf*4 float64: cos/sin at t_e=(n+1)dt then at t_h=(n+1/2)dt, windowed.
+ * Replaces monitors_setup._flux_mon_entry (`openem/monitors_setup.py:162-208`) and
+ * accumulate_dft / dft_snap_plane / dft_flush_plane (`openem/kernels/source_dft.cu:371-430`). */
+int oem_add_flux_plane(oem_ctx *ctx, int32_t axis, int32_t km, int32_t nf, const double *phase,
+                       int32_t *mon_id);
+
+/* Box DFT monitor, all six components at native Yee positions.  Replaces
+ * monitors_setup._field_mon_entry (`openem/monitors_setup.py:211-256`) and
+ * accumulate_dft_box / dft_snap_box / dft_flush_box (`openem/kernels/source_dft.cu:216-271`). */
+int oem_add_field_box(oem_ctx *ctx, const int32_t *origin, const int32_t *box, int32_t nf,
+                      const double *phase, int32_t *mon_id);
+
+/* Freeze the tables, build the launch plan.  Must precede oem_run. */
+int oem_prepare(oem_ctx *ctx);
+
+/* Advance n steps (graph-captured blocks of dft_batch steps where possible) and flush the
+ * DFT accumulators.  Replaces solver._time_loop (`openem/solver.py:1165-1213`). */
+int oem_run(oem_ctx *ctx, int64_t n_steps);
+
+/* As oem_run but launched kernel by kernel with HIP events on the context stream around
+ * every launch; ms[k] receives the summed device time of kernel family k and calls[k] the
+ * launch count (arrays of OEM_K_COUNT).  Replaces device.Profiler (`openem/device.py:29-139`). */
+int oem_profile_run(oem_ctx *ctx, int64_t n_steps, double *ms, int64_t *calls);
+
+/* Wall time of oem_run(ctx, n_steps) measured with HIP events on the context stream. */
+int oem_timed_run(oem_ctx *ctx, int64_t n_steps, double *ms);
+
+/* Steps completed so far. */
+int oem_step_index(oem_ctx *ctx, int64_t *n);
+
+/* Monitor accumulators: count = ncomp*nf*npoints doubles each, layout (comp, f, points) with
+ * points in C order of the plane (n1, n2) or the box (ni, nj, nk). */
+int oem_read_monitor(oem_ctx *ctx, int32_t mon_id, double *re, double *im, size_t count);
+
+/* Field component 0..5 (Ex Ey Ez Hx Hy Hz) to/from host, ncell floats. */
+int oem_read_field(oem_ctx *ctx, int32_t comp, float *out, size_t count);
+int oem_write_field(oem_ctx *ctx, int32_t comp, const float *in, size_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPENEM_HIP_H */

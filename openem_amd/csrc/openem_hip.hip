@@ -1,0 +1,784 @@
+// C-ABI implementation of include/openem_hip.h for MI355X (gfx950).
+//
+// One context = one scene resident in HBM: fields, CPML psi slabs, material classes, ADE pole
+// state, source and monitor tables.  oem_run advances it with the launch sequence of the
+// reference's _step_body (`openem/solver.py:1145-1162`):
+//
+//   update_h (+ plane-wave H correction) -> magnetic dipoles -> ADE pre -> update_e
+//   (+ plane-wave E correction) -> electric dipoles -> ADE post -> DFT snapshots -> step++
+//
+// with the DFT flushed every dft_batch steps.  Blocks of dft_batch steps aligned to the batch
+// are captured once into a hipGraph and replayed; the device-side step counter lets one graph
+// serve every block (the reference does the same with a CUDA graph, `openem/solver.py:1165-1182`).
+
+#include "../../include/openem_hip.h"
+#include "oem_kernels.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? OEM_ENOMEM : OEM_EHIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);               \
+    } while (0)
+
+struct Monitor {
+    MonDev d;
+    size_t count;     // ncomp*nf*npts
+};
+
+struct Dipoles {
+    DipoleDev d;
+};
+
+}  // namespace
+
+struct oem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    oem_dims dims{};
+    size_t ncell = 0;
+    float *E = nullptr, *H = nullptr;
+    PsiDev psi{};
+    GridDev g{};
+    bool axis_set[3] = {false, false, false};
+    MatDev mat{};
+    bool mat_set = false;
+    std::vector<char> class_flagged;   // LUT classes that read the ADE history term
+    DispDev disp{};
+    bool has_disp = false;
+    PlaneSrcDev pw{};
+    std::vector<Dipoles> dipoles;
+    std::vector<Monitor> mons;
+    int *d_step = nullptr;
+    int64_t step = 0;
+    int pending = 0;
+    bool prepared = false;
+    hipGraphExec_t graph = nullptr;
+    std::vector<void *> allocs;
+
+    ~oem_ctx() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (void *p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    int alloc(T **out, size_t n, bool zero) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        HIP_OK(hipMalloc(&p, n * sizeof(T)));
+        allocs.push_back(p);
+        if (zero) HIP_OK(hipMemsetAsync(p, 0, n * sizeof(T), stream));
+        *out = static_cast<T *>(p);
+        return OEM_OK;
+    }
+
+    template <typename T>
+    int upload(T **out, const T *host, size_t n) {
+        int rc = alloc(out, n, false);
+        if (rc) return rc;
+        HIP_OK(hipMemcpyAsync(*out, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return OEM_OK;
+    }
+};
+
+#define CHECK_CTX(c)                                                           \
+    do {                                                                       \
+        if (!(c)) return fail(OEM_EINVAL, "null context");                     \
+        HIP_OK(hipSetDevice((c)->device));                                     \
+    } while (0)
+
+#define TRY(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_) return rc_;       \
+    } while (0)
+
+extern "C" int oem_version(void) { return OEM_ABI_VERSION; }
+
+extern "C" int oem_last_error(char *buf, size_t n) {
+    if (!buf || n == 0) return OEM_EINVAL;
+    std::snprintf(buf, n, "%s", g_err.c_str());
+    return OEM_OK;
+}
+
+extern "C" int oem_device_count(int32_t *n) {
+    if (!n) return fail(OEM_EINVAL, "null output");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return OEM_OK;
+}
+
+extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) {
+    if (!d || !out) return fail(OEM_EINVAL, "null argument");
+    *out = nullptr;
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1)
+        return fail(OEM_EINVAL, "grid %d x %d x %d", d->nx, d->ny, d->nz);
+    if ((int64_t)d->ny * d->nz > (int64_t)1 << 30)
+        return fail(OEM_EINVAL, "plane ny*nz = %lld too large", (long long)d->ny * d->nz);
+    if (d->complex_fields) return fail(OEM_ENOTSUP, "complex (Bloch) fields use oem_bloch_*");
+    if (d->n_steps < 1) return fail(OEM_EINVAL, "n_steps %d", d->n_steps);
+    if (d->dft_batch < 1 || d->dft_batch > 64)
+        return fail(OEM_EINVAL, "dft_batch %d outside 1..64", d->dft_batch);
+    const int n[3] = {d->nx, d->ny, d->nz};
+    for (int a = 0; a < 3; ++a)
+        if (d->pml_lo[a] < 0 || d->pml_hi[a] < 0 || d->pml_lo[a] + d->pml_hi[a] > n[a])
+            return fail(OEM_EINVAL, "axis %d: pml %d+%d exceeds %d cells", a, d->pml_lo[a],
+                        d->pml_hi[a], n[a]);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(OEM_EHIP, "no HIP device visible");
+    if (device < 0 || device >= ndev) return fail(OEM_EINVAL, "device %d of %d", device, ndev);
+    HIP_OK(hipSetDevice(device));
+    oem_ctx *c = new oem_ctx();
+    c->device = device;
+    c->dims = *d;
+    c->ncell = (size_t)d->nx * d->ny * d->nz;
+    int rc = OEM_OK;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(OEM_EHIP, "hipStreamCreate failed");
+    }
+    const size_t nc = c->ncell;
+    const size_t slab[3] = {(size_t)(d->pml_lo[0] + d->pml_hi[0]) * d->ny * d->nz,
+                            (size_t)d->nx * (d->pml_lo[1] + d->pml_hi[1]) * d->nz,
+                            (size_t)d->nx * d->ny * (d->pml_lo[2] + d->pml_hi[2])};
+    rc = c->alloc(&c->E, 3 * nc, true);
+    if (!rc) rc = c->alloc(&c->H, 3 * nc, true);
+    for (int a = 0; a < 3 && !rc; ++a) {
+        for (int m = 0; m < 2 && !rc; ++m) {
+            const size_t len = slab[a] ? slab[a] : 1;
+            rc = c->alloc(&c->psi.h[2 * a + m], len, true);
+            if (!rc) rc = c->alloc(&c->psi.e[2 * a + m], len, true);
+        }
+    }
+    if (!rc) rc = c->alloc(&c->d_step, 1, true);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OEM_EHIP, "sync failed");
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return OEM_OK;
+}
+
+extern "C" int oem_ctx_destroy(oem_ctx *ctx) {
+    if (!ctx) return OEM_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return OEM_OK;
+}
+
+extern "C" int oem_set_axis(oem_ctx *ctx, int32_t axis, const float *idl_p, const float *idl_d,
+                            const int32_t *nxt, const int32_t *prv, const float *mnx,
+                            const float *mpv, const float *pec, const float *a_e,
+                            const float *b_e, const float *k_e, const float *a_h,
+                            const float *b_h, const float *k_h) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (axis < 0 || axis > 2) return fail(OEM_EINVAL, "axis %d", axis);
+    const int n = axis == 0 ? ctx->dims.nx : (axis == 1 ? ctx->dims.ny : ctx->dims.nz);
+    const float *fa[] = {idl_p, idl_d, mnx, mpv, pec, a_e, b_e, k_e, a_h, b_h, k_h};
+    for (const float *p : fa)
+        if (!p) return fail(OEM_EINVAL, "axis %d: null table", axis);
+    if (!nxt || !prv) return fail(OEM_EINVAL, "axis %d: null neighbour table", axis);
+    for (int i = 0; i < n; ++i)
+        if (nxt[i] < 0 || nxt[i] >= n || prv[i] < 0 || prv[i] >= n)
+            return fail(OEM_EINVAL, "axis %d: neighbour index out of range at %d", axis, i);
+    // the x march carries plane i+1 (H) / i-1 (E) between iterations: interior neighbours
+    // must be the adjacent planes
+    for (int i = 0; i + 1 < n; ++i)
+        if (nxt[i] != i + 1 || prv[i + 1] != i)
+            return fail(OEM_EINVAL, "axis %d: interior neighbours must be i+-1 (at %d)", axis, i);
+    AxisDev &A = ctx->g.ax[axis];
+    const float **dst[] = {&A.idl_p, &A.idl_d, &A.mnx, &A.mpv, &A.pec, &A.a_e,
+                           &A.b_e,   &A.k_e,   &A.a_h, &A.b_h, &A.k_h};
+    for (int t = 0; t < 11; ++t) {
+        float *p = nullptr;
+        TRY(ctx->upload(&p, fa[t], (size_t)n));
+        *dst[t] = p;
+    }
+    int *pn = nullptr, *pp = nullptr;
+    TRY(ctx->upload(&pn, (const int *)nxt, (size_t)n));
+    TRY(ctx->upload(&pp, (const int *)prv, (size_t)n));
+    A.nxt = pn;
+    A.prv = pp;
+    A.n = n;
+    A.lo = ctx->dims.pml_lo[axis];
+    A.hi = ctx->dims.pml_hi[axis];
+    ctx->axis_set[axis] = true;
+    return OEM_OK;
+}
+
+extern "C" int oem_set_materials_lut(oem_ctx *ctx, const uint16_t *cls, int32_t n_classes,
+                                     const float *lut) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (!cls || !lut || n_classes < 1 || n_classes > 65535)
+        return fail(OEM_EINVAL, "bad class table (n_classes=%d)", n_classes);
+    for (size_t i = 0; i < ctx->ncell; ++i)
+        if (cls[i] >= n_classes) return fail(OEM_EINVAL, "class %u at cell %zu >= %d", cls[i], i, n_classes);
+    uint16_t *dc = nullptr;
+    float *dl = nullptr;
+    TRY(ctx->upload(&dc, cls, ctx->ncell));
+    TRY(ctx->upload(&dl, lut, (size_t)n_classes * 8));
+    ctx->class_flagged.assign(n_classes, 0);
+    for (int q = 0; q < n_classes; ++q) {
+        uint32_t fl;
+        std::memcpy(&fl, lut + 8 * (size_t)q + 6, 4);
+        if (fl & ~7u) return fail(OEM_EINVAL, "class %d: flag bits 0x%x beyond 0..2", q, fl);
+        ctx->class_flagged[q] = fl != 0;
+    }
+    ctx->mat.mode = 0;
+    ctx->mat.cls = dc;
+    ctx->mat.lut = dl;
+    ctx->mat_set = true;
+    return OEM_OK;
+}
+
+extern "C" int oem_set_materials_dense(oem_ctx *ctx, const float *ca, const float *cb) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (!ca || !cb) return fail(OEM_EINVAL, "null ca/cb");
+    float *a = nullptr, *b = nullptr;
+    TRY(ctx->upload(&a, ca, 3 * ctx->ncell));
+    TRY(ctx->upload(&b, cb, 3 * ctx->ncell));
+    ctx->mat.mode = 1;
+    ctx->mat.ca = a;
+    ctx->mat.cb = b;
+    ctx->mat_set = true;
+    return OEM_OK;
+}
+
+extern "C" int oem_set_dispersion(oem_ctx *ctx, int32_t n_entry, const int32_t *comp,
+                                  const int32_t *cell, const int32_t *ofs, int32_t n_pole,
+                                  const int32_t *cidx, int32_t n_coef, const float *am1_re,
+                                  const float *am1_im, const float *b_re, const float *b_im,
+                                  const int32_t *hist_box) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (n_entry < 1 || n_pole < 0 || n_coef < 1 || !comp || !cell || !ofs || !hist_box ||
+        (n_pole && !cidx) || !am1_re || !am1_im || !b_re || !b_im)
+        return fail(OEM_EINVAL, "bad dispersion tables");
+    const oem_dims &d = ctx->dims;
+    const int hb[6] = {hist_box[0], hist_box[1], hist_box[2], hist_box[3], hist_box[4], hist_box[5]};
+    if (hb[0] < 0 || hb[1] > d.nx || hb[0] >= hb[1] || hb[2] < 0 || hb[3] > d.ny ||
+        hb[2] >= hb[3] || hb[4] < 0 || hb[5] > d.nz || hb[4] >= hb[5])
+        return fail(OEM_EINVAL, "hist box outside the grid");
+    if (ofs[0] != 0 || ofs[n_entry] != n_pole) return fail(OEM_EINVAL, "ofs endpoints");
+    const int pl = d.ny * d.nz;
+    for (int e = 0; e < n_entry; ++e) {
+        if (ofs[e + 1] < ofs[e]) return fail(OEM_EINVAL, "ofs not monotonic at %d", e);
+        if (comp[e] < 0 || comp[e] > 2) return fail(OEM_EINVAL, "comp out of range at %d", e);
+        if (cell[e] < 0 || (size_t)cell[e] >= ctx->ncell)
+            return fail(OEM_EINVAL, "cell out of range at %d", e);
+        const int i = cell[e] / pl, r = cell[e] % pl, j = r / d.nz, k = r % d.nz;
+        if (i < hb[0] || i >= hb[1] || j < hb[2] || j >= hb[3] || k < hb[4] || k >= hb[5])
+            return fail(OEM_EINVAL, "entry %d outside the hist box", e);
+    }
+    for (int t = 0; t < n_pole; ++t)
+        if (cidx[t] < 0 || cidx[t] >= n_coef) return fail(OEM_EINVAL, "cidx out of range at %d", t);
+    DispDev &D = ctx->disp;
+    D.n_entry = n_entry;
+    int *p_comp, *p_cell, *p_ofs, *p_cidx;
+    float *ar, *ai, *br, *bi;
+    TRY(ctx->upload(&p_comp, (const int *)comp, (size_t)n_entry));
+    TRY(ctx->upload(&p_cell, (const int *)cell, (size_t)n_entry));
+    TRY(ctx->upload(&p_ofs, (const int *)ofs, (size_t)n_entry + 1));
+    if (n_pole) {
+        TRY(ctx->upload(&p_cidx, (const int *)cidx, (size_t)n_pole));
+    } else {
+        TRY(ctx->alloc(&p_cidx, 1, true));
+    }
+    TRY(ctx->upload(&ar, am1_re, (size_t)n_coef));
+    TRY(ctx->upload(&ai, am1_im, (size_t)n_coef));
+    TRY(ctx->upload(&br, b_re, (size_t)n_coef));
+    TRY(ctx->upload(&bi, b_im, (size_t)n_coef));
+    D.comp = p_comp; D.cell = p_cell; D.ofs = p_ofs; D.cidx = p_cidx;
+    D.am1_re = ar; D.am1_im = ai; D.b_re = br; D.b_im = bi;
+    TRY(ctx->alloc(&D.p_re, (size_t)n_pole, true));
+    TRY(ctx->alloc(&D.p_im, (size_t)n_pole, true));
+    const size_t hsz = 3ull * (hb[1] - hb[0]) * (hb[3] - hb[2]) * (hb[5] - hb[4]);
+    TRY(ctx->alloc(&D.hist, hsz, true));
+    for (int q = 0; q < 6; ++q) { D.hb[q] = hb[q]; ctx->mat.hb[q] = hb[q]; }
+    ctx->mat.hist = D.hist;
+    ctx->has_disp = true;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return OEM_OK;
+}
+
+extern "C" int oem_add_plane_source(oem_ctx *ctx, int32_t pol, int32_t kh, int32_t ks,
+                                    float coef_h, float coef_e, const double *ex_inc,
+                                    const double *hy_inc) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (ctx->pw.n >= OEM_MAX_PW) return fail(OEM_ENOTSUP, "at most %d plane sources", OEM_MAX_PW);
+    if (pol < 0 || pol > 1) return fail(OEM_ENOTSUP, "plane wave polarisation %d (Ex/Ey only)", pol);
+    if (kh < 0 || kh >= ctx->dims.nz || ks < 0 || ks >= ctx->dims.nz)
+        return fail(OEM_EINVAL, "plane indices kh=%d ks=%d outside nz=%d", kh, ks, ctx->dims.nz);
+    if (!ex_inc || !hy_inc) return fail(OEM_EINVAL, "null incident table");
+    double *ex, *hy;
+    TRY(ctx->upload(&ex, ex_inc, (size_t)ctx->dims.n_steps + 1));
+    TRY(ctx->upload(&hy, hy_inc, (size_t)ctx->dims.n_steps + 1));
+    const int s = ctx->pw.n++;
+    ctx->pw.pol[s] = pol;
+    ctx->pw.kh[s] = kh;
+    ctx->pw.ks[s] = ks;
+    ctx->pw.coef_h[s] = coef_h;
+    ctx->pw.coef_e[s] = coef_e;
+    ctx->pw.ex_inc[s] = ex;
+    ctx->pw.hy_inc[s] = hy;
+    return OEM_OK;
+}
+
+extern "C" int oem_add_dipoles(oem_ctx *ctx, int32_t magnetic, int32_t n_pts,
+                               const int32_t *comp, const int32_t *cell, const float *coef,
+                               const int32_t *src_of, int32_t n_src, const double *amp) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (n_pts < 1 || n_src < 1 || !comp || !cell || !coef || !src_of || !amp)
+        return fail(OEM_EINVAL, "bad dipole tables");
+    for (int q = 0; q < n_pts; ++q) {
+        if (comp[q] < 0 || comp[q] > 2) return fail(OEM_EINVAL, "dipole comp at %d", q);
+        if (cell[q] < 0 || (size_t)cell[q] >= ctx->ncell) return fail(OEM_EINVAL, "dipole cell at %d", q);
+        if (src_of[q] < 0 || src_of[q] >= n_src) return fail(OEM_EINVAL, "dipole src at %d", q);
+    }
+    // group points by (comp, cell), keeping input order inside a group
+    std::vector<int> order(n_pts);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return comp[a] != comp[b] ? comp[a] < comp[b] : cell[a] < cell[b];
+    });
+    std::vector<int> gc, gl, go;
+    std::vector<float> cf;
+    std::vector<int> sr;
+    for (int t = 0; t < n_pts; ++t) {
+        const int q = order[t];
+        if (t == 0 || comp[q] != gc.back() || cell[q] != gl.back()) {
+            gc.push_back(comp[q]);
+            gl.push_back(cell[q]);
+            go.push_back(t);
+        }
+        cf.push_back(coef[q]);
+        sr.push_back(src_of[q]);
+    }
+    go.push_back(n_pts);
+    Dipoles dp{};
+    dp.d.n_groups = (int)gc.size();
+    dp.d.nsteps1 = ctx->dims.n_steps + 1;
+    dp.d.magnetic = magnetic ? 1 : 0;
+    int *a, *b, *o, *s;
+    float *c;
+    double *am;
+    TRY(ctx->upload(&a, gc.data(), gc.size()));
+    TRY(ctx->upload(&b, gl.data(), gl.size()));
+    TRY(ctx->upload(&o, go.data(), go.size()));
+    TRY(ctx->upload(&c, cf.data(), cf.size()));
+    TRY(ctx->upload(&s, sr.data(), sr.size()));
+    TRY(ctx->upload(&am, amp, (size_t)n_src * dp.d.nsteps1));
+    dp.d.g_comp = a; dp.d.g_cell = b; dp.d.g_ofs = o; dp.d.coef = c; dp.d.src = s; dp.d.amp = am;
+    ctx->dipoles.push_back(dp);
+    return OEM_OK;
+}
+
+static int add_monitor(oem_ctx *ctx, MonDev m, const double *phase, int32_t *mon_id) {
+    if (!phase) return fail(OEM_EINVAL, "null phase table");
+    if (m.nf < 1 || m.nf > 65535) return fail(OEM_EINVAL, "nf=%d outside 1..65535", m.nf);
+    m.T = ctx->dims.dft_batch;
+    const size_t cnt = (size_t)m.ncomp * m.nf * m.npts;
+    TRY(ctx->alloc(&m.snap, (size_t)m.T * m.ncomp * m.npts, true));
+    TRY(ctx->alloc(&m.re, cnt, true));
+    TRY(ctx->alloc(&m.im, cnt, true));
+    double *ph;
+    TRY(ctx->upload(&ph, phase, (size_t)ctx->dims.n_steps * m.nf * 4));
+    m.phase = ph;
+    Monitor mon{m, cnt};
+    ctx->mons.push_back(mon);
+    if (mon_id) *mon_id = (int32_t)ctx->mons.size() - 1;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return OEM_OK;
+}
+
+extern "C" int oem_add_flux_plane(oem_ctx *ctx, int32_t axis, int32_t km, int32_t nf,
+                                  const double *phase, int32_t *mon_id) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    const int n[3] = {ctx->dims.nx, ctx->dims.ny, ctx->dims.nz};
+    if (axis < 0 || axis > 2) return fail(OEM_EINVAL, "axis %d", axis);
+    if (km < 1 || km >= n[axis]) return fail(OEM_EINVAL, "plane %d needs 1 <= km < %d", km, n[axis]);
+    MonDev m{};
+    m.kind = 0;
+    m.axis = axis;
+    m.km = km;
+    m.n1 = axis == 0 ? n[1] : n[0];
+    m.n2 = axis == 2 ? n[1] : n[2];
+    m.ncomp = 4;
+    m.npts = m.n1 * m.n2;
+    m.nf = nf;
+    return add_monitor(ctx, m, phase, mon_id);
+}
+
+extern "C" int oem_add_field_box(oem_ctx *ctx, const int32_t *origin, const int32_t *box,
+                                 int32_t nf, const double *phase, int32_t *mon_id) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (!origin || !box) return fail(OEM_EINVAL, "null box");
+    const int n[3] = {ctx->dims.nx, ctx->dims.ny, ctx->dims.nz};
+    MonDev m{};
+    m.kind = 1;
+    size_t pts = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (origin[a] < 0 || box[a] < 1 || origin[a] + box[a] > n[a])
+            return fail(OEM_EINVAL, "box outside the grid on axis %d", a);
+        m.o[a] = origin[a];
+        m.b[a] = box[a];
+        pts *= (size_t)box[a];
+    }
+    if (pts > (size_t)1 << 30) return fail(OEM_EINVAL, "box too large");
+    m.ncomp = 6;
+    m.npts = (int)pts;
+    m.nf = nf;
+    return add_monitor(ctx, m, phase, mon_id);
+}
+
+extern "C" int oem_prepare(oem_ctx *ctx) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return OEM_OK;
+    for (int a = 0; a < 3; ++a)
+        if (!ctx->axis_set[a]) return fail(OEM_ESTATE, "axis %d tables not set", a);
+    if (!ctx->mat_set) return fail(OEM_ESTATE, "materials not set");
+    if (ctx->mat.mode == 0) {
+        // every cell whose class reads the ADE history must lie inside the history box
+        bool any = false;
+        for (char f : ctx->class_flagged) any |= f != 0;
+        if (any) {
+            if (!ctx->has_disp) return fail(OEM_EINVAL, "classes flag dispersion but no ADE tables set");
+            std::vector<uint16_t> cls(ctx->ncell);
+            HIP_OK(hipMemcpy(cls.data(), ctx->mat.cls, ctx->ncell * sizeof(uint16_t),
+                             hipMemcpyDeviceToHost));
+            const int ny = ctx->dims.ny, nz = ctx->dims.nz;
+            const int *hb = ctx->mat.hb;
+            for (size_t id = 0; id < ctx->ncell; ++id) {
+                if (!ctx->class_flagged[cls[id]]) continue;
+                const int i = (int)(id / ((size_t)ny * nz));
+                const int r = (int)(id - (size_t)i * ny * nz), j = r / nz, k = r % nz;
+                if (i < hb[0] || i >= hb[1] || j < hb[2] || j >= hb[3] || k < hb[4] || k >= hb[5])
+                    return fail(OEM_EINVAL, "cell %zu reads the ADE history outside its box", id);
+            }
+        }
+    }
+    GridDev &g = ctx->g;
+    g.nx = ctx->dims.nx;
+    g.ny = ctx->dims.ny;
+    g.nz = ctx->dims.nz;
+    g.plane = g.ny * g.nz;
+    g.tiles = (g.plane + OEM_TILE - 1) / OEM_TILE;
+    int chunk;
+    const char *env = std::getenv("OEM_CHUNK");
+    if (env && std::atoi(env) > 0) {
+        chunk = std::atoi(env);
+    } else {
+        const int want_blocks = 4096;
+        const int chunks_needed = std::max(1, (want_blocks + g.tiles - 1) / g.tiles);
+        chunk = std::max(4, (g.nx + chunks_needed - 1) / chunks_needed);
+    }
+    g.chunk = std::min(chunk, g.nx);
+    const long long nb = (long long)g.tiles * ((g.nx + g.chunk - 1) / g.chunk);
+    if (nb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
+    g.nblocks = (int)nb;
+    ctx->prepared = true;
+    return OEM_OK;
+}
+
+namespace {
+
+struct Prof {
+    hipEvent_t ev[64][2];
+    int kid[64];
+    int n = 0;
+};
+
+inline int blocks_for(int n, int bs) { return (n + bs - 1) / bs; }
+
+int launch_flush(oem_ctx *c, int cnt) {
+    for (auto &m : c->mons) {
+        dim3 grid(blocks_for(m.d.npts, 256), m.d.nf);
+        if (m.d.ncomp == 4)
+            hipLaunchKernelGGL(k_dft_flush<4>, grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
+        else
+            hipLaunchKernelGGL(k_dft_flush<6>, grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
+    }
+    HIP_OK(hipGetLastError());
+    return OEM_OK;
+}
+
+// One time step.  `flush_after`: number of snapshots to flush after the step counter advanced
+// (0 = none).  With `pr`, every launch is bracketed by events.
+int launch_step(oem_ctx *c, int flush_after, Prof *pr) {
+    float *E = c->E, *H = c->H;
+    const size_t nc = c->ncell;
+    float *Ex = E, *Ey = E + nc, *Ez = E + 2 * nc;
+    float *Hx = H, *Hy = H + nc, *Hz = H + 2 * nc;
+    auto mark = [&](int kid, bool begin) -> int {
+        if (!pr) return OEM_OK;
+        if (begin) {
+            if (pr->n >= 64) return fail(OEM_EINVAL, "too many launches to profile");
+            pr->kid[pr->n] = kid;
+            HIP_OK(hipEventRecord(pr->ev[pr->n][0], c->stream));
+        } else {
+            HIP_OK(hipEventRecord(pr->ev[pr->n][1], c->stream));
+            pr->n++;
+        }
+        return OEM_OK;
+    };
+    const dim3 yb(OEM_TILE), yg(c->g.nblocks);
+    TRY(mark(OEM_K_UPDATE_H, true));
+    hipLaunchKernelGGL(k_update_h, yg, yb, 0, c->stream, Hx, Hy, Hz, Ex, Ey, Ez, c->psi, c->g,
+                       c->dims.ch, c->pw, c->d_step);
+    TRY(mark(OEM_K_UPDATE_H, false));
+    for (auto &dp : c->dipoles) {
+        if (!dp.d.magnetic) continue;
+        TRY(mark(OEM_K_DIPOLE_H, true));
+        hipLaunchKernelGGL(k_dipoles, dim3(blocks_for(dp.d.n_groups, 256)), dim3(256), 0,
+                           c->stream, Hx, Hy, Hz, dp.d, c->mat, c->dims.ch, nc, c->d_step);
+        TRY(mark(OEM_K_DIPOLE_H, false));
+    }
+    if (c->has_disp) {
+        TRY(mark(OEM_K_DISP_PRE, true));
+        hipLaunchKernelGGL(k_disp_pre, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
+                           c->stream, c->disp, Ex, Ey, Ez, c->dims.ny, c->dims.nz);
+        TRY(mark(OEM_K_DISP_PRE, false));
+    }
+    TRY(mark(OEM_K_UPDATE_E, true));
+    hipLaunchKernelGGL(k_update_e, yg, yb, 0, c->stream, Ex, Ey, Ez, Hx, Hy, Hz, c->psi, c->g,
+                       c->mat, c->dims.cw, c->pw, c->d_step);
+    TRY(mark(OEM_K_UPDATE_E, false));
+    for (auto &dp : c->dipoles) {
+        if (dp.d.magnetic) continue;
+        TRY(mark(OEM_K_DIPOLE_E, true));
+        hipLaunchKernelGGL(k_dipoles, dim3(blocks_for(dp.d.n_groups, 256)), dim3(256), 0,
+                           c->stream, Ex, Ey, Ez, dp.d, c->mat, c->dims.ch, nc, c->d_step);
+        TRY(mark(OEM_K_DIPOLE_E, false));
+    }
+    if (c->has_disp) {
+        TRY(mark(OEM_K_DISP_POST, true));
+        hipLaunchKernelGGL(k_disp_post, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
+                           c->stream, c->disp, Ex, Ey, Ez);
+        TRY(mark(OEM_K_DISP_POST, false));
+    }
+    for (auto &m : c->mons) {
+        TRY(mark(OEM_K_DFT_SNAP, true));
+        hipLaunchKernelGGL(k_dft_snap, dim3(blocks_for(m.d.npts, 256)), dim3(256), 0, c->stream,
+                           m.d, Ex, Ey, Ez, Hx, Hy, Hz, c->dims.ny, c->dims.nz, c->d_step);
+        TRY(mark(OEM_K_DFT_SNAP, false));
+    }
+    TRY(mark(OEM_K_STEP, true));
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, c->stream, c->d_step);
+    TRY(mark(OEM_K_STEP, false));
+    if (flush_after > 0 && !c->mons.empty()) {
+        TRY(mark(OEM_K_DFT_FLUSH, true));
+        TRY(launch_flush(c, flush_after));
+        TRY(mark(OEM_K_DFT_FLUSH, false));
+    }
+    HIP_OK(hipGetLastError());
+    return OEM_OK;
+}
+
+}  // namespace
+
+static int check_run(oem_ctx *ctx, int64_t n) {
+    if (!ctx->prepared) return fail(OEM_ESTATE, "oem_prepare not called");
+    if (n < 0) return fail(OEM_EINVAL, "negative step count");
+    if (ctx->step + n > ctx->dims.n_steps)
+        return fail(OEM_EINVAL, "step %lld + %lld exceeds the %d-step tables",
+                    (long long)ctx->step, (long long)n, ctx->dims.n_steps);
+    return OEM_OK;
+}
+
+static int run_steps(oem_ctx *ctx, int64_t n) {
+    const int T = ctx->dims.dft_batch;
+    const bool mons = !ctx->mons.empty();
+    const char *ng = std::getenv("OEM_NO_GRAPH");
+    const bool use_graph = !(ng && ng[0] == '1');
+    int64_t left = n;
+    while (left > 0) {
+        if (use_graph && ctx->pending == 0 && ctx->step % T == 0 && left >= T) {
+            if (!ctx->graph) {
+                hipGraph_t gr = nullptr;
+                HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                int rc = OEM_OK;
+                for (int s = 0; s < T && !rc; ++s) rc = launch_step(ctx, s == T - 1 ? T : 0, nullptr);
+                hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
+                if (rc) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return rc;
+                }
+                if (ec != hipSuccess) return fail(OEM_EHIP, "graph capture: %s", hipGetErrorString(ec));
+                hipError_t ei = hipGraphInstantiate(&ctx->graph, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (ei != hipSuccess) {
+                    ctx->graph = nullptr;
+                    return fail(OEM_EHIP, "graph instantiate: %s", hipGetErrorString(ei));
+                }
+            }
+            HIP_OK(hipGraphLaunch(ctx->graph, ctx->stream));
+            ctx->step += T;
+            left -= T;
+        } else {
+            int fl = 0;
+            if (mons) {
+                ctx->pending++;
+                if (ctx->pending == T) fl = T;
+            }
+            TRY(launch_step(ctx, fl, nullptr));
+            if (fl) ctx->pending = 0;
+            ctx->step++;
+            left--;
+        }
+    }
+    if (mons && ctx->pending > 0) {
+        TRY(launch_flush(ctx, ctx->pending));
+        ctx->pending = 0;
+    }
+    return OEM_OK;
+}
+
+extern "C" int oem_run(oem_ctx *ctx, int64_t n_steps) {
+    CHECK_CTX(ctx);
+    TRY(check_run(ctx, n_steps));
+    TRY(run_steps(ctx, n_steps));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return OEM_OK;
+}
+
+extern "C" int oem_timed_run(oem_ctx *ctx, int64_t n_steps, double *ms) {
+    CHECK_CTX(ctx);
+    if (!ms) return fail(OEM_EINVAL, "null output");
+    TRY(check_run(ctx, n_steps));
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    HIP_OK(hipEventRecord(a, ctx->stream));
+    int rc = run_steps(ctx, n_steps);
+    if (!rc) {
+        HIP_OK(hipEventRecord(b, ctx->stream));
+        HIP_OK(hipEventSynchronize(b));
+        float t = 0.f;
+        HIP_OK(hipEventElapsedTime(&t, a, b));
+        *ms = t;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
+extern "C" int oem_profile_run(oem_ctx *ctx, int64_t n_steps, double *ms, int64_t *calls) {
+    CHECK_CTX(ctx);
+    if (!ms || !calls) return fail(OEM_EINVAL, "null output");
+    TRY(check_run(ctx, n_steps));
+    for (int k = 0; k < OEM_K_COUNT; ++k) { ms[k] = 0.0; calls[k] = 0; }
+    Prof *pr = new Prof();
+    int rc = OEM_OK;
+    for (int q = 0; q < 64 && !rc; ++q)
+        for (int e = 0; e < 2; ++e)
+            if (hipEventCreate(&pr->ev[q][e]) != hipSuccess) rc = fail(OEM_EHIP, "event create");
+    const int T = ctx->dims.dft_batch;
+    const bool mons = !ctx->mons.empty();
+    for (int64_t s = 0; s < n_steps && !rc; ++s) {
+        pr->n = 0;
+        int fl = 0;
+        if (mons) {
+            ctx->pending++;
+            if (ctx->pending == T || s == n_steps - 1) fl = ctx->pending;
+        }
+        rc = launch_step(ctx, fl, pr);
+        if (rc) break;
+        if (fl) ctx->pending = 0;
+        ctx->step++;
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) { rc = fail(OEM_EHIP, "sync"); break; }
+        for (int q = 0; q < pr->n; ++q) {
+            float t = 0.f;
+            (void)hipEventElapsedTime(&t, pr->ev[q][0], pr->ev[q][1]);
+            ms[pr->kid[q]] += t;
+            calls[pr->kid[q]] += 1;
+        }
+    }
+    for (int q = 0; q < 64; ++q)
+        for (int e = 0; e < 2; ++e)
+            if (pr->ev[q][e]) (void)hipEventDestroy(pr->ev[q][e]);
+    delete pr;
+    return rc;
+}
+
+extern "C" int oem_step_index(oem_ctx *ctx, int64_t *n) {
+    if (!ctx || !n) return fail(OEM_EINVAL, "null argument");
+    *n = ctx->step;
+    return OEM_OK;
+}
+
+extern "C" int oem_read_monitor(oem_ctx *ctx, int32_t mon_id, double *re, double *im,
+                                size_t count) {
+    CHECK_CTX(ctx);
+    if (mon_id < 0 || mon_id >= (int)ctx->mons.size()) return fail(OEM_EINVAL, "monitor %d", mon_id);
+    const Monitor &m = ctx->mons[mon_id];
+    if (count != m.count || !re || !im)
+        return fail(OEM_EINVAL, "monitor %d holds %zu values, asked %zu", mon_id, m.count, count);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipMemcpy(re, m.d.re, count * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(im, m.d.im, count * sizeof(double), hipMemcpyDeviceToHost));
+    return OEM_OK;
+}
+
+static int field_ptr(oem_ctx *ctx, int32_t comp, size_t count, float **p) {
+    if (comp < 0 || comp > 5) return fail(OEM_EINVAL, "component %d", comp);
+    if (count != ctx->ncell) return fail(OEM_EINVAL, "field holds %zu cells, got %zu", ctx->ncell, count);
+    *p = (comp < 3 ? ctx->E : ctx->H) + (size_t)(comp % 3) * ctx->ncell;
+    return OEM_OK;
+}
+
+extern "C" int oem_read_field(oem_ctx *ctx, int32_t comp, float *out, size_t count) {
+    CHECK_CTX(ctx);
+    float *p;
+    TRY(field_ptr(ctx, comp, count, &p));
+    if (!out) return fail(OEM_EINVAL, "null output");
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipMemcpy(out, p, count * sizeof(float), hipMemcpyDeviceToHost));
+    return OEM_OK;
+}
+
+extern "C" int oem_write_field(oem_ctx *ctx, int32_t comp, const float *in, size_t count) {
+    CHECK_CTX(ctx);
+    float *p;
+    TRY(field_ptr(ctx, comp, count, &p));
+    if (!in) return fail(OEM_EINVAL, "null input");
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipMemcpy(p, in, count * sizeof(float), hipMemcpyHostToDevice));
+    return OEM_OK;
+}

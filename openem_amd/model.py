@@ -1,0 +1,270 @@
+"""Scene data model: the numpy contract between scene building and the stepper.
+
+Field names and meanings follow the reference's ``openem/model.py`` so a ``Scene`` built for the
+reference maps one to one (``PlaneWaveSource`` :23, ``TFSFSource`` :63, ``ModeSource`` :143,
+``PointDipole`` :198, ``FieldMonitor`` :232, ``FluxMonitor`` :387, ``Dispersion`` :462,
+``Scene`` :858). Units are SI (metres, seconds). Flat cell index ``(i*ny + j)*nz + k``.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import cpml, waveform
+from .grid import Grid
+
+COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+@dataclass(frozen=True)
+class PlaneWaveSource:
+    """One-face TF/SF plane wave (reference ``openem/model.py:23-60``).
+
+    ``ex_inc[n]`` is ``E_inc(n dt)`` on the source plane, ``hy_inc[n]`` the incident H at the
+    upstream half cell and time ``(n+1/2) dt``, already divided by the wave impedance.
+    """
+
+    axis: int
+    plane_index: int
+    direction: int
+    pol_axis: int
+    waveform: waveform.Waveform
+    ex_inc: np.ndarray
+    hy_inc: np.ndarray
+    angle_theta: float = 0.0
+    num_freqs: int = 1
+
+
+@dataclass(frozen=True)
+class TFSFSource:
+    """TF/SF box driven by a 1D auxiliary line (reference ``openem/model.py:63-140``).
+
+    Only the fields the stepper needs are kept: the box, the injection axis, the polarisation
+    basis and the incident tables on the box columns (see :mod:`openem_amd.tfsf`).
+    """
+
+    direction: int
+    box_lo: tuple[int, int, int]
+    box_hi: tuple[int, int, int]
+    waveform: waveform.Waveform
+    axis: int = 2
+    pol_u: float = 1.0
+    pol_v: float = 0.0
+    k_hat: tuple[float, float, float] | None = None
+    e_hat: tuple[float, float, float] | None = None
+    open_axes: tuple[int, ...] = ()
+    #: ``(n_steps+1, ncol_e)`` incident E^n on the 1D line columns and ``(n_steps, ncol_h)``
+    #: incident H^{n+1/2}; plus the mapping of 3D positions onto columns.
+    ex_tab: np.ndarray | None = None
+    hy_tab: np.ndarray | None = None
+    col_step: float = 0.0
+    proj0: float = 0.0
+    amp_scale: float = 1.0
+
+
+@dataclass(frozen=True)
+class ModeSource:
+    """Mode injection on one plane (reference ``openem/model.py:143-195``): complex profiles of
+    the tangential fields in cyclic order (t1, t2) = ((axis+1)%3, (axis+2)%3)."""
+
+    plane_index: int
+    direction: int
+    ey_inc: np.ndarray
+    ez_inc: np.ndarray
+    hy_inc: np.ndarray
+    hz_inc: np.ndarray
+    amp_e: np.ndarray
+    amp_h: np.ndarray
+    n_eff: complex
+    axis: int = 0
+
+
+@dataclass(frozen=True)
+class PointDipole:
+    """Point current spread over Yee points (reference ``openem/model.py:198-229``).
+
+    ``indices`` ``(nw, 3)`` stencil cells, ``coef`` ``(nw,)`` = weight / dual volume.
+    """
+
+    component: int
+    indices: np.ndarray
+    coef: np.ndarray
+    waveform: waveform.Waveform
+    magnetic: bool = False
+
+
+@dataclass(frozen=True)
+class FieldMonitor:
+    """Frequency-domain box monitor storing the six components at their Yee positions
+    (reference ``openem/model.py:232-262``)."""
+
+    name: str
+    freqs: np.ndarray
+    origin: tuple[int, int, int]
+    box: tuple[int, int, int]
+    apodization: tuple | None = None
+
+
+@dataclass(frozen=True)
+class FluxMonitor:
+    """Planar flux monitor (reference ``openem/model.py:387-418``): tangential phasors
+    (E_t1, E_t2, H_t1, H_t2) with H colocated onto the E plane."""
+
+    name: str
+    axis: int
+    plane_index: int
+    normal_dir: int
+    freqs: np.ndarray
+    transverse: tuple[tuple[int, int], tuple[int, int]] | None = None
+    apodization: tuple | None = None
+
+
+@dataclass(frozen=True)
+class Dispersion:
+    """Complex-pole ADE data in CSR form (reference ``openem/model.py:462-527``).
+
+    Entry ``e`` is component ``comp[e]`` at flat cell ``cell[e]`` and owns poles
+    ``pole_ofs[e]:pole_ofs[e+1]``; ``am1 = A-1`` and ``b = B`` (complex) per pole;
+    ``g[e] = sum 2 Re(B)`` is folded into ca/cb.
+    """
+
+    comp: np.ndarray
+    cell: np.ndarray
+    pole_ofs: np.ndarray
+    am1: np.ndarray
+    b: np.ndarray
+    g: np.ndarray
+
+    @property
+    def n_entry(self) -> int:
+        return int(self.comp.size)
+
+    @property
+    def n_pole(self) -> int:
+        return int(self.am1.size)
+
+    def validate(self, shape: tuple[int, int, int]) -> None:
+        """Shapes, ranges, CSR monotonicity, and uniqueness of (comp, cell): a duplicate would
+        let one entry overwrite another's history term (reference ``openem/model.py:421-458``)."""
+        n = self.n_entry
+        ncell = int(np.prod(shape))
+        if self.cell.shape != (n,) or self.g.shape != (n,):
+            raise ValueError("Dispersion: comp/cell/g lengths differ")
+        if self.pole_ofs.shape != (n + 1,) or self.pole_ofs[0] != 0 \
+                or self.pole_ofs[-1] != self.n_pole or np.any(np.diff(self.pole_ofs) < 0):
+            raise ValueError("Dispersion: bad pole_ofs")
+        if self.b.shape != self.am1.shape:
+            raise ValueError("Dispersion: am1/b lengths differ")
+        if n == 0:
+            return
+        if self.comp.min() < 0 or self.comp.max() > 2:
+            raise ValueError("Dispersion: comp out of range")
+        if self.cell.min() < 0 or self.cell.max() >= ncell:
+            raise ValueError("Dispersion: cell out of range")
+        key = self.comp.astype(np.int64) * ncell + self.cell
+        if np.unique(key).size != n:
+            raise ValueError("Dispersion: duplicate (comp, cell) entries")
+
+    def g_dense(self, shape: tuple[int, int, int]) -> tuple[np.ndarray, ...]:
+        out = [np.zeros(int(np.prod(shape))) for _ in range(3)]
+        for c in range(3):
+            m = self.comp == c
+            out[c][self.cell[m]] = self.g[m]
+        return tuple(o.reshape(shape) for o in out)
+
+
+@dataclass(frozen=True)
+class Scene:
+    """Everything the stepper consumes (reference ``openem/model.py:858-1000``)."""
+
+    grid: Grid
+    dt: float
+    num_time_steps: int
+    eps_ex: np.ndarray
+    eps_ey: np.ndarray
+    eps_ez: np.ndarray
+    pml: dict = field(default_factory=dict)
+    shutoff: float = 0.0
+    sigma_ex: np.ndarray | None = None
+    sigma_ey: np.ndarray | None = None
+    sigma_ez: np.ndarray | None = None
+    pec_ex: np.ndarray | None = None
+    pec_ey: np.ndarray | None = None
+    pec_ez: np.ndarray | None = None
+    dispersion: Dispersion | None = None
+    bloch_k: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    sources: list = field(default_factory=list)
+    tfsf_sources: list = field(default_factory=list)
+    dipoles: list = field(default_factory=list)
+    mode_sources: list = field(default_factory=list)
+    flux_monitors: list = field(default_factory=list)
+    field_monitors: list = field(default_factory=list)
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        return self.grid.shape
+
+    @property
+    def any_dispersion(self) -> bool:
+        return self.dispersion is not None and self.dispersion.n_entry > 0
+
+    @property
+    def any_bloch(self) -> bool:
+        return any(k != 0.0 for k in self.bloch_k)
+
+    @property
+    def any_loss(self) -> bool:
+        return self.sigma_ex is not None
+
+
+def lorentz_poles(de: float, w0: float, delta: float) -> list[tuple[complex, complex]]:
+    """One Lorentz term ``de*w0^2/(w0^2 - 2i w delta - w^2)`` (angular units) as the pole pair
+    ``(a, c)`` of ``eps = eps_inf - sum[c/(iw+a) + c*/(iw+a*)]`` (the PoleResidue form the
+    reference reduces every medium to, ``openem/kernels/dispersion.cu:6-15``)."""
+    if delta < w0:
+        r = np.sqrt(w0 * w0 - delta * delta)
+        return [(complex(-delta, -r), complex(0.0, de * w0 * w0 / (2.0 * r)))]
+    r = np.sqrt(delta * delta - w0 * w0)
+    c0 = de * w0 * w0 / (4.0 * r)
+    return [(complex(-delta + r, 0.0), complex(c0, 0.0)), (complex(-delta - r, 0.0), complex(-c0, 0.0))]
+
+
+def drude_poles(wp: float, gamma: float) -> list[tuple[complex, complex]]:
+    """One Drude term ``-wp^2/(w^2 + i w gamma)`` (angular units) as pole pairs: a DC pole
+    ``(0, wp^2/(2 gamma))`` and ``(-gamma, -wp^2/(2 gamma))``."""
+    c0 = wp * wp / (2.0 * gamma)
+    return [(0j, complex(c0, 0.0)), (complex(-gamma, 0.0), complex(-c0, 0.0))]
+
+
+def split_dc_pole(poles, dt: float, rtol: float = 1e-9):
+    """Move poles at ``a = 0`` to an equivalent conductivity ``sigma = 2 Re(c) eps0``
+    (reference ``openem/scene/poles.py:156-182``): such a pole is a pure integrator."""
+    keep, sigma = [], 0.0
+    for a, c in poles:
+        if abs(a) <= rtol / dt:
+            sigma += 2.0 * complex(c).real * cpml.EPSILON_0
+        else:
+            keep.append((complex(a), complex(c)))
+    return keep, sigma
+
+
+def trap_coeffs(poles, dt: float) -> tuple[np.ndarray, np.ndarray]:
+    """``(A-1, B)`` of the trapezoidal discretisation of ``dP/dt = qP + rE``
+    (reference ``openem/scene/poles.py:284-295``)."""
+    am1, b = [], []
+    for q, r in poles:
+        den = 1.0 - q * dt / 2.0
+        am1.append((1.0 + q * dt / 2.0) / den - 1.0)
+        b.append((r * dt / 2.0) / den)
+    return np.array(am1, dtype=np.complex128), np.array(b, dtype=np.complex128)
+
+
+def eps_poles(eps_inf: float, poles, omega: np.ndarray) -> np.ndarray:
+    """Continuous ``eps(omega) = eps_inf - sum[c/(i w + a) + c*/(i w + a*)]``."""
+    w = np.asarray(omega, dtype=np.float64)
+    out = np.full(w.shape, complex(eps_inf))
+    for a, c in poles:
+        out -= c / (1j * w + a) + np.conj(c) / (1j * w + np.conj(a))
+    return out

@@ -1,0 +1,26 @@
+"""Run results (reference ``openem/results.py:18-84``)."""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Phasors:
+    """Plane phasors ``(4, nf, n1, n2)`` complex128 in the order E_t1, E_t2, H_t1, H_t2."""
+
+    data: np.ndarray
+    freqs: np.ndarray
+
+
+@dataclass
+class Result:
+    phasors: dict = field(default_factory=dict)
+    field_phasors: dict = field(default_factory=dict)
+    steps_run: int = 0
+    wall_seconds: float = 0.0
+    fields: np.ndarray | None = None
+    shutoff_triggered: bool = False
+    stop_reason: str = "pinned step count"

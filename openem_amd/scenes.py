@@ -1,0 +1,311 @@
+"""Synthetic scene builders for the benchmark workloads and the parity tests.
+
+The reference builds a ``Scene`` from a tidy3d ``Simulation`` (``openem/scene/build.py:224``);
+tidy3d is not available, so these builders produce the same ``Scene`` contract directly from
+numpy, the pattern the reference's own tests use for tidy3d-free scenes
+(``tests/test_bloch.py:42-99``, ``tests/test_fold.py:74-235``):
+
+- :func:`slab_scene` - 0.6x0.6x3.0 um slab transmission (dl = 20 nm, periodic x/y, 12-layer
+  PML on z outside the domain, eps=4 slab of 0.3 um, +z plane wave, flux planes);
+- :func:`box_soup_scene` - N^3 grid with 12-layer CPML on all faces (inside the N^3), 32 seeded
+  axis-aligned boxes (plain eps or 2-pole Lorentz), a centred Ez dipole, a field monitor on the
+  central xy plane;
+- :func:`mini_scene` - small grids mixing every feature for oracle parity.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import cpml, waveform
+from .grid import Axis, Grid, nearest_edge_index, uniform_axis
+from .model import (Dispersion, FieldMonitor, FluxMonitor, PlaneWaveSource, PointDipole, Scene,
+                    lorentz_poles, trap_coeffs)
+
+UM = 1e-6
+
+
+def num_steps_for(run_time: float, dt: float) -> int:
+    """Step count covering ``run_time``: ``floor(run_time/dt) + 1``."""
+    return int(np.floor(run_time / dt + 1e-9)) + 1
+
+
+def incident_tables(source_time, dt: float, num_steps: int, dl_half: float, eps_r: float,
+                    direction: int = +1, sigma: float = 0.0, freq0: float | None = None):
+    """The two TF/SF injection tables (reference ``openem/scene/sources.py:19-75``).
+
+    ``ex_inc[n] = Re a(n dt)``; ``hy_inc[n] = dir * Re(a((n+1/2)dt + dl_half/v) * n_c/eta0 * g)``
+    where ``n_c = sqrt(eps_r + i sigma/(w0 eps0))``, ``v = c/Re(n_c)`` and ``g`` the half-cell
+    gain ``exp(Im(n_c) w0 dl_half / c)`` of a lossy medium.
+    """
+    eps_c = complex(eps_r)
+    if sigma:
+        if freq0 is None:
+            raise ValueError("a lossy source plane needs freq0")
+        eps_c += 1j * sigma / (2 * np.pi * freq0 * cpml.EPSILON_0)
+    n_c = np.sqrt(eps_c)
+    v = cpml.C_0 / n_c.real
+    n = np.arange(num_steps + 1, dtype=np.float64)
+    ex_inc = np.real(np.asarray(source_time.amp_time(n * dt)))
+    t_h = (n + 0.5) * dt + dl_half / v
+    gain = 1.0
+    if sigma:
+        gain = float(np.exp(n_c.imag * 2 * np.pi * freq0 * dl_half / cpml.C_0))
+    hy_inc = direction * np.real(np.asarray(source_time.amp_time(t_h)) * (n_c / cpml.ETA_0 * gain))
+    return np.ascontiguousarray(ex_inc), np.ascontiguousarray(hy_inc)
+
+
+def plane_wave(grid: Grid, eps_pol: np.ndarray, dt: float, num_steps: int, z: float,
+               direction: int, pol_axis: int, source_time) -> PlaneWaveSource:
+    ks = nearest_edge_index(grid.z.edges, z)
+    kh = ks - 1 if direction > 0 else ks
+    ex, hy = incident_tables(source_time, dt, num_steps, float(grid.z.dl[kh]) / 2.0,
+                             float(eps_pol[0, 0, ks]), direction)
+    return PlaneWaveSource(axis=2, plane_index=ks, direction=direction, pol_axis=pol_axis,
+                           waveform=waveform.sample(source_time, dt, num_steps),
+                           ex_inc=ex, hy_inc=hy)
+
+
+def _interval_fill(lo_c: np.ndarray, hi_c: np.ndarray, a: float, b: float) -> np.ndarray:
+    """Fraction of each interval ``[lo_c, hi_c]`` covered by ``[a, b]``."""
+    ov = np.clip(np.minimum(hi_c, b) - np.maximum(lo_c, a), 0.0, None)
+    return ov / (hi_c - lo_c)
+
+
+def layer_eps_z(grid: Grid, z0: float, z1: float, eps_in: float, eps_out: float = 1.0):
+    """``(eps_ex, eps_ey, eps_ez)`` of a z-layered slab ``[z0, z1]`` with exact 1D averaging:
+    tangential components (on z edges) take the arithmetic mean over their dual interval, the
+    normal component (at z centres) the harmonic mean over its primal cell."""
+    e = grid.z.edges
+    dl = grid.z.dl
+    lo_d = np.concatenate([[e[0] - 0.5 * dl[0]], e[1:-1] - 0.5 * dl[:-1]])
+    hi_d = e[:-1] + 0.5 * dl
+    f_t = _interval_fill(lo_d, hi_d, z0, z1)
+    f_n = _interval_fill(e[:-1], e[1:], z0, z1)
+    eps_t = eps_out + (eps_in - eps_out) * f_t
+    eps_n = 1.0 / ((1.0 - f_n) / eps_out + f_n / eps_in)
+    shp = grid.shape
+    full = lambda v: np.broadcast_to(v[None, None, :], shp).copy()  # noqa: E731
+    return full(eps_t), full(eps_t), full(eps_n)
+
+
+def pml_faces(dt: float, dls, axes_sides, n_layers: int = 12) -> dict:
+    p = cpml.PMLParams(num_layers=n_layers)
+    return {(a, s): cpml.build(p, dls[a], s, dt) for a, s in axes_sides}
+
+
+def slab_scene(dl: float = 0.02 * UM, size=(0.6 * UM, 0.6 * UM, 3.0 * UM), eps_slab: float = 4.0,
+               thick: float = 0.3 * UM, f0: float = 2.998e14, fwidth_rel: float = 0.2,
+               n_pml: int = 12, src_z: float = -1.0 * UM, t_z: float = 1.0 * UM,
+               r_z: float | None = -1.2 * UM, nf: int = 11, run_time: float = 4e-13,
+               num_steps: int | None = None, pol_axis: int = 0,
+               direction: int = +1, homogeneous: bool = False) -> Scene:
+    """Slab transmission scene. ``homogeneous=True`` gives the no-slab reference run."""
+    nx, ny, nz = (int(round(s / dl)) for s in size)
+    ax = uniform_axis(nx, dl, -size[0] / 2, "Periodic")
+    ay = uniform_axis(ny, dl, -size[1] / 2, "Periodic")
+    az = uniform_axis(nz + 2 * n_pml, dl, -size[2] / 2 - n_pml * dl, "PML")
+    grid = Grid(ax, ay, az)
+    dt = cpml.courant_dt((dl, dl, dl))
+    n_steps = num_steps if num_steps is not None else num_steps_for(run_time, dt)
+    if homogeneous:
+        ones = np.ones(grid.shape)
+        ex, ey, ez = ones, ones.copy(), ones.copy()
+    else:
+        ex, ey, ez = layer_eps_z(grid, -thick / 2, thick / 2, eps_slab)
+    pulse = waveform.GaussianPulse(freq0=f0, fwidth=fwidth_rel * f0)
+    src = plane_wave(grid, ex if pol_axis == 0 else ey, dt, n_steps, src_z, direction,
+                     pol_axis, pulse)
+    freqs = np.linspace(f0 * 0.9, f0 * 1.1, nf)
+    mons = [FluxMonitor("T", 2, nearest_edge_index(az.edges, t_z), +1, freqs)]
+    if r_z is not None:
+        mons.append(FluxMonitor("R", 2, nearest_edge_index(az.edges, r_z), -1, freqs))
+    return Scene(grid=grid, dt=dt, num_time_steps=n_steps, eps_ex=ex, eps_ey=ey, eps_ez=ez,
+                 pml=pml_faces(dt, (dl, dl, dl), [(2, "lo"), (2, "hi")], n_pml),
+                 sources=[src], flux_monitors=mons)
+
+
+def dipole_stencil(grid: Grid, comp: int, pos, magnetic: bool = False):
+    """Multilinear stencil of a point current at ``pos`` on component ``comp``'s Yee points:
+    ``(indices (nw,3), weights/dV (nw,))`` (reference ``openem/model.py:198-229``)."""
+    coords = grid.yee_coords(comp, magnetic)
+    per_axis = []
+    for a in range(3):
+        c = coords[a]
+        x = float(pos[a])
+        i = int(np.argmin(np.abs(c - x)))
+        if abs(float(c[i]) - x) <= 1e-12 * max(abs(x), 1e-9):
+            per_axis.append([(i, 1.0)])
+            continue
+        j = i + 1 if c[i] < x else i - 1
+        w = (x - float(c[i])) / (float(c[j]) - float(c[i]))
+        per_axis.append([(i, 1.0 - w), (j, w)])
+    vols = (grid.yee_h_dual_volumes() if magnetic else grid.yee_dual_volumes())[comp]
+    idx, coef = [], []
+    for i, wi in per_axis[0]:
+        for j, wj in per_axis[1]:
+            for k, wk in per_axis[2]:
+                idx.append((i, j, k))
+                coef.append(wi * wj * wk / vols[i, j, k])
+    return np.array(idx, dtype=np.int64), np.array(coef, dtype=np.float64)
+
+
+def box_soup_scene(n: int = 512, dl: float = 20e-9, n_pml: int = 12, n_boxes: int = 32,
+                   seed: int = 0, f0: float = 2e14, fwidth_rel: float = 0.5,
+                   num_steps: int = 4000, nf: int = 8, moment: float = 1e-20,
+                   shape: tuple | None = None) -> Scene:
+    """The roofline workload: ``shape`` (default ``(n, n, n)``) cells including the CPML on all
+    six faces, 32 seeded boxes (half plain eps in [1, 12], half 2-pole Lorentz with
+    eps_inf in [1, 4], w0 in [1e14, 8e14] rad/s, delta = 1e13 rad/s, de in [0.5, 3]), an Ez
+    point dipole at the centre and a field monitor (2 z layers) on the central xy plane."""
+    nx, ny, nz = shape if shape is not None else (n, n, n)
+    rng = np.random.default_rng(seed)
+    axes = [uniform_axis(m, dl, 0.0, "PML") for m in (nx, ny, nz)]
+    grid = Grid(*axes)
+    dt = cpml.courant_dt((dl, dl, dl))
+    eps = np.ones((nx, ny, nz))
+    mat = np.full((nx, ny, nz), -1, dtype=np.int32)      # Lorentz material id per cell
+    lorentz = []
+    lo_in, hi_in = n_pml, (nx - n_pml, ny - n_pml, nz - n_pml)
+    for _ in range(n_boxes):
+        sides = rng.integers(4, 65, size=3)
+        cen = [int(rng.integers(lo_in, hi_in[a])) for a in range(3)]
+        lo = [max(lo_in, cen[a] - int(sides[a]) // 2) for a in range(3)]
+        hi = [min(hi_in[a], lo[a] + int(sides[a])) for a in range(3)]
+        sl = tuple(slice(lo[a], hi[a]) for a in range(3))
+        if rng.random() < 0.5:
+            eps[sl] = rng.uniform(1.0, 12.0)
+            mat[sl] = -1
+        else:
+            eps_inf = rng.uniform(1.0, 4.0)
+            poles = []
+            for _p in range(2):
+                poles += lorentz_poles(rng.uniform(0.5, 3.0), rng.uniform(1e14, 8e14), 1e13)
+            eps[sl] = eps_inf
+            mat[sl] = len(lorentz)
+            lorentz.append(poles)
+    disp = _staircase_dispersion(mat, lorentz, dt) if lorentz else None
+    if disp is not None and disp.n_entry == 0:
+        disp = None
+    pulse = waveform.GaussianPulse(freq0=f0, fwidth=fwidth_rel * f0)
+    wf = waveform.sample(pulse, dt, num_steps)
+    centre = [0.5 * (a.edges[0] + a.edges[-1]) for a in axes]
+    idx, coef = dipole_stencil(grid, 2, centre)
+    dip = PointDipole(component=2, indices=idx, coef=coef * moment, waveform=wf)
+    kc = nz // 2 - 1
+    freqs = np.linspace(f0 * (1 - fwidth_rel), f0 * (1 + fwidth_rel), nf)
+    fm = FieldMonitor("xy_center", freqs, (0, 0, kc), (nx, ny, 2))
+    sides_all = [(a, s) for a in range(3) for s in ("lo", "hi")]
+    return Scene(grid=grid, dt=dt, num_time_steps=num_steps, eps_ex=eps, eps_ey=eps, eps_ez=eps,
+                 pml=pml_faces(dt, (dl, dl, dl), sides_all, n_pml), dispersion=disp,
+                 dipoles=[dip], field_monitors=[fm])
+
+
+def _staircase_dispersion(mat: np.ndarray, materials: list, dt: float) -> Dispersion:
+    """CSR entries for every cell whose material id >= 0, all three components, the material's
+    poles at full weight."""
+    cells = np.flatnonzero(mat.reshape(-1) >= 0)
+    mids = mat.reshape(-1)[cells]
+    coeffs = [trap_coeffs(p, dt) for p in materials]
+    npole = np.array([len(p) for p in materials], dtype=np.int64)
+    comp = np.repeat(np.arange(3, dtype=np.int32), cells.size)
+    cell = np.tile(cells.astype(np.int32), 3)
+    mid3 = np.tile(mids, 3)
+    counts = npole[mid3]
+    ofs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    am1 = np.concatenate([coeffs[m][0] for m in mid3]) if mid3.size else np.zeros(0, complex)
+    b = np.concatenate([coeffs[m][1] for m in mid3]) if mid3.size else np.zeros(0, complex)
+    gm = np.array([2.0 * c[1].real.sum() for c in coeffs])
+    return Dispersion(comp=comp, cell=cell, pole_ofs=ofs, am1=am1, b=b, g=gm[mid3])
+
+
+def mini_scene(shape=(10, 12, 40), dl: float = 20e-9, n_pml: int = 6, seed: int = 1,
+               num_steps: int = 300, plane_wave_on: bool = True, dipole_on: bool = True,
+               magnetic_dipole: bool = False, lorentz_on: bool = True, sigma_on: bool = False,
+               pec_on: bool = False, periodic_xy: bool = True, pml_xy: bool = False,
+               nf: int = 4, pol_axis: int = 0, direction: int = +1,
+               field_box: bool = True, nonuniform_z: bool = False) -> Scene:
+    """Small scene exercising every real-path feature at once, for oracle parity."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    if pml_xy:
+        bx = by = "PML"
+    else:
+        bx = by = "Periodic" if periodic_xy else "PECBoundary"
+    ax = uniform_axis(nx, dl, 0.0, bx)
+    ay = uniform_axis(ny, dl * 1.1, 0.0, by)
+    if nonuniform_z:
+        dz = dl * (1.0 + 0.3 * np.sin(np.linspace(0, 3, nz)) ** 2)
+        dz[:n_pml] = dz[n_pml]
+        dz[-n_pml:] = dz[-n_pml - 1]
+        az = Axis(np.concatenate([[0.0], np.cumsum(dz)]), "PML", "PML")
+    else:
+        az = uniform_axis(nz, dl, 0.0, "PML")
+    grid = Grid(ax, ay, az)
+    dt = cpml.courant_dt((dl, dl * 1.1, float(az.dl.min())))
+    f0 = 3e14
+    pulse = waveform.GaussianPulse(freq0=f0, fwidth=0.3 * f0)
+    src_k = n_pml + 3
+    mon_k = nz - n_pml - 3
+    eps = [np.ones(shape) for _ in range(3)]
+    k0, k1 = nz // 2 - 3, nz // 2 + 2
+    for c in range(3):
+        eps[c][:, :, k0:k1] = 2.0 + rng.uniform(0, 2, size=(nx, ny, k1 - k0))
+    sig = None
+    if sigma_on:
+        sig = [np.zeros(shape) for _ in range(3)]
+        for c in range(3):
+            sig[c][:, :, k0:k1] = rng.uniform(0, 2e5, size=(nx, ny, k1 - k0))
+    disp = None
+    if lorentz_on:
+        mat = np.full(shape, -1, np.int32)
+        mat[1:nx - 1, 2:ny - 2, k1 + 1:k1 + 5] = 0
+        mat[2:nx - 2, 1:ny - 1, k0 - 5:k0 - 2] = 1
+        mats = [lorentz_poles(1.5, 2.5e15, 2e14) + lorentz_poles(0.7, 1.2e15, 5e13),
+                lorentz_poles(2.0, 1.8e15, 1e14)]
+        disp = _staircase_dispersion(mat, mats, dt)
+        for c in range(3):
+            eps[c][mat >= 0] = 1.7
+    pec = None
+    if pec_on:
+        pe = np.flatnonzero((np.arange(nx * ny * nz).reshape(shape) % 97) == 5)
+        pec = (pe, pe.copy(), pe.copy())
+    srcs = []
+    if plane_wave_on:
+        srcs.append(plane_wave(grid, eps[pol_axis], dt, num_steps,
+                               float(az.edges[src_k if direction > 0 else mon_k + 1]),
+                               direction, pol_axis, pulse))
+    dips = []
+    if dipole_on:
+        wf = waveform.sample(waveform.GaussianPulse(freq0=f0 * 1.1, fwidth=0.4 * f0,
+                                                    phase=0.3), dt, num_steps)
+        pos = (0.5 * (ax.edges[0] + ax.edges[-1]) + 0.3 * dl, 0.4 * (ay.edges[0] + ay.edges[-1]),
+               float(az.edges[nz // 2 + 6]) + 0.25 * dl)
+        for comp in (0, 2):
+            idx, coef = dipole_stencil(grid, comp, pos)
+            dips.append(PointDipole(comp, idx, coef * 1e-21 * (1 + comp), wf))
+        if magnetic_dipole:
+            idx, coef = dipole_stencil(grid, 1, pos, magnetic=True)
+            dips.append(PointDipole(1, idx, coef * 1e-21 * 377.0, wf, magnetic=True))
+    freqs = np.linspace(0.8 * f0, 1.2 * f0, nf)
+    mons = [FluxMonitor("T", 2, mon_k, +1, freqs),
+            FluxMonitor("R", 2, src_k - 1, -1, freqs),
+            FluxMonitor("X", 0, nx // 2, +1, freqs[:2]),
+            FluxMonitor("Y", 1, ny // 2, +1, freqs[1:3])]
+    fms = []
+    if field_box:
+        fms.append(FieldMonitor("box", freqs[:3], (1, 2, nz // 2 - 4), (nx - 2, ny - 3, 9),
+                                apodization=(None, dt * num_steps * 0.7, dt * 30)))
+    sides = [(2, "lo"), (2, "hi")]
+    if pml_xy:
+        sides += [(0, "lo"), (0, "hi"), (1, "lo"), (1, "hi")]
+    return Scene(grid=grid, dt=dt, num_time_steps=num_steps, eps_ex=eps[0], eps_ey=eps[1],
+                 eps_ez=eps[2], pml=pml_faces(dt, (dl, dl * 1.1, float(az.dl[0])), sides,
+                                              n_pml if not pml_xy else min(n_pml, 3)),
+                 sigma_ex=None if sig is None else sig[0],
+                 sigma_ey=None if sig is None else sig[1],
+                 sigma_ez=None if sig is None else sig[2],
+                 pec_ex=None if pec is None else pec[0], pec_ey=None if pec is None else pec[1],
+                 pec_ez=None if pec is None else pec[2],
+                 dispersion=disp, sources=srcs, dipoles=dips, flux_monitors=mons,
+                 field_monitors=fms)

@@ -1,0 +1,216 @@
+"""Time stepping on the MI355X through the C ABI.
+
+:func:`run` mirrors the reference's ``openem.solver.run(sc, num_steps=None, use_shutoff=True,
+kernels=None, verbose=True, return_fields=False, policy=None, force_complex=False) -> Result``
+(``openem/solver.py:403-511``): it builds the tables on the host, uploads them once, advances
+the scene on the GPU and reads back float64 phasors. :class:`Stepper` is the lower-level handle
+(one resident scene) the bench and the serving loop use.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib, tables as tables_mod
+from .model import Scene
+from .results import Phasors, Result
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a).real if np.iscomplexobj(a) else a, dtype=np.float32)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Stepper:
+    """One scene resident on the GPU (real fields)."""
+
+    def __init__(self, tabs: tables_mod.StepTables, device: int = 0, dft_batch: int = 16,
+                 dense_materials: bool = False):
+        if tabs.bloch:
+            raise NotImplementedError("Bloch scenes go through openem_amd.bloch")
+        self.lib = lib = _lib.load()
+        self.tabs = tabs
+        nx, ny, nz = tabs.shape
+        d = _lib.Dims()
+        d.nx, d.ny, d.nz = nx, ny, nz
+        d.n_steps = tabs.n_total
+        for a in range(3):
+            d.pml_lo[a] = tabs.axes[a].pml_lo
+            d.pml_hi[a] = tabs.axes[a].pml_hi
+        d.complex_fields = 0
+        d.dft_batch = int(dft_batch)
+        d.ch = np.float32(tabs.ch)
+        d.cw = np.float32(tabs.cw)
+        self.ctx = C.c_void_p()
+        _lib.check(lib, lib.oem_ctx_create(C.byref(d), int(device), C.byref(self.ctx)))
+        self._keep = []
+        try:
+            self._upload(dense_materials)
+            _lib.check(lib, lib.oem_prepare(self.ctx))
+        except Exception:
+            self.close()
+            raise
+
+    def _upload(self, dense: bool):
+        lib, t, ctx = self.lib, self.tabs, self.ctx
+        for a, ax in enumerate(t.axes):
+            arrs = [_f32(ax.idl_p), _f32(ax.idl_d), _i32(ax.nxt), _i32(ax.prv), _f32(ax.mnx),
+                    _f32(ax.mpv), _f32(ax.pec), _f32(ax.a_e), _f32(ax.b_e), _f32(ax.k_e),
+                    _f32(ax.a_h), _f32(ax.b_h), _f32(ax.k_h)]
+            _lib.check(lib, lib.oem_set_axis(ctx, a, *[_lib.ptr(x) for x in arrs]))
+        mc = None if dense else t.material_classes()
+        if mc is not None:
+            cls, lut = mc
+            cls = np.ascontiguousarray(cls, dtype=np.uint16)
+            lut = np.ascontiguousarray(lut, dtype=np.float32)
+            _lib.check(lib, lib.oem_set_materials_lut(ctx, _lib.ptr(cls), lut.shape[0],
+                                                      _lib.ptr(lut)))
+            self.n_classes = int(lut.shape[0])
+        else:
+            ca, cb = _f32(t.ca), _f32(t.cb)
+            _lib.check(lib, lib.oem_set_materials_dense(ctx, _lib.ptr(ca), _lib.ptr(cb)))
+            self.n_classes = 0
+        if t.disp is not None:
+            dp = t.disp
+            arrs = [_i32(dp.comp), _i32(dp.cell), _i32(dp.ofs), _i32(dp.cidx),
+                    _f32(dp.lut_am1.real), _f32(dp.lut_am1.imag), _f32(dp.lut_b.real),
+                    _f32(dp.lut_b.imag), _i32(dp.hist_box)]
+            _lib.check(lib, lib.oem_set_dispersion(
+                ctx, dp.comp.size, _lib.ptr(arrs[0]), _lib.ptr(arrs[1]), _lib.ptr(arrs[2]),
+                dp.am1.size, _lib.ptr(arrs[3]), dp.lut_am1.size, _lib.ptr(arrs[4]),
+                _lib.ptr(arrs[5]), _lib.ptr(arrs[6]), _lib.ptr(arrs[7]), _lib.ptr(arrs[8])))
+        for ps in t.plane_srcs:
+            ex, hy = _f64(ps.ex_inc[:t.n_total + 1]), _f64(ps.hy_inc[:t.n_total + 1])
+            _lib.check(lib, lib.oem_add_plane_source(ctx, ps.pol, ps.kh, ps.ks,
+                                                     np.float32(ps.coef_h),
+                                                     np.float32(ps.coef_e),
+                                                     _lib.ptr(ex), _lib.ptr(hy)))
+        for dp in t.dipoles:
+            arrs = [_i32(dp.comp), _i32(dp.cell), _f32(dp.coef), _i32(dp.src_of), _f64(dp.amp)]
+            _lib.check(lib, lib.oem_add_dipoles(ctx, int(dp.magnetic), dp.comp.size,
+                                                _lib.ptr(arrs[0]), _lib.ptr(arrs[1]),
+                                                _lib.ptr(arrs[2]), _lib.ptr(arrs[3]),
+                                                dp.amp.shape[0], _lib.ptr(arrs[4])))
+        self.mon_ids = []
+        for m in t.monitors:
+            ph = _f64(m.phase)
+            mid = C.c_int32(-1)
+            if m.kind == "plane":
+                _lib.check(lib, lib.oem_add_flux_plane(ctx, m.axis, m.km, m.nf, _lib.ptr(ph),
+                                                       C.byref(mid)))
+            else:
+                o, b = _i32(m.origin), _i32(m.box)
+                _lib.check(lib, lib.oem_add_field_box(ctx, _lib.ptr(o), _lib.ptr(b), m.nf,
+                                                      _lib.ptr(ph), C.byref(mid)))
+            self.mon_ids.append(int(mid.value))
+
+    # -- stepping ---------------------------------------------------------------------------
+    def run(self, n_steps: int) -> None:
+        _lib.check(self.lib, self.lib.oem_run(self.ctx, int(n_steps)))
+
+    def timed_run(self, n_steps: int) -> float:
+        ms = C.c_double(0.0)
+        _lib.check(self.lib, self.lib.oem_timed_run(self.ctx, int(n_steps), C.byref(ms)))
+        return float(ms.value)
+
+    def profile_run(self, n_steps: int) -> dict:
+        ms = np.zeros(_lib.K_COUNT, np.float64)
+        calls = np.zeros(_lib.K_COUNT, np.int64)
+        _lib.check(self.lib, self.lib.oem_profile_run(self.ctx, int(n_steps), _lib.ptr(ms),
+                                                      _lib.ptr(calls)))
+        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(_lib.KERNELS)}
+
+    @property
+    def step_index(self) -> int:
+        n = C.c_int64(0)
+        _lib.check(self.lib, self.lib.oem_step_index(self.ctx, C.byref(n)))
+        return int(n.value)
+
+    # -- readout ----------------------------------------------------------------------------
+    def read_field(self, comp: int) -> np.ndarray:
+        out = np.empty(self.tabs.shape, np.float32)
+        _lib.check(self.lib, self.lib.oem_read_field(self.ctx, comp, _lib.ptr(out), out.size))
+        return out
+
+    def write_field(self, comp: int, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        _lib.check(self.lib, self.lib.oem_write_field(self.ctx, comp, _lib.ptr(a), a.size))
+
+    def read_monitor(self, i: int) -> np.ndarray:
+        m = self.tabs.monitors[i]
+        if m.kind == "plane":
+            shp = (4, m.nf, m.n1, m.n2)
+        else:
+            shp = (6, m.nf) + tuple(m.box)
+        re = np.empty(shp, np.float64)
+        im = np.empty(shp, np.float64)
+        _lib.check(self.lib, self.lib.oem_read_monitor(self.ctx, self.mon_ids[i], _lib.ptr(re),
+                                                       _lib.ptr(im), re.size))
+        return re + 1j * im
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.lib.oem_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kernels=None,
+        verbose: bool = True, return_fields: bool = False, policy=None,
+        force_complex: bool = False, device: int = 0) -> Result:
+    """Run one simulation; see the module docstring. ``use_shutoff``/``policy`` are accepted
+    for signature parity: this stepper always runs the pinned step count (no early stop), and
+    ``kernels`` is ignored (the library is loaded once per process)."""
+    if sc.any_bloch or force_complex:
+        from . import bloch
+        return bloch.run(sc, num_steps=num_steps, return_fields=return_fields, device=device)
+    n_total = int(num_steps if num_steps is not None else sc.num_time_steps)
+    t0 = time.time()
+    tabs = tables_mod.build(sc, n_total)
+    st = Stepper(tabs, device=device)
+    try:
+        t1 = time.time()
+        st.run(n_total)
+        wall = time.time() - t1
+        phasors, fphas = {}, {}
+        for i, m in enumerate(tabs.monitors):
+            data = st.read_monitor(i)
+            if m.kind == "plane":
+                phasors[m.name] = Phasors(data=data, freqs=m.freqs)
+            else:
+                fphas[m.name] = data
+        fields = None
+        if return_fields:
+            fields = np.stack([st.read_field(c) for c in range(6)])
+    finally:
+        st.close()
+    if verbose:
+        print(f"[openem_amd] {sc.shape} x {n_total} steps: setup {t1 - t0:.2f}s, "
+              f"stepping {wall:.2f}s")
+    _warn_nonfinite(phasors, fphas)
+    return Result(phasors=phasors, field_phasors=fphas, steps_run=n_total,
+                  wall_seconds=wall, fields=fields)
+
+
+def _warn_nonfinite(ph, fph) -> None:
+    """NaN/Inf in the outputs is warned, not raised (reference ``openem/solver.py:347-370``)."""
+    bad = [k for k, v in ph.items() if not np.all(np.isfinite(v.data))]
+    bad += [k for k, v in fph.items() if not np.all(np.isfinite(v))]
+    if bad:
+        import warnings
+        warnings.warn(f"non-finite values in monitors {bad}", RuntimeWarning, stacklevel=3)

@@ -1,0 +1,389 @@
+"""Scene -> stepping tables (numpy, float64 masters).
+
+One builder feeds both consumers: the float64 oracle reads the float64 masters, the HIP library
+receives their float32 roundings. The table semantics follow the reference's setup chain:
+
+- per-axis CPML coefficient arrays with identity fill outside the PML
+  (``openem/coeffs.py:21-45``), neighbour/mask tables (``openem/grid.py:219-304``);
+- ``ca = (eps - s)/(eps + s + G)``, ``cb = (dt/eps0)/(eps + s + G)`` with ``s = sigma dt/(2 eps0)``
+  and ``G`` the ADE coupling (``openem/coeffs.py:98-136``), zeroed on PEC cells
+  (``openem/coeffs.py:139-171``);
+- plane-wave TF/SF coefficients (``openem/sources_setup.py:93-119``);
+- dipole stencils with the electric amplitude at half steps and the magnetic at whole steps
+  (``openem/kernels/source_dft.cu:147-206``);
+- DFT phase tables: E at ``t=(n+1)dt``, H at ``(n+1/2)dt``, apodisation multiplied in
+  (``openem/monitors_setup.py:74-101``, ``openem/kernels/source_dft.cu:332-362``).
+
+Differences in representation, not in values:
+
+- CPML psi lives only in the PML slabs: ``pml_lo/pml_hi`` give each axis's slab thickness.
+  Outside the slabs the coefficients are the identity, where psi stays 0 and the stretched
+  derivative equals the plain one, so the restriction changes no value.
+- ``(ca, cb)`` of all three components plus a per-component "dispersive" flag are deduplicated
+  into one uint16 *material class* per cell and a LUT (the reference dedups each component into
+  its own uint16 index, ``openem/coeffs.py:174-208``); a lookup returns the same float32 values.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import cpml
+from .grid import transverse_axes
+from .model import Scene
+
+LUT_STRIDE = 8          # ca_x cb_x ca_y cb_y ca_z cb_z flags pad
+MAX_CLASSES = 65535
+
+
+@dataclass
+class AxisTables:
+    n: int
+    idl_p: np.ndarray
+    idl_d: np.ndarray
+    nxt: np.ndarray
+    prv: np.ndarray
+    mnx: np.ndarray         # float64, or complex128 on a Bloch axis
+    mpv: np.ndarray
+    pec: np.ndarray
+    a_e: np.ndarray
+    b_e: np.ndarray
+    k_e: np.ndarray
+    a_h: np.ndarray
+    b_h: np.ndarray
+    k_h: np.ndarray
+    pml_lo: int
+    pml_hi: int
+
+
+@dataclass
+class PlaneSrcTables:
+    pol: int                # 0 -> (Ex, Hy), 1 -> (Ey, Hx)
+    kh: int
+    ks: int
+    coef_h: float
+    coef_e: float
+    ex_inc: np.ndarray      # (n_total+1,) float64
+    hy_inc: np.ndarray
+
+
+@dataclass
+class DipoleTables:
+    magnetic: bool
+    comp: np.ndarray        # (n_pts,) int32
+    cell: np.ndarray        # (n_pts,) int32, flat
+    coef: np.ndarray        # (n_pts,) float64
+    src_of: np.ndarray      # (n_pts,) int32
+    amp: np.ndarray         # (n_src, n_total+1) float64
+
+
+@dataclass
+class MonitorTables:
+    name: str
+    kind: str               # "plane" or "box"
+    nf: int
+    freqs: np.ndarray
+    phase: np.ndarray       # (n_total, nf, 4) float64: cos_e, sin_e, cos_h, sin_h (windowed)
+    axis: int = 2
+    km: int = 0
+    n1: int = 0
+    n2: int = 0
+    origin: tuple = (0, 0, 0)
+    box: tuple = (0, 0, 0)
+
+
+@dataclass
+class DispTables:
+    comp: np.ndarray        # (n_entry,) int32
+    cell: np.ndarray        # (n_entry,) int32 flat
+    ofs: np.ndarray         # (n_entry+1,) int32
+    am1: np.ndarray         # (n_pole,) complex128
+    b: np.ndarray           # (n_pole,) complex128
+    cidx: np.ndarray        # (n_pole,) int32 into the deduplicated coefficient LUT
+    lut_am1: np.ndarray     # (n_coef,) complex64 values as stored on the device
+    lut_b: np.ndarray
+    hist_box: tuple         # (x0, x1, y0, y1, z0, z1)
+
+
+@dataclass
+class StepTables:
+    shape: tuple
+    n_total: int
+    dt: float
+    ch: float
+    cw: float
+    axes: list
+    ca: np.ndarray          # (3, nx, ny, nz) float64
+    cb: np.ndarray
+    disp_flag: np.ndarray   # (3, nx, ny, nz) bool
+    disp: DispTables | None = None
+    plane_srcs: list = field(default_factory=list)
+    dipoles: list = field(default_factory=list)
+    monitors: list = field(default_factory=list)
+    bloch: bool = False
+
+    # material classes (filled lazily by material_classes())
+    _cls: np.ndarray | None = None
+    _lut: np.ndarray | None = None
+
+    def material_classes(self):
+        """``(cls (nx,ny,nz) uint16, lut (n_classes, 8) float32)``, or ``None`` when more than
+        65535 distinct classes exist (the caller then uploads dense ca/cb)."""
+        if self._cls is None:
+            self._cls, self._lut = _material_classes(self.ca, self.cb, self.disp_flag)
+        if self._cls.size == 0 and self.ca[0].size:
+            return None
+        return self._cls, self._lut
+
+
+def _pml_axis_arrays(sc: Scene, axis: int, at: str):
+    n = sc.grid.axes[axis].n
+    a = np.zeros(n)
+    b = np.ones(n)
+    k = np.ones(n)
+    for side in ("lo", "hi"):
+        c = sc.pml.get((axis, side))
+        if c is None:
+            continue
+        m = c.num_layers
+        sl = slice(0, m) if side == "lo" else slice(n - m, n)
+        a[sl] = getattr(c, f"a_{at}")
+        b[sl] = getattr(c, f"b_{at}")
+        k[sl] = getattr(c, f"inv_kappa_{at}")
+    return a, b, k
+
+
+def axis_tables(sc: Scene) -> list[AxisTables]:
+    out = []
+    for ax_i, ax in enumerate(sc.grid.axes):
+        t = ax.index_tables(sc.bloch_k[ax_i])
+        a_e, b_e, k_e = _pml_axis_arrays(sc, ax_i, "E")
+        a_h, b_h, k_h = _pml_axis_arrays(sc, ax_i, "H")
+        lo = sc.pml.get((ax_i, "lo"))
+        hi = sc.pml.get((ax_i, "hi"))
+        nlo = lo.num_layers if lo is not None else 0
+        nhi = hi.num_layers if hi is not None else 0
+        if nlo + nhi > ax.n:
+            raise ValueError(f"axis {ax_i}: PML layers {nlo}+{nhi} exceed {ax.n} cells")
+        out.append(AxisTables(
+            n=ax.n, idl_p=1.0 / ax.dl, idl_d=1.0 / ax.dl_dual,
+            nxt=t["nxt"], prv=t["prv"], mnx=t["mnx"], mpv=t["mpv"], pec=t["pec"],
+            a_e=a_e, b_e=b_e, k_e=k_e, a_h=a_h, b_h=b_h, k_h=k_h, pml_lo=nlo, pml_hi=nhi))
+    return out
+
+
+def e_coeffs(sc: Scene) -> tuple[np.ndarray, np.ndarray]:
+    """``(ca, cb)`` each ``(3, nx, ny, nz)`` float64 (reference ``openem/coeffs.py:98-136``)."""
+    shape = sc.shape
+    gg = sc.dispersion.g_dense(shape) if sc.any_dispersion else (np.zeros(shape),) * 3
+    eps = (sc.eps_ex, sc.eps_ey, sc.eps_ez)
+    sig = (sc.sigma_ex, sc.sigma_ey, sc.sigma_ez)
+    ca = np.empty((3,) + shape)
+    cb = np.empty((3,) + shape)
+    for c in range(3):
+        e = np.asarray(eps[c], dtype=np.float64)
+        s = np.zeros(shape) if sig[c] is None else np.asarray(sig[c]) * sc.dt / (2.0 * cpml.EPSILON_0)
+        den = e + s + gg[c]
+        ca[c] = (e - s) / den
+        cb[c] = (sc.dt / cpml.EPSILON_0) / den
+    for c, pidx in enumerate((sc.pec_ex, sc.pec_ey, sc.pec_ez)):
+        if pidx is None or np.asarray(pidx).size == 0:
+            continue
+        pidx = np.asarray(pidx, dtype=np.int64)
+        if pidx.min() < 0 or pidx.max() >= ca[c].size:
+            raise ValueError(f"pec_e{'xyz'[c]} flat index out of range")
+        ca[c].reshape(-1)[pidx] = 0.0
+        cb[c].reshape(-1)[pidx] = 0.0
+    return ca, cb
+
+
+def _material_classes(ca, cb, flag):
+    """Deduplicate the per-cell sextuple of float32 (ca, cb) bit patterns + dispersive flags."""
+    ncell = ca[0].size
+    empty = (np.zeros(0, np.uint16), np.zeros((0, LUT_STRIDE), np.float32))
+    # per component: dedup the (ca, cb) float32 bit pair -> small index
+    pairs, idx = [], []
+    for c in range(3):
+        a32 = ca[c].astype(np.float32).reshape(-1).view(np.uint32).astype(np.uint64)
+        b32 = cb[c].astype(np.float32).reshape(-1).view(np.uint32).astype(np.uint64)
+        u, inv = np.unique((a32 << np.uint64(32)) | b32, return_inverse=True)
+        if u.size > MAX_CLASSES:
+            return empty
+        pairs.append(u)
+        idx.append(inv.reshape(-1).astype(np.int64))
+    fl = (flag[0].reshape(-1).astype(np.int64) | (flag[1].reshape(-1).astype(np.int64) << 1)
+          | (flag[2].reshape(-1).astype(np.int64) << 2))
+    key = ((idx[0] * 65536 + idx[1]) * 65536 + idx[2]) * 8 + fl
+    ukey, inv = np.unique(key, return_inverse=True)
+    if ukey.size > MAX_CLASSES:
+        return empty
+    lut = np.zeros((ukey.size, LUT_STRIDE), np.uint32)
+    parts = (ukey // (8 * 65536 * 65536), (ukey // (8 * 65536)) % 65536, (ukey // 8) % 65536)
+    for c in range(3):
+        pr = pairs[c][parts[c]]
+        lut[:, 2 * c] = (pr >> np.uint64(32)).astype(np.uint32)
+        lut[:, 2 * c + 1] = (pr & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    lut[:, 6] = (ukey % 8).astype(np.uint32)
+    return inv.reshape(ca[0].shape).astype(np.uint16), lut.view(np.float32)
+
+
+def disp_tables(sc: Scene) -> DispTables | None:
+    if not sc.any_dispersion:
+        return None
+    d = sc.dispersion
+    d.validate(sc.shape)
+    am1_32 = d.am1.astype(np.complex64)
+    b_32 = d.b.astype(np.complex64)
+    key = np.stack([am1_32.real.view(np.uint32), am1_32.imag.view(np.uint32),
+                    b_32.real.view(np.uint32), b_32.imag.view(np.uint32)], axis=1)
+    uniq, inv = np.unique(key, axis=0, return_inverse=True)
+    u = uniq.view(np.float32)
+    nx, ny, nz = sc.shape
+    ii, jj, kk = np.unravel_index(d.cell, sc.shape)
+    hb = (int(ii.min()), int(ii.max()) + 1, int(jj.min()), int(jj.max()) + 1,
+          int(kk.min()), int(kk.max()) + 1)
+    return DispTables(
+        comp=d.comp.astype(np.int32), cell=d.cell.astype(np.int32),
+        ofs=d.pole_ofs.astype(np.int32), am1=d.am1.astype(np.complex128),
+        b=d.b.astype(np.complex128), cidx=inv.reshape(-1).astype(np.int32),
+        lut_am1=(u[:, 0] + 1j * u[:, 1]).astype(np.complex64),
+        lut_b=(u[:, 2] + 1j * u[:, 3]).astype(np.complex64), hist_box=hb)
+
+
+SIGNS = {+1: (+1.0, +1.0), -1: (-1.0, -1.0)}
+
+
+def plane_source_tables(sc: Scene) -> list[PlaneSrcTables]:
+    """Reference ``openem/sources_setup.py:57-119``: only z propagation with Ex/Ey polarisation;
+    sources of one polarisation on the same plane are summed."""
+    if not sc.sources:
+        return []
+    for s in sc.sources:
+        if s.axis != 2 or s.pol_axis not in (0, 1):
+            raise NotImplementedError("plane waves support z propagation with Ex/Ey polarisation")
+    keys = {(int(s.plane_index), int(s.direction)) for s in sc.sources}
+    if len(keys) != 1:
+        raise NotImplementedError(f"plane-wave sources must share plane and direction: {keys}")
+    az = sc.grid.z
+    groups: dict[int, list] = {}
+    for s in sc.sources:
+        groups.setdefault(int(s.pol_axis), []).append(s)
+    out = []
+    for pol, grp in sorted(groups.items()):
+        s0 = grp[0]
+        ks = int(s0.plane_index)
+        kh = ks - 1 if s0.direction > 0 else ks
+        sh, se = SIGNS[s0.direction]
+        sp = -1.0 if pol == 1 else 1.0
+        out.append(PlaneSrcTables(
+            pol=pol, kh=kh, ks=ks,
+            coef_h=sp * sh * sc.dt / cpml.MU_0 * float(1.0 / az.dl[kh]),
+            coef_e=se * float(1.0 / az.dl_dual[ks]),
+            ex_inc=np.ascontiguousarray(sum(x.ex_inc for x in grp), dtype=np.float64),
+            hy_inc=np.ascontiguousarray(sum(x.hy_inc for x in grp), dtype=np.float64)))
+    return out
+
+
+def dipole_tables(sc: Scene, n_total: int) -> list[DipoleTables]:
+    """Electric and magnetic dipole stencils (reference ``openem/sources_setup.py:419-457``)."""
+    out = []
+    ny, nz = sc.shape[1], sc.shape[2]
+    for magnetic in (False, True):
+        ds = [d for d in sc.dipoles if bool(d.magnetic) == magnetic]
+        if not ds:
+            continue
+        comp, cell, coef, src_of, amp = [], [], [], [], []
+        for s, d in enumerate(ds):
+            idx = np.asarray(d.indices, dtype=np.int64).reshape(-1, 3)
+            fl = (idx[:, 0] * ny + idx[:, 1]) * nz + idx[:, 2]
+            comp.append(np.full(fl.size, d.component, np.int32))
+            cell.append(fl.astype(np.int32))
+            coef.append(np.asarray(d.coef, dtype=np.float64))
+            src_of.append(np.full(fl.size, s, np.int32))
+            w = d.waveform.amp_int if magnetic else d.waveform.amp_half
+            if w.size < n_total + 1:
+                raise ValueError(f"dipole amplitude table holds {w.size} samples, "
+                                 f"{n_total} steps need {n_total + 1}")
+            amp.append(np.asarray(w[:n_total + 1], dtype=np.float64))
+        out.append(DipoleTables(magnetic, np.concatenate(comp), np.concatenate(cell),
+                                np.concatenate(coef), np.concatenate(src_of), np.stack(amp)))
+    return out
+
+
+def apod_window(t: np.ndarray, spec) -> np.ndarray:
+    """Flat-top window with Gaussian ramps, ``exp(-0.5((t-edge)/width)^2)`` outside
+    ``[start, end]`` (reference ``openem/apodization.py:27-45``)."""
+    w = np.ones_like(t)
+    if spec is None:
+        return w
+    start, end, width = spec
+    if start is not None:
+        m = t < start
+        w[m] = np.exp(-0.5 * ((t[m] - start) / width) ** 2)
+    if end is not None:
+        m = t > end
+        w[m] = np.exp(-0.5 * ((t[m] - end) / width) ** 2)
+    return w
+
+
+def phase_table(freqs: np.ndarray, dt: float, n_total: int, apod=None) -> np.ndarray:
+    """``(n_total, nf, 4)``: ``[cos w t_e, sin w t_e, cos w t_h, sin w t_h]`` times the window,
+    ``t_e = (n+1) dt``, ``t_h = (n+1/2) dt``."""
+    n = np.arange(n_total, dtype=np.float64)
+    te = (n + 1.0) * dt
+    th = (n + 0.5) * dt
+    we = apod_window(te, apod)
+    wh = apod_window(th, apod)
+    w = 2.0 * np.pi * np.asarray(freqs, dtype=np.float64)
+    out = np.empty((n_total, w.size, 4))
+    ae = w[None, :] * te[:, None]
+    ah = w[None, :] * th[:, None]
+    out[:, :, 0] = np.cos(ae) * we[:, None]
+    out[:, :, 1] = np.sin(ae) * we[:, None]
+    out[:, :, 2] = np.cos(ah) * wh[:, None]
+    out[:, :, 3] = np.sin(ah) * wh[:, None]
+    return out
+
+
+def monitor_tables(sc: Scene, n_total: int) -> list[MonitorTables]:
+    out = []
+    shape = sc.shape
+    for m in sc.flux_monitors:
+        t1, t2 = transverse_axes(m.axis)
+        if not (1 <= m.plane_index < shape[m.axis]):
+            raise ValueError(f"flux monitor {m.name}: plane {m.plane_index} needs a cell below it")
+        out.append(MonitorTables(
+            name=m.name, kind="plane", nf=len(m.freqs), freqs=np.asarray(m.freqs, np.float64),
+            phase=phase_table(m.freqs, sc.dt, n_total, m.apodization),
+            axis=int(m.axis), km=int(m.plane_index), n1=shape[t1], n2=shape[t2]))
+    for m in sc.field_monitors:
+        o, b = tuple(int(v) for v in m.origin), tuple(int(v) for v in m.box)
+        for a in range(3):
+            if o[a] < 0 or b[a] < 1 or o[a] + b[a] > shape[a]:
+                raise ValueError(f"field monitor {m.name}: box outside the grid")
+        out.append(MonitorTables(
+            name=m.name, kind="box", nf=len(m.freqs), freqs=np.asarray(m.freqs, np.float64),
+            phase=phase_table(m.freqs, sc.dt, n_total, m.apodization), origin=o, box=b))
+    return out
+
+
+def build(sc: Scene, n_total: int | None = None) -> StepTables:
+    n_total = int(sc.num_time_steps if n_total is None else n_total)
+    for s in sc.sources:
+        if s.ex_inc.size < n_total + 1 or s.hy_inc.size < n_total + 1:
+            raise ValueError(f"plane-wave tables hold {s.ex_inc.size} samples, "
+                             f"{n_total} steps need {n_total + 1}")
+    ca, cb = e_coeffs(sc)
+    flag = np.zeros((3,) + sc.shape, dtype=bool)
+    if sc.any_dispersion:
+        d = sc.dispersion
+        for c in range(3):
+            flag[c].reshape(-1)[d.cell[d.comp == c]] = True
+    return StepTables(
+        shape=sc.shape, n_total=n_total, dt=sc.dt, ch=sc.dt / cpml.MU_0,
+        cw=cpml.EPSILON_0 / sc.dt, axes=axis_tables(sc), ca=ca, cb=cb, disp_flag=flag,
+        disp=disp_tables(sc), plane_srcs=plane_source_tables(sc),
+        dipoles=dipole_tables(sc, n_total), monitors=monitor_tables(sc, n_total),
+        bloch=sc.any_bloch)

@@ -1,0 +1,190 @@
+"""Float64 NumPy oracle of the FDTD time step.  TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this module, and only as the checker / the reported CPU baseline. The product path
+(``openem_amd``) never calls it and fails loudly when the HIP library is missing.
+
+It restates the reference's step term by term, on the float64 master tables that
+``openem_amd.tables.build`` produces (the HIP library receives their float32 roundings):
+
+- H update: ``openem/kernels/yee.cu:76-137`` - forward difference ``(F[nxt]*mnx - F)/dl``,
+  CPML stretch ``s = invk*d + psi`` with ``psi <- b*psi + a*d`` (``yee.cu:60-74``), full-domain
+  psi with identity coefficients outside the PML;
+- E update: ``yee.cu:154-240`` - backward difference over the dual spacing,
+  ``E = (ca*E + cb*(s1 - s2 - cw*hist)) * pec_a * pec_b``;
+- ADE poles: ``openem/kernels/dispersion.cu:68-147`` - pre on E^n (``dP = (A-1)P + B E``,
+  ``hist = sum 2 Re dP``), post after every E-side injection (``P += B E^{n+1}``), the order of
+  ``openem/solver.py:1026-1041``;
+- plane-wave TF/SF: ``openem/kernels/source_dft.cu:40-73`` (H side after update_h, E side after
+  update_e with the cell's cb);
+- point dipoles: ``source_dft.cu:147-206`` (electric ``E -= cb*coef*amp_half[n]``, magnetic
+  ``H -= ch*coef*amp_int[n]``);
+- running DFT: ``source_dft.cu:216-271`` (box, native Yee positions) and ``:371-430`` (plane,
+  H colocated as ``(H[k]+H[k-1])/2``), E at ``(n+1)dt``, H at ``(n+1/2)dt``;
+- launch order of ``openem/solver.py:1145-1162``.
+
+Pinned by (see tests/test_oracle.py): discrete div(curl)=0, the Yee dispersion relation, the
+CPML reflection bound (``tests/test_pml_reflection.py:57-60`` of the reference), TF/SF
+one-sidedness, T+R=1 and the slab transmission against the reference's published value, the
+dispersive == equivalent-lossy identity (``tests/test_dispersion.py:224-274``), zero-residue
+poles as no-ops, the windowed DFT == tensordot of the time series
+(``tests/test_apodization.py:171-205``), and brute-force loops on tiny grids.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+_PSI = {  # (field comp, derivative axis) pairs, H and E sides
+    "hx_y": (0, 1), "hx_z": (0, 2), "hy_z": (1, 2), "hy_x": (1, 0), "hz_x": (2, 0), "hz_y": (2, 1),
+}
+
+
+def _bc(v: np.ndarray, axis: int) -> np.ndarray:
+    shp = [1, 1, 1]
+    shp[axis] = -1
+    return np.asarray(v).reshape(shp)
+
+
+class Oracle64:
+    """Real-field (non-Bloch) stepper in float64."""
+
+    def __init__(self, tabs):
+        if tabs.bloch:
+            raise NotImplementedError("use oracle.bloch64 for Bloch scenes")
+        self.t = tabs
+        shp = tabs.shape
+        self.E = np.zeros((3,) + shp)
+        self.H = np.zeros((3,) + shp)
+        self.psi_h = {k: np.zeros(shp) for k in _PSI}
+        self.psi_e = {k: np.zeros(shp) for k in _PSI}
+        ax = tabs.axes
+        self._m = [dict(nxt=a.nxt, prv=a.prv, mnx=_bc(a.mnx.real, i), mpv=_bc(a.mpv.real, i),
+                        idp=_bc(a.idl_p, i), idd=_bc(a.idl_d, i), pec=_bc(a.pec, i),
+                        aE=_bc(a.a_e, i), bE=_bc(a.b_e, i), kE=_bc(a.k_e, i),
+                        aH=_bc(a.a_h, i), bH=_bc(a.b_h, i), kH=_bc(a.k_h, i))
+                   for i, a in enumerate(ax)]
+        d = tabs.disp
+        self.hist = np.zeros((3,) + shp)
+        if d is not None:
+            self.P = np.zeros(d.am1.size, dtype=np.complex128)
+            self._pole_entry = np.repeat(np.arange(d.comp.size), np.diff(d.ofs))
+            self._idx4 = (d.comp.astype(np.int64),) + np.unravel_index(
+                d.cell.astype(np.int64), shp)
+        self.acc = []
+        for m in tabs.monitors:
+            if m.kind == "plane":
+                self.acc.append(np.zeros((4, m.nf, m.n1, m.n2), dtype=np.complex128))
+            else:
+                self.acc.append(np.zeros((6, m.nf) + tuple(m.box), dtype=np.complex128))
+        self.n = 0
+
+    # -- derivatives ------------------------------------------------------------------------
+    def _dfwd(self, F, a):
+        m = self._m[a]
+        return (np.take(F, m["nxt"], axis=a) * m["mnx"] - F) * m["idp"]
+
+    def _dbwd(self, F, a):
+        m = self._m[a]
+        return (F - np.take(F, m["prv"], axis=a) * m["mpv"]) * m["idd"]
+
+    def _stretch(self, psi, d, a, side):
+        m = self._m[a]
+        psi *= m["b" + side]
+        psi += m["a" + side] * d
+        return m["k" + side] * d + psi
+
+    # -- one step ---------------------------------------------------------------------------
+    def step(self):
+        t, n = self.t, self.n
+        if n >= t.n_total:
+            raise ValueError(f"step {n} beyond the {t.n_total}-step tables")
+        E, H = self.E, self.H
+        ch = t.ch
+        ph, st = self.psi_h, self._stretch
+        sy = st(ph["hx_y"], self._dfwd(E[2], 1), 1, "H")
+        sz = st(ph["hx_z"], self._dfwd(E[1], 2), 2, "H")
+        hx_new = H[0] - ch * (sy - sz)
+        sz = st(ph["hy_z"], self._dfwd(E[0], 2), 2, "H")
+        sx = st(ph["hy_x"], self._dfwd(E[2], 0), 0, "H")
+        hy_new = H[1] - ch * (sz - sx)
+        sx = st(ph["hz_x"], self._dfwd(E[1], 0), 0, "H")
+        sy = st(ph["hz_y"], self._dfwd(E[0], 1), 1, "H")
+        H[2] -= ch * (sx - sy)
+        H[0] = hx_new
+        H[1] = hy_new
+        for dp in t.dipoles:
+            if dp.magnetic:
+                v = ch * dp.coef * dp.amp[dp.src_of, n]
+                for c in range(3):
+                    sel = dp.comp == c
+                    np.subtract.at(H[c].reshape(-1), dp.cell[sel].astype(np.int64), v[sel])
+        for ps in t.plane_srcs:
+            hc = 1 if ps.pol == 0 else 0
+            H[hc][:, :, ps.kh] += ps.coef_h * ps.ex_inc[n]
+        # ---- E phase ----
+        d = t.disp
+        if d is not None:
+            en = E[self._idx4]
+            dP = d.am1 * self.P + d.b * en[self._pole_entry]
+            self.P += dP
+            w = np.bincount(self._pole_entry, weights=2.0 * dP.real, minlength=d.comp.size)
+            self.hist[self._idx4] = w
+        pe = self.psi_e
+        m = self._m
+        ca, cb, cw, hist = t.ca, t.cb, t.cw, self.hist
+        sy = st(pe["hx_y"], self._dbwd(H[2], 1), 1, "E")
+        sz = st(pe["hx_z"], self._dbwd(H[1], 2), 2, "E")
+        ex_new = (ca[0] * E[0] + cb[0] * (sy - sz - cw * hist[0])) * m[1]["pec"] * m[2]["pec"]
+        sz = st(pe["hy_z"], self._dbwd(H[0], 2), 2, "E")
+        sx = st(pe["hy_x"], self._dbwd(H[2], 0), 0, "E")
+        ey_new = (ca[1] * E[1] + cb[1] * (sz - sx - cw * hist[1])) * m[2]["pec"] * m[0]["pec"]
+        sx = st(pe["hz_x"], self._dbwd(H[1], 0), 0, "E")
+        sy = st(pe["hz_y"], self._dbwd(H[0], 1), 1, "E")
+        E[2] = (ca[2] * E[2] + cb[2] * (sx - sy - cw * hist[2])) * m[0]["pec"] * m[1]["pec"]
+        E[0] = ex_new
+        E[1] = ey_new
+        for ps in t.plane_srcs:
+            E[ps.pol][:, :, ps.ks] += cb[ps.pol][:, :, ps.ks] * ps.coef_e * ps.hy_inc[n]
+        for dp in t.dipoles:
+            if not dp.magnetic:
+                for c in range(3):
+                    sel = dp.comp == c
+                    cells = dp.cell[sel].astype(np.int64)
+                    v = cb[c].reshape(-1)[cells] * dp.coef[sel] * dp.amp[dp.src_of[sel], n]
+                    np.subtract.at(E[c].reshape(-1), cells, v)
+        if d is not None:
+            en = E[self._idx4]
+            self.P += d.b * en[self._pole_entry]
+        self._dft(n)
+        self.n += 1
+
+    def _dft(self, n):
+        E, H = self.E, self.H
+        for m, acc in zip(self.t.monitors, self.acc):
+            p = m.phase[n]                                  # (nf, 4)
+            we = (p[:, 0] + 1j * p[:, 1])[:, None, None]
+            wh = (p[:, 2] + 1j * p[:, 3])[:, None, None]
+            if m.kind == "plane":
+                a = m.axis
+                t1, t2 = [i for i in range(3) if i != a]
+                sl = [slice(None)] * 3
+                sl[a] = m.km
+                slm = list(sl)
+                slm[a] = m.km - 1
+                sl, slm = tuple(sl), tuple(slm)
+                vals = (E[t1][sl], E[t2][sl], 0.5 * (H[t1][sl] + H[t1][slm]),
+                        0.5 * (H[t2][sl] + H[t2][slm]))
+                for c in range(4):
+                    acc[c] += vals[c][None] * (we if c < 2 else wh)
+            else:
+                (i0, j0, k0), (ni, nj, nk) = m.origin, m.box
+                sl = (slice(i0, i0 + ni), slice(j0, j0 + nj), slice(k0, k0 + nk))
+                for c in range(6):
+                    F = E[c] if c < 3 else H[c - 3]
+                    acc[c] += F[sl][None] * (we[..., None] if c < 3 else wh[..., None])
+
+    def run(self, n_steps: int):
+        for _ in range(int(n_steps)):
+            self.step()
+        return self
