@@ -17,33 +17,39 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define OEM_TILE 256
 #define OEM_NXCD 8
 #define OEM_MAX_PW 2
 
+// Per-axis tables interleaved as tab[i*AX_W + field]: one pointer per axis keeps the kernel
+// argument block (and the SGPRs holding it) small.
+enum { AX_IDLP = 0, AX_IDLD, AX_MNX, AX_MPV, AX_PEC, AX_AE, AX_BE, AX_KE, AX_AH, AX_BH, AX_KH,
+       AX_W = 12 };
+
 struct AxisDev {
-    const float *idl_p, *idl_d;
+    const float *tab;       // n * AX_W
     const int *nxt, *prv;
-    const float *mnx, *mpv, *pec;
-    const float *a_e, *b_e, *k_e;
-    const float *a_h, *b_h, *k_h;
     int n, lo, hi;
 };
+
+__device__ __forceinline__ float axv(const AxisDev &A, int i, int f) { return A.tab[i * AX_W + f]; }
 
 struct GridDev {
     AxisDev ax[3];
     int nx, ny, nz;
     int plane;          // ny*nz
-    int tiles;          // ceil(plane / OEM_TILE)
+    int tiles;          // k-tiles * j-tiles of the (ny, nz) plane
     int chunk;          // i-planes per block
     int nblocks;        // tiles * ceil(nx / chunk)
 };
 
-// psi slabs; index [2*axis + c] where c selects the field component pair member:
+// psi slabs: one allocation per side (H, E) holding six sub-arrays at element offsets
+// off[2*axis + c], where c selects the field component pair member:
 //   H side: x -> (Hy, Hz), y -> (Hx, Hz), z -> (Hx, Hy); E side likewise with E.
+// x slab (wx, ny, nz), y slab (nx, wy, nz), z slab (nx, ny, wz), w = lo + hi layers.
 struct PsiDev {
-    float *h[6];
-    float *e[6];
+    float *h, *e;
+    uint32_t off[6];
+    uint32_t total;     // elements per side
 };
 
 struct PlaneSrcDev {
@@ -55,6 +61,7 @@ struct PlaneSrcDev {
 
 struct MatDev {
     int mode;           // 0: class LUT, 1: dense ca/cb
+    int n_classes;
     const uint16_t *cls;
     const float *lut;   // n_classes * 8
     const float *ca, *cb;   // dense: 3*ncell each
@@ -83,38 +90,89 @@ __device__ __forceinline__ float stretch(float *psi, size_t id, float d, float a
 
 __device__ __forceinline__ int step_now(const int *stp) { return *stp; }
 
+// Buffer resources: an SGPR descriptor + a fixed 32-bit per-lane VGPR offset + a per-iteration
+// SGPR offset.  This keeps the x march's address arithmetic out of the VGPRs (the compiler
+// would otherwise carry one 64-bit pointer per stream).  gfx9-family descriptor word 3.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+#define OEM_RSRC_W3 0x00020000
+
+__device__ __forceinline__ rsrc_t mk_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             OEM_RSRC_W3);
+}
+__device__ __forceinline__ float bld(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ uint32_t bld_u16(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+}
+__device__ __forceinline__ void bst(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
+}
+
+// Tile of the (ny, nz) plane owned by one block: OEM_TK k-lanes x OEM_TJ j-rows.
+#define OEM_TK 64
+#define OEM_TJ 4
+#define OEM_LDS_CLASSES 512
+
+struct TileCoord {
+    int j, k, i0, i1;
+    bool ok;
+};
+
+__device__ __forceinline__ TileCoord tile_coord(const GridDev &g) {
+    const int lb = xcd_remap(blockIdx.x, g.nblocks);
+    const int nkt = (g.nz + OEM_TK - 1) / OEM_TK;
+    const int tile = lb % g.tiles, chunk = lb / g.tiles;
+    const int jt = tile / nkt, kt = tile - jt * nkt;
+    TileCoord t;
+    t.k = kt * OEM_TK + (int)threadIdx.x;
+    t.j = jt * OEM_TJ + (int)threadIdx.y;
+    t.i0 = chunk * g.chunk;
+    t.i1 = min(t.i0 + g.chunk, g.nx);
+    t.ok = t.k < g.nz && t.j < g.ny;
+    return t;
+}
+
+// CPML recursion on one psi element already loaded: psi' = b*psi + a*d, s = invk*d + psi'.
+__device__ __forceinline__ float stretch_v(float &psi, float d, float a, float b, float invk) {
+    psi = b * psi + a * d;
+    return invk * d + psi;
+}
+
 // ------------------------------------------------------------------------------------------
 // H half step (reference update_h, `openem/kernels/yee.cu:76-137`) fused with the H side of
 // the plane-wave TF/SF correction (`openem/kernels/source_dft.cu:40-55`).
 // ------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(OEM_TILE)
-k_update_h(float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
-           const float *__restrict__ Ex, const float *__restrict__ Ey,
-           const float *__restrict__ Ez, PsiDev ps, GridDev g, float ch, PlaneSrcDev pw,
-           const int *__restrict__ stp)
+__global__ void __launch_bounds__(OEM_TK * OEM_TJ)
+k_update_h(float *__restrict__ H, const float *__restrict__ E, PsiDev ps, GridDev g, float ch,
+           PlaneSrcDev pw, const int *__restrict__ stp)
 {
-    const int lb = xcd_remap(blockIdx.x, g.nblocks);
-    const int tile = lb % g.tiles, chunk = lb / g.tiles;
-    const int p = tile * OEM_TILE + threadIdx.x;
-    if (p >= g.plane) return;
+    const TileCoord tc = tile_coord(g);
+    if (!tc.ok) return;
+    const int j = tc.j, k = tc.k;
     const int nz = g.nz, ny = g.ny, nx = g.nx;
-    const int j = p / nz, k = p - j * nz;
-    const int i0 = chunk * g.chunk;
-    const int i1 = min(i0 + g.chunk, nx);
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const size_t ncell = (size_t)g.plane * nx;
+    const uint32_t cb = (uint32_t)(ncell * 4);
+    const rsrc_t rEx = mk_rsrc(E, cb), rEy = mk_rsrc(E + ncell, cb), rEz = mk_rsrc(E + 2 * ncell, cb);
+    const rsrc_t rHx = mk_rsrc(H, cb), rHy = mk_rsrc(H + ncell, cb), rHz = mk_rsrc(H + 2 * ncell, cb);
+    const rsrc_t rP = mk_rsrc(ps.h, ps.total * 4u);
 
-    const int pj = Y.nxt[j] * nz + k;      // (j+1, k) inside the plane
-    const int pk = j * nz + Z.nxt[k];      // (j, k+1)
-    const float mj = Y.mnx[j], mk = Z.mnx[k];
-    const float idy = Y.idl_p[j], idz = Z.idl_p[k];
+    const uint32_t op = (uint32_t)(j * nz + k) * 4u;              // own cell in the plane
+    const uint32_t oj = (uint32_t)(Y.nxt[j] * nz + k) * 4u;       // (j+1, k)
+    const uint32_t ok = (uint32_t)(j * nz + Z.nxt[k]) * 4u;       // (j, k+1)
+    const float mj = axv(Y, j, AX_MNX), mk = axv(Z, k, AX_MNX);
+    const float idy = axv(Y, j, AX_IDLP), idz = axv(Z, k, AX_IDLP);
     const int sy = slab_of(j, ny, Y.lo, Y.hi);
     const int sz = slab_of(k, nz, Z.lo, Z.hi);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
     float ay = 0.f, by = 1.f, ky = 1.f, az = 0.f, bz = 1.f, kz = 1.f;
-    if (sy >= 0) { ay = Y.a_h[j]; by = Y.b_h[j]; ky = Y.k_h[j]; }
-    if (sz >= 0) { az = Z.a_h[k]; bz = Z.b_h[k]; kz = Z.k_h[k]; }
+    if (sy >= 0) { ay = axv(Y, j, AX_AH); by = axv(Y, j, AX_BH); ky = axv(Y, j, AX_KH); }
+    if (sz >= 0) { az = axv(Z, k, AX_AH); bz = axv(Z, k, AX_BH); kz = axv(Z, k, AX_KH); }
+    const uint32_t oy = (uint32_t)(max(sy, 0) * nz + k) * 4u;     // psi y-slab, within plane i
+    const uint32_t oz = (uint32_t)(j * wz + max(sz, 0)) * 4u;     // psi z-slab, within plane i
 
-    // plane-wave H correction of this column (k == kh), amplitude of this step
     float inj_hx = 0.f, inj_hy = 0.f;
     if (pw.n) {
         const int n = step_now(stp);
@@ -126,20 +184,29 @@ k_update_h(float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ H
         }
     }
 
-    const size_t plane = (size_t)g.plane;
-    float ey = Ey[(size_t)i0 * plane + p];
-    float ez = Ez[(size_t)i0 * plane + p];
-    for (int i = i0; i < i1; ++i) {
-        const size_t base = (size_t)i * plane;
-        const size_t id = base + p;
-        const size_t idn = (size_t)X.nxt[i] * plane + p;
-        const float mi = X.mnx[i], idx = X.idl_p[i];
-        const float ex = Ex[id];
-        const float ex_j = Ex[base + pj], ex_k = Ex[base + pk];
-        const float ey_k = Ey[base + pk];
-        const float ez_j = Ez[base + pj];
-        const float ey_n = Ey[idn], ez_n = Ez[idn];
+    const uint32_t plane4 = (uint32_t)g.plane * 4u;
+    float ey = bld(rEy, op, (uint32_t)tc.i0 * plane4);
+    float ez = bld(rEz, op, (uint32_t)tc.i0 * plane4);
+    for (int i = tc.i0; i < tc.i1; ++i) {
+        const uint32_t so = (uint32_t)i * plane4;
+        const uint32_t sn = (uint32_t)X.nxt[i] * plane4;
+        const float ex = bld(rEx, op, so), ex_j = bld(rEx, oj, so), ex_k = bld(rEx, ok, so);
+        const float ey_k = bld(rEy, ok, so), ez_j = bld(rEz, oj, so);
+        const float ey_n = bld(rEy, op, sn), ez_n = bld(rEz, op, sn);
+        float hx = bld(rHx, op, so), hy = bld(rHy, op, so), hz = bld(rHz, op, so);
+        const int sx = slab_of(i, nx, X.lo, X.hi);
+        const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * 4u;
+        const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * 4u;
+        const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * 4u;
+        const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * 4u;
+        const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * 4u;
+        const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * 4u;
+        float psx0 = 0.f, psx1 = 0.f, psy0 = 0.f, psy1 = 0.f, psz0 = 0.f, psz1 = 0.f;
+        if (sx >= 0) { psx0 = bld(rP, op, sx0); psx1 = bld(rP, op, sx1); }
+        if (sy >= 0) { psy0 = bld(rP, oy, sy0); psy1 = bld(rP, oy, sy1); }
+        if (sz >= 0) { psz0 = bld(rP, oz, sz0); psz1 = bld(rP, oz, sz1); }
 
+        const float mi = axv(X, i, AX_MNX), idx = axv(X, i, AX_IDLP);
         const float dzy = (ez_j * mj - ez) * idy;
         const float dyz = (ey_k * mk - ey) * idz;
         const float dxz = (ex_k * mk - ex) * idz;
@@ -148,26 +215,31 @@ k_update_h(float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ H
         const float dxy = (ex_j * mj - ex) * idy;
 
         float s_hx_y = dzy, s_hz_y = dxy, s_hx_z = dyz, s_hy_z = dxz, s_hy_x = dzx, s_hz_x = dyx;
-        const int sx = slab_of(i, nx, X.lo, X.hi);
         if (sx >= 0) {
-            const size_t q = ((size_t)sx * ny + j) * nz + k;
-            const float ax_ = X.a_h[i], bx_ = X.b_h[i], kx_ = X.k_h[i];
-            s_hy_x = stretch(ps.h[0], q, dzx, ax_, bx_, kx_);
-            s_hz_x = stretch(ps.h[1], q, dyx, ax_, bx_, kx_);
+            const float ax_ = axv(X, i, AX_AH), bx_ = axv(X, i, AX_BH), kx_ = axv(X, i, AX_KH);
+            s_hy_x = stretch_v(psx0, dzx, ax_, bx_, kx_);
+            s_hz_x = stretch_v(psx1, dyx, ax_, bx_, kx_);
+            bst(rP, psx0, op, sx0);
+            bst(rP, psx1, op, sx1);
         }
         if (sy >= 0) {
-            const size_t q = ((size_t)i * wy + sy) * nz + k;
-            s_hx_y = stretch(ps.h[2], q, dzy, ay, by, ky);
-            s_hz_y = stretch(ps.h[3], q, dxy, ay, by, ky);
+            s_hx_y = stretch_v(psy0, dzy, ay, by, ky);
+            s_hz_y = stretch_v(psy1, dxy, ay, by, ky);
+            bst(rP, psy0, oy, sy0);
+            bst(rP, psy1, oy, sy1);
         }
         if (sz >= 0) {
-            const size_t q = ((size_t)i * ny + j) * wz + sz;
-            s_hx_z = stretch(ps.h[4], q, dyz, az, bz, kz);
-            s_hy_z = stretch(ps.h[5], q, dxz, az, bz, kz);
+            s_hx_z = stretch_v(psz0, dyz, az, bz, kz);
+            s_hy_z = stretch_v(psz1, dxz, az, bz, kz);
+            bst(rP, psz0, oz, sz0);
+            bst(rP, psz1, oz, sz1);
         }
-        Hx[id] = Hx[id] - ch * (s_hx_y - s_hx_z) + inj_hx;
-        Hy[id] = Hy[id] - ch * (s_hy_z - s_hy_x) + inj_hy;
-        Hz[id] = Hz[id] - ch * (s_hz_x - s_hz_y);
+        hx = hx - ch * (s_hx_y - s_hx_z) + inj_hx;
+        hy = hy - ch * (s_hy_z - s_hy_x) + inj_hy;
+        hz = hz - ch * (s_hz_x - s_hz_y);
+        bst(rHx, hx, op, so);
+        bst(rHy, hy, op, so);
+        bst(rHz, hz, op, so);
         ey = ey_n;
         ez = ez_n;
     }
@@ -175,37 +247,52 @@ k_update_h(float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ H
 
 // ------------------------------------------------------------------------------------------
 // E whole step (reference update_e / update_e_lut, `openem/kernels/yee.cu:154-337`) fused with
-// the E side of the plane-wave TF/SF correction (`openem/kernels/source_dft.cu:58-73`).
+// the E side of the plane-wave TF/SF correction (`openem/kernels/source_dft.cu:58-73`).  The
+// class LUT is staged in LDS and the next plane's class index is prefetched, so the per-cell
+// coefficient gather never waits on a global load issued in the same iteration.
 // ------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(OEM_TILE)
-k_update_e(float *__restrict__ Ex, float *__restrict__ Ey, float *__restrict__ Ez,
-           const float *__restrict__ Hx, const float *__restrict__ Hy,
-           const float *__restrict__ Hz, PsiDev ps, GridDev g, MatDev mat, float cw,
-           PlaneSrcDev pw, const int *__restrict__ stp)
+__global__ void __launch_bounds__(OEM_TK * OEM_TJ)
+k_update_e(float *__restrict__ E, const float *__restrict__ H, PsiDev ps, GridDev g,
+           MatDev mat, float cw, PlaneSrcDev pw, const int *__restrict__ stp)
 {
-    const int lb = xcd_remap(blockIdx.x, g.nblocks);
-    const int tile = lb % g.tiles, chunk = lb / g.tiles;
-    const int p = tile * OEM_TILE + threadIdx.x;
-    if (p >= g.plane) return;
+    __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
+    const bool lds_lut = mat.mode == 0 && mat.n_classes <= OEM_LDS_CLASSES;
+    if (lds_lut) {
+        const int tid = threadIdx.y * OEM_TK + threadIdx.x;
+        const float4 *src = reinterpret_cast<const float4 *>(mat.lut);
+        for (int q = tid; q < 2 * mat.n_classes; q += OEM_TK * OEM_TJ) s_lut[q] = src[q];
+        __syncthreads();
+    }
+    const TileCoord tc = tile_coord(g);
+    if (!tc.ok) return;
+    const int j = tc.j, k = tc.k;
     const int nz = g.nz, ny = g.ny, nx = g.nx;
-    const int j = p / nz, k = p - j * nz;
-    const int i0 = chunk * g.chunk;
-    const int i1 = min(i0 + g.chunk, nx);
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const size_t ncell = (size_t)g.plane * nx;
+    const uint32_t cbytes = (uint32_t)(ncell * 4);
+    const rsrc_t rEx = mk_rsrc(E, cbytes), rEy = mk_rsrc(E + ncell, cbytes),
+                 rEz = mk_rsrc(E + 2 * ncell, cbytes);
+    const rsrc_t rHx = mk_rsrc(H, cbytes), rHy = mk_rsrc(H + ncell, cbytes),
+                 rHz = mk_rsrc(H + 2 * ncell, cbytes);
+    const rsrc_t rP = mk_rsrc(ps.e, ps.total * 4u);
+    const rsrc_t rC = mk_rsrc(mat.cls, (uint32_t)(ncell * 2));
 
-    const int pj = Y.prv[j] * nz + k;      // (j-1, k)
-    const int pk = j * nz + Z.prv[k];      // (j, k-1)
-    const float mj = Y.mpv[j], mk = Z.mpv[k];
-    const float idy = Y.idl_d[j], idz = Z.idl_d[k];
-    const float pecy = Y.pec[j], pecz = Z.pec[k];
+    const uint32_t op = (uint32_t)(j * nz + k) * 4u;
+    const uint32_t oj = (uint32_t)(Y.prv[j] * nz + k) * 4u;       // (j-1, k)
+    const uint32_t ok = (uint32_t)(j * nz + Z.prv[k]) * 4u;       // (j, k-1)
+    const float mj = axv(Y, j, AX_MPV), mk = axv(Z, k, AX_MPV);
+    const float idy = axv(Y, j, AX_IDLD), idz = axv(Z, k, AX_IDLD);
+    const float pecy = axv(Y, j, AX_PEC), pecz = axv(Z, k, AX_PEC);
     const int sy = slab_of(j, ny, Y.lo, Y.hi);
     const int sz = slab_of(k, nz, Z.lo, Z.hi);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
     float ay = 0.f, by = 1.f, ky = 1.f, az = 0.f, bz = 1.f, kz = 1.f;
-    if (sy >= 0) { ay = Y.a_e[j]; by = Y.b_e[j]; ky = Y.k_e[j]; }
-    if (sz >= 0) { az = Z.a_e[k]; bz = Z.b_e[k]; kz = Z.k_e[k]; }
+    if (sy >= 0) { ay = axv(Y, j, AX_AE); by = axv(Y, j, AX_BE); ky = axv(Y, j, AX_KE); }
+    if (sz >= 0) { az = axv(Z, k, AX_AE); bz = axv(Z, k, AX_BE); kz = axv(Z, k, AX_KE); }
+    const uint32_t oy = (uint32_t)(max(sy, 0) * nz + k) * 4u;
+    const uint32_t oz = (uint32_t)(j * wz + max(sz, 0)) * 4u;
 
-    float inj_ex = 0.f, inj_ey = 0.f;       // coef_e * Hy_inc of this column (k == ks)
+    float inj_ex = 0.f, inj_ey = 0.f;
     bool inj_x = false, inj_y = false;
     if (pw.n) {
         const int n = step_now(stp);
@@ -217,25 +304,65 @@ k_update_e(float *__restrict__ Ex, float *__restrict__ Ey, float *__restrict__ E
             }
         }
     }
-    const bool hrow = mat.hist && j >= mat.hb[2] && j < mat.hb[3] && k >= mat.hb[4] &&
-                      k < mat.hb[5];
     const int hbx = mat.hb[1] - mat.hb[0], hby = mat.hb[3] - mat.hb[2],
               hbz = mat.hb[5] - mat.hb[4];
-    const size_t hcomp = (size_t)hbx * hby * hbz;
+    const bool hrow = mat.hist && j >= mat.hb[2] && j < mat.hb[3] && k >= mat.hb[4] &&
+                      k < mat.hb[5];
+    const uint32_t hcomp = (uint32_t)(hbx * hby * hbz);
+    const rsrc_t rHist = mk_rsrc(mat.hist, 3u * hcomp * 4u);
+    const uint32_t oh = hrow ? (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * 4u : 0u;
+    const uint32_t oc = (uint32_t)(j * nz + k) * 2u;               // uint16 class index
 
-    const size_t plane = (size_t)g.plane;
-    const size_t ncell = plane * nx;
-    const size_t idp0 = (size_t)X.prv[i0] * plane + p;
-    float hy_m = Hy[idp0], hz_m = Hz[idp0];
-    for (int i = i0; i < i1; ++i) {
-        const size_t base = (size_t)i * plane;
-        const size_t id = base + p;
-        const float mi = X.mpv[i], idx = X.idl_d[i], pecx = X.pec[i];
-        const float hx = Hx[id], hy = Hy[id], hz = Hz[id];
-        const float hx_j = Hx[base + pj], hx_k = Hx[base + pk];
-        const float hy_k = Hy[base + pk];
-        const float hz_j = Hz[base + pj];
+    const uint32_t plane4 = (uint32_t)g.plane * 4u, plane2 = (uint32_t)g.plane * 2u;
+    const uint32_t sp0 = (uint32_t)X.prv[tc.i0] * plane4;
+    float hy_m = bld(rHy, op, sp0), hz_m = bld(rHz, op, sp0);
+    uint32_t cls_next = mat.mode == 0 ? bld_u16(rC, oc, (uint32_t)tc.i0 * plane2) : 0u;
+    for (int i = tc.i0; i < tc.i1; ++i) {
+        const uint32_t so = (uint32_t)i * plane4;
+        const uint32_t cls = cls_next;
+        if (mat.mode == 0 && i + 1 < tc.i1) cls_next = bld_u16(rC, oc, so / 2u + plane2);
+        const float hx = bld(rHx, op, so), hy = bld(rHy, op, so), hz = bld(rHz, op, so);
+        const float hx_j = bld(rHx, oj, so), hx_k = bld(rHx, ok, so);
+        const float hy_k = bld(rHy, ok, so), hz_j = bld(rHz, oj, so);
+        float ex = bld(rEx, op, so), ey = bld(rEy, op, so), ez = bld(rEz, op, so);
+        const int sx = slab_of(i, nx, X.lo, X.hi);
+        const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * 4u;
+        const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * 4u;
+        const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * 4u;
+        const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * 4u;
+        const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * 4u;
+        const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * 4u;
+        float psx0 = 0.f, psx1 = 0.f, psy0 = 0.f, psy1 = 0.f, psz0 = 0.f, psz1 = 0.f;
+        if (sx >= 0) { psx0 = bld(rP, op, sx0); psx1 = bld(rP, op, sx1); }
+        if (sy >= 0) { psy0 = bld(rP, oy, sy0); psy1 = bld(rP, oy, sy1); }
+        if (sz >= 0) { psz0 = bld(rP, oz, sz0); psz1 = bld(rP, oz, sz1); }
 
+        float cax, cbx, cay, cby, caz, cbz;
+        uint32_t fl;
+        if (lds_lut) {
+            const float4 r0 = s_lut[2 * cls], r1 = s_lut[2 * cls + 1];
+            cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
+            fl = __float_as_uint(r1.z);
+        } else if (mat.mode == 0) {
+            const float4 *row = reinterpret_cast<const float4 *>(mat.lut + 8u * cls);
+            const float4 r0 = row[0], r1 = row[1];
+            cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
+            fl = __float_as_uint(r1.z);
+        } else {
+            const size_t id = (size_t)i * g.plane + (size_t)j * nz + k;
+            cax = mat.ca[id]; cay = mat.ca[ncell + id]; caz = mat.ca[2 * ncell + id];
+            cbx = mat.cb[id]; cby = mat.cb[ncell + id]; cbz = mat.cb[2 * ncell + id];
+            fl = (hrow && i >= mat.hb[0] && i < mat.hb[1]) ? 7u : 0u;
+        }
+        float phx = 0.f, phy = 0.f, phz = 0.f;
+        if (fl) {
+            const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * 4u;
+            if (fl & 1u) phx = bld(rHist, oh, sh);
+            if (fl & 2u) phy = bld(rHist, oh, sh + hcomp * 4u);
+            if (fl & 4u) phz = bld(rHist, oh, sh + 2u * hcomp * 4u);
+        }
+
+        const float mi = axv(X, i, AX_MPV), idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
         const float dzy = (hz - hz_j * mj) * idy;
         const float dyz = (hy - hy_k * mk) * idz;
         const float dxz = (hx - hx_k * mk) * idz;
@@ -244,52 +371,33 @@ k_update_e(float *__restrict__ Ex, float *__restrict__ Ey, float *__restrict__ E
         const float dxy = (hx - hx_j * mj) * idy;
 
         float s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
-        const int sx = slab_of(i, nx, X.lo, X.hi);
         if (sx >= 0) {
-            const size_t q = ((size_t)sx * ny + j) * nz + k;
-            const float ax_ = X.a_e[i], bx_ = X.b_e[i], kx_ = X.k_e[i];
-            s_ey_x = stretch(ps.e[0], q, dzx, ax_, bx_, kx_);
-            s_ez_x = stretch(ps.e[1], q, dyx, ax_, bx_, kx_);
+            const float ax_ = axv(X, i, AX_AE), bx_ = axv(X, i, AX_BE), kx_ = axv(X, i, AX_KE);
+            s_ey_x = stretch_v(psx0, dzx, ax_, bx_, kx_);
+            s_ez_x = stretch_v(psx1, dyx, ax_, bx_, kx_);
+            bst(rP, psx0, op, sx0);
+            bst(rP, psx1, op, sx1);
         }
         if (sy >= 0) {
-            const size_t q = ((size_t)i * wy + sy) * nz + k;
-            s_ex_y = stretch(ps.e[2], q, dzy, ay, by, ky);
-            s_ez_y = stretch(ps.e[3], q, dxy, ay, by, ky);
+            s_ex_y = stretch_v(psy0, dzy, ay, by, ky);
+            s_ez_y = stretch_v(psy1, dxy, ay, by, ky);
+            bst(rP, psy0, oy, sy0);
+            bst(rP, psy1, oy, sy1);
         }
         if (sz >= 0) {
-            const size_t q = ((size_t)i * ny + j) * wz + sz;
-            s_ex_z = stretch(ps.e[4], q, dyz, az, bz, kz);
-            s_ey_z = stretch(ps.e[5], q, dxz, az, bz, kz);
+            s_ex_z = stretch_v(psz0, dyz, az, bz, kz);
+            s_ey_z = stretch_v(psz1, dxz, az, bz, kz);
+            bst(rP, psz0, oz, sz0);
+            bst(rP, psz1, oz, sz1);
         }
-
-        float cax, cbx, cay, cby, caz, cbz;
-        uint32_t fl;
-        if (mat.mode == 0) {
-            const float4 *row = reinterpret_cast<const float4 *>(mat.lut + 8 * (size_t)mat.cls[id]);
-            const float4 r0 = row[0], r1 = row[1];
-            cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
-            fl = __float_as_uint(r1.z);
-        } else {
-            cax = mat.ca[id]; cay = mat.ca[ncell + id]; caz = mat.ca[2 * ncell + id];
-            cbx = mat.cb[id]; cby = mat.cb[ncell + id]; cbz = mat.cb[2 * ncell + id];
-            fl = (hrow && i >= mat.hb[0] && i < mat.hb[1]) ? 7u : 0u;
-        }
-        float phx = 0.f, phy = 0.f, phz = 0.f;
-        if (fl) {
-            const size_t hq = ((size_t)(i - mat.hb[0]) * hby + (j - mat.hb[2])) * hbz +
-                              (k - mat.hb[4]);
-            if (fl & 1u) phx = mat.hist[hq];
-            if (fl & 2u) phy = mat.hist[hcomp + hq];
-            if (fl & 4u) phz = mat.hist[2 * hcomp + hq];
-        }
-        float ex = (cax * Ex[id] + cbx * (s_ex_y - s_ex_z - cw * phx)) * pecy * pecz;
-        float ey = (cay * Ey[id] + cby * (s_ey_z - s_ey_x - cw * phy)) * pecz * pecx;
-        const float ez = (caz * Ez[id] + cbz * (s_ez_x - s_ez_y - cw * phz)) * pecx * pecy;
+        ex = (cax * ex + cbx * (s_ex_y - s_ex_z - cw * phx)) * pecy * pecz;
+        ey = (cay * ey + cby * (s_ey_z - s_ey_x - cw * phy)) * pecz * pecx;
+        ez = (caz * ez + cbz * (s_ez_x - s_ez_y - cw * phz)) * pecx * pecy;
         if (inj_x) ex += cbx * inj_ex;
         if (inj_y) ey += cby * inj_ey;
-        Ex[id] = ex;
-        Ey[id] = ey;
-        Ez[id] = ez;
+        bst(rEx, ex, op, so);
+        bst(rEy, ey, op, so);
+        bst(rEz, ez, op, so);
         hy_m = hy;
         hz_m = hz;
     }

@@ -166,18 +166,28 @@ extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) 
         return fail(OEM_EHIP, "hipStreamCreate failed");
     }
     const size_t nc = c->ncell;
+    if (nc * 4 > 0xffffffffull) {
+        delete c;
+        return fail(OEM_ENOTSUP, "%zu cells: one field component exceeds 4 GiB (limit 2^30 cells)", nc);
+    }
     const size_t slab[3] = {(size_t)(d->pml_lo[0] + d->pml_hi[0]) * d->ny * d->nz,
                             (size_t)d->nx * (d->pml_lo[1] + d->pml_hi[1]) * d->nz,
                             (size_t)d->nx * d->ny * (d->pml_lo[2] + d->pml_hi[2])};
+    size_t total = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int m = 0; m < 2; ++m) {
+            c->psi.off[2 * a + m] = (uint32_t)total;
+            total += slab[a];
+        }
+    if (total * 4 > 0xffffffffull) {
+        delete c;
+        return fail(OEM_ENOTSUP, "CPML slabs exceed 4 GiB per side");
+    }
+    c->psi.total = (uint32_t)(total ? total : 1);
     rc = c->alloc(&c->E, 3 * nc, true);
     if (!rc) rc = c->alloc(&c->H, 3 * nc, true);
-    for (int a = 0; a < 3 && !rc; ++a) {
-        for (int m = 0; m < 2 && !rc; ++m) {
-            const size_t len = slab[a] ? slab[a] : 1;
-            rc = c->alloc(&c->psi.h[2 * a + m], len, true);
-            if (!rc) rc = c->alloc(&c->psi.e[2 * a + m], len, true);
-        }
-    }
+    if (!rc) rc = c->alloc(&c->psi.h, c->psi.total, true);
+    if (!rc) rc = c->alloc(&c->psi.e, c->psi.total, true);
     if (!rc) rc = c->alloc(&c->d_step, 1, true);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OEM_EHIP, "sync failed");
     if (rc) {
@@ -218,13 +228,14 @@ extern "C" int oem_set_axis(oem_ctx *ctx, int32_t axis, const float *idl_p, cons
         if (nxt[i] != i + 1 || prv[i + 1] != i)
             return fail(OEM_EINVAL, "axis %d: interior neighbours must be i+-1 (at %d)", axis, i);
     AxisDev &A = ctx->g.ax[axis];
-    const float **dst[] = {&A.idl_p, &A.idl_d, &A.mnx, &A.mpv, &A.pec, &A.a_e,
-                           &A.b_e,   &A.k_e,   &A.a_h, &A.b_h, &A.k_h};
-    for (int t = 0; t < 11; ++t) {
-        float *p = nullptr;
-        TRY(ctx->upload(&p, fa[t], (size_t)n));
-        *dst[t] = p;
-    }
+    std::vector<float> tab((size_t)n * AX_W, 0.f);
+    const int fid[11] = {AX_IDLP, AX_IDLD, AX_MNX, AX_MPV, AX_PEC, AX_AE,
+                         AX_BE,   AX_KE,   AX_AH,  AX_BH,  AX_KH};
+    for (int t = 0; t < 11; ++t)
+        for (int i = 0; i < n; ++i) tab[(size_t)i * AX_W + fid[t]] = fa[t][i];
+    float *dt = nullptr;
+    TRY(ctx->upload(&dt, tab.data(), tab.size()));
+    A.tab = dt;
     int *pn = nullptr, *pp = nullptr;
     TRY(ctx->upload(&pn, (const int *)nxt, (size_t)n));
     TRY(ctx->upload(&pp, (const int *)prv, (size_t)n));
@@ -257,6 +268,7 @@ extern "C" int oem_set_materials_lut(oem_ctx *ctx, const uint16_t *cls, int32_t 
         ctx->class_flagged[q] = fl != 0;
     }
     ctx->mat.mode = 0;
+    ctx->mat.n_classes = n_classes;
     ctx->mat.cls = dc;
     ctx->mat.lut = dl;
     ctx->mat_set = true;
@@ -326,6 +338,7 @@ extern "C" int oem_set_dispersion(oem_ctx *ctx, int32_t n_entry, const int32_t *
     TRY(ctx->alloc(&D.p_re, (size_t)n_pole, true));
     TRY(ctx->alloc(&D.p_im, (size_t)n_pole, true));
     const size_t hsz = 3ull * (hb[1] - hb[0]) * (hb[3] - hb[2]) * (hb[5] - hb[4]);
+    if (hsz * 4 > 0xffffffffull) return fail(OEM_ENOTSUP, "ADE history box exceeds 4 GiB");
     TRY(ctx->alloc(&D.hist, hsz, true));
     for (int q = 0; q < 6; ++q) { D.hb[q] = hb[q]; ctx->mat.hb[q] = hb[q]; }
     ctx->mat.hist = D.hist;
@@ -499,7 +512,9 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     g.ny = ctx->dims.ny;
     g.nz = ctx->dims.nz;
     g.plane = g.ny * g.nz;
-    g.tiles = (g.plane + OEM_TILE - 1) / OEM_TILE;
+    if ((size_t)g.plane * 4 >= ((size_t)1 << 32))
+        return fail(OEM_EINVAL, "plane ny*nz=%d too large for 32-bit in-plane offsets", g.plane);
+    g.tiles = ((g.nz + OEM_TK - 1) / OEM_TK) * ((g.ny + OEM_TJ - 1) / OEM_TJ);
     int chunk;
     const char *env = std::getenv("OEM_CHUNK");
     if (env && std::atoi(env) > 0) {
@@ -558,9 +573,9 @@ int launch_step(oem_ctx *c, int flush_after, Prof *pr) {
         }
         return OEM_OK;
     };
-    const dim3 yb(OEM_TILE), yg(c->g.nblocks);
+    const dim3 yb(OEM_TK, OEM_TJ), yg(c->g.nblocks);
     TRY(mark(OEM_K_UPDATE_H, true));
-    hipLaunchKernelGGL(k_update_h, yg, yb, 0, c->stream, Hx, Hy, Hz, Ex, Ey, Ez, c->psi, c->g,
+    hipLaunchKernelGGL(k_update_h, yg, yb, 0, c->stream, H, (const float *)E, c->psi, c->g,
                        c->dims.ch, c->pw, c->d_step);
     TRY(mark(OEM_K_UPDATE_H, false));
     for (auto &dp : c->dipoles) {
@@ -577,7 +592,7 @@ int launch_step(oem_ctx *c, int flush_after, Prof *pr) {
         TRY(mark(OEM_K_DISP_PRE, false));
     }
     TRY(mark(OEM_K_UPDATE_E, true));
-    hipLaunchKernelGGL(k_update_e, yg, yb, 0, c->stream, Ex, Ey, Ez, Hx, Hy, Hz, c->psi, c->g,
+    hipLaunchKernelGGL(k_update_e, yg, yb, 0, c->stream, E, (const float *)H, c->psi, c->g,
                        c->mat, c->dims.cw, c->pw, c->d_step);
     TRY(mark(OEM_K_UPDATE_E, false));
     for (auto &dp : c->dipoles) {
