@@ -48,6 +48,20 @@ def algorithmic_bytes(tabs) -> dict:
     return {"update_h": h, "update_e": e, "cells": n, "slab_cells": slab_cells}
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (profiles/yee_box_soup_512.md), or None when absent."""
+    p = ROOT / "profiles" / "yee_box_soup_512.md"
+    if not p.exists():
+        return None
+    text = p.read_text()
+    try:
+        blob = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(blob)["traffic_bytes_per_launch"].get("k_" + kernel)
+    except (IndexError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(n: int, steps: int = 2, nx_sample: int = 64) -> dict:
     from openem_amd import scenes, tables
     from oracle.yee64 import Oracle64
@@ -122,7 +136,8 @@ def main() -> int:
                    "cells": cells, "dft_batch": args.dft_batch},
         "roofline": {"bound": "hbm", "kernel": k_dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(k_dom) if args.n == 512 else None,
                      "algorithmic_bytes_per_launch": ab[k_dom], "avg_launch_ms": round(avg_ms, 5)},
         "step_hbm_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "cpu_baseline": cpu,

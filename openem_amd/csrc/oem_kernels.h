@@ -497,6 +497,33 @@ k_disp_post(DispDev d, const float *__restrict__ Ex, const float *__restrict__ E
     }
 }
 
+// post(n) + pre(n+1) in one pass (reference dispersion_step, `openem/kernels/dispersion.cu:168-207`):
+// nothing writes E between the two, so the same E^{n+1} feeds both and P is read and written
+// once instead of twice.
+__global__ void __launch_bounds__(256)
+k_disp_step(DispDev d, const float *__restrict__ Ex, const float *__restrict__ Ey,
+            const float *__restrict__ Ez, int ny, int nz)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.n_entry) return;
+    const int c = d.comp[e], id = d.cell[e];
+    const float en = c == 0 ? Ex[id] : (c == 1 ? Ey[id] : Ez[id]);
+    float w = 0.f;
+    for (int t = d.ofs[e]; t < d.ofs[e + 1]; ++t) {
+        const int ci = d.cidx[t];
+        const float br = d.b_re[ci], bi = d.b_im[ci];
+        const float pr = d.p_re[t] + br * en;
+        const float pi = d.p_im[t] + bi * en;
+        const float ar = d.am1_re[ci], ai = d.am1_im[ci];
+        const float dr = ar * pr - ai * pi + br * en;
+        const float di = ar * pi + ai * pr + bi * en;
+        d.p_re[t] = pr + dr;
+        d.p_im[t] = pi + di;
+        w += 2.0f * dr;
+    }
+    d.hist[hist_index(d, c, id, ny, nz)] = w;
+}
+
 // ------------------------------------------------------------------------------------------
 // Running DFT, batched: every step a snapshot of the monitored values goes into a ring of T
 // float32 rows; every T steps one flush replays them in step order against the float64 phase

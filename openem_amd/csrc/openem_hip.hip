@@ -76,6 +76,7 @@ struct oem_ctx {
     int *d_step = nullptr;
     int64_t step = 0;
     int pending = 0;
+    bool pre_ready = false;     // ADE: hist/P already advanced by the previous fused step
     bool prepared = false;
     hipGraphExec_t graph = nullptr;
     std::vector<void *> allocs;
@@ -555,8 +556,9 @@ int launch_flush(oem_ctx *c, int cnt) {
 }
 
 // One time step.  `flush_after`: number of snapshots to flush after the step counter advanced
-// (0 = none).  With `pr`, every launch is bracketed by events.
-int launch_step(oem_ctx *c, int flush_after, Prof *pr) {
+// (0 = none).  `last`: the final step of a run call ends the ADE with a plain post, otherwise
+// the post is fused with the next step's pre.  With `pr`, every launch is bracketed by events.
+int launch_step(oem_ctx *c, int flush_after, Prof *pr, bool last) {
     float *E = c->E, *H = c->H;
     const size_t nc = c->ncell;
     float *Ex = E, *Ey = E + nc, *Ez = E + 2 * nc;
@@ -585,7 +587,7 @@ int launch_step(oem_ctx *c, int flush_after, Prof *pr) {
                            c->stream, Hx, Hy, Hz, dp.d, c->mat, c->dims.ch, nc, c->d_step);
         TRY(mark(OEM_K_DIPOLE_H, false));
     }
-    if (c->has_disp) {
+    if (c->has_disp && !c->pre_ready) {
         TRY(mark(OEM_K_DISP_PRE, true));
         hipLaunchKernelGGL(k_disp_pre, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
                            c->stream, c->disp, Ex, Ey, Ez, c->dims.ny, c->dims.nz);
@@ -604,9 +606,14 @@ int launch_step(oem_ctx *c, int flush_after, Prof *pr) {
     }
     if (c->has_disp) {
         TRY(mark(OEM_K_DISP_POST, true));
-        hipLaunchKernelGGL(k_disp_post, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
-                           c->stream, c->disp, Ex, Ey, Ez);
+        if (last)
+            hipLaunchKernelGGL(k_disp_post, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
+                               c->stream, c->disp, Ex, Ey, Ez);
+        else
+            hipLaunchKernelGGL(k_disp_step, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
+                               c->stream, c->disp, Ex, Ey, Ez, c->dims.ny, c->dims.nz);
         TRY(mark(OEM_K_DISP_POST, false));
+        c->pre_ready = !last;
     }
     for (auto &m : c->mons) {
         TRY(mark(OEM_K_DFT_SNAP, true));
@@ -644,12 +651,14 @@ static int run_steps(oem_ctx *ctx, int64_t n) {
     const bool use_graph = !(ng && ng[0] == '1');
     int64_t left = n;
     while (left > 0) {
-        if (use_graph && ctx->pending == 0 && ctx->step % T == 0 && left >= T) {
+        const bool disp_ok = !ctx->has_disp || ctx->pre_ready;
+        if (use_graph && ctx->pending == 0 && ctx->step % T == 0 && left > T && disp_ok) {
             if (!ctx->graph) {
                 hipGraph_t gr = nullptr;
                 HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
                 int rc = OEM_OK;
-                for (int s = 0; s < T && !rc; ++s) rc = launch_step(ctx, s == T - 1 ? T : 0, nullptr);
+                for (int s = 0; s < T && !rc; ++s)
+                    rc = launch_step(ctx, s == T - 1 ? T : 0, nullptr, false);
                 hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
                 if (rc) {
                     if (gr) (void)hipGraphDestroy(gr);
@@ -672,7 +681,7 @@ static int run_steps(oem_ctx *ctx, int64_t n) {
                 ctx->pending++;
                 if (ctx->pending == T) fl = T;
             }
-            TRY(launch_step(ctx, fl, nullptr));
+            TRY(launch_step(ctx, fl, nullptr, left == 1));
             if (fl) ctx->pending = 0;
             ctx->step++;
             left--;
@@ -733,7 +742,7 @@ extern "C" int oem_profile_run(oem_ctx *ctx, int64_t n_steps, double *ms, int64_
             ctx->pending++;
             if (ctx->pending == T || s == n_steps - 1) fl = ctx->pending;
         }
-        rc = launch_step(ctx, fl, pr);
+        rc = launch_step(ctx, fl, pr, s == n_steps - 1);
         if (rc) break;
         if (fl) ctx->pending = 0;
         ctx->step++;

@@ -1,0 +1,345 @@
+"""Pinning the float64 oracle (CPU only).
+
+Each test checks the oracle against something the mathematics or the reference's own published
+numbers fix, chosen so that a plausible mistake (a dropped term, a wrong sign, index or operand)
+fails:
+
+- the discrete divergence is exactly conserved by the Yee curl pair (div curl = 0);
+- a single Fourier mode oscillates at the Yee dispersion relation;
+- the slab transmission reproduces the reference's published T = 0.8310044973694679
+  (README.md:62) and T + R = 1 (docs/tutorial.md:103);
+- the CPML reflection bound of the reference's test (tests/test_pml_reflection.py:57-60) and its
+  improvement with layer count; TF/SF one-sidedness;
+- the ADE: dispersive == equivalent lossy medium solved from the discrete transfer function
+  (reference tests/test_dispersion.py:224-274), zero-residue poles are no-ops;
+- a pure-Python triple loop over a tiny grid (brute force, independent indexing).
+"""
+
+import numpy as np
+import pytest
+
+from openem_amd import cpml, flux, scenes, tables
+from openem_amd.grid import Grid, uniform_axis
+from openem_amd.model import Dispersion, Scene, lorentz_poles, trap_coeffs
+from oracle.yee64 import Oracle64
+
+UM = 1e-6
+
+
+def _periodic_scene(shape=(6, 5, 7), dls=(20e-9, 23e-9, 17e-9)):
+    axes = [uniform_axis(n, d, 0.0, "Periodic") for n, d in zip(shape, dls)]
+    g = Grid(*axes)
+    dt = cpml.courant_dt(dls)
+    one = np.ones(shape)
+    return Scene(grid=g, dt=dt, num_time_steps=50, eps_ex=one, eps_ey=one.copy(),
+                 eps_ez=one.copy())
+
+
+def _div_fwd(F, dls):
+    return sum((np.roll(F[a], -1, axis=a) - F[a]) / dls[a] for a in range(3))
+
+
+def _div_bwd(F, dls):
+    return sum((F[a] - np.roll(F[a], 1, axis=a)) / dls[a] for a in range(3))
+
+
+def test_discrete_divergence_is_conserved():
+    """div_f(H) and div_b(E) are invariants of the source-free Yee update on a periodic grid:
+    any sign or neighbour error in one curl term breaks them at O(1)."""
+    dls = (20e-9, 23e-9, 17e-9)
+    sc = _periodic_scene(dls=dls)
+    o = Oracle64(tables.build(sc, 40))
+    rng = np.random.default_rng(3)
+    o.E[:] = rng.standard_normal(o.E.shape)
+    o.H[:] = rng.standard_normal(o.H.shape) / 377.0
+    de0, dh0 = _div_bwd(o.E, dls), _div_fwd(o.H, dls)
+    o.run(40)
+    de1, dh1 = _div_bwd(o.E, dls), _div_fwd(o.H, dls)
+    assert np.abs(de1 - de0).max() <= 1e-9 * np.abs(de0).max()
+    assert np.abs(dh1 - dh0).max() <= 1e-9 * np.abs(dh0).max()
+    # the fields themselves did change
+    assert np.abs(o.E).max() > 0
+
+
+@pytest.mark.parametrize("m", [1, 2, 5])
+def test_single_mode_follows_yee_dispersion(m):
+    """Ez = cos(k x) standing wave: E^{n+1} + E^{n-1} = 2 cos(w dt) E^n with
+    sin(w dt / 2) = (c dt / dx) |sin(k dx / 2)| (reference tests/test_bloch.py:291-297)."""
+    nx, dx = 16, 20e-9
+    axes = [uniform_axis(nx, dx, 0.0, "Periodic"), uniform_axis(1, dx, 0.0, "Periodic"),
+            uniform_axis(1, dx, 0.0, "Periodic")]
+    g = Grid(*axes)
+    dt = cpml.courant_dt((dx, dx, dx))
+    one = np.ones((nx, 1, 1))
+    sc = Scene(grid=g, dt=dt, num_time_steps=10, eps_ex=one, eps_ey=one, eps_ez=one)
+    o = Oracle64(tables.build(sc, 10))
+    k = 2 * np.pi * m / (nx * dx)
+    x = axes[0].edges[:-1]                       # Ez sits on x edges
+    o.E[2, :, 0, 0] = np.cos(k * x)
+    e = [o.E[2, 0, 0, 0]]
+    for _ in range(4):
+        o.step()
+        e.append(o.E[2, 0, 0, 0])
+    cos_wdt = (e[3] + e[1]) / (2 * e[2])
+    s = cpml.C_0 * dt / dx * abs(np.sin(k * dx / 2))
+    want = 1 - 2 * s * s                          # cos(w dt) = 1 - 2 sin^2(w dt/2)
+    assert abs(cos_wdt - want) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def slab_runs():
+    """Config-0 physics on a 2x2 cross-section (normal incidence is x/y-uniform): homogeneous
+    reference run and slab run, with an extra monitor 'S' on the source plane."""
+    from dataclasses import replace
+    from openem_amd.model import FluxMonitor
+    out = {}
+    for hom in (True, False, "ey"):
+        sc = scenes.slab_scene(size=(0.04 * UM, 0.04 * UM, 3.0 * UM), homogeneous=bool(hom),
+                               pol_axis=1 if hom == "ey" else 0)
+        s_mon = FluxMonitor("S", 2, sc.sources[0].plane_index, +1, sc.flux_monitors[0].freqs)
+        sc = replace(sc, flux_monitors=list(sc.flux_monitors) + [s_mon])
+        t = tables.build(sc)
+        o = Oracle64(t).run(sc.num_time_steps)
+        out[hom] = (sc, [flux.monitor_flux(a, sc.grid, m)
+                         for a, m in zip(o.acc, sc.flux_monitors)][:2], o.acc)
+    return out
+
+
+def test_slab_transmission_matches_reference(slab_runs):
+    """The reference publishes T(lambda = 1 um) = 0.8310044973694679 for this scene
+    (README.md:62); the analytic thin-film value is 0.8373."""
+    sc, (t_hom, r_hom), _ = slab_runs[True]
+    _, (t_slab, r_slab), _ = slab_runs[False]
+    T = t_slab / t_hom
+    R = -r_slab / t_hom
+    assert abs(T[5] - 0.8310044973694679) < 1e-5
+    assert np.all(np.abs(T + R - 1.0) < 1e-4)            # reference: 1.0000 - 1.0001
+    assert abs(R[5] - 0.1689) < 2e-4                      # docs/tutorial.md:102
+
+
+def test_tfsf_injection_is_one_sided(slab_runs):
+    """Homogeneous run: the flux upstream of the source plane (scattered-field side) is the
+    leakage of the TF/SF injection plus the PML reflection, < 1e-8 of the injected flux."""
+    _, (t_hom, r_hom), _ = slab_runs[True]
+    assert np.all(np.abs(r_hom / t_hom) < 1e-8)
+
+
+def _yee_kappa(sc, f):
+    dz, dt = sc.grid.z.dl[0], sc.dt
+    return 2 / dz * np.arcsin(np.sin(2 * np.pi * f * dt / 2) * dz / (cpml.C_0 * dt))
+
+
+def test_colocated_h_over_e_is_the_discrete_impedance(slab_runs):
+    """One-way wave in vacuum: with H averaged over the two planes around the E plane and each
+    DFT'd at its own time ((n+1/2)dt vs (n+1)dt), H/E = cos(kappa dz/2)/eta0 exactly, kappa from
+    the Yee dispersion relation.  Residual: the PML reflection amplitude (~1e-4)."""
+    sc, _, acc = slab_runs[True]
+    f = sc.flux_monitors[0].freqs
+    want = np.cos(_yee_kappa(sc, f) * sc.grid.z.dl[0] / 2) / cpml.ETA_0
+    ratio = acc[0][3, :, 0, 0] / acc[0][0, :, 0, 0]          # Hy / Ex on the T plane
+    assert np.abs(ratio / want - 1).max() < 1e-3
+    _, _, acc_y = slab_runs["ey"]
+    ratio_y = acc_y[0][2, :, 0, 0] / acc_y[0][1, :, 0, 0]    # Hx / Ey: -1/eta for +z travel
+    assert np.abs(-ratio_y / want - 1).max() < 1e-3
+
+
+def test_dft_time_convention_on_the_source_plane(slab_runs):
+    """On the injection plane the total field is the incident table: the E phasor equals
+    sum_n ex_inc[n+1] exp(i w (n+1) dt) (E^{n+1} sampled at t = (n+1)dt)."""
+    sc, _, acc = slab_runs[True]
+    f = sc.flux_monitors[0].freqs
+    n = np.arange(sc.num_time_steps)
+    ex = sc.sources[0].ex_inc
+    dft = (ex[n + 1][None, :] * np.exp(1j * 2 * np.pi * f[:, None] * (n + 1) * sc.dt)).sum(1)
+    got = acc[2][0, :, 0, 0]
+    assert np.abs(got - dft).max() / np.abs(dft).max() < 1e-3
+
+
+def _pml_reflection(n_layers):
+    """Reference tests/test_pml_reflection.py:31-53: vacuum, -z plane wave at z=0, 'back'
+    monitor at z=+0.4 um; dl = 10 nm, 5e14 Hz, run 6e-13 s."""
+    dl = 0.01 * UM
+    sc = scenes.slab_scene(dl=dl, size=(2 * dl, 2 * dl, 2.0 * UM), homogeneous=True,
+                           f0=5e14, src_z=0.0, direction=-1, t_z=-0.12 * UM, r_z=0.4 * UM,
+                           nf=1, run_time=6e-13, n_pml=n_layers)
+    sc_freqs = np.array([5e14])
+    from dataclasses import replace
+    sc = replace(sc, flux_monitors=[replace(m, freqs=sc_freqs) for m in sc.flux_monitors])
+    o = Oracle64(tables.build(sc)).run(sc.num_time_steps)
+    fin, fback = [abs(flux.monitor_flux(a, sc.grid, m)[0])
+                  for a, m in zip(o.acc, sc.flux_monitors)]
+    return fback / fin
+
+
+def test_pml_reflection_bound_and_layer_scaling():
+    r12 = _pml_reflection(12)
+    r6 = _pml_reflection(6)
+    assert r12 < 1e-8                 # reference measured 3.8e-9
+    assert r12 < r6 / 100
+
+
+def _chi_disc(poles, f, dt):
+    z = np.exp(-1j * 2 * np.pi * f * dt)
+
+    def one(q, r):
+        return r * dt * (z + 1) / 2 / (z * (1 - q * dt / 2) - (1 + q * dt / 2))
+
+    return sum(one(q, r) + one(np.conj(q), np.conj(r)) for q, r in poles)
+
+
+def _equivalent_lossy(eps_inf, poles, f, dt):
+    z = np.exp(-1j * 2 * np.pi * f * dt)
+    target = (eps_inf + _chi_disc(poles, f, dt)) * (z - 1.0)
+    m = np.array([[(z - 1).real, (z + 1).real], [(z - 1).imag, (z + 1).imag]])
+    eps_b, s_b = np.linalg.solve(m, np.array([target.real, target.imag]))
+    return float(eps_b), float(s_b * 2.0 * cpml.EPSILON_0 / dt)
+
+
+def _uniform_line_scene(n_steps, eps, sigma=None, disp=None, f0=3e14):
+    """2x2xN periodic-in-xy line, PML on z, an Ex current sheet in the middle (a dipole per
+    cell of one plane), a flux plane on each side."""
+    from openem_amd import waveform
+    from openem_amd.model import FieldMonitor, PointDipole
+    nz, dl = 120, 20e-9
+    g = Grid(uniform_axis(2, dl, 0, "Periodic"), uniform_axis(2, dl, 0, "Periodic"),
+             uniform_axis(nz, dl, 0, "PML"))
+    dt = cpml.courant_dt((dl, dl, dl))
+    shape = g.shape
+    e = np.full(shape, eps)
+    sig = None if sigma is None else np.full(shape, sigma)
+    wf = waveform.sample(waveform.GaussianPulse(freq0=f0, fwidth=0.4 * f0), dt, n_steps)
+    dips = [PointDipole(0, np.array([[i, j, nz // 2]]), np.array([1e6]), wf)
+            for i in range(2) for j in range(2)]
+    fm = FieldMonitor("line", np.array([f0]), (0, 0, 20), (2, 2, 80))
+    return Scene(grid=g, dt=dt, num_time_steps=n_steps, eps_ex=e, eps_ey=e, eps_ez=e,
+                 pml=scenes.pml_faces(dt, (dl, dl, dl), [(2, "lo"), (2, "hi")]),
+                 sigma_ex=sig, sigma_ey=sig, sigma_ez=sig, dispersion=disp, dipoles=dips,
+                 field_monitors=[fm])
+
+
+def _all_cells_dispersion(shape, poles, dt):
+    am1, b = trap_coeffs(poles, dt)
+    n = int(np.prod(shape))
+    comp = np.repeat(np.arange(3, dtype=np.int32), n)
+    cell = np.tile(np.arange(n, dtype=np.int32), 3)
+    ofs = np.arange(0, 3 * n * len(poles) + 1, len(poles), dtype=np.int32)
+    g = np.full(3 * n, 2.0 * b.real.sum())
+    return Dispersion(comp=comp, cell=cell, pole_ofs=ofs, am1=np.tile(am1, 3 * n),
+                      b=np.tile(b, 3 * n), g=g)
+
+
+def test_dispersive_equals_equivalent_lossy():
+    """At one frequency a dispersive medium is indistinguishable from the lossy medium with the
+    same discrete transfer function; the residual is the finite DFT window's transient and must
+    fall as the window grows (reference tests/test_dispersion.py:224-274)."""
+    f0 = 3e14
+    poles = lorentz_poles(1.2, 2 * np.pi * 2.0e14, 2 * np.pi * 0.4e14)
+    eps_inf = 2.0
+    errs = []
+    for n in (1500, 6000):
+        sc0 = _uniform_line_scene(n, eps_inf)
+        disp = _all_cells_dispersion(sc0.shape, poles, sc0.dt)
+        scd = _uniform_line_scene(n, eps_inf, disp=disp)
+        eps_b, sig_b = _equivalent_lossy(eps_inf, poles, f0, sc0.dt)
+        assert sig_b > 0                                  # passive poles
+        scl = _uniform_line_scene(n, eps_b, sigma=sig_b)
+        ad = Oracle64(tables.build(scd)).run(n).acc[0]
+        al = Oracle64(tables.build(scl)).run(n).acc[0]
+        errs.append(np.abs(ad[0] - al[0]).max() / np.abs(al[0]).max())
+    assert errs[0] < 1e-2
+    assert errs[1] < errs[0] / 3 and errs[1] < 1e-3
+
+
+def test_zero_residue_poles_are_noops():
+    sc0 = _uniform_line_scene(300, 2.0)
+    poles = [(complex(-1e13, -2e15), 0j)]
+    scd = _uniform_line_scene(300, 2.0, disp=_all_cells_dispersion(sc0.shape, poles, sc0.dt))
+    a = Oracle64(tables.build(sc0)).run(300)
+    b = Oracle64(tables.build(scd)).run(300)
+    assert np.array_equal(a.E, b.E) and np.array_equal(a.H, b.H)
+
+
+def _brute_step(t, E, H, psi_h, psi_e, n):
+    """One step written as explicit loops straight from the reference kernel text
+    (openem/kernels/yee.cu:76-137 and :154-240); no sources, no dispersion."""
+    nx, ny, nz = t.shape
+    ax = t.axes
+    Hn = H.copy()
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                ip, jp, kp = ax[0].nxt[i], ax[1].nxt[j], ax[2].nxt[k]
+                mi, mj, mk = ax[0].mnx[i], ax[1].mnx[j], ax[2].mnx[k]
+
+                def st(key, d, a, idx):
+                    A = (ax[a].a_h, ax[a].b_h, ax[a].k_h)
+                    psi_h[key][i, j, k] = A[1][idx] * psi_h[key][i, j, k] + A[0][idx] * d
+                    return A[2][idx] * d + psi_h[key][i, j, k]
+
+                dzy = (E[2][i, jp, k] * mj - E[2][i, j, k]) * ax[1].idl_p[j]
+                dyz = (E[1][i, j, kp] * mk - E[1][i, j, k]) * ax[2].idl_p[k]
+                Hn[0][i, j, k] -= t.ch * (st("hx_y", dzy, 1, j) - st("hx_z", dyz, 2, k))
+                dxz = (E[0][i, j, kp] * mk - E[0][i, j, k]) * ax[2].idl_p[k]
+                dzx = (E[2][ip, j, k] * mi - E[2][i, j, k]) * ax[0].idl_p[i]
+                Hn[1][i, j, k] -= t.ch * (st("hy_z", dxz, 2, k) - st("hy_x", dzx, 0, i))
+                dyx = (E[1][ip, j, k] * mi - E[1][i, j, k]) * ax[0].idl_p[i]
+                dxy = (E[0][i, jp, k] * mj - E[0][i, j, k]) * ax[1].idl_p[j]
+                Hn[2][i, j, k] -= t.ch * (st("hz_x", dyx, 0, i) - st("hz_y", dxy, 1, j))
+    En = E.copy()
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                im, jm, km = ax[0].prv[i], ax[1].prv[j], ax[2].prv[k]
+                mi, mj, mk = ax[0].mpv[i], ax[1].mpv[j], ax[2].mpv[k]
+
+                def st(key, d, a, idx):
+                    A = (ax[a].a_e, ax[a].b_e, ax[a].k_e)
+                    psi_e[key][i, j, k] = A[1][idx] * psi_e[key][i, j, k] + A[0][idx] * d
+                    return A[2][idx] * d + psi_e[key][i, j, k]
+
+                px, py, pz = ax[0].pec[i], ax[1].pec[j], ax[2].pec[k]
+                dzy = (Hn[2][i, j, k] - Hn[2][i, jm, k] * mj) * ax[1].idl_d[j]
+                dyz = (Hn[1][i, j, k] - Hn[1][i, j, km] * mk) * ax[2].idl_d[k]
+                En[0][i, j, k] = (t.ca[0][i, j, k] * E[0][i, j, k] + t.cb[0][i, j, k] * (
+                    st("hx_y", dzy, 1, j) - st("hx_z", dyz, 2, k))) * py * pz
+                dxz = (Hn[0][i, j, k] - Hn[0][i, j, km] * mk) * ax[2].idl_d[k]
+                dzx = (Hn[2][i, j, k] - Hn[2][im, j, k] * mi) * ax[0].idl_d[i]
+                En[1][i, j, k] = (t.ca[1][i, j, k] * E[1][i, j, k] + t.cb[1][i, j, k] * (
+                    st("hy_z", dxz, 2, k) - st("hy_x", dzx, 0, i))) * pz * px
+                dyx = (Hn[1][i, j, k] - Hn[1][im, j, k] * mi) * ax[0].idl_d[i]
+                dxy = (Hn[0][i, j, k] - Hn[0][i, jm, k] * mj) * ax[1].idl_d[j]
+                En[2][i, j, k] = (t.ca[2][i, j, k] * E[2][i, j, k] + t.cb[2][i, j, k] * (
+                    st("hz_x", dyx, 0, i) - st("hz_y", dxy, 1, j))) * px * py
+    return En, Hn
+
+
+def test_brute_force_loops_match_vectorised_oracle():
+    """Tiny grid with CPML on x and z, PEC walls on y, random fields and psi: three steps of the
+    explicit loops == three steps of the vectorised oracle."""
+    shape = (5, 4, 8)
+    dl = 20e-9
+    g = Grid(uniform_axis(5, dl, 0, "PML"), uniform_axis(4, dl, 0, "PECBoundary"),
+             uniform_axis(8, dl, 0, "PML"))
+    dt = cpml.courant_dt((dl, dl, dl))
+    rng = np.random.default_rng(7)
+    eps = [1 + 3 * rng.random(shape) for _ in range(3)]
+    sc = Scene(grid=g, dt=dt, num_time_steps=4, eps_ex=eps[0], eps_ey=eps[1], eps_ez=eps[2],
+               pml=scenes.pml_faces(dt, (dl, dl, dl), [(0, "lo"), (0, "hi"), (2, "lo"),
+                                                        (2, "hi")], 2),
+               sigma_ex=rng.random(shape) * 1e5, sigma_ey=rng.random(shape) * 1e5,
+               sigma_ez=rng.random(shape) * 1e5)
+    t = tables.build(sc, 4)
+    o = Oracle64(t)
+    o.E[:] = rng.standard_normal(o.E.shape)
+    o.H[:] = rng.standard_normal(o.H.shape) / 377
+    for key in o.psi_h:
+        o.psi_h[key][:] = rng.standard_normal(shape) * 1e6
+        o.psi_e[key][:] = rng.standard_normal(shape) * 1e3
+    E, H = o.E.copy(), o.H.copy()
+    ph = {k: v.copy() for k, v in o.psi_h.items()}
+    pe = {k: v.copy() for k, v in o.psi_e.items()}
+    for n in range(3):
+        E, H = _brute_step(t, E, H, ph, pe, n)
+        o.step()
+    assert np.allclose(o.E, E, rtol=1e-12, atol=1e-12 * np.abs(E).max())
+    assert np.allclose(o.H, H, rtol=1e-12, atol=1e-12 * np.abs(H).max())
