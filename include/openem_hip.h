@@ -39,7 +39,7 @@ enum oem_status {
 enum oem_kernel_id {
     OEM_K_UPDATE_H = 0, OEM_K_UPDATE_E = 1, OEM_K_DIPOLE_E = 2, OEM_K_DIPOLE_H = 3,
     OEM_K_DISP_PRE = 4, OEM_K_DISP_POST = 5, OEM_K_DFT_SNAP = 6, OEM_K_DFT_FLUSH = 7,
-    OEM_K_STEP = 8, OEM_K_COUNT = 9
+    OEM_K_STEP = 8, OEM_K_TFSF_H = 9, OEM_K_TFSF_E = 10, OEM_K_COUNT = 11
 };
 
 typedef struct oem_ctx oem_ctx;
@@ -49,7 +49,7 @@ typedef struct oem_dims {
     int32_t n_steps;           /* length of every per-step table (>= steps ever run)          */
     int32_t pml_lo[3];         /* CPML slab thickness per axis at the low / high end;          */
     int32_t pml_hi[3];         /*   psi is stored only inside these slabs                     */
-    int32_t complex_fields;    /* 1 = Bloch complex fields (not in ABI v1: OEM_ENOTSUP)       */
+    int32_t complex_fields;    /* 1 = Bloch: complex fields, (re, im) float pairs per element */
     int32_t dft_batch;         /* DFT snapshot depth T (steps per fp64 flush), 1..64          */
     float ch;                  /* dt / mu0                                                    */
     float cw;                  /* eps0 / dt                                                   */
@@ -71,13 +71,16 @@ int oem_ctx_destroy(oem_ctx *ctx);
 
 /* One axis (0=x,1=y,2=z): 1/dl primal and dual, neighbour tables nxt/prv (int32), masks
  * mnx/mpv/pec, CPML (a, b, 1/kappa) at E and H positions, each of length n with the identity
- * (0, 1, 1) outside the PML.  Replaces grid.Axis.index_tables (`openem/grid.py:219-304`),
- * coeffs._pml_axis_arrays (`openem/coeffs.py:21-45`) and device_tables._axis_device_tables
- * (`openem/device_tables.py:118-165`). */
+ * (0, 1, 1) outside the PML; mnx_im/mpv_im (may be NULL) are the imaginary parts of the masks,
+ * non-zero only at a Bloch wrap (complex contexts only).  Replaces grid.Axis.index_tables
+ * (`openem/grid.py:219-304`), coeffs._pml_axis_arrays (`openem/coeffs.py:21-45`),
+ * device_tables._axis_device_tables (`openem/device_tables.py:118-165`) and the complex masks
+ * of the Bloch path (`openem/kernels/bloch.cu:3-23`). */
 int oem_set_axis(oem_ctx *ctx, int32_t axis, const float *idl_p, const float *idl_d,
                  const int32_t *nxt, const int32_t *prv, const float *mnx, const float *mpv,
                  const float *pec, const float *a_e, const float *b_e, const float *k_e,
-                 const float *a_h, const float *b_h, const float *k_h);
+                 const float *a_h, const float *b_h, const float *k_h, const float *mnx_im,
+                 const float *mpv_im);
 
 /* Material classes: one uint16 class per cell and a LUT of n_classes rows of 8 floats
  * (ca_x, cb_x, ca_y, cb_y, ca_z, cb_z, flags as uint32 bits 0..2 = "component is dispersive",
@@ -107,13 +110,26 @@ int oem_add_plane_source(oem_ctx *ctx, int32_t pol, int32_t kh, int32_t ks, floa
                          float coef_e, const double *ex_inc, const double *hy_inc);
 
 /* Point dipoles (electric: E -= cb*coef*amp[src][n]; magnetic: H -= ch*coef*amp[src][n]).
- * amp is n_src rows of n_steps+1 float64 samples.  Points on the same (comp, cell) are
- * applied by one thread in input order (no atomics, deterministic).  Replaces
- * sources_setup._dipole_table (`openem/sources_setup.py:419-457`) and inject_dipole /
- * inject_dipole_h (`openem/kernels/source_dft.cu:147-206`). */
+ * amp is n_src rows of n_steps+1 float64 samples, (re, im) pairs when amp_complex (complex
+ * contexts inject the complex waveform, real ones its real part).  Points on the same
+ * (comp, cell) are applied by one thread in input order (no atomics, deterministic).
+ * Replaces sources_setup._dipole_table (`openem/sources_setup.py:419-457`, `:521`) and
+ * inject_dipole / inject_dipole_h (`openem/kernels/source_dft.cu:147-206`,
+ * `openem/kernels/bloch.cu:226-278`). */
 int oem_add_dipoles(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const int32_t *comp,
                     const int32_t *cell, const float *coef, const int32_t *src_of,
-                    int32_t n_src, const double *amp);
+                    int32_t n_src, const double *amp, int32_t amp_complex);
+
+/* TF/SF box face corrections as a point list: F[comp][cell] += scale*coef*(table[n][col]*phase)
+ * with scale = dt/mu0 for H points (magnetic=1, applied after update_h with the incident E at
+ * time n) and the cell's cb for E points (applied after update_e with the incident H at
+ * n+1/2).  table is (n_steps+1) x ncol complex float64 (re, im) from the 1D auxiliary line;
+ * phase is the per-point transverse Bloch phase (re, im) float32 (imaginary parts must be 0 on
+ * real contexts).  Replaces _tfsf_box / tfsf1d.tables (`openem/sources_setup.py:236`,
+ * `openem/tfsf1d.py:327`) and tfsf_corr_h / tfsf_corr_e (`openem/kernels/tfsf.cu:49-83`). */
+int oem_add_tfsf_points(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const int32_t *comp,
+                        const int32_t *cell, const float *coef, const int32_t *col,
+                        const float *phase, int32_t ncol, const double *table);
 
 /* Plane DFT monitor (normal `axis`, E plane index km >= 1): accumulates
  * (E_t1, E_t2, (H_t1[km]+H_t1[km-1])/2, (H_t2..)/2) for nf frequencies in float64.
@@ -151,7 +167,8 @@ int oem_step_index(oem_ctx *ctx, int64_t *n);
  * points in C order of the plane (n1, n2) or the box (ni, nj, nk). */
 int oem_read_monitor(oem_ctx *ctx, int32_t mon_id, double *re, double *im, size_t count);
 
-/* Field component 0..5 (Ex Ey Ez Hx Hy Hz) to/from host, ncell floats. */
+/* Field component 0..5 (Ex Ey Ez Hx Hy Hz) to/from host: ncell floats, or 2*ncell (re, im)
+ * pairs on a complex context. */
 int oem_read_field(oem_ctx *ctx, int32_t comp, float *out, size_t count);
 int oem_write_field(oem_ctx *ctx, int32_t comp, const float *in, size_t count);
 

@@ -17,14 +17,14 @@ LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 OEM_OK, OEM_EINVAL, OEM_ENOMEM, OEM_EHIP, OEM_ESTATE, OEM_ENOTSUP = range(6)
 KERNELS = ("update_h", "update_e", "dipole_e", "dipole_h", "disp_pre", "disp_post",
-           "dft_snap", "dft_flush", "step_advance")
+           "dft_snap", "dft_flush", "step_advance", "tfsf_h", "tfsf_e")
 K_COUNT = len(KERNELS)
 
 #: every symbol include/openem_hip.h declares (checked by the CPU test suite)
 EXPORTS = ("oem_version", "oem_last_error", "oem_device_count", "oem_ctx_create",
            "oem_ctx_destroy", "oem_set_axis", "oem_set_materials_lut",
            "oem_set_materials_dense", "oem_set_dispersion", "oem_add_plane_source",
-           "oem_add_dipoles", "oem_add_flux_plane", "oem_add_field_box", "oem_prepare",
+           "oem_add_dipoles", "oem_add_tfsf_points", "oem_add_flux_plane", "oem_add_field_box", "oem_prepare",
            "oem_run", "oem_profile_run", "oem_timed_run", "oem_step_index", "oem_read_monitor",
            "oem_read_field", "oem_write_field")
 
@@ -53,14 +53,17 @@ _SIGS = {
     "oem_device_count": ([_P], C.c_int),
     "oem_ctx_create": ([_P, C.c_int32, _P], C.c_int),
     "oem_ctx_destroy": ([_P], C.c_int),
-    "oem_set_axis": ([_P, C.c_int32] + [_P] * 13, C.c_int),
+    "oem_set_axis": ([_P, C.c_int32] + [_P] * 15, C.c_int),
     "oem_set_materials_lut": ([_P, _P, C.c_int32, _P], C.c_int),
     "oem_set_materials_dense": ([_P, _P, _P], C.c_int),
     "oem_set_dispersion": ([_P, C.c_int32, _P, _P, _P, C.c_int32, _P, C.c_int32,
                             _P, _P, _P, _P, _P], C.c_int),
     "oem_add_plane_source": ([_P, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
                               _P, _P], C.c_int),
-    "oem_add_dipoles": ([_P, C.c_int32, C.c_int32, _P, _P, _P, _P, C.c_int32, _P], C.c_int),
+    "oem_add_dipoles": ([_P, C.c_int32, C.c_int32, _P, _P, _P, _P, C.c_int32, _P, C.c_int32],
+                        C.c_int),
+    "oem_add_tfsf_points": ([_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, C.c_int32, _P],
+                            C.c_int),
     "oem_add_flux_plane": ([_P, C.c_int32, C.c_int32, C.c_int32, _P, _P], C.c_int),
     "oem_add_field_box": ([_P, _P, _P, C.c_int32, _P, _P], C.c_int),
     "oem_prepare": ([_P], C.c_int),

@@ -1,13 +1,17 @@
 // Device kernels of the MI355X (gfx950, CDNA4) FDTD stepper.
 //
 // The Yee update is HBM-bound (about 1 flop per byte), so the design goal is: every field
-// byte crosses HBM once per sweep.  Each work-group owns a tile of 256 consecutive positions
-// p = j*nz + k of the (ny, nz) plane and marches along x (the slowest axis) over a chunk of
-// i-planes.  The x neighbour plane that the curl needs is carried in registers from one
-// iteration to the next; the y/z neighbours are in the same or an adjacent plane tile and are
-// served by L1/L2.  The linear block id is remapped so that consecutive tiles land on the same
-// XCD (MI355X dispatches work-groups round-robin over 8 XCDs, each with its own L2), which keeps
-// the halo rows of neighbouring tiles in one L2.
+// byte crosses HBM once per sweep.  Each work-group owns a 64 (k) x 4 (j) tile of the (ny, nz)
+// plane and marches along x (the slowest axis) over a chunk of i-planes.  The x neighbour plane
+// the curl needs is carried in registers from one iteration to the next; the y/z neighbours are
+// in the same or an adjacent tile and are served by L1/L2.  The linear block id is remapped so
+// that consecutive tiles land on the same XCD (MI355X dispatches work-groups round-robin over 8
+// XCDs, each with its own L2), which keeps the halo rows of neighbouring tiles in one L2.
+//
+// Every kernel is a template on the field type F: float for real scenes, cf2 (re, im
+// interleaved) for Bloch scenes.  All coefficients are real; the Bloch phase enters only through
+// the complex masks at the periodic wrap (the reference's layout of the complex path,
+// `openem/kernels/bloch.cu:1-23`).
 //
 // CPML psi is stored only inside the PML slabs of each axis.  Outside a slab the reference's
 // identity coefficients (a=0, b=1, 1/kappa=1, `openem/kernels/yee.cu:60-74`) make the stretched
@@ -19,11 +23,80 @@
 
 #define OEM_NXCD 8
 #define OEM_MAX_PW 2
+#define OEM_TK 64
+#define OEM_TJ 4
+#define OEM_LDS_CLASSES 512
 
+// ---------------------------------------------------------------- complex field element ----
+struct cf2 {
+    float r, i;
+};
+__device__ __forceinline__ cf2 operator+(cf2 a, cf2 b) { return {a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ cf2 operator-(cf2 a, cf2 b) { return {a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ cf2 operator*(float s, cf2 a) { return {s * a.r, s * a.i}; }
+__device__ __forceinline__ cf2 operator*(cf2 a, float s) { return {a.r * s, a.i * s}; }
+
+// ---------------------------------------------------------------- buffer resources ----------
+// An SGPR descriptor + a fixed 32-bit per-lane VGPR offset + a per-iteration SGPR offset keeps
+// the x march's address arithmetic out of the VGPRs.  gfx9-family descriptor word 3.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+#define OEM_RSRC_W3 0x00020000
+
+__device__ __forceinline__ rsrc_t mk_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             OEM_RSRC_W3);
+}
+__device__ __forceinline__ uint32_t bld_u16(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+}
+
+// The 64-bit buffer builtins take and return a GCC vector of two uint32; converting that
+// implicitly to or from an ext_vector type splats one lane, so every conversion goes through
+// __builtin_bit_cast.
+typedef unsigned int oem_u32x2 __attribute__((vector_size(8)));
+
+template <typename F> struct FT;
+template <> struct FT<float> {
+    static constexpr uint32_t B = 4;
+    static __device__ __forceinline__ float ld(rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+    }
+    static __device__ __forceinline__ void st(rsrc_t r, float x, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, x), r, v, s, 0);
+    }
+    static __device__ __forceinline__ float mmul(float x, float mr, float) { return x * mr; }
+    static __device__ __forceinline__ float zero() { return 0.f; }
+    static __device__ __forceinline__ float addre(float x, float a) { return x + a; }
+    // x + s * Re(v) (real fields keep the real part of a complex correction)
+    static __device__ __forceinline__ float addc(float x, float s, float vr, float) {
+        return x + s * vr;
+    }
+};
+template <> struct FT<cf2> {
+    static constexpr uint32_t B = 8;
+    static __device__ __forceinline__ cf2 ld(rsrc_t r, uint32_t v, uint32_t s) {
+        const oem_u32x2 q = __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0);
+        return __builtin_bit_cast(cf2, q);
+    }
+    static __device__ __forceinline__ void st(rsrc_t r, cf2 x, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(oem_u32x2, x), r, v, s, 0);
+    }
+    static __device__ __forceinline__ cf2 mmul(cf2 x, float mr, float mi) {
+        return {x.r * mr - x.i * mi, x.r * mi + x.i * mr};
+    }
+    static __device__ __forceinline__ cf2 zero() { return {0.f, 0.f}; }
+    static __device__ __forceinline__ cf2 addre(cf2 x, float a) { return {x.r + a, x.i}; }
+    static __device__ __forceinline__ cf2 addc(cf2 x, float s, float vr, float vi) {
+        return {x.r + s * vr, x.i + s * vi};
+    }
+};
+
+// ---------------------------------------------------------------- tables -------------------
 // Per-axis tables interleaved as tab[i*AX_W + field]: one pointer per axis keeps the kernel
-// argument block (and the SGPRs holding it) small.
+// argument block (and the SGPRs holding it) small.  MNXI/MPVI: imaginary parts of the masks
+// (non-zero only at a Bloch wrap).
 enum { AX_IDLP = 0, AX_IDLD, AX_MNX, AX_MPV, AX_PEC, AX_AE, AX_BE, AX_KE, AX_AH, AX_BH, AX_KH,
-       AX_W = 12 };
+       AX_MNXI, AX_MPVI, AX_W = 16 };
 
 struct AxisDev {
     const float *tab;       // n * AX_W
@@ -47,7 +120,7 @@ struct GridDev {
 //   H side: x -> (Hy, Hz), y -> (Hx, Hz), z -> (Hx, Hy); E side likewise with E.
 // x slab (wx, ny, nz), y slab (nx, wy, nz), z slab (nx, ny, wz), w = lo + hi layers.
 struct PsiDev {
-    float *h, *e;
+    void *h, *e;
     uint32_t off[6];
     uint32_t total;     // elements per side
 };
@@ -65,7 +138,7 @@ struct MatDev {
     const uint16_t *cls;
     const float *lut;   // n_classes * 8
     const float *ca, *cb;   // dense: 3*ncell each
-    float *hist;        // (3, bx, by, bz) inside hist box, or nullptr
+    void *hist;         // (3, bx, by, bz) elements of F inside the hist box, or nullptr
     int hb[6];          // x0 x1 y0 y1 z0 z1
 };
 
@@ -80,40 +153,7 @@ __device__ __forceinline__ int slab_of(int i, int n, int lo, int hi) {
     return i < lo ? i : (i >= n - hi ? i - (n - hi) + lo : -1);
 }
 
-// CPML recursion on one psi element: psi <- b*psi + a*d; returns invk*d + psi.
-__device__ __forceinline__ float stretch(float *psi, size_t id, float d, float a, float b,
-                                         float invk) {
-    const float p = b * psi[id] + a * d;
-    psi[id] = p;
-    return invk * d + p;
-}
-
 __device__ __forceinline__ int step_now(const int *stp) { return *stp; }
-
-// Buffer resources: an SGPR descriptor + a fixed 32-bit per-lane VGPR offset + a per-iteration
-// SGPR offset.  This keeps the x march's address arithmetic out of the VGPRs (the compiler
-// would otherwise carry one 64-bit pointer per stream).  gfx9-family descriptor word 3.
-typedef __amdgpu_buffer_rsrc_t rsrc_t;
-#define OEM_RSRC_W3 0x00020000
-
-__device__ __forceinline__ rsrc_t mk_rsrc(const void *p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
-                                             OEM_RSRC_W3);
-}
-__device__ __forceinline__ float bld(rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-__device__ __forceinline__ uint32_t bld_u16(rsrc_t r, uint32_t voff, uint32_t soff) {
-    return (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
-}
-__device__ __forceinline__ void bst(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
-}
-
-// Tile of the (ny, nz) plane owned by one block: OEM_TK k-lanes x OEM_TJ j-rows.
-#define OEM_TK 64
-#define OEM_TJ 4
-#define OEM_LDS_CLASSES 512
 
 struct TileCoord {
     int j, k, i0, i1;
@@ -135,34 +175,39 @@ __device__ __forceinline__ TileCoord tile_coord(const GridDev &g) {
 }
 
 // CPML recursion on one psi element already loaded: psi' = b*psi + a*d, s = invk*d + psi'.
-__device__ __forceinline__ float stretch_v(float &psi, float d, float a, float b, float invk) {
+template <typename F>
+__device__ __forceinline__ F stretch_v(F &psi, F d, float a, float b, float invk) {
     psi = b * psi + a * d;
     return invk * d + psi;
 }
 
 // ------------------------------------------------------------------------------------------
-// H half step (reference update_h, `openem/kernels/yee.cu:76-137`) fused with the H side of
-// the plane-wave TF/SF correction (`openem/kernels/source_dft.cu:40-55`).
+// H half step (reference update_h, `openem/kernels/yee.cu:76-137`; complex variant
+// update_h_c, `openem/kernels/bloch.cu:38-126`) fused with the H side of the plane-wave TF/SF
+// correction (`openem/kernels/source_dft.cu:40-55`).
 // ------------------------------------------------------------------------------------------
+template <typename F>
 __global__ void __launch_bounds__(OEM_TK * OEM_TJ)
-k_update_h(float *__restrict__ H, const float *__restrict__ E, PsiDev ps, GridDev g, float ch,
+k_update_h(F *__restrict__ H, const F *__restrict__ E, PsiDev ps, GridDev g, float ch,
            PlaneSrcDev pw, const int *__restrict__ stp)
 {
+    typedef FT<F> T;
     const TileCoord tc = tile_coord(g);
     if (!tc.ok) return;
     const int j = tc.j, k = tc.k;
     const int nz = g.nz, ny = g.ny, nx = g.nx;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const size_t ncell = (size_t)g.plane * nx;
-    const uint32_t cb = (uint32_t)(ncell * 4);
+    const uint32_t cb = (uint32_t)(ncell * T::B);
     const rsrc_t rEx = mk_rsrc(E, cb), rEy = mk_rsrc(E + ncell, cb), rEz = mk_rsrc(E + 2 * ncell, cb);
     const rsrc_t rHx = mk_rsrc(H, cb), rHy = mk_rsrc(H + ncell, cb), rHz = mk_rsrc(H + 2 * ncell, cb);
-    const rsrc_t rP = mk_rsrc(ps.h, ps.total * 4u);
+    const rsrc_t rP = mk_rsrc(ps.h, ps.total * T::B);
 
-    const uint32_t op = (uint32_t)(j * nz + k) * 4u;              // own cell in the plane
-    const uint32_t oj = (uint32_t)(Y.nxt[j] * nz + k) * 4u;       // (j+1, k)
-    const uint32_t ok = (uint32_t)(j * nz + Z.nxt[k]) * 4u;       // (j, k+1)
+    const uint32_t op = (uint32_t)(j * nz + k) * T::B;            // own cell in the plane
+    const uint32_t oj = (uint32_t)(Y.nxt[j] * nz + k) * T::B;     // (j+1, k)
+    const uint32_t ok = (uint32_t)(j * nz + Z.nxt[k]) * T::B;     // (j, k+1)
     const float mj = axv(Y, j, AX_MNX), mk = axv(Z, k, AX_MNX);
+    const float mji = axv(Y, j, AX_MNXI), mki = axv(Z, k, AX_MNXI);
     const float idy = axv(Y, j, AX_IDLP), idz = axv(Z, k, AX_IDLP);
     const int sy = slab_of(j, ny, Y.lo, Y.hi);
     const int sz = slab_of(k, nz, Z.lo, Z.hi);
@@ -170,8 +215,8 @@ k_update_h(float *__restrict__ H, const float *__restrict__ E, PsiDev ps, GridDe
     float ay = 0.f, by = 1.f, ky = 1.f, az = 0.f, bz = 1.f, kz = 1.f;
     if (sy >= 0) { ay = axv(Y, j, AX_AH); by = axv(Y, j, AX_BH); ky = axv(Y, j, AX_KH); }
     if (sz >= 0) { az = axv(Z, k, AX_AH); bz = axv(Z, k, AX_BH); kz = axv(Z, k, AX_KH); }
-    const uint32_t oy = (uint32_t)(max(sy, 0) * nz + k) * 4u;     // psi y-slab, within plane i
-    const uint32_t oz = (uint32_t)(j * wz + max(sz, 0)) * 4u;     // psi z-slab, within plane i
+    const uint32_t oy = (uint32_t)(max(sy, 0) * nz + k) * T::B;   // psi y-slab, within plane i
+    const uint32_t oz = (uint32_t)(j * wz + max(sz, 0)) * T::B;   // psi z-slab, within plane i
 
     float inj_hx = 0.f, inj_hy = 0.f;
     if (pw.n) {
@@ -184,77 +229,81 @@ k_update_h(float *__restrict__ H, const float *__restrict__ E, PsiDev ps, GridDe
         }
     }
 
-    const uint32_t plane4 = (uint32_t)g.plane * 4u;
-    float ey = bld(rEy, op, (uint32_t)tc.i0 * plane4);
-    float ez = bld(rEz, op, (uint32_t)tc.i0 * plane4);
+    const uint32_t planeB = (uint32_t)g.plane * T::B;
+    F ey = T::ld(rEy, op, (uint32_t)tc.i0 * planeB);
+    F ez = T::ld(rEz, op, (uint32_t)tc.i0 * planeB);
     for (int i = tc.i0; i < tc.i1; ++i) {
-        const uint32_t so = (uint32_t)i * plane4;
-        const uint32_t sn = (uint32_t)X.nxt[i] * plane4;
-        const float ex = bld(rEx, op, so), ex_j = bld(rEx, oj, so), ex_k = bld(rEx, ok, so);
-        const float ey_k = bld(rEy, ok, so), ez_j = bld(rEz, oj, so);
-        const float ey_n = bld(rEy, op, sn), ez_n = bld(rEz, op, sn);
-        float hx = bld(rHx, op, so), hy = bld(rHy, op, so), hz = bld(rHz, op, so);
+        const uint32_t so = (uint32_t)i * planeB;
+        const uint32_t sn = (uint32_t)X.nxt[i] * planeB;
+        const F ex = T::ld(rEx, op, so), ex_j = T::ld(rEx, oj, so), ex_k = T::ld(rEx, ok, so);
+        const F ey_k = T::ld(rEy, ok, so), ez_j = T::ld(rEz, oj, so);
+        const F ey_n = T::ld(rEy, op, sn), ez_n = T::ld(rEz, op, sn);
+        F hx = T::ld(rHx, op, so), hy = T::ld(rHy, op, so), hz = T::ld(rHz, op, so);
         const int sx = slab_of(i, nx, X.lo, X.hi);
-        const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * 4u;
-        const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * 4u;
-        const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * 4u;
-        const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * 4u;
-        const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * 4u;
-        const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * 4u;
-        float psx0 = 0.f, psx1 = 0.f, psy0 = 0.f, psy1 = 0.f, psz0 = 0.f, psz1 = 0.f;
-        if (sx >= 0) { psx0 = bld(rP, op, sx0); psx1 = bld(rP, op, sx1); }
-        if (sy >= 0) { psy0 = bld(rP, oy, sy0); psy1 = bld(rP, oy, sy1); }
-        if (sz >= 0) { psz0 = bld(rP, oz, sz0); psz1 = bld(rP, oz, sz1); }
+        const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
+        const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
+        const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
+        const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
+        const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
+        const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
+        F psx0 = T::zero(), psx1 = T::zero(), psy0 = T::zero(), psy1 = T::zero(),
+          psz0 = T::zero(), psz1 = T::zero();
+        if (sx >= 0) { psx0 = T::ld(rP, op, sx0); psx1 = T::ld(rP, op, sx1); }
+        if (sy >= 0) { psy0 = T::ld(rP, oy, sy0); psy1 = T::ld(rP, oy, sy1); }
+        if (sz >= 0) { psz0 = T::ld(rP, oz, sz0); psz1 = T::ld(rP, oz, sz1); }
 
-        const float mi = axv(X, i, AX_MNX), idx = axv(X, i, AX_IDLP);
-        const float dzy = (ez_j * mj - ez) * idy;
-        const float dyz = (ey_k * mk - ey) * idz;
-        const float dxz = (ex_k * mk - ex) * idz;
-        const float dzx = (ez_n * mi - ez) * idx;
-        const float dyx = (ey_n * mi - ey) * idx;
-        const float dxy = (ex_j * mj - ex) * idy;
+        const float mi = axv(X, i, AX_MNX), mii = axv(X, i, AX_MNXI), idx = axv(X, i, AX_IDLP);
+        const F dzy = (T::mmul(ez_j, mj, mji) - ez) * idy;
+        const F dyz = (T::mmul(ey_k, mk, mki) - ey) * idz;
+        const F dxz = (T::mmul(ex_k, mk, mki) - ex) * idz;
+        const F dzx = (T::mmul(ez_n, mi, mii) - ez) * idx;
+        const F dyx = (T::mmul(ey_n, mi, mii) - ey) * idx;
+        const F dxy = (T::mmul(ex_j, mj, mji) - ex) * idy;
 
-        float s_hx_y = dzy, s_hz_y = dxy, s_hx_z = dyz, s_hy_z = dxz, s_hy_x = dzx, s_hz_x = dyx;
+        F s_hx_y = dzy, s_hz_y = dxy, s_hx_z = dyz, s_hy_z = dxz, s_hy_x = dzx, s_hz_x = dyx;
         if (sx >= 0) {
             const float ax_ = axv(X, i, AX_AH), bx_ = axv(X, i, AX_BH), kx_ = axv(X, i, AX_KH);
             s_hy_x = stretch_v(psx0, dzx, ax_, bx_, kx_);
             s_hz_x = stretch_v(psx1, dyx, ax_, bx_, kx_);
-            bst(rP, psx0, op, sx0);
-            bst(rP, psx1, op, sx1);
+            T::st(rP, psx0, op, sx0);
+            T::st(rP, psx1, op, sx1);
         }
         if (sy >= 0) {
             s_hx_y = stretch_v(psy0, dzy, ay, by, ky);
             s_hz_y = stretch_v(psy1, dxy, ay, by, ky);
-            bst(rP, psy0, oy, sy0);
-            bst(rP, psy1, oy, sy1);
+            T::st(rP, psy0, oy, sy0);
+            T::st(rP, psy1, oy, sy1);
         }
         if (sz >= 0) {
             s_hx_z = stretch_v(psz0, dyz, az, bz, kz);
             s_hy_z = stretch_v(psz1, dxz, az, bz, kz);
-            bst(rP, psz0, oz, sz0);
-            bst(rP, psz1, oz, sz1);
+            T::st(rP, psz0, oz, sz0);
+            T::st(rP, psz1, oz, sz1);
         }
-        hx = hx - ch * (s_hx_y - s_hx_z) + inj_hx;
-        hy = hy - ch * (s_hy_z - s_hy_x) + inj_hy;
+        hx = T::addre(hx - ch * (s_hx_y - s_hx_z), inj_hx);
+        hy = T::addre(hy - ch * (s_hy_z - s_hy_x), inj_hy);
         hz = hz - ch * (s_hz_x - s_hz_y);
-        bst(rHx, hx, op, so);
-        bst(rHy, hy, op, so);
-        bst(rHz, hz, op, so);
+        T::st(rHx, hx, op, so);
+        T::st(rHy, hy, op, so);
+        T::st(rHz, hz, op, so);
         ey = ey_n;
         ez = ez_n;
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// E whole step (reference update_e / update_e_lut, `openem/kernels/yee.cu:154-337`) fused with
-// the E side of the plane-wave TF/SF correction (`openem/kernels/source_dft.cu:58-73`).  The
-// class LUT is staged in LDS and the next plane's class index is prefetched, so the per-cell
-// coefficient gather never waits on a global load issued in the same iteration.
+// E whole step (reference update_e / update_e_lut, `openem/kernels/yee.cu:154-337`; complex
+// update_e_c, `openem/kernels/bloch.cu:128-221`) fused with the E side of the plane-wave TF/SF
+// correction (`openem/kernels/source_dft.cu:58-73`).  The class LUT is staged in LDS and the
+// next plane's class index is prefetched, so the per-cell coefficient gather never waits on a
+// global load issued in the same iteration.
 // ------------------------------------------------------------------------------------------
+template <typename F>
 __global__ void __launch_bounds__(OEM_TK * OEM_TJ)
-k_update_e(float *__restrict__ E, const float *__restrict__ H, PsiDev ps, GridDev g,
-           MatDev mat, float cw, PlaneSrcDev pw, const int *__restrict__ stp)
+k_update_e(F *__restrict__ E, const F *__restrict__ H, PsiDev ps, GridDev g, MatDev mat,
+           float cw, PlaneSrcDev pw, const int *__restrict__ stp)
 {
+    typedef FT<F> T;
     __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
     const bool lds_lut = mat.mode == 0 && mat.n_classes <= OEM_LDS_CLASSES;
     if (lds_lut) {
@@ -269,18 +318,19 @@ k_update_e(float *__restrict__ E, const float *__restrict__ H, PsiDev ps, GridDe
     const int nz = g.nz, ny = g.ny, nx = g.nx;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const size_t ncell = (size_t)g.plane * nx;
-    const uint32_t cbytes = (uint32_t)(ncell * 4);
+    const uint32_t cbytes = (uint32_t)(ncell * T::B);
     const rsrc_t rEx = mk_rsrc(E, cbytes), rEy = mk_rsrc(E + ncell, cbytes),
                  rEz = mk_rsrc(E + 2 * ncell, cbytes);
     const rsrc_t rHx = mk_rsrc(H, cbytes), rHy = mk_rsrc(H + ncell, cbytes),
                  rHz = mk_rsrc(H + 2 * ncell, cbytes);
-    const rsrc_t rP = mk_rsrc(ps.e, ps.total * 4u);
+    const rsrc_t rP = mk_rsrc(ps.e, ps.total * T::B);
     const rsrc_t rC = mk_rsrc(mat.cls, (uint32_t)(ncell * 2));
 
-    const uint32_t op = (uint32_t)(j * nz + k) * 4u;
-    const uint32_t oj = (uint32_t)(Y.prv[j] * nz + k) * 4u;       // (j-1, k)
-    const uint32_t ok = (uint32_t)(j * nz + Z.prv[k]) * 4u;       // (j, k-1)
+    const uint32_t op = (uint32_t)(j * nz + k) * T::B;
+    const uint32_t oj = (uint32_t)(Y.prv[j] * nz + k) * T::B;     // (j-1, k)
+    const uint32_t ok = (uint32_t)(j * nz + Z.prv[k]) * T::B;     // (j, k-1)
     const float mj = axv(Y, j, AX_MPV), mk = axv(Z, k, AX_MPV);
+    const float mji = axv(Y, j, AX_MPVI), mki = axv(Z, k, AX_MPVI);
     const float idy = axv(Y, j, AX_IDLD), idz = axv(Z, k, AX_IDLD);
     const float pecy = axv(Y, j, AX_PEC), pecz = axv(Z, k, AX_PEC);
     const int sy = slab_of(j, ny, Y.lo, Y.hi);
@@ -289,8 +339,8 @@ k_update_e(float *__restrict__ E, const float *__restrict__ H, PsiDev ps, GridDe
     float ay = 0.f, by = 1.f, ky = 1.f, az = 0.f, bz = 1.f, kz = 1.f;
     if (sy >= 0) { ay = axv(Y, j, AX_AE); by = axv(Y, j, AX_BE); ky = axv(Y, j, AX_KE); }
     if (sz >= 0) { az = axv(Z, k, AX_AE); bz = axv(Z, k, AX_BE); kz = axv(Z, k, AX_KE); }
-    const uint32_t oy = (uint32_t)(max(sy, 0) * nz + k) * 4u;
-    const uint32_t oz = (uint32_t)(j * wz + max(sz, 0)) * 4u;
+    const uint32_t oy = (uint32_t)(max(sy, 0) * nz + k) * T::B;
+    const uint32_t oz = (uint32_t)(j * wz + max(sz, 0)) * T::B;
 
     float inj_ex = 0.f, inj_ey = 0.f;
     bool inj_x = false, inj_y = false;
@@ -309,33 +359,34 @@ k_update_e(float *__restrict__ E, const float *__restrict__ H, PsiDev ps, GridDe
     const bool hrow = mat.hist && j >= mat.hb[2] && j < mat.hb[3] && k >= mat.hb[4] &&
                       k < mat.hb[5];
     const uint32_t hcomp = (uint32_t)(hbx * hby * hbz);
-    const rsrc_t rHist = mk_rsrc(mat.hist, 3u * hcomp * 4u);
-    const uint32_t oh = hrow ? (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * 4u : 0u;
+    const rsrc_t rHist = mk_rsrc(mat.hist, 3u * hcomp * T::B);
+    const uint32_t oh = hrow ? (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * T::B : 0u;
     const uint32_t oc = (uint32_t)(j * nz + k) * 2u;               // uint16 class index
 
-    const uint32_t plane4 = (uint32_t)g.plane * 4u, plane2 = (uint32_t)g.plane * 2u;
-    const uint32_t sp0 = (uint32_t)X.prv[tc.i0] * plane4;
-    float hy_m = bld(rHy, op, sp0), hz_m = bld(rHz, op, sp0);
+    const uint32_t planeB = (uint32_t)g.plane * T::B, plane2 = (uint32_t)g.plane * 2u;
+    const uint32_t sp0 = (uint32_t)X.prv[tc.i0] * planeB;
+    F hy_m = T::ld(rHy, op, sp0), hz_m = T::ld(rHz, op, sp0);
     uint32_t cls_next = mat.mode == 0 ? bld_u16(rC, oc, (uint32_t)tc.i0 * plane2) : 0u;
     for (int i = tc.i0; i < tc.i1; ++i) {
-        const uint32_t so = (uint32_t)i * plane4;
+        const uint32_t so = (uint32_t)i * planeB;
         const uint32_t cls = cls_next;
-        if (mat.mode == 0 && i + 1 < tc.i1) cls_next = bld_u16(rC, oc, so / 2u + plane2);
-        const float hx = bld(rHx, op, so), hy = bld(rHy, op, so), hz = bld(rHz, op, so);
-        const float hx_j = bld(rHx, oj, so), hx_k = bld(rHx, ok, so);
-        const float hy_k = bld(rHy, ok, so), hz_j = bld(rHz, oj, so);
-        float ex = bld(rEx, op, so), ey = bld(rEy, op, so), ez = bld(rEz, op, so);
+        if (mat.mode == 0 && i + 1 < tc.i1) cls_next = bld_u16(rC, oc, (uint32_t)(i + 1) * plane2);
+        const F hx = T::ld(rHx, op, so), hy = T::ld(rHy, op, so), hz = T::ld(rHz, op, so);
+        const F hx_j = T::ld(rHx, oj, so), hx_k = T::ld(rHx, ok, so);
+        const F hy_k = T::ld(rHy, ok, so), hz_j = T::ld(rHz, oj, so);
+        F ex = T::ld(rEx, op, so), ey = T::ld(rEy, op, so), ez = T::ld(rEz, op, so);
         const int sx = slab_of(i, nx, X.lo, X.hi);
-        const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * 4u;
-        const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * 4u;
-        const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * 4u;
-        const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * 4u;
-        const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * 4u;
-        const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * 4u;
-        float psx0 = 0.f, psx1 = 0.f, psy0 = 0.f, psy1 = 0.f, psz0 = 0.f, psz1 = 0.f;
-        if (sx >= 0) { psx0 = bld(rP, op, sx0); psx1 = bld(rP, op, sx1); }
-        if (sy >= 0) { psy0 = bld(rP, oy, sy0); psy1 = bld(rP, oy, sy1); }
-        if (sz >= 0) { psz0 = bld(rP, oz, sz0); psz1 = bld(rP, oz, sz1); }
+        const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
+        const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
+        const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
+        const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
+        const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
+        const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
+        F psx0 = T::zero(), psx1 = T::zero(), psy0 = T::zero(), psy1 = T::zero(),
+          psz0 = T::zero(), psz1 = T::zero();
+        if (sx >= 0) { psx0 = T::ld(rP, op, sx0); psx1 = T::ld(rP, op, sx1); }
+        if (sy >= 0) { psy0 = T::ld(rP, oy, sy0); psy1 = T::ld(rP, oy, sy1); }
+        if (sz >= 0) { psz0 = T::ld(rP, oz, sz0); psz1 = T::ld(rP, oz, sz1); }
 
         float cax, cbx, cay, cby, caz, cbz;
         uint32_t fl;
@@ -354,102 +405,133 @@ k_update_e(float *__restrict__ E, const float *__restrict__ H, PsiDev ps, GridDe
             cbx = mat.cb[id]; cby = mat.cb[ncell + id]; cbz = mat.cb[2 * ncell + id];
             fl = (hrow && i >= mat.hb[0] && i < mat.hb[1]) ? 7u : 0u;
         }
-        float phx = 0.f, phy = 0.f, phz = 0.f;
+        F phx = T::zero(), phy = T::zero(), phz = T::zero();
         if (fl) {
-            const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * 4u;
-            if (fl & 1u) phx = bld(rHist, oh, sh);
-            if (fl & 2u) phy = bld(rHist, oh, sh + hcomp * 4u);
-            if (fl & 4u) phz = bld(rHist, oh, sh + 2u * hcomp * 4u);
+            const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * T::B;
+            if (fl & 1u) phx = T::ld(rHist, oh, sh);
+            if (fl & 2u) phy = T::ld(rHist, oh, sh + hcomp * T::B);
+            if (fl & 4u) phz = T::ld(rHist, oh, sh + 2u * hcomp * T::B);
         }
 
-        const float mi = axv(X, i, AX_MPV), idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
-        const float dzy = (hz - hz_j * mj) * idy;
-        const float dyz = (hy - hy_k * mk) * idz;
-        const float dxz = (hx - hx_k * mk) * idz;
-        const float dzx = (hz - hz_m * mi) * idx;
-        const float dyx = (hy - hy_m * mi) * idx;
-        const float dxy = (hx - hx_j * mj) * idy;
+        const float mi = axv(X, i, AX_MPV), mii = axv(X, i, AX_MPVI);
+        const float idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
+        const F dzy = (hz - T::mmul(hz_j, mj, mji)) * idy;
+        const F dyz = (hy - T::mmul(hy_k, mk, mki)) * idz;
+        const F dxz = (hx - T::mmul(hx_k, mk, mki)) * idz;
+        const F dzx = (hz - T::mmul(hz_m, mi, mii)) * idx;
+        const F dyx = (hy - T::mmul(hy_m, mi, mii)) * idx;
+        const F dxy = (hx - T::mmul(hx_j, mj, mji)) * idy;
 
-        float s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
+        F s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
         if (sx >= 0) {
             const float ax_ = axv(X, i, AX_AE), bx_ = axv(X, i, AX_BE), kx_ = axv(X, i, AX_KE);
             s_ey_x = stretch_v(psx0, dzx, ax_, bx_, kx_);
             s_ez_x = stretch_v(psx1, dyx, ax_, bx_, kx_);
-            bst(rP, psx0, op, sx0);
-            bst(rP, psx1, op, sx1);
+            T::st(rP, psx0, op, sx0);
+            T::st(rP, psx1, op, sx1);
         }
         if (sy >= 0) {
             s_ex_y = stretch_v(psy0, dzy, ay, by, ky);
             s_ez_y = stretch_v(psy1, dxy, ay, by, ky);
-            bst(rP, psy0, oy, sy0);
-            bst(rP, psy1, oy, sy1);
+            T::st(rP, psy0, oy, sy0);
+            T::st(rP, psy1, oy, sy1);
         }
         if (sz >= 0) {
             s_ex_z = stretch_v(psz0, dyz, az, bz, kz);
             s_ey_z = stretch_v(psz1, dxz, az, bz, kz);
-            bst(rP, psz0, oz, sz0);
-            bst(rP, psz1, oz, sz1);
+            T::st(rP, psz0, oz, sz0);
+            T::st(rP, psz1, oz, sz1);
         }
         ex = (cax * ex + cbx * (s_ex_y - s_ex_z - cw * phx)) * pecy * pecz;
         ey = (cay * ey + cby * (s_ey_z - s_ey_x - cw * phy)) * pecz * pecx;
         ez = (caz * ez + cbz * (s_ez_x - s_ez_y - cw * phz)) * pecx * pecy;
-        if (inj_x) ex += cbx * inj_ex;
-        if (inj_y) ey += cby * inj_ey;
-        bst(rEx, ex, op, so);
-        bst(rEy, ey, op, so);
-        bst(rEz, ez, op, so);
+        if (inj_x) ex = T::addre(ex, cbx * inj_ex);
+        if (inj_y) ey = T::addre(ey, cby * inj_ey);
+        T::st(rEx, ex, op, so);
+        T::st(rEy, ey, op, so);
+        T::st(rEz, ez, op, so);
         hy_m = hy;
         hz_m = hz;
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// Point dipoles, one thread per distinct (comp, cell); its points are applied in input order
-// (reference inject_dipole / inject_dipole_h, `openem/kernels/source_dft.cu:147-206`, which
-// uses atomicAdd instead).
+// Grouped point corrections: one thread per distinct (component, cell), its points applied in
+// input order (no atomics, deterministic).  Used for
+//   * point dipoles (reference inject_dipole / inject_dipole_h,
+//     `openem/kernels/source_dft.cu:147-206`, which use atomicAdd):
+//     F -= scale * coef * amp[src][n]
+//   * TF/SF box face corrections (reference tfsf_corr_e / tfsf_corr_h,
+//     `openem/kernels/tfsf.cu:49-83`):  F += scale * coef * (table[n][col] * phase)
+// scale is ch on the H side and the cell's cb on the E side.
 // ------------------------------------------------------------------------------------------
-struct DipoleDev {
-    int n_groups, nsteps1, magnetic;
-    const int *g_comp, *g_cell, *g_ofs;     // per group; ofs (n_groups+1) into the point lists
+struct PointsDev {
+    int n_groups, nsteps1, ncol;
+    int e_side;                             // 1: scale by the cell's cb; 0: by ch
+    int tfsf;                               // 0: dipole amplitudes, 1: TF/SF table lookups
+    int cplx_amp;                           // dipole amplitude entries are complex doubles
+    const int *g_comp, *g_cell, *g_ofs;     // per group; ofs (n_groups+1) into the points
     const float *coef;                      // per point
-    const int *src;                         // per point
-    const double *amp;                      // n_src * nsteps1
+    const int *col;                         // per point: source (dipole) or table column
+    const float *phase;                     // per point (re, im) for TF/SF, or nullptr
+    const double *tab;                      // dipole: n_src*nsteps1 (x2 if complex);
+                                            // TF/SF: nsteps1*ncol complex (re, im)
 };
 
+template <typename F>
 __global__ void __launch_bounds__(256)
-k_dipoles(float *__restrict__ F0, float *__restrict__ F1, float *__restrict__ F2, DipoleDev d,
-          MatDev mat, float ch, size_t ncell, const int *__restrict__ stp)
+k_points(F *__restrict__ Fld, PointsDev d, MatDev mat, float ch, size_t ncell,
+         const int *__restrict__ stp)
 {
+    typedef FT<F> T;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.n_groups) return;
     const int n = step_now(stp);
     if (n >= d.nsteps1) return;
     const int c = d.g_comp[t], id = d.g_cell[t];
-    float *F = c == 0 ? F0 : (c == 1 ? F1 : F2);
+    F *Fp = Fld + (size_t)c * ncell;
     float scale;
-    if (d.magnetic) {
+    if (!d.e_side) {
         scale = ch;
     } else if (mat.mode == 0) {
         scale = mat.lut[8 * (size_t)mat.cls[id] + 2 * c + 1];
     } else {
         scale = mat.cb[(size_t)c * ncell + id];
     }
-    float v = F[id];
-    for (int q = d.g_ofs[t]; q < d.g_ofs[t + 1]; ++q)
-        v -= scale * d.coef[q] * (float)d.amp[(size_t)d.src[q] * d.nsteps1 + n];
-    F[id] = v;
+    F v = Fp[id];
+    for (int q = d.g_ofs[t]; q < d.g_ofs[t + 1]; ++q) {
+        float ar, ai;
+        if (d.tfsf) {
+            const double *e = d.tab + 2 * ((size_t)n * d.ncol + d.col[q]);
+            const double pr = d.phase[2 * q], pi = d.phase[2 * q + 1];
+            ar = (float)(e[0] * pr - e[1] * pi);
+            ai = (float)(e[0] * pi + e[1] * pr);
+        } else if (d.cplx_amp) {
+            const double *e = d.tab + 2 * ((size_t)d.col[q] * d.nsteps1 + n);
+            ar = -(float)e[0];
+            ai = -(float)e[1];
+        } else {
+            ar = -(float)d.tab[(size_t)d.col[q] * d.nsteps1 + n];
+            ai = 0.f;
+        }
+        v = T::addc(v, scale * d.coef[q], ar, ai);
+    }
+    Fp[id] = v;
 }
 
 // ------------------------------------------------------------------------------------------
-// Complex-pole ADE (reference dispersion_pre / dispersion_post,
-// `openem/kernels/dispersion.cu:68-147`): one thread per entry.
+// Complex-pole ADE (reference dispersion_pre / dispersion_post / dispersion_step,
+// `openem/kernels/dispersion.cu:68-207`): one thread per entry.  On the complex (Bloch) path
+// Re E and Im E each drive their own pole state (p / q), since the poles act cell-wise and every
+// coefficient of the update is real.
 // ------------------------------------------------------------------------------------------
 struct DispDev {
     int n_entry;
     const int *comp, *cell, *ofs, *cidx;
     const float *am1_re, *am1_im, *b_re, *b_im;
-    float *p_re, *p_im;
-    float *hist;
+    float *p_re, *p_im;     // pole state driven by Re E
+    float *q_re, *q_im;     // pole state driven by Im E (complex fields only)
+    void *hist;             // F elements, (3, bx, by, bz)
     int hb[6];
 };
 
@@ -459,98 +541,83 @@ __device__ __forceinline__ size_t hist_index(const DispDev &d, int c, int cell, 
     return (((size_t)c * bx + (i - d.hb[0])) * by + (j - d.hb[2])) * bz + (k - d.hb[4]);
 }
 
-__global__ void __launch_bounds__(256)
-k_disp_pre(DispDev d, const float *__restrict__ Ex, const float *__restrict__ Ey,
-           const float *__restrict__ Ez, int ny, int nz)
-{
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= d.n_entry) return;
-    const int c = d.comp[e], id = d.cell[e];
-    const float en = c == 0 ? Ex[id] : (c == 1 ? Ey[id] : Ez[id]);
-    float w = 0.f;
-    for (int t = d.ofs[e]; t < d.ofs[e + 1]; ++t) {
-        const int ci = d.cidx[t];
-        const float pr = d.p_re[t], pi = d.p_im[t];
-        const float ar = d.am1_re[ci], ai = d.am1_im[ci];
-        const float br = d.b_re[ci], bi = d.b_im[ci];
-        const float dr = ar * pr - ai * pi + br * en;
-        const float di = ar * pi + ai * pr + bi * en;
-        d.p_re[t] = pr + dr;
-        d.p_im[t] = pi + di;
-        w += 2.0f * dr;
+// one pole: mode 0 = pre (P += dP with dP = (A-1)P + B e, returns 2 Re dP), 1 = post
+// (P += B e), 2 = post then pre with the same e (the fused step).
+__device__ __forceinline__ float pole_update(float *pr_, float *pi_, int t, int ci,
+                                            const DispDev &d, float e, int mode) {
+    const float br = d.b_re[ci], bi = d.b_im[ci];
+    float pr = pr_[t], pi = pi_[t];
+    if (mode >= 1) {
+        pr = pr + br * e;
+        pi = pi + bi * e;
+        if (mode == 1) {
+            pr_[t] = pr;
+            pi_[t] = pi;
+            return 0.f;
+        }
     }
-    d.hist[hist_index(d, c, id, ny, nz)] = w;
+    const float ar = d.am1_re[ci], ai = d.am1_im[ci];
+    const float dr = ar * pr - ai * pi + br * e;
+    const float di = ar * pi + ai * pr + bi * e;
+    pr_[t] = pr + dr;
+    pi_[t] = pi + di;
+    return 2.0f * dr;
 }
 
-__global__ void __launch_bounds__(256)
-k_disp_post(DispDev d, const float *__restrict__ Ex, const float *__restrict__ Ey,
-            const float *__restrict__ Ez)
-{
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= d.n_entry) return;
-    const int c = d.comp[e], id = d.cell[e];
-    const float en = c == 0 ? Ex[id] : (c == 1 ? Ey[id] : Ez[id]);
-    for (int t = d.ofs[e]; t < d.ofs[e + 1]; ++t) {
-        const int ci = d.cidx[t];
-        d.p_re[t] = d.p_re[t] + d.b_re[ci] * en;
-        d.p_im[t] = d.p_im[t] + d.b_im[ci] * en;
-    }
+__device__ __forceinline__ void hist_store(void *h, size_t q, float w, float) {
+    static_cast<float *>(h)[q] = w;
 }
 
-// post(n) + pre(n+1) in one pass (reference dispersion_step, `openem/kernels/dispersion.cu:168-207`):
-// nothing writes E between the two, so the same E^{n+1} feeds both and P is read and written
-// once instead of twice.
+template <typename F>
 __global__ void __launch_bounds__(256)
-k_disp_step(DispDev d, const float *__restrict__ Ex, const float *__restrict__ Ey,
-            const float *__restrict__ Ez, int ny, int nz)
+k_disp(DispDev d, const F *__restrict__ E, size_t ncell, int ny, int nz, int mode)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= d.n_entry) return;
     const int c = d.comp[e], id = d.cell[e];
-    const float en = c == 0 ? Ex[id] : (c == 1 ? Ey[id] : Ez[id]);
-    float w = 0.f;
-    for (int t = d.ofs[e]; t < d.ofs[e + 1]; ++t) {
-        const int ci = d.cidx[t];
-        const float br = d.b_re[ci], bi = d.b_im[ci];
-        const float pr = d.p_re[t] + br * en;
-        const float pi = d.p_im[t] + bi * en;
-        const float ar = d.am1_re[ci], ai = d.am1_im[ci];
-        const float dr = ar * pr - ai * pi + br * en;
-        const float di = ar * pi + ai * pr + bi * en;
-        d.p_re[t] = pr + dr;
-        d.p_im[t] = pi + di;
-        w += 2.0f * dr;
+    const F en = E[(size_t)c * ncell + id];
+    if constexpr (sizeof(F) == 4) {
+        float w = 0.f;
+        for (int t = d.ofs[e]; t < d.ofs[e + 1]; ++t)
+            w += pole_update(d.p_re, d.p_im, t, d.cidx[t], d, en, mode);
+        if (mode != 1) static_cast<float *>(d.hist)[hist_index(d, c, id, ny, nz)] = w;
+    } else {
+        float wr = 0.f, wi = 0.f;
+        for (int t = d.ofs[e]; t < d.ofs[e + 1]; ++t) {
+            wr += pole_update(d.p_re, d.p_im, t, d.cidx[t], d, en.r, mode);
+            wi += pole_update(d.q_re, d.q_im, t, d.cidx[t], d, en.i, mode);
+        }
+        if (mode != 1) static_cast<cf2 *>(d.hist)[hist_index(d, c, id, ny, nz)] = cf2{wr, wi};
     }
-    d.hist[hist_index(d, c, id, ny, nz)] = w;
 }
 
 // ------------------------------------------------------------------------------------------
 // Running DFT, batched: every step a snapshot of the monitored values goes into a ring of T
-// float32 rows; every T steps one flush replays them in step order against the float64 phase
-// table and adds into the float64 accumulators (reference accumulate_dft / accumulate_dft_box
-// and the batched dft_snap_* / dft_flush_*, `openem/kernels/source_dft.cu:216-271`,
-// `:371-430`, `:453-779`; there the partial sums are float32, here they are float64).
+// rows; every T steps one flush replays them in step order against the float64 phase table and
+// adds into the float64 accumulators (reference accumulate_dft / accumulate_dft_box and the
+// batched dft_snap_* / dft_flush_*, `openem/kernels/source_dft.cu:216-271`, `:371-430`,
+// `:453-779`; there the partial sums are float32, here they are float64).  Complex fields
+// accumulate (u_r + i u_i) e^{i w t}, the reference's complex DFT (`bloch.cu:389-481`).
 // ------------------------------------------------------------------------------------------
 struct MonDev {
     int kind;                // 0 plane, 1 box
     int axis, km, n1, n2;    // plane
     int o[3], b[3];          // box
     int ncomp, npts, nf, T;
-    float *snap;             // T * ncomp * npts
+    void *snap;              // T * ncomp * npts elements of F
     double *re, *im;         // ncomp * nf * npts
     const double *phase;     // n_steps * nf * 4
 };
 
+template <typename F>
 __global__ void __launch_bounds__(256)
-k_dft_snap(MonDev m, const float *__restrict__ Ex, const float *__restrict__ Ey,
-           const float *__restrict__ Ez, const float *__restrict__ Hx,
-           const float *__restrict__ Hy, const float *__restrict__ Hz, int ny, int nz,
-           const int *__restrict__ stp)
+k_dft_snap(MonDev m, const F *__restrict__ E, const F *__restrict__ H, size_t ncell, int ny,
+           int nz, const int *__restrict__ stp)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.npts) return;
     const int slot = step_now(stp) % m.T;
-    float *row = m.snap + (size_t)slot * m.ncomp * m.npts;
+    F *row = static_cast<F *>(m.snap) + (size_t)slot * m.ncomp * m.npts;
     if (m.kind == 0) {
         const int a = q / m.n2, b = q - a * m.n2;
         int c[3];
@@ -559,29 +626,35 @@ k_dft_snap(MonDev m, const float *__restrict__ Ex, const float *__restrict__ Ey,
         const size_t id = ((size_t)c[0] * ny + c[1]) * nz + c[2];
         c[m.axis] = m.km - 1;
         const size_t idm = ((size_t)c[0] * ny + c[1]) * nz + c[2];
-        const float *Ep[3] = {Ex, Ey, Ez};
-        const float *Hp[3] = {Hx, Hy, Hz};
-        row[q] = Ep[t1][id];
-        row[m.npts + q] = Ep[t2][id];
-        row[2 * m.npts + q] = 0.5f * (Hp[t1][id] + Hp[t1][idm]);
-        row[3 * m.npts + q] = 0.5f * (Hp[t2][id] + Hp[t2][idm]);
+        row[q] = E[(size_t)t1 * ncell + id];
+        row[m.npts + q] = E[(size_t)t2 * ncell + id];
+        row[2 * m.npts + q] = 0.5f * (H[(size_t)t1 * ncell + id] + H[(size_t)t1 * ncell + idm]);
+        row[3 * m.npts + q] = 0.5f * (H[(size_t)t2 * ncell + id] + H[(size_t)t2 * ncell + idm]);
     } else {
         const int bjk = m.b[1] * m.b[2];
         const int a = q / bjk, r = q - a * bjk, b = r / m.b[2], cc = r - b * m.b[2];
         const size_t id = ((size_t)(m.o[0] + a) * ny + (m.o[1] + b)) * nz + (m.o[2] + cc);
-        row[q] = Ex[id];
-        row[m.npts + q] = Ey[id];
-        row[2 * (size_t)m.npts + q] = Ez[id];
-        row[3 * (size_t)m.npts + q] = Hx[id];
-        row[4 * (size_t)m.npts + q] = Hy[id];
-        row[5 * (size_t)m.npts + q] = Hz[id];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            row[(size_t)c * m.npts + q] = E[(size_t)c * ncell + id];
+            row[(size_t)(3 + c) * m.npts + q] = H[(size_t)c * ncell + id];
+        }
     }
+}
+
+__device__ __forceinline__ void dft_acc(double &sr, double &si, float v, double c, double s) {
+    sr += (double)v * c;
+    si += (double)v * s;
+}
+__device__ __forceinline__ void dft_acc(double &sr, double &si, cf2 v, double c, double s) {
+    sr += (double)v.r * c - (double)v.i * s;
+    si += (double)v.r * s + (double)v.i * c;
 }
 
 // grid: x over points, y over frequencies.  Launched after the step counter advanced past the
 // last snapshot: the cnt snapshots ending at step *stp - 1 are replayed in step order; NC
 // components of which the first NC/2 are E (whole-step phase).
-template <int NC>
+template <typename F, int NC>
 __global__ void __launch_bounds__(256)
 k_dft_flush(MonDev m, int cnt, const int *__restrict__ stp)
 {
@@ -594,14 +667,13 @@ k_dft_flush(MonDev m, int cnt, const int *__restrict__ stp)
 #pragma unroll
     for (int c = 0; c < NC; ++c) { sre[c] = 0.0; sim[c] = 0.0; }
     for (int s = first; s <= last; ++s) {
-        const float *row = m.snap + (size_t)(s % m.T) * NC * m.npts;
+        const F *row = static_cast<const F *>(m.snap) + (size_t)(s % m.T) * NC * m.npts;
         const double *ph = m.phase + ((size_t)s * m.nf + f) * 4;
         const double ce = ph[0], se = ph[1], chh = ph[2], shh = ph[3];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            const double v = (double)row[(size_t)c * m.npts + q];
-            sre[c] += v * (c < NC / 2 ? ce : chh);
-            sim[c] += v * (c < NC / 2 ? se : shh);
+            const F v = row[(size_t)c * m.npts + q];
+            dft_acc(sre[c], sim[c], v, c < NC / 2 ? ce : chh, c < NC / 2 ? se : shh);
         }
     }
 #pragma unroll

@@ -4,8 +4,9 @@
 // state, source and monitor tables.  oem_run advances it with the launch sequence of the
 // reference's _step_body (`openem/solver.py:1145-1162`):
 //
-//   update_h (+ plane-wave H correction) -> magnetic dipoles -> ADE pre -> update_e
-//   (+ plane-wave E correction) -> electric dipoles -> ADE post -> DFT snapshots -> step++
+//   update_h (+ plane-wave H correction) -> magnetic dipoles / TF/SF H faces -> ADE pre ->
+//   update_e (+ plane-wave E correction) -> TF/SF E faces -> electric dipoles -> ADE post
+//   (fused with the next pre) -> DFT snapshots -> step++
 //
 // with the DFT flushed every dft_batch steps.  Blocks of dft_batch steps aligned to the batch
 // are captured once into a hipGraph and replayed; the device-side step counter lets one graph
@@ -50,8 +51,10 @@ struct Monitor {
     size_t count;     // ncomp*nf*npts
 };
 
-struct Dipoles {
-    DipoleDev d;
+// One grouped point list (dipoles or TF/SF face corrections) and which field it updates.
+struct Points {
+    PointsDev d;
+    int magnetic;       // 1: applied to H after update_h; 0: to E after update_e
 };
 
 }  // namespace
@@ -61,6 +64,8 @@ struct oem_ctx {
     hipStream_t stream = nullptr;
     oem_dims dims{};
     size_t ncell = 0;
+    bool cplx = false;          // Bloch: complex fields, (re, im) interleaved
+    size_t esz = 1;             // floats per field element
     float *E = nullptr, *H = nullptr;
     PsiDev psi{};
     GridDev g{};
@@ -71,7 +76,7 @@ struct oem_ctx {
     DispDev disp{};
     bool has_disp = false;
     PlaneSrcDev pw{};
-    std::vector<Dipoles> dipoles;
+    std::vector<Points> points;
     std::vector<Monitor> mons;
     int *d_step = nullptr;
     int64_t step = 0;
@@ -143,7 +148,6 @@ extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) 
         return fail(OEM_EINVAL, "grid %d x %d x %d", d->nx, d->ny, d->nz);
     if ((int64_t)d->ny * d->nz > (int64_t)1 << 30)
         return fail(OEM_EINVAL, "plane ny*nz = %lld too large", (long long)d->ny * d->nz);
-    if (d->complex_fields) return fail(OEM_ENOTSUP, "complex (Bloch) fields use oem_bloch_*");
     if (d->n_steps < 1) return fail(OEM_EINVAL, "n_steps %d", d->n_steps);
     if (d->dft_batch < 1 || d->dft_batch > 64)
         return fail(OEM_EINVAL, "dft_batch %d outside 1..64", d->dft_batch);
@@ -161,15 +165,18 @@ extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) 
     c->device = device;
     c->dims = *d;
     c->ncell = (size_t)d->nx * d->ny * d->nz;
+    c->cplx = d->complex_fields != 0;
+    c->esz = c->cplx ? 2 : 1;
     int rc = OEM_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(OEM_EHIP, "hipStreamCreate failed");
     }
     const size_t nc = c->ncell;
-    if (nc * 4 > 0xffffffffull) {
+    if (nc * 4 * c->esz > 0xffffffffull) {
         delete c;
-        return fail(OEM_ENOTSUP, "%zu cells: one field component exceeds 4 GiB (limit 2^30 cells)", nc);
+        return fail(OEM_ENOTSUP, "%zu cells: one field component exceeds 4 GiB (limit 2^%d cells)",
+                    nc, c->cplx ? 29 : 30);
     }
     const size_t slab[3] = {(size_t)(d->pml_lo[0] + d->pml_hi[0]) * d->ny * d->nz,
                             (size_t)d->nx * (d->pml_lo[1] + d->pml_hi[1]) * d->nz,
@@ -180,15 +187,18 @@ extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) 
             c->psi.off[2 * a + m] = (uint32_t)total;
             total += slab[a];
         }
-    if (total * 4 > 0xffffffffull) {
+    if (total * 4 * c->esz > 0xffffffffull) {
         delete c;
         return fail(OEM_ENOTSUP, "CPML slabs exceed 4 GiB per side");
     }
     c->psi.total = (uint32_t)(total ? total : 1);
-    rc = c->alloc(&c->E, 3 * nc, true);
-    if (!rc) rc = c->alloc(&c->H, 3 * nc, true);
-    if (!rc) rc = c->alloc(&c->psi.h, c->psi.total, true);
-    if (!rc) rc = c->alloc(&c->psi.e, c->psi.total, true);
+    rc = c->alloc(&c->E, 3 * nc * c->esz, true);
+    if (!rc) rc = c->alloc(&c->H, 3 * nc * c->esz, true);
+    float *ph = nullptr, *pe = nullptr;
+    if (!rc) rc = c->alloc(&ph, (size_t)c->psi.total * c->esz, true);
+    if (!rc) rc = c->alloc(&pe, (size_t)c->psi.total * c->esz, true);
+    c->psi.h = ph;
+    c->psi.e = pe;
     if (!rc) rc = c->alloc(&c->d_step, 1, true);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OEM_EHIP, "sync failed");
     if (rc) {
@@ -211,7 +221,8 @@ extern "C" int oem_set_axis(oem_ctx *ctx, int32_t axis, const float *idl_p, cons
                             const int32_t *nxt, const int32_t *prv, const float *mnx,
                             const float *mpv, const float *pec, const float *a_e,
                             const float *b_e, const float *k_e, const float *a_h,
-                            const float *b_h, const float *k_h) {
+                            const float *b_h, const float *k_h, const float *mnx_im,
+                            const float *mpv_im) {
     CHECK_CTX(ctx);
     if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
     if (axis < 0 || axis > 2) return fail(OEM_EINVAL, "axis %d", axis);
@@ -234,6 +245,13 @@ extern "C" int oem_set_axis(oem_ctx *ctx, int32_t axis, const float *idl_p, cons
                          AX_BE,   AX_KE,   AX_AH,  AX_BH,  AX_KH};
     for (int t = 0; t < 11; ++t)
         for (int i = 0; i < n; ++i) tab[(size_t)i * AX_W + fid[t]] = fa[t][i];
+    for (int i = 0; i < n; ++i) {
+        const float a = mnx_im ? mnx_im[i] : 0.f, b = mpv_im ? mpv_im[i] : 0.f;
+        if ((a != 0.f || b != 0.f) && !ctx->cplx)
+            return fail(OEM_EINVAL, "axis %d: complex (Bloch) masks need complex fields", axis);
+        tab[(size_t)i * AX_W + AX_MNXI] = a;
+        tab[(size_t)i * AX_W + AX_MPVI] = b;
+    }
     float *dt = nullptr;
     TRY(ctx->upload(&dt, tab.data(), tab.size()));
     A.tab = dt;
@@ -338,9 +356,15 @@ extern "C" int oem_set_dispersion(oem_ctx *ctx, int32_t n_entry, const int32_t *
     D.am1_re = ar; D.am1_im = ai; D.b_re = br; D.b_im = bi;
     TRY(ctx->alloc(&D.p_re, (size_t)n_pole, true));
     TRY(ctx->alloc(&D.p_im, (size_t)n_pole, true));
+    if (ctx->cplx) {
+        TRY(ctx->alloc(&D.q_re, (size_t)n_pole, true));
+        TRY(ctx->alloc(&D.q_im, (size_t)n_pole, true));
+    }
     const size_t hsz = 3ull * (hb[1] - hb[0]) * (hb[3] - hb[2]) * (hb[5] - hb[4]);
-    if (hsz * 4 > 0xffffffffull) return fail(OEM_ENOTSUP, "ADE history box exceeds 4 GiB");
-    TRY(ctx->alloc(&D.hist, hsz, true));
+    if (hsz * 4 * ctx->esz > 0xffffffffull) return fail(OEM_ENOTSUP, "ADE history box exceeds 4 GiB");
+    float *hist = nullptr;
+    TRY(ctx->alloc(&hist, hsz * ctx->esz, true));
+    D.hist = hist;
     for (int q = 0; q < 6; ++q) { D.hb[q] = hb[q]; ctx->mat.hb[q] = hb[q]; }
     ctx->mat.hist = D.hist;
     ctx->has_disp = true;
@@ -372,27 +396,18 @@ extern "C" int oem_add_plane_source(oem_ctx *ctx, int32_t pol, int32_t kh, int32
     return OEM_OK;
 }
 
-extern "C" int oem_add_dipoles(oem_ctx *ctx, int32_t magnetic, int32_t n_pts,
-                               const int32_t *comp, const int32_t *cell, const float *coef,
-                               const int32_t *src_of, int32_t n_src, const double *amp) {
-    CHECK_CTX(ctx);
-    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
-    if (n_pts < 1 || n_src < 1 || !comp || !cell || !coef || !src_of || !amp)
-        return fail(OEM_EINVAL, "bad dipole tables");
-    for (int q = 0; q < n_pts; ++q) {
-        if (comp[q] < 0 || comp[q] > 2) return fail(OEM_EINVAL, "dipole comp at %d", q);
-        if (cell[q] < 0 || (size_t)cell[q] >= ctx->ncell) return fail(OEM_EINVAL, "dipole cell at %d", q);
-        if (src_of[q] < 0 || src_of[q] >= n_src) return fail(OEM_EINVAL, "dipole src at %d", q);
-    }
-    // group points by (comp, cell), keeping input order inside a group
+// Group points by (comp, cell), keeping input order inside a group, upload, and append.
+static int add_points(oem_ctx *ctx, int magnetic, int tfsf, int cplx_amp, int32_t n_pts,
+                      const int32_t *comp, const int32_t *cell, const float *coef,
+                      const int32_t *col, const float *phase, int32_t ncol, const double *tab,
+                      size_t tab_doubles) {
     std::vector<int> order(n_pts);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return comp[a] != comp[b] ? comp[a] < comp[b] : cell[a] < cell[b];
     });
-    std::vector<int> gc, gl, go;
-    std::vector<float> cf;
-    std::vector<int> sr;
+    std::vector<int> gc, gl, go, cl;
+    std::vector<float> cf, ph;
     for (int t = 0; t < n_pts; ++t) {
         const int q = order[t];
         if (t == 0 || comp[q] != gc.back() || cell[q] != gl.back()) {
@@ -401,25 +416,75 @@ extern "C" int oem_add_dipoles(oem_ctx *ctx, int32_t magnetic, int32_t n_pts,
             go.push_back(t);
         }
         cf.push_back(coef[q]);
-        sr.push_back(src_of[q]);
+        cl.push_back(col[q]);
+        if (phase) {
+            ph.push_back(phase[2 * q]);
+            ph.push_back(phase[2 * q + 1]);
+        }
     }
     go.push_back(n_pts);
-    Dipoles dp{};
-    dp.d.n_groups = (int)gc.size();
-    dp.d.nsteps1 = ctx->dims.n_steps + 1;
-    dp.d.magnetic = magnetic ? 1 : 0;
-    int *a, *b, *o, *s;
-    float *c;
-    double *am;
-    TRY(ctx->upload(&a, gc.data(), gc.size()));
-    TRY(ctx->upload(&b, gl.data(), gl.size()));
-    TRY(ctx->upload(&o, go.data(), go.size()));
-    TRY(ctx->upload(&c, cf.data(), cf.size()));
-    TRY(ctx->upload(&s, sr.data(), sr.size()));
-    TRY(ctx->upload(&am, amp, (size_t)n_src * dp.d.nsteps1));
-    dp.d.g_comp = a; dp.d.g_cell = b; dp.d.g_ofs = o; dp.d.coef = c; dp.d.src = s; dp.d.amp = am;
-    ctx->dipoles.push_back(dp);
+    Points P{};
+    P.magnetic = magnetic ? 1 : 0;
+    PointsDev &d = P.d;
+    d.n_groups = (int)gc.size();
+    d.nsteps1 = ctx->dims.n_steps + 1;
+    d.ncol = ncol;
+    d.e_side = magnetic ? 0 : 1;
+    d.tfsf = tfsf;
+    d.cplx_amp = cplx_amp;
+    int *pa, *pb, *po, *pc;
+    float *pf, *pp = nullptr;
+    double *pt;
+    TRY(ctx->upload(&pa, gc.data(), gc.size()));
+    TRY(ctx->upload(&pb, gl.data(), gl.size()));
+    TRY(ctx->upload(&po, go.data(), go.size()));
+    TRY(ctx->upload(&pf, cf.data(), cf.size()));
+    TRY(ctx->upload(&pc, cl.data(), cl.size()));
+    if (phase) TRY(ctx->upload(&pp, ph.data(), ph.size()));
+    TRY(ctx->upload(&pt, tab, tab_doubles));
+    d.g_comp = pa; d.g_cell = pb; d.g_ofs = po; d.coef = pf; d.col = pc; d.phase = pp; d.tab = pt;
+    ctx->points.push_back(P);
     return OEM_OK;
+}
+
+static int check_points(oem_ctx *ctx, int32_t n_pts, const int32_t *comp, const int32_t *cell,
+                        const float *coef, const int32_t *col, int32_t ncol) {
+    if (n_pts < 1 || !comp || !cell || !coef || !col) return fail(OEM_EINVAL, "bad point tables");
+    for (int q = 0; q < n_pts; ++q) {
+        if (comp[q] < 0 || comp[q] > 2) return fail(OEM_EINVAL, "point %d: component %d", q, comp[q]);
+        if (cell[q] < 0 || (size_t)cell[q] >= ctx->ncell) return fail(OEM_EINVAL, "point %d: cell", q);
+        if (col[q] < 0 || col[q] >= ncol) return fail(OEM_EINVAL, "point %d: column %d of %d", q, col[q], ncol);
+    }
+    return OEM_OK;
+}
+
+extern "C" int oem_add_dipoles(oem_ctx *ctx, int32_t magnetic, int32_t n_pts,
+                               const int32_t *comp, const int32_t *cell, const float *coef,
+                               const int32_t *src_of, int32_t n_src, const double *amp,
+                               int32_t amp_complex) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (n_src < 1 || !amp) return fail(OEM_EINVAL, "bad dipole amplitude table");
+    TRY(check_points(ctx, n_pts, comp, cell, coef, src_of, n_src));
+    const size_t nd = (size_t)n_src * (ctx->dims.n_steps + 1) * (amp_complex ? 2 : 1);
+    return add_points(ctx, magnetic, 0, amp_complex ? 1 : 0, n_pts, comp, cell, coef, src_of,
+                      nullptr, n_src, amp, nd);
+}
+
+extern "C" int oem_add_tfsf_points(oem_ctx *ctx, int32_t magnetic, int32_t n_pts,
+                                   const int32_t *comp, const int32_t *cell, const float *coef,
+                                   const int32_t *col, const float *phase, int32_t ncol,
+                                   const double *table) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (ncol < 1 || !table || !phase) return fail(OEM_EINVAL, "bad TF/SF table");
+    TRY(check_points(ctx, n_pts, comp, cell, coef, col, ncol));
+    if (!ctx->cplx)
+        for (int q = 0; q < n_pts; ++q)
+            if (phase[2 * q + 1] != 0.f)
+                return fail(OEM_EINVAL, "point %d: complex phase needs complex fields", q);
+    const size_t nd = (size_t)(ctx->dims.n_steps + 1) * ncol * 2;
+    return add_points(ctx, magnetic, 1, 1, n_pts, comp, cell, coef, col, phase, ncol, table, nd);
 }
 
 static int add_monitor(oem_ctx *ctx, MonDev m, const double *phase, int32_t *mon_id) {
@@ -427,7 +492,9 @@ static int add_monitor(oem_ctx *ctx, MonDev m, const double *phase, int32_t *mon
     if (m.nf < 1 || m.nf > 65535) return fail(OEM_EINVAL, "nf=%d outside 1..65535", m.nf);
     m.T = ctx->dims.dft_batch;
     const size_t cnt = (size_t)m.ncomp * m.nf * m.npts;
-    TRY(ctx->alloc(&m.snap, (size_t)m.T * m.ncomp * m.npts, true));
+    float *snap = nullptr;
+    TRY(ctx->alloc(&snap, (size_t)m.T * m.ncomp * m.npts * ctx->esz, true));
+    m.snap = snap;
     TRY(ctx->alloc(&m.re, cnt, true));
     TRY(ctx->alloc(&m.im, cnt, true));
     double *ph;
@@ -543,26 +610,30 @@ struct Prof {
 
 inline int blocks_for(int n, int bs) { return (n + bs - 1) / bs; }
 
-int launch_flush(oem_ctx *c, int cnt) {
+template <typename F>
+int launch_flush_t(oem_ctx *c, int cnt) {
     for (auto &m : c->mons) {
         dim3 grid(blocks_for(m.d.npts, 256), m.d.nf);
         if (m.d.ncomp == 4)
-            hipLaunchKernelGGL(k_dft_flush<4>, grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
+            hipLaunchKernelGGL((k_dft_flush<F, 4>), grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
         else
-            hipLaunchKernelGGL(k_dft_flush<6>, grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
+            hipLaunchKernelGGL((k_dft_flush<F, 6>), grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
     }
     HIP_OK(hipGetLastError());
     return OEM_OK;
 }
 
+int launch_flush(oem_ctx *c, int cnt) {
+    return c->cplx ? launch_flush_t<cf2>(c, cnt) : launch_flush_t<float>(c, cnt);
+}
+
 // One time step.  `flush_after`: number of snapshots to flush after the step counter advanced
 // (0 = none).  `last`: the final step of a run call ends the ADE with a plain post, otherwise
 // the post is fused with the next step's pre.  With `pr`, every launch is bracketed by events.
-int launch_step(oem_ctx *c, int flush_after, Prof *pr, bool last) {
-    float *E = c->E, *H = c->H;
+template <typename F>
+int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
+    F *E = reinterpret_cast<F *>(c->E), *H = reinterpret_cast<F *>(c->H);
     const size_t nc = c->ncell;
-    float *Ex = E, *Ey = E + nc, *Ez = E + 2 * nc;
-    float *Hx = H, *Hy = H + nc, *Hz = H + 2 * nc;
     auto mark = [&](int kid, bool begin) -> int {
         if (!pr) return OEM_OK;
         if (begin) {
@@ -577,48 +648,50 @@ int launch_step(oem_ctx *c, int flush_after, Prof *pr, bool last) {
     };
     const dim3 yb(OEM_TK, OEM_TJ), yg(c->g.nblocks);
     TRY(mark(OEM_K_UPDATE_H, true));
-    hipLaunchKernelGGL(k_update_h, yg, yb, 0, c->stream, H, (const float *)E, c->psi, c->g,
+    hipLaunchKernelGGL((k_update_h<F>), yg, yb, 0, c->stream, H, (const F *)E, c->psi, c->g,
                        c->dims.ch, c->pw, c->d_step);
     TRY(mark(OEM_K_UPDATE_H, false));
-    for (auto &dp : c->dipoles) {
-        if (!dp.d.magnetic) continue;
-        TRY(mark(OEM_K_DIPOLE_H, true));
-        hipLaunchKernelGGL(k_dipoles, dim3(blocks_for(dp.d.n_groups, 256)), dim3(256), 0,
-                           c->stream, Hx, Hy, Hz, dp.d, c->mat, c->dims.ch, nc, c->d_step);
-        TRY(mark(OEM_K_DIPOLE_H, false));
+    for (auto &pt : c->points) {
+        if (!pt.magnetic) continue;
+        TRY(mark(pt.d.tfsf ? OEM_K_TFSF_H : OEM_K_DIPOLE_H, true));
+        hipLaunchKernelGGL((k_points<F>), dim3(blocks_for(pt.d.n_groups, 256)), dim3(256), 0,
+                           c->stream, H, pt.d, c->mat, c->dims.ch, nc, c->d_step);
+        TRY(mark(pt.d.tfsf ? OEM_K_TFSF_H : OEM_K_DIPOLE_H, false));
     }
+    const dim3 db(256), dg(c->has_disp ? blocks_for(c->disp.n_entry, 256) : 1);
     if (c->has_disp && !c->pre_ready) {
         TRY(mark(OEM_K_DISP_PRE, true));
-        hipLaunchKernelGGL(k_disp_pre, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
-                           c->stream, c->disp, Ex, Ey, Ez, c->dims.ny, c->dims.nz);
+        hipLaunchKernelGGL((k_disp<F>), dg, db, 0, c->stream, c->disp, (const F *)E, nc,
+                           c->dims.ny, c->dims.nz, 0);
         TRY(mark(OEM_K_DISP_PRE, false));
     }
     TRY(mark(OEM_K_UPDATE_E, true));
-    hipLaunchKernelGGL(k_update_e, yg, yb, 0, c->stream, E, (const float *)H, c->psi, c->g,
+    hipLaunchKernelGGL((k_update_e<F>), yg, yb, 0, c->stream, E, (const F *)H, c->psi, c->g,
                        c->mat, c->dims.cw, c->pw, c->d_step);
     TRY(mark(OEM_K_UPDATE_E, false));
-    for (auto &dp : c->dipoles) {
-        if (dp.d.magnetic) continue;
-        TRY(mark(OEM_K_DIPOLE_E, true));
-        hipLaunchKernelGGL(k_dipoles, dim3(blocks_for(dp.d.n_groups, 256)), dim3(256), 0,
-                           c->stream, Ex, Ey, Ez, dp.d, c->mat, c->dims.ch, nc, c->d_step);
-        TRY(mark(OEM_K_DIPOLE_E, false));
+    // E-side corrections: TF/SF faces first, then electric dipoles (reference order,
+    // `openem/solver.py:960-970`)
+    for (int pass = 0; pass < 2; ++pass) {
+        for (auto &pt : c->points) {
+            if (pt.magnetic || (pass == 0) != (pt.d.tfsf == 1)) continue;
+            TRY(mark(pt.d.tfsf ? OEM_K_TFSF_E : OEM_K_DIPOLE_E, true));
+            hipLaunchKernelGGL((k_points<F>), dim3(blocks_for(pt.d.n_groups, 256)), dim3(256), 0,
+                               c->stream, E, pt.d, c->mat, c->dims.ch, nc, c->d_step);
+            TRY(mark(pt.d.tfsf ? OEM_K_TFSF_E : OEM_K_DIPOLE_E, false));
+        }
     }
     if (c->has_disp) {
         TRY(mark(OEM_K_DISP_POST, true));
-        if (last)
-            hipLaunchKernelGGL(k_disp_post, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
-                               c->stream, c->disp, Ex, Ey, Ez);
-        else
-            hipLaunchKernelGGL(k_disp_step, dim3(blocks_for(c->disp.n_entry, 256)), dim3(256), 0,
-                               c->stream, c->disp, Ex, Ey, Ez, c->dims.ny, c->dims.nz);
+        hipLaunchKernelGGL((k_disp<F>), dg, db, 0, c->stream, c->disp, (const F *)E, nc,
+                           c->dims.ny, c->dims.nz, last ? 1 : 2);
         TRY(mark(OEM_K_DISP_POST, false));
         c->pre_ready = !last;
     }
     for (auto &m : c->mons) {
         TRY(mark(OEM_K_DFT_SNAP, true));
-        hipLaunchKernelGGL(k_dft_snap, dim3(blocks_for(m.d.npts, 256)), dim3(256), 0, c->stream,
-                           m.d, Ex, Ey, Ez, Hx, Hy, Hz, c->dims.ny, c->dims.nz, c->d_step);
+        hipLaunchKernelGGL((k_dft_snap<F>), dim3(blocks_for(m.d.npts, 256)), dim3(256), 0,
+                           c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
+                           c->dims.nz, c->d_step);
         TRY(mark(OEM_K_DFT_SNAP, false));
     }
     TRY(mark(OEM_K_STEP, true));
@@ -631,6 +704,11 @@ int launch_step(oem_ctx *c, int flush_after, Prof *pr, bool last) {
     }
     HIP_OK(hipGetLastError());
     return OEM_OK;
+}
+
+int launch_step(oem_ctx *c, int flush_after, Prof *pr, bool last) {
+    return c->cplx ? launch_step_t<cf2>(c, flush_after, pr, last)
+                   : launch_step_t<float>(c, flush_after, pr, last);
 }
 
 }  // namespace
@@ -782,8 +860,9 @@ extern "C" int oem_read_monitor(oem_ctx *ctx, int32_t mon_id, double *re, double
 
 static int field_ptr(oem_ctx *ctx, int32_t comp, size_t count, float **p) {
     if (comp < 0 || comp > 5) return fail(OEM_EINVAL, "component %d", comp);
-    if (count != ctx->ncell) return fail(OEM_EINVAL, "field holds %zu cells, got %zu", ctx->ncell, count);
-    *p = (comp < 3 ? ctx->E : ctx->H) + (size_t)(comp % 3) * ctx->ncell;
+    if (count != ctx->ncell * ctx->esz)
+        return fail(OEM_EINVAL, "field holds %zu floats, got %zu", ctx->ncell * ctx->esz, count);
+    *p = (comp < 3 ? ctx->E : ctx->H) + (size_t)(comp % 3) * ctx->ncell * ctx->esz;
     return OEM_OK;
 }
 

@@ -39,29 +39,32 @@ class PlaneWaveSource:
 
 @dataclass(frozen=True)
 class TFSFSource:
-    """TF/SF box driven by a 1D auxiliary line (reference ``openem/model.py:63-140``).
+    """TF/SF box driven by a 1D auxiliary line along z (reference ``openem/model.py:63-140``).
 
-    Only the fields the stepper needs are kept: the box, the injection axis, the polarisation
-    basis and the incident tables on the box columns (see :mod:`openem_amd.tfsf`).
+    Attributes:
+        box_lo / box_hi: primal edge indices of the faces; the total-field region is cells
+            ``lo..hi-1`` (edges ``lo..hi``) on every axis that is not open.
+        direction: +1 / -1, propagation along z.
+        waveform: sampled source waveform; the line is driven by a tangential current sheet
+            ``jxy * amp_half[n]``.
+        open_axes: periodic axes the box spans completely (those faces do not exist).
+        kxy: fixed transverse wave vector (rad/m); non-zero only on a Bloch cell, where it
+            must equal ``2 pi bloch_k / L`` on each axis.
+        jxy: direction of the driving sheet current (sets the polarisation).
+        eps_line: optional ``(eps at z edges, eps at z centres)`` background of the line; by
+            default the column at the box corner.
     """
 
-    direction: int
     box_lo: tuple[int, int, int]
     box_hi: tuple[int, int, int]
+    direction: int
     waveform: waveform.Waveform
-    axis: int = 2
-    pol_u: float = 1.0
-    pol_v: float = 0.0
-    k_hat: tuple[float, float, float] | None = None
-    e_hat: tuple[float, float, float] | None = None
     open_axes: tuple[int, ...] = ()
-    #: ``(n_steps+1, ncol_e)`` incident E^n on the 1D line columns and ``(n_steps, ncol_h)``
-    #: incident H^{n+1/2}; plus the mapping of 3D positions onto columns.
-    ex_tab: np.ndarray | None = None
-    hy_tab: np.ndarray | None = None
-    col_step: float = 0.0
-    proj0: float = 0.0
+    kxy: tuple[float, float] = (0.0, 0.0)
+    jxy: tuple[float, float] = (1.0, 0.0)
+    eps_line: tuple | None = None
     amp_scale: float = 1.0
+    axis: int = 2
 
 
 @dataclass(frozen=True)

@@ -309,3 +309,115 @@ def mini_scene(shape=(10, 12, 40), dl: float = 20e-9, n_pml: int = 6, seed: int 
                  pec_ez=None if pec is None else pec[2],
                  dispersion=disp, sources=srcs, dipoles=dips, flux_monitors=mons,
                  field_monitors=fms)
+
+
+def metal_poles(wp: float = 1.37e16, gamma: float = 1e14,
+                lorentz=((1.0, 4e15, 2e14), (0.6, 6e15, 3e14))):
+    """Drude + Lorentz pole pairs (angular units) of the Bloch-cell metal."""
+    from .model import drude_poles
+    poles = drude_poles(wp, gamma)
+    for de, w0, dl in lorentz:
+        poles += lorentz_poles(de, w0, dl)
+    return poles
+
+
+def bloch_cell_scene(nxy: int = 256, nz: int = 600, dl: float = 10e-9, n_pml: int = 12,
+                     seed: int = 0, theta_deg: float = 30.0, phi_deg: float = 45.0,
+                     f0: float = 4e14, fwidth_rel: float = 0.4, num_steps: int = 8000,
+                     nf: int = 21, sub_frac: float = 0.35, height: float = 200e-9,
+                     eps_sub: float = 2.1, with_metal: bool = True, n_cyl: int = 3,
+                     with_tfsf: bool = True) -> Scene:
+    """Bloch / oblique periodic unit cell: Bloch x/y for (theta, phi) incidence at f0, CPML on
+    z, eps = 2.1 substrate, a 3x3 array of metal cylinders (Drude + 2 Lorentz pole-residue
+    metal; the Drude DC pole is a conductivity) of seeded radius 20-60 nm on the substrate, an
+    oblique TF/SF box (open in x/y) driven from above, and flux planes of 21 frequencies."""
+    from .model import split_dc_pole
+    from .tfsf import EPS3  # noqa: F401  (module import check)
+    from .model import TFSFSource
+    rng = np.random.default_rng(seed)
+    ax = uniform_axis(nxy, dl, 0.0, "BlochBoundary")
+    ay = uniform_axis(nxy, dl, 0.0, "BlochBoundary")
+    az = uniform_axis(nz, dl, 0.0, "PML")
+    grid = Grid(ax, ay, az)
+    dt = cpml.courant_dt((dl, dl, dl))
+    L = nxy * dl
+    k0 = 2 * np.pi * f0 / cpml.C_0
+    th, ph = np.deg2rad(theta_deg), np.deg2rad(phi_deg)
+    kx, ky = k0 * np.sin(th) * np.cos(ph), k0 * np.sin(th) * np.sin(ph)
+    bk = (kx * L / (2 * np.pi), ky * L / (2 * np.pi), 0.0)
+    shape = grid.shape
+    k_sub = int(round(sub_frac * nz))                   # substrate top edge index
+    eps = np.ones(shape)
+    eps[:, :, :k_sub] = eps_sub                          # Ex/Ey at edges k < k_sub, Ez centres
+    eps_ez = np.ones(shape)
+    eps_ez[:, :, :k_sub] = eps_sub
+    sigma = None
+    disp = None
+    kc_hi = k_sub + int(round(height / dl))
+    if with_metal:
+        poles = metal_poles()
+        keep, sig_dc = split_dc_pole(poles, dt)
+        mat = np.full(shape, -1, np.int32)
+        xc = ax.centers
+        pitch = L / n_cyl
+        for ci in range(n_cyl):
+            for cj in range(n_cyl):
+                r = rng.uniform(20e-9, 60e-9)
+                cx, cy = (ci + 0.5) * pitch, (cj + 0.5) * pitch
+                inside = ((xc[:, None] - cx) ** 2 + (xc[None, :] - cy) ** 2) <= r * r
+                for k in range(k_sub, kc_hi):
+                    mat[:, :, k][inside] = 0
+        disp = _staircase_dispersion(mat, [keep], dt)
+        sigma = np.where(mat >= 0, sig_dc, 0.0)
+        for e_ in (eps, eps_ez):
+            e_[mat >= 0] = 1.0
+    pulse = waveform.GaussianPulse(freq0=f0, fwidth=fwidth_rel * f0)
+    wf = waveform.sample(pulse, dt, num_steps)
+    tfsf = []
+    lo_k, hi_k = max(n_pml + 4, k_sub - 20), min(nz - n_pml - 8, kc_hi + 30)
+    if with_tfsf:
+        line_e = np.where(np.arange(nz) < k_sub, eps_sub, 1.0)
+        tfsf.append(TFSFSource(box_lo=(0, 0, lo_k), box_hi=(nxy, nxy, hi_k), direction=-1,
+                               waveform=wf, open_axes=(0, 1), kxy=(kx, ky),
+                               jxy=(np.cos(ph), np.sin(ph)), eps_line=(line_e, line_e)))
+    freqs = np.linspace(f0 * (1 - fwidth_rel / 2), f0 * (1 + fwidth_rel / 2), nf)
+    mons = [FluxMonitor("R", 2, min(hi_k + 4, nz - n_pml - 2), +1, freqs),
+            FluxMonitor("T", 2, max(lo_k - 4, n_pml + 2), -1, freqs),
+            FluxMonitor("box_zlo", 2, lo_k + 3, -1, freqs),
+            FluxMonitor("box_zhi", 2, hi_k - 3, +1, freqs)]
+    return Scene(grid=grid, dt=dt, num_time_steps=num_steps, eps_ex=eps, eps_ey=eps.copy(),
+                 eps_ez=eps_ez, sigma_ex=sigma, sigma_ey=None if sigma is None else sigma.copy(),
+                 sigma_ez=None if sigma is None else sigma.copy(),
+                 pml=pml_faces(dt, (dl, dl, dl), [(2, "lo"), (2, "hi")], n_pml),
+                 dispersion=disp, bloch_k=bk, tfsf_sources=tfsf, flux_monitors=mons)
+
+
+def tfsf_box_scene(shape=(26, 24, 44), dl: float = 20e-9, n_pml: int = 6, num_steps: int = 300,
+                   jxy=(1.0, 0.0), direction: int = +1, scatterer: bool = False,
+                   periodic_xy: bool = False) -> Scene:
+    """Small real-field scene with a 6-face (or z-face-only when periodic_xy) TF/SF box at
+    normal incidence along z, for leakage and parity tests."""
+    from .model import TFSFSource
+    nx, ny, nz = shape
+    b = "Periodic" if periodic_xy else "PML"
+    grid = Grid(uniform_axis(nx, dl, 0, b), uniform_axis(ny, dl, 0, b),
+                uniform_axis(nz, dl, 0, "PML"))
+    dt = cpml.courant_dt((dl, dl, dl))
+    eps = np.ones(shape)
+    if scatterer:
+        eps[nx // 2 - 2:nx // 2 + 2, ny // 2 - 2:ny // 2 + 2, nz // 2 - 3:nz // 2 + 3] = 4.0
+    f0 = 3e14
+    wf = waveform.sample(waveform.GaussianPulse(freq0=f0, fwidth=0.4 * f0), dt, num_steps)
+    m = 0 if periodic_xy else n_pml + 3
+    lo = (m, m, n_pml + 6)
+    hi = (nx - m if not periodic_xy else nx, ny - m if not periodic_xy else ny, nz - n_pml - 6)
+    src = TFSFSource(box_lo=lo, box_hi=hi, direction=direction, waveform=wf,
+                     open_axes=(0, 1) if periodic_xy else (), jxy=jxy)
+    sides = [(2, "lo"), (2, "hi")] + ([] if periodic_xy else
+                                      [(0, "lo"), (0, "hi"), (1, "lo"), (1, "hi")])
+    freqs = np.linspace(0.8 * f0, 1.2 * f0, 3)
+    mons = [FluxMonitor("zin", 2, nz // 2, +1, freqs),
+            FluxMonitor("zout", 2, nz - n_pml - 3, +1, freqs)]
+    return Scene(grid=grid, dt=dt, num_time_steps=num_steps, eps_ex=eps, eps_ey=eps.copy(),
+                 eps_ez=eps.copy(), pml=pml_faces(dt, (dl, dl, dl), sides, n_pml),
+                 tfsf_sources=[src], flux_monitors=mons)

@@ -36,9 +36,8 @@ class Stepper:
 
     def __init__(self, tabs: tables_mod.StepTables, device: int = 0, dft_batch: int = 16,
                  dense_materials: bool = False):
-        if tabs.bloch:
-            raise NotImplementedError("Bloch scenes go through openem_amd.bloch")
         self.lib = lib = _lib.load()
+        self.cplx = bool(tabs.bloch)
         self.tabs = tabs
         nx, ny, nz = tabs.shape
         d = _lib.Dims()
@@ -47,7 +46,7 @@ class Stepper:
         for a in range(3):
             d.pml_lo[a] = tabs.axes[a].pml_lo
             d.pml_hi[a] = tabs.axes[a].pml_hi
-        d.complex_fields = 0
+        d.complex_fields = 1 if self.cplx else 0
         d.dft_batch = int(dft_batch)
         d.ch = np.float32(tabs.ch)
         d.cw = np.float32(tabs.cw)
@@ -66,7 +65,9 @@ class Stepper:
         for a, ax in enumerate(t.axes):
             arrs = [_f32(ax.idl_p), _f32(ax.idl_d), _i32(ax.nxt), _i32(ax.prv), _f32(ax.mnx),
                     _f32(ax.mpv), _f32(ax.pec), _f32(ax.a_e), _f32(ax.b_e), _f32(ax.k_e),
-                    _f32(ax.a_h), _f32(ax.b_h), _f32(ax.k_h)]
+                    _f32(ax.a_h), _f32(ax.b_h), _f32(ax.k_h),
+                    np.ascontiguousarray(np.imag(ax.mnx), dtype=np.float32),
+                    np.ascontiguousarray(np.imag(ax.mpv), dtype=np.float32)]
             _lib.check(lib, lib.oem_set_axis(ctx, a, *[_lib.ptr(x) for x in arrs]))
         mc = None if dense else t.material_classes()
         if mc is not None:
@@ -96,11 +97,28 @@ class Stepper:
                                                      np.float32(ps.coef_e),
                                                      _lib.ptr(ex), _lib.ptr(hy)))
         for dp in t.dipoles:
-            arrs = [_i32(dp.comp), _i32(dp.cell), _f32(dp.coef), _i32(dp.src_of), _f64(dp.amp)]
+            cplx_amp = np.iscomplexobj(dp.amp)
+            amp = (np.ascontiguousarray(dp.amp, dtype=np.complex128).view(np.float64)
+                   if cplx_amp else _f64(dp.amp))
+            arrs = [_i32(dp.comp), _i32(dp.cell), _f32(dp.coef), _i32(dp.src_of), amp]
             _lib.check(lib, lib.oem_add_dipoles(ctx, int(dp.magnetic), dp.comp.size,
                                                 _lib.ptr(arrs[0]), _lib.ptr(arrs[1]),
                                                 _lib.ptr(arrs[2]), _lib.ptr(arrs[3]),
-                                                dp.amp.shape[0], _lib.ptr(arrs[4])))
+                                                dp.amp.shape[0], _lib.ptr(arrs[4]),
+                                                int(cplx_amp)))
+        for tf in t.tfsf:
+            ncol = len(tf.cols)
+            for magnetic, (comp, cell, coef, col, ph, tab) in (
+                    (1, (tf.h_comp, tf.h_cell, tf.h_coef, tf.h_col, tf.h_phase, tf.tab_h)),
+                    (0, (tf.e_comp, tf.e_cell, tf.e_coef, tf.e_col, tf.e_phase, tf.tab_e))):
+                if comp.size == 0:
+                    continue
+                phase = np.ascontiguousarray(ph, dtype=np.complex64).view(np.float32)
+                table = np.ascontiguousarray(tab, dtype=np.complex128).view(np.float64)
+                arrs = [_i32(comp), _i32(cell), _f32(coef), _i32(col), phase, table]
+                _lib.check(lib, lib.oem_add_tfsf_points(
+                    ctx, magnetic, comp.size, *[_lib.ptr(x) for x in arrs[:5]], ncol,
+                    _lib.ptr(arrs[5])))
         self.mon_ids = []
         for m in t.monitors:
             ph = _f64(m.phase)
@@ -138,12 +156,16 @@ class Stepper:
 
     # -- readout ----------------------------------------------------------------------------
     def read_field(self, comp: int) -> np.ndarray:
-        out = np.empty(self.tabs.shape, np.float32)
-        _lib.check(self.lib, self.lib.oem_read_field(self.ctx, comp, _lib.ptr(out), out.size))
+        """Field component ``comp`` (0..5 = Ex..Hz): float32, or complex64 on a Bloch scene."""
+        dt = np.complex64 if self.cplx else np.float32
+        out = np.empty(self.tabs.shape, dt)
+        raw = out.view(np.float32)
+        _lib.check(self.lib, self.lib.oem_read_field(self.ctx, comp, _lib.ptr(raw), raw.size))
         return out
 
     def write_field(self, comp: int, arr: np.ndarray) -> None:
-        a = np.ascontiguousarray(arr, dtype=np.float32)
+        dt = np.complex64 if self.cplx else np.float32
+        a = np.ascontiguousarray(arr, dtype=dt).view(np.float32)
         _lib.check(self.lib, self.lib.oem_write_field(self.ctx, comp, _lib.ptr(a), a.size))
 
     def read_monitor(self, i: int) -> np.ndarray:
@@ -176,9 +198,8 @@ def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kerne
     """Run one simulation; see the module docstring. ``use_shutoff``/``policy`` are accepted
     for signature parity: this stepper always runs the pinned step count (no early stop), and
     ``kernels`` is ignored (the library is loaded once per process)."""
-    if sc.any_bloch or force_complex:
-        from . import bloch
-        return bloch.run(sc, num_steps=num_steps, return_fields=return_fields, device=device)
+    if force_complex and not sc.any_bloch:
+        raise NotImplementedError("force_complex needs a Bloch scene (set bloch_k)")
     n_total = int(num_steps if num_steps is not None else sc.num_time_steps)
     t0 = time.time()
     tabs = tables_mod.build(sc, n_total)

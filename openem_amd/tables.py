@@ -31,6 +31,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import cpml
+from . import tfsf as tfsf_mod
 from .grid import transverse_axes
 from .model import Scene
 
@@ -76,7 +77,7 @@ class DipoleTables:
     cell: np.ndarray        # (n_pts,) int32, flat
     coef: np.ndarray        # (n_pts,) float64
     src_of: np.ndarray      # (n_pts,) int32
-    amp: np.ndarray         # (n_src, n_total+1) float64
+    amp: np.ndarray         # (n_src, n_total+1) float64, complex128 on a Bloch scene
 
 
 @dataclass
@@ -122,7 +123,9 @@ class StepTables:
     plane_srcs: list = field(default_factory=list)
     dipoles: list = field(default_factory=list)
     monitors: list = field(default_factory=list)
+    tfsf: list = field(default_factory=list)
     bloch: bool = False
+    coords: list | None = None   # per axis (edge coordinates, centre coordinates)
 
     # material classes (filled lazily by material_classes())
     _cls: np.ndarray | None = None
@@ -287,7 +290,8 @@ def plane_source_tables(sc: Scene) -> list[PlaneSrcTables]:
 
 
 def dipole_tables(sc: Scene, n_total: int) -> list[DipoleTables]:
-    """Electric and magnetic dipole stencils (reference ``openem/sources_setup.py:419-457``)."""
+    """Electric and magnetic dipole stencils (reference ``openem/sources_setup.py:419-457``;
+    the complex waveform on a Bloch scene, ``:521``)."""
     out = []
     ny, nz = sc.shape[1], sc.shape[2]
     for magnetic in (False, True):
@@ -302,11 +306,14 @@ def dipole_tables(sc: Scene, n_total: int) -> list[DipoleTables]:
             cell.append(fl.astype(np.int32))
             coef.append(np.asarray(d.coef, dtype=np.float64))
             src_of.append(np.full(fl.size, s, np.int32))
-            w = d.waveform.amp_int if magnetic else d.waveform.amp_half
+            if sc.any_bloch:
+                w = d.waveform.amp_int_complex if magnetic else d.waveform.amp_half_complex
+            else:
+                w = d.waveform.amp_int if magnetic else d.waveform.amp_half
             if w.size < n_total + 1:
                 raise ValueError(f"dipole amplitude table holds {w.size} samples, "
                                  f"{n_total} steps need {n_total + 1}")
-            amp.append(np.asarray(w[:n_total + 1], dtype=np.float64))
+            amp.append(np.asarray(w[:n_total + 1]))
         out.append(DipoleTables(magnetic, np.concatenate(comp), np.concatenate(cell),
                                 np.concatenate(coef), np.concatenate(src_of), np.stack(amp)))
     return out
@@ -369,8 +376,14 @@ def monitor_tables(sc: Scene, n_total: int) -> list[MonitorTables]:
     return out
 
 
-def build(sc: Scene, n_total: int | None = None) -> StepTables:
+def build(sc: Scene, n_total: int | None = None, force_complex: bool = False) -> StepTables:
+    """All tables of a scene for ``n_total`` steps.  ``force_complex`` takes the complex
+    (Bloch) field path at k = 0 (reference ``solver.run(force_complex=True)``,
+    ``openem/solver.py:403-420``)."""
     n_total = int(sc.num_time_steps if n_total is None else n_total)
+    for k_ax, a in enumerate(sc.grid.axes):
+        if sc.bloch_k[k_ax] != 0.0 and not a.is_periodic:
+            raise ValueError(f"bloch_k on axis {k_ax} needs a periodic axis")
     for s in sc.sources:
         if s.ex_inc.size < n_total + 1 or s.hy_inc.size < n_total + 1:
             raise ValueError(f"plane-wave tables hold {s.ex_inc.size} samples, "
@@ -386,4 +399,6 @@ def build(sc: Scene, n_total: int | None = None) -> StepTables:
         cw=cpml.EPSILON_0 / sc.dt, axes=axis_tables(sc), ca=ca, cb=cb, disp_flag=flag,
         disp=disp_tables(sc), plane_srcs=plane_source_tables(sc),
         dipoles=dipole_tables(sc, n_total), monitors=monitor_tables(sc, n_total),
-        bloch=sc.any_bloch)
+        tfsf=[tfsf_mod.tables(sc, s, n_total) for s in sc.tfsf_sources],
+        bloch=sc.any_bloch or force_complex,
+        coords=[(a.edges[:-1], a.centers) for a in sc.grid.axes])

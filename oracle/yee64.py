@@ -35,6 +35,15 @@ from __future__ import annotations
 
 import numpy as np
 
+_EPS3 = {(0, 1, 2): 1, (1, 2, 0): 1, (2, 0, 1): 1, (0, 2, 1): -1, (2, 1, 0): -1, (1, 0, 2): -1}
+
+
+def _rng(lo, hi, shape, opn, axis, on_edge):
+    if axis in opn:
+        return np.arange(shape[axis])
+    return np.arange(lo[axis], hi[axis] + 1) if on_edge else np.arange(lo[axis], hi[axis])
+
+
 _PSI = {  # (field comp, derivative axis) pairs, H and E sides
     "hx_y": (0, 1), "hx_z": (0, 2), "hy_z": (1, 2), "hy_x": (1, 0), "hz_x": (2, 0), "hz_y": (2, 1),
 }
@@ -47,30 +56,34 @@ def _bc(v: np.ndarray, axis: int) -> np.ndarray:
 
 
 class Oracle64:
-    """Real-field (non-Bloch) stepper in float64."""
+    """Stepper in float64 (real fields) or complex128 (Bloch scenes: the wrap masks carry
+    ``exp(+-i 2 pi k)``, reference ``openem/kernels/bloch.cu:1-23``, ``openem/grid.py:219-260``)."""
 
     def __init__(self, tabs):
-        if tabs.bloch:
-            raise NotImplementedError("use oracle.bloch64 for Bloch scenes")
         self.t = tabs
         shp = tabs.shape
-        self.E = np.zeros((3,) + shp)
-        self.H = np.zeros((3,) + shp)
-        self.psi_h = {k: np.zeros(shp) for k in _PSI}
-        self.psi_e = {k: np.zeros(shp) for k in _PSI}
+        dt = np.complex128 if tabs.bloch else np.float64
+        self.E = np.zeros((3,) + shp, dtype=dt)
+        self.H = np.zeros((3,) + shp, dtype=dt)
+        self.psi_h = {k: np.zeros(shp, dtype=dt) for k in _PSI}
+        self.psi_e = {k: np.zeros(shp, dtype=dt) for k in _PSI}
         ax = tabs.axes
-        self._m = [dict(nxt=a.nxt, prv=a.prv, mnx=_bc(a.mnx.real, i), mpv=_bc(a.mpv.real, i),
+        mk = (lambda v: v) if tabs.bloch else (lambda v: np.real(v))
+        self._m = [dict(nxt=a.nxt, prv=a.prv, mnx=_bc(mk(a.mnx), i), mpv=_bc(mk(a.mpv), i),
                         idp=_bc(a.idl_p, i), idd=_bc(a.idl_d, i), pec=_bc(a.pec, i),
                         aE=_bc(a.a_e, i), bE=_bc(a.b_e, i), kE=_bc(a.k_e, i),
                         aH=_bc(a.a_h, i), bH=_bc(a.b_h, i), kH=_bc(a.k_h, i))
                    for i, a in enumerate(ax)]
         d = tabs.disp
-        self.hist = np.zeros((3,) + shp)
+        self.hist = np.zeros((3,) + shp, dtype=dt)
         if d is not None:
+            # Re E and Im E each drive a pole state (every update coefficient is real)
             self.P = np.zeros(d.am1.size, dtype=np.complex128)
+            self.Q = np.zeros(d.am1.size, dtype=np.complex128)
             self._pole_entry = np.repeat(np.arange(d.comp.size), np.diff(d.ofs))
             self._idx4 = (d.comp.astype(np.int64),) + np.unravel_index(
                 d.cell.astype(np.int64), shp)
+        self._edges_ctrs = getattr(tabs, "coords", None) or [(None, None)] * 3
         self.acc = []
         for m in tabs.monitors:
             if m.kind == "plane":
@@ -123,12 +136,19 @@ class Oracle64:
             hc = 1 if ps.pol == 0 else 0
             H[hc][:, :, ps.kh] += ps.coef_h * ps.ex_inc[n]
         # ---- E phase ----
+        for tf in t.tfsf:
+            self._tfsf(tf, n, magnetic=True)
         d = t.disp
         if d is not None:
             en = E[self._idx4]
-            dP = d.am1 * self.P + d.b * en[self._pole_entry]
+            dP = d.am1 * self.P + d.b * en.real[self._pole_entry]
             self.P += dP
             w = np.bincount(self._pole_entry, weights=2.0 * dP.real, minlength=d.comp.size)
+            if t.bloch:
+                dQ = d.am1 * self.Q + d.b * en.imag[self._pole_entry]
+                self.Q += dQ
+                w = w + 1j * np.bincount(self._pole_entry, weights=2.0 * dQ.real,
+                                         minlength=d.comp.size)
             self.hist[self._idx4] = w
         pe = self.psi_e
         m = self._m
@@ -146,6 +166,8 @@ class Oracle64:
         E[1] = ey_new
         for ps in t.plane_srcs:
             E[ps.pol][:, :, ps.ks] += cb[ps.pol][:, :, ps.ks] * ps.coef_e * ps.hy_inc[n]
+        for tf in t.tfsf:
+            self._tfsf(tf, n, magnetic=False)
         for dp in t.dipoles:
             if not dp.magnetic:
                 for c in range(3):
@@ -155,9 +177,71 @@ class Oracle64:
                     np.subtract.at(E[c].reshape(-1), cells, v)
         if d is not None:
             en = E[self._idx4]
-            self.P += d.b * en[self._pole_entry]
+            self.P += d.b * en.real[self._pole_entry]
+            if t.bloch:
+                self.Q += d.b * en.imag[self._pole_entry]
         self._dft(n)
         self.n += 1
+
+    def _tfsf(self, tf, n, magnetic):
+        """TF/SF face corrections of ``openem/kernels/tfsf.cu:49-83`` restated per face with
+        array slices (lo face: H_c[p-1] += ch e_cab E_b/dl, E_c[p] -= cb e_cab H_b/dl_dual; hi
+        face: the opposite signs on H_c[p], E_c[p])."""
+        t, src = self.t, tf.src
+        grid_axes = [self._coords(a) for a in range(3)]
+        colmap = {key: q for q, key in enumerate(tf.cols)}
+        row = (tf.tab_h if magnetic else tf.tab_e)[n]
+        kx, ky = src.kxy
+        lo, hi, opn = src.box_lo, src.box_hi, tuple(src.open_axes)
+        for a in range(3):
+            if a in opn:
+                continue
+            for side in ("lo", "hi"):
+                p = lo[a] if side == "lo" else hi[a]
+                for c in range(3):
+                    if c == a:
+                        continue
+                    b = 3 - a - c
+                    sgn = _EPS3[(c, a, b)]
+                    if magnetic:
+                        plane = p - 1 if side == "lo" else p
+                        coef = (sgn if side == "lo" else -sgn) * t.axes[a].idl_p[plane] * t.ch
+                        c_edge, b_edge, comp_inc = True, False, b
+                        zsrc = p
+                    else:
+                        plane = p
+                        coef = (-sgn if side == "lo" else sgn) * t.axes[a].idl_d[p]
+                        c_edge, b_edge, comp_inc = False, True, 3 + b
+                        zsrc = p - 1 if side == "lo" else p
+                    rc = _rng(lo, hi, t.shape, opn, c, c_edge)
+                    rb = _rng(lo, hi, t.shape, opn, b, b_edge)
+                    sl = [None, None, None]
+                    sl[a] = plane
+                    sl[c] = rc
+                    sl[b] = rb
+                    ii, jj, kk = np.meshgrid(*[np.atleast_1d(np.asarray(v)) for v in sl],
+                                             indexing="ij")
+                    if a == 2:
+                        zidx = np.full(ii.shape, zsrc)
+                    else:
+                        zidx = kk
+                    cols = np.vectorize(lambda z: colmap[(comp_inc, int(z))])(zidx)
+                    inc = row[cols]
+                    if kx or ky:
+                        x_edge = c_edge if c == 0 else b_edge
+                        y_edge = c_edge if c == 1 else b_edge
+                        xs = grid_axes[0][0 if x_edge else 1][ii]
+                        ys = grid_axes[1][0 if y_edge else 1][jj]
+                        inc = inc * np.exp(1j * (kx * xs + ky * ys))
+                    F = self.H if magnetic else self.E
+                    if magnetic:
+                        F[c][ii, jj, kk] += coef * (inc if t.bloch else inc.real)
+                    else:
+                        scale = t.cb[c][ii, jj, kk] * coef
+                        F[c][ii, jj, kk] += scale * (inc if t.bloch else inc.real)
+
+    def _coords(self, a):
+        return self._edges_ctrs[a]
 
     def _dft(self, n):
         E, H = self.E, self.H
