@@ -20,13 +20,14 @@ KERNELS = ("update_h", "update_e", "dipole_e", "dipole_h", "disp_pre", "disp_pos
            "dft_snap", "dft_flush", "step_advance", "tfsf_h", "tfsf_e")
 K_COUNT = len(KERNELS)
 
-#: every symbol include/openem_hip.h declares (checked by the CPU test suite)
+#: every symbol include/openem_hip.h and include/openem_host.h declare (checked by the CPU
+#: test suite)
 EXPORTS = ("oem_version", "oem_last_error", "oem_device_count", "oem_ctx_create",
            "oem_ctx_destroy", "oem_set_axis", "oem_set_materials_lut",
            "oem_set_materials_dense", "oem_set_dispersion", "oem_add_plane_source",
            "oem_add_dipoles", "oem_add_tfsf_points", "oem_add_flux_plane", "oem_add_field_box", "oem_prepare",
            "oem_run", "oem_profile_run", "oem_timed_run", "oem_step_index", "oem_read_monitor",
-           "oem_read_field", "oem_write_field")
+           "oem_read_field", "oem_write_field", "oem_host_factorize")
 
 
 class OpenEMLibraryError(RuntimeError):
@@ -74,9 +75,44 @@ _SIGS = {
     "oem_read_monitor": ([_P, C.c_int32, _P, _P, C.c_size_t], C.c_int),
     "oem_read_field": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "oem_write_field": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "oem_host_factorize": ([C.c_int64, C.c_int32, _P, _P, _P, _P, C.c_int64, _P], C.c_int),
 }
 
 _lib = None
+
+
+def factorize(cols, max_uniq: int):
+    """Distinct rows over the columns ``cols`` (arrays of one length with 4- or 8-byte
+    elements; an 8-byte column counts as two 32-bit words), numbered in order of first
+    appearance: ``(inv int32 (n,), uniq uint32 (n_uniq, n_words))``, or ``None`` when there are
+    more than ``max_uniq`` distinct rows."""
+    lib = load()
+    keep, ptrs, strides = [], [], []
+    n = None
+    for c in cols:
+        a = np.ascontiguousarray(np.asarray(c).reshape(-1))
+        if a.dtype.itemsize not in (4, 8):
+            raise ValueError(f"factorize: {a.dtype} columns are not supported")
+        if n is None:
+            n = a.size
+        elif a.size != n:
+            raise ValueError("factorize: columns differ in length")
+        keep.append(a)
+        w = a.itemsize // 4
+        for h in range(w):
+            ptrs.append(a.ctypes.data + 4 * h)
+            strides.append(w)
+    pa = (C.c_void_p * len(ptrs))(*ptrs)
+    sa = np.asarray(strides, np.int32)
+    inv = np.empty(n, np.int32)
+    cap = int(min(max_uniq, max(n, 1)))
+    uniq = np.empty((cap, len(ptrs)), np.uint32)
+    nu = C.c_int64(0)
+    check(lib, lib.oem_host_factorize(n, len(ptrs), pa, ptr(sa), ptr(inv), ptr(uniq), cap,
+                                      C.byref(nu)))
+    if nu.value < 0:
+        return None
+    return inv, uniq[:nu.value].copy()
 
 
 def load(path: str | os.PathLike | None = None):
@@ -84,6 +120,8 @@ def load(path: str | os.PathLike | None = None):
     global _lib
     if _lib is not None and path is None:
         return _lib
+    if path is None and os.environ.get("OPENEM_HIP_LIB"):
+        path = os.environ["OPENEM_HIP_LIB"]      # an alternative build of the same ABI
     p = Path(path) if path is not None else LIB_PATH
     if not p.exists():
         raise OpenEMLibraryError(
@@ -99,8 +137,7 @@ def load(path: str | os.PathLike | None = None):
         fn.restype = res
     if lib.oem_version() != 1:
         raise OpenEMLibraryError(f"ABI version {lib.oem_version()} != 1")
-    if path is None:
-        _lib = lib
+    _lib = lib if (path is None or os.environ.get("OPENEM_HIP_LIB") == str(path)) else _lib
     return lib
 
 

@@ -24,7 +24,9 @@
 #define OEM_NXCD 8
 #define OEM_MAX_PW 2
 #define OEM_TK 64
+#ifndef OEM_TJ
 #define OEM_TJ 4
+#endif
 #define OEM_LDS_CLASSES 512
 
 // ---------------------------------------------------------------- complex field element ----

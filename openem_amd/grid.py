@@ -164,6 +164,14 @@ class Grid:
         xd, yd, zd = (a.dl_dual for a in self.axes)
         return outer3(xp, yd, zd), outer3(xd, yp, zd), outer3(xd, yd, zp)
 
+    def dual_volume_at(self, comp: int, magnetic: bool, i: int, j: int, k: int) -> float:
+        """One entry of :meth:`yee_dual_volumes` (``magnetic``: :meth:`yee_h_dual_volumes`),
+        with the same multiplication order."""
+        primal = (comp != 0, comp != 1, comp != 2) if magnetic else \
+            (comp == 0, comp == 1, comp == 2)
+        f = [a.dl if p else a.dl_dual for a, p in zip(self.axes, primal)]
+        return float((f[0][i] * f[1][j]) * f[2][k])
+
     def yee_h_dual_volumes(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """Dual volume of Hx/Hy/Hz (reference ``openem/grid.py:370-386``)."""
         xp, yp, zp = (a.dl for a in self.axes)

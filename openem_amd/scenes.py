@@ -140,13 +140,12 @@ def dipole_stencil(grid: Grid, comp: int, pos, magnetic: bool = False):
         j = i + 1 if c[i] < x else i - 1
         w = (x - float(c[i])) / (float(c[j]) - float(c[i]))
         per_axis.append([(i, 1.0 - w), (j, w)])
-    vols = (grid.yee_h_dual_volumes() if magnetic else grid.yee_dual_volumes())[comp]
     idx, coef = [], []
     for i, wi in per_axis[0]:
         for j, wj in per_axis[1]:
             for k, wk in per_axis[2]:
                 idx.append((i, j, k))
-                coef.append(wi * wj * wk / vols[i, j, k])
+                coef.append(wi * wj * wk / grid.dual_volume_at(comp, magnetic, i, j, k))
     return np.array(idx, dtype=np.int64), np.array(coef, dtype=np.float64)
 
 

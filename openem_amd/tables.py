@@ -30,7 +30,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import cpml
+from . import _lib, cpml
 from . import tfsf as tfsf_mod
 from .grid import transverse_axes
 from .model import Scene
@@ -116,9 +116,7 @@ class StepTables:
     ch: float
     cw: float
     axes: list
-    ca: np.ndarray          # (3, nx, ny, nz) float64
-    cb: np.ndarray
-    disp_flag: np.ndarray   # (3, nx, ny, nz) bool
+    scene: Scene            # the source of the material coefficients
     disp: DispTables | None = None
     plane_srcs: list = field(default_factory=list)
     dipoles: list = field(default_factory=list)
@@ -127,18 +125,49 @@ class StepTables:
     bloch: bool = False
     coords: list | None = None   # per axis (edge coordinates, centre coordinates)
 
-    # material classes (filled lazily by material_classes())
+    # filled lazily: float64 (ca, cb) and the material classes
+    _cacb: tuple | None = None
+    _flag: np.ndarray | None = None
     _cls: np.ndarray | None = None
     _lut: np.ndarray | None = None
+
+    @property
+    def ca(self) -> np.ndarray:
+        """(3, nx, ny, nz) float64 ca (reference ``openem/coeffs.py:98-136``)."""
+        if self._cacb is None:
+            self._cacb = e_coeffs(self.scene)
+        return self._cacb[0]
+
+    @property
+    def cb(self) -> np.ndarray:
+        if self._cacb is None:
+            self._cacb = e_coeffs(self.scene)
+        return self._cacb[1]
+
+    @property
+    def disp_flag(self) -> np.ndarray:
+        """(3, nx, ny, nz) bool: the cell of this component carries ADE pole entries."""
+        if self._flag is None:
+            self._flag = _disp_flags(self.scene)
+        return self._flag
 
     def material_classes(self):
         """``(cls (nx,ny,nz) uint16, lut (n_classes, 8) float32)``, or ``None`` when more than
         65535 distinct classes exist (the caller then uploads dense ca/cb)."""
         if self._cls is None:
-            self._cls, self._lut = _material_classes(self.ca, self.cb, self.disp_flag)
-        if self._cls.size == 0 and self.ca[0].size:
+            self._cls, self._lut = _material_classes(self.scene)
+        if self._cls.size == 0:
             return None
         return self._cls, self._lut
+
+
+def _disp_flags(sc: Scene) -> np.ndarray:
+    flag = np.zeros((3,) + sc.shape, dtype=bool)
+    if sc.any_dispersion:
+        d = sc.dispersion
+        for c in range(3):
+            flag[c].reshape(-1)[d.cell[d.comp == c]] = True
+    return flag
 
 
 def _pml_axis_arrays(sc: Scene, axis: int, at: str):
@@ -180,17 +209,23 @@ def axis_tables(sc: Scene) -> list[AxisTables]:
 def e_coeffs(sc: Scene) -> tuple[np.ndarray, np.ndarray]:
     """``(ca, cb)`` each ``(3, nx, ny, nz)`` float64 (reference ``openem/coeffs.py:98-136``)."""
     shape = sc.shape
-    gg = sc.dispersion.g_dense(shape) if sc.any_dispersion else (np.zeros(shape),) * 3
+    gg = sc.dispersion.g_dense(shape) if sc.any_dispersion else None
     eps = (sc.eps_ex, sc.eps_ey, sc.eps_ez)
     sig = (sc.sigma_ex, sc.sigma_ey, sc.sigma_ez)
     ca = np.empty((3,) + shape)
     cb = np.empty((3,) + shape)
+    k0 = sc.dt / cpml.EPSILON_0
     for c in range(3):
-        e = np.asarray(eps[c], dtype=np.float64)
-        s = np.zeros(shape) if sig[c] is None else np.asarray(sig[c]) * sc.dt / (2.0 * cpml.EPSILON_0)
-        den = e + s + gg[c]
-        ca[c] = (e - s) / den
-        cb[c] = (sc.dt / cpml.EPSILON_0) / den
+        e = np.asarray(eps[c], dtype=np.float64).reshape(shape)
+        if sig[c] is None:
+            num, den = e, (e + gg[c] if gg is not None else e.copy())
+        else:
+            s = np.asarray(sig[c]) * (sc.dt / (2.0 * cpml.EPSILON_0))
+            num, den = e - s, e + s
+            if gg is not None:
+                den += gg[c]
+        np.divide(num, den, out=ca[c])
+        np.divide(k0, den, out=cb[c])
     for c, pidx in enumerate((sc.pec_ex, sc.pec_ey, sc.pec_ez)):
         if pidx is None or np.asarray(pidx).size == 0:
             continue
@@ -202,34 +237,82 @@ def e_coeffs(sc: Scene) -> tuple[np.ndarray, np.ndarray]:
     return ca, cb
 
 
-def _material_classes(ca, cb, flag):
-    """Deduplicate the per-cell sextuple of float32 (ca, cb) bit patterns + dispersive flags."""
-    ncell = ca[0].size
+def _cell_flags(sc: Scene) -> np.ndarray:
+    """Per cell uint32: bit c = component c carries ADE entries, bit 3+c = component c is PEC."""
+    fl = np.zeros(sc.grid.num_cells, np.uint32)
+    if sc.any_dispersion:
+        d = sc.dispersion
+        for c in range(3):
+            fl[d.cell[d.comp == c]] |= np.uint32(1 << c)
+    for c, pidx in enumerate((sc.pec_ex, sc.pec_ey, sc.pec_ez)):
+        if pidx is not None and np.asarray(pidx).size:
+            pidx = np.asarray(pidx, dtype=np.int64)
+            if pidx.min() < 0 or pidx.max() >= fl.size:
+                raise ValueError(f"pec_e{'xyz'[c]} flat index out of range")
+            fl[pidx] |= np.uint32(8 << c)
+    return fl
+
+
+def _material_classes(sc: Scene):
+    """Per-cell material class (uint16) and its LUT rows (ca_x, cb_x, ca_y, cb_y, ca_z, cb_z,
+    ADE flags) as float32 bits, computed from the distinct per-cell inputs (eps, sigma, G,
+    flags): the float64 ca/cb of each distinct input row follow :func:`e_coeffs`' formula term
+    by term, so every class row equals the float32 cast of ``e_coeffs`` at its cells.  Classes
+    are numbered in order of first appearance (native hash factorization)."""
     empty = (np.zeros(0, np.uint16), np.zeros((0, LUT_STRIDE), np.float32))
-    # per component: dedup the (ca, cb) float32 bit pair -> small index
-    pairs, idx = [], []
+    shape = sc.shape
+    gg = sc.dispersion.g_dense(shape) if sc.any_dispersion else None
+    eps = (sc.eps_ex, sc.eps_ey, sc.eps_ez)
+    sig = (sc.sigma_ex, sc.sigma_ey, sc.sigma_ez)
+    cols, layout = [], []
     for c in range(3):
-        a32 = ca[c].astype(np.float32).reshape(-1).view(np.uint32).astype(np.uint64)
-        b32 = cb[c].astype(np.float32).reshape(-1).view(np.uint32).astype(np.uint64)
-        u, inv = np.unique((a32 << np.uint64(32)) | b32, return_inverse=True)
-        if u.size > MAX_CLASSES:
-            return empty
-        pairs.append(u)
-        idx.append(inv.reshape(-1).astype(np.int64))
-    fl = (flag[0].reshape(-1).astype(np.int64) | (flag[1].reshape(-1).astype(np.int64) << 1)
-          | (flag[2].reshape(-1).astype(np.int64) << 2))
-    key = ((idx[0] * 65536 + idx[1]) * 65536 + idx[2]) * 8 + fl
-    ukey, inv = np.unique(key, return_inverse=True)
-    if ukey.size > MAX_CLASSES:
+        cols.append(np.asarray(eps[c], dtype=np.float64))
+        has_s, has_g = sig[c] is not None, gg is not None
+        if has_s:
+            cols.append(np.asarray(sig[c], dtype=np.float64))
+        if has_g:
+            cols.append(gg[c])
+        layout.append((has_s, has_g))
+    cols.append(_cell_flags(sc))
+    r = _lib.factorize(cols, 1 << 20)
+    if r is None:
         return empty
-    lut = np.zeros((ukey.size, LUT_STRIDE), np.uint32)
-    parts = (ukey // (8 * 65536 * 65536), (ukey // (8 * 65536)) % 65536, (ukey // 8) % 65536)
+    inv, rows = r
+    # decode the distinct input rows (float64 = two words) and evaluate e_coeffs on them
+    w64 = rows[:, :-1].copy().view(np.float64)
+    fl = rows[:, -1]
+    k0 = sc.dt / cpml.EPSILON_0
+    lut = np.zeros((rows.shape[0], LUT_STRIDE), np.float32)
+    q = 0
     for c in range(3):
-        pr = pairs[c][parts[c]]
-        lut[:, 2 * c] = (pr >> np.uint64(32)).astype(np.uint32)
-        lut[:, 2 * c + 1] = (pr & np.uint64(0xFFFFFFFF)).astype(np.uint32)
-    lut[:, 6] = (ukey % 8).astype(np.uint32)
-    return inv.reshape(ca[0].shape).astype(np.uint16), lut.view(np.float32)
+        has_s, has_g = layout[c]
+        e = w64[:, q]; q += 1
+        s = g = None
+        if has_s:
+            s = w64[:, q] * (sc.dt / (2.0 * cpml.EPSILON_0)); q += 1
+        if has_g:
+            g = w64[:, q]; q += 1
+        if s is None:
+            num, den = e, (e + g if g is not None else e.copy())
+        else:
+            num, den = e - s, e + s
+            if g is not None:
+                den += g
+        ca, cb = num / den, k0 / den
+        pec = (fl & np.uint32(8 << c)) != 0
+        ca[pec] = 0.0
+        cb[pec] = 0.0
+        lut[:, 2 * c] = ca
+        lut[:, 2 * c + 1] = cb
+    lut.view(np.uint32)[:, 6] = fl & np.uint32(7)
+    # distinct input rows can share one coefficient row: deduplicate the LUT itself
+    r2 = _lib.factorize([lut[:, i].copy() for i in range(7)], MAX_CLASSES)
+    if r2 is None:
+        return empty
+    inv2, rows2 = r2
+    out = np.zeros((rows2.shape[0], LUT_STRIDE), np.uint32)
+    out[:, :7] = rows2
+    return inv2[inv].reshape(shape).astype(np.uint16), out.view(np.float32)
 
 
 def disp_tables(sc: Scene) -> DispTables | None:
@@ -239,9 +322,9 @@ def disp_tables(sc: Scene) -> DispTables | None:
     d.validate(sc.shape)
     am1_32 = d.am1.astype(np.complex64)
     b_32 = d.b.astype(np.complex64)
-    key = np.stack([am1_32.real.view(np.uint32), am1_32.imag.view(np.uint32),
-                    b_32.real.view(np.uint32), b_32.imag.view(np.uint32)], axis=1)
-    uniq, inv = np.unique(key, axis=0, return_inverse=True)
+    cols = [np.ascontiguousarray(x, dtype=np.float32).view(np.uint32)
+            for x in (am1_32.real, am1_32.imag, b_32.real, b_32.imag)]
+    inv, uniq = _lib.factorize(cols, max(am1_32.size, 1))
     u = uniq.view(np.float32)
     nx, ny, nz = sc.shape
     ii, jj, kk = np.unravel_index(d.cell, sc.shape)
@@ -388,16 +471,9 @@ def build(sc: Scene, n_total: int | None = None, force_complex: bool = False) ->
         if s.ex_inc.size < n_total + 1 or s.hy_inc.size < n_total + 1:
             raise ValueError(f"plane-wave tables hold {s.ex_inc.size} samples, "
                              f"{n_total} steps need {n_total + 1}")
-    ca, cb = e_coeffs(sc)
-    flag = np.zeros((3,) + sc.shape, dtype=bool)
-    if sc.any_dispersion:
-        d = sc.dispersion
-        for c in range(3):
-            flag[c].reshape(-1)[d.cell[d.comp == c]] = True
     return StepTables(
         shape=sc.shape, n_total=n_total, dt=sc.dt, ch=sc.dt / cpml.MU_0,
-        cw=cpml.EPSILON_0 / sc.dt, axes=axis_tables(sc), ca=ca, cb=cb, disp_flag=flag,
-        disp=disp_tables(sc), plane_srcs=plane_source_tables(sc),
+        cw=cpml.EPSILON_0 / sc.dt, axes=axis_tables(sc), scene=sc, disp=disp_tables(sc), plane_srcs=plane_source_tables(sc),
         dipoles=dipole_tables(sc, n_total), monitors=monitor_tables(sc, n_total),
         tfsf=[tfsf_mod.tables(sc, s, n_total) for s in sc.tfsf_sources],
         bloch=sc.any_bloch or force_complex,

@@ -15,7 +15,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 
 def _header_functions():
-    text = (ROOT / "include" / "openem_hip.h").read_text()
+    text = "".join(h.read_text() for h in sorted((ROOT / "include").glob("*.h")))
     return sorted(set(re.findall(r"^int (oem_\w+)\(", text, flags=re.M)))
 
 
