@@ -8,7 +8,11 @@ Yee time step of that scene (H sweep, ADE, E sweep, dipole, DFT), inputs residen
 Prints one JSON line.  ``value`` = Mcell-steps/s over the timed steps (HIP events on the
 stepper's stream).  ``roofline`` is for the dominant kernel: algorithmic bytes per launch /
 its average device time from event pairs around each launch.  ``cpu_baseline`` is the float64
-NumPy oracle on a bounded sample of the same workload.
+NumPy oracle on a bounded sample of the same workload.  ``serving`` is the batched-serving
+half of the metric: the first ``--serve-requests`` requests of the seeded serving stream (Poisson
+arrivals at 20 / s, cubic sides log-uniform in [64, 320], 2000-8000 steps, 1-3 flux monitors of
+5-21 frequencies, slab / box-soup / Lorentz-sphere scenes) served by continuous batching and by
+the serial plain loop on the same GPU: sims/s and p50/p99 time-to-result of each.
 """
 
 from __future__ import annotations
@@ -80,6 +84,34 @@ def cpu_baseline(n: int, steps: int = 2, nx_sample: int = 64) -> dict:
                       f"monitor), single process"}
 
 
+def serving_bench(n_req: int, arena_gib: int, workers: int) -> dict:
+    from openem_amd import scenes, serving
+    specs = scenes.serving_requests(256, rate=20.0, seed=0)[:n_req]
+    cs = sum(sp["side"] ** 3 * sp["n_steps"] for sp in specs)
+    print(f"[bench] serving {n_req} requests ({cs / 1e12:.2f} Tcell-steps): batched",
+          file=sys.stderr, flush=True)
+    b = serving.serve(specs, arena_bytes=arena_gib << 30, workers=workers)
+    sb = serving.summary(b)
+    print(f"[bench] batched {sb}, peak running {b['peak_running']}; serial", file=sys.stderr,
+          flush=True)
+    s = serving.serve_serial(specs)
+    ss = serving.summary(s)
+    print(f"[bench] serial {ss}", file=sys.stderr, flush=True)
+    worst = 0.0
+    for rb, rs in zip(b["requests"], s["requests"]):
+        for x, y in zip(rb.flux, rs.flux):
+            worst = max(worst, float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300)))
+    return {"requests": n_req, "stream": "seed 0, Poisson 20/s, side log-U[64,320], "
+                                          "2000-8000 steps, 1-3 flux monitors x 5-21 freqs",
+            "cell_steps": cs, "sims_per_s": round(sb["sims_per_s"], 4),
+            "p50_s": round(sb["p50_s"], 3), "p99_s": round(sb["p99_s"], 3),
+            "serial_sims_per_s": round(ss["sims_per_s"], 4),
+            "serial_p50_s": round(ss["p50_s"], 3), "serial_p99_s": round(ss["p99_s"], 3),
+            "speedup": round(sb["sims_per_s"] / ss["sims_per_s"], 3),
+            "peak_batch": b["peak_running"], "failed": sb["failed"] + ss["failed"],
+            "max_rel_flux_diff_vs_serial": worst, "arena_gib": arena_gib}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +121,11 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dft-batch", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=32,
+                    help="requests of the serving stream to run batched and serially (0: skip)")
+    ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--arena-gib", type=int, default=96)
+    ap.add_argument("--serve-workers", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -97,6 +134,11 @@ def main() -> int:
 
     from openem_amd import scenes, tables
     from openem_amd.solver import Stepper
+
+    if args.serve_only:
+        print(json.dumps({"metric": METRIC, "serving": serving_bench(
+            args.serve_requests, args.arena_gib, args.serve_workers)}), flush=True)
+        return 0
 
     n_total = args.warmup + args.steps + args.profile_steps + 8
     t0 = time.time()
@@ -126,6 +168,8 @@ def main() -> int:
     print(f"[bench] timed {args.steps} steps: {ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall)",
           file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.n)
+    srv = (serving_bench(args.serve_requests, args.arena_gib, args.serve_workers)
+           if args.serve_requests > 0 else None)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -141,6 +185,7 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": ab[k_dom], "avg_launch_ms": round(avg_ms, 5)},
         "step_hbm_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "cpu_baseline": cpu,
+        "serving": srv,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -43,6 +43,7 @@ enum oem_kernel_id {
 };
 
 typedef struct oem_ctx oem_ctx;
+typedef struct oem_pool oem_pool;
 
 typedef struct oem_dims {
     int32_t nx, ny, nz;        /* cells (PML layers included)                                 */
@@ -145,6 +146,11 @@ int oem_add_flux_plane(oem_ctx *ctx, int32_t axis, int32_t km, int32_t nf, const
 int oem_add_field_box(oem_ctx *ctx, const int32_t *origin, const int32_t *box, int32_t nf,
                       const double *phase, int32_t *mon_id);
 
+/* Launch shape: x-planes each work-group marches (0 = the library's choice).  Before
+ * oem_prepare.  No counterpart in the reference (its launch shapes are fixed in the kernels'
+ * callers, `openem/solver.py:876-1162`); it is the knob the launch autotuner sets. */
+int oem_set_chunk(oem_ctx *ctx, int32_t chunk);
+
 /* Freeze the tables, build the launch plan.  Must precede oem_run. */
 int oem_prepare(oem_ctx *ctx);
 
@@ -171,6 +177,66 @@ int oem_read_monitor(oem_ctx *ctx, int32_t mon_id, double *re, double *im, size_
  * pairs on a complex context. */
 int oem_read_field(oem_ctx *ctx, int32_t comp, float *out, size_t count);
 int oem_write_field(oem_ctx *ctx, int32_t comp, const float *in, size_t count);
+
+/* ---- Float64 reductions: shutoff energies and plane flux -----------------------------
+ * Tables for the volume-weighted energy sums: per axis the primal and dual cell lengths (n
+ * doubles each) and eps per E component, either per material class (eps_per_class = 1:
+ * n_classes rows of 4 floats, eps_x eps_y eps_z pad, LUT materials only) or dense (3 * ncell
+ * floats).  Replaces device_tables._dual_volumes_eps (`openem/device_tables.py:320-343`). */
+int oem_set_energy_tables(oem_ctx *ctx, const double *dl_x, const double *dl_y,
+                          const double *dl_z, const double *dd_x, const double *dd_y,
+                          const double *dd_z, const float *eps, int32_t eps_per_class);
+
+/* out[0] = sum V|E|^2, out[1] = sum V eps |E|^2, out[2] = sum V |H|^2 over every cell and
+ * component (V = the E component's Yee dual volume as float32, each term formed in float32,
+ * accumulated in float64, deterministic order).  Replaces solver._sum2 inside _shutoff_check
+ * (`openem/solver.py:107-122`, `:162-177`). */
+int oem_energy(oem_ctx *ctx, double *out);
+
+/* Face weights (npts doubles each, the monitor's (n1, n2) layout) for oem_monitor_flux. */
+int oem_set_flux_weights(oem_ctx *ctx, int32_t mon_id, const double *w1, const double *w2);
+
+/* Flux terms of a plane monitor from its float64 accumulators, reduced on the device:
+ * out[2f] = sum w1 Re(E_t1 conj H_t2), out[2f+1] = sum w2 Re(E_t2 conj H_t1); the flux is
+ * (out[2f] - out[2f+1]) / 2.  count = 2 * nf.  Replaces the host reduction of flux.plane_flux
+ * (`openem/flux.py:75-110`) on the shutoff and readout paths. */
+int oem_monitor_flux(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
+
+/* ---- Serving pool: many resident scenes advanced by the same launches ------------------
+ * No counterpart in the reference, which runs one scene per openem.solver.run call
+ * (`openem/solver.py:403-511`) with its own allocations.  A pool owns one device arena of
+ * arena_bytes (the admission budget) and a hipGraph of dft_batch steps whose persistent
+ * kernels walk the tiles, point lists, ADE entries and monitors of every attached scene; each
+ * scene keeps its own device step counter, so scenes at different steps share every launch and
+ * one that reaches its n_steps simply stops.  Contexts made by oem_pool_ctx_create are built
+ * and uploaded with the single-scene calls above (from any host thread), then attached. */
+int oem_pool_create(int32_t device, uint64_t arena_bytes, int32_t dft_batch, oem_pool **out);
+
+/* Fails with OEM_ESTATE while contexts of the pool are alive. */
+int oem_pool_destroy(oem_pool *pool);
+
+/* As oem_ctx_create, with every device buffer carved out of the pool's arena; OEM_ENOMEM when
+ * the arena cannot hold it (nothing stays allocated). */
+int oem_pool_ctx_create(oem_pool *pool, const oem_dims *d, oem_ctx **out);
+
+/* Join the batch from the next block on.  The context must be prepared, at step 0, of real
+ * fields, with dft_batch equal to the pool's and at most one point list per kind. */
+int oem_pool_attach(oem_pool *pool, oem_ctx *ctx);
+
+/* Enqueue n_blocks blocks of dft_batch steps for every attached scene that has steps left;
+ * returns without waiting. */
+int oem_pool_launch(oem_pool *pool, int32_t n_blocks);
+
+/* *done = 1 once every step of an attached context has run on the device (wait = 1 blocks
+ * until then; OEM_ESTATE if the launches enqueued so far do not finish it). */
+int oem_pool_ctx_done(oem_pool *pool, oem_ctx *ctx, int32_t wait, int32_t *done);
+
+/* Wait until at most max_pending enqueued blocks are unfinished (0: all finished). */
+int oem_pool_wait(oem_pool *pool, int32_t max_pending);
+
+/* Arena bytes in use, arena capacity, attached scenes with steps left, blocks launched. */
+int oem_pool_stats(oem_pool *pool, uint64_t *used, uint64_t *capacity, int32_t *n_running,
+                   int64_t *blocks);
 
 #ifdef __cplusplus
 }

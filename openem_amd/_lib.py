@@ -27,7 +27,10 @@ EXPORTS = ("oem_version", "oem_last_error", "oem_device_count", "oem_ctx_create"
            "oem_set_materials_dense", "oem_set_dispersion", "oem_add_plane_source",
            "oem_add_dipoles", "oem_add_tfsf_points", "oem_add_flux_plane", "oem_add_field_box", "oem_prepare",
            "oem_run", "oem_profile_run", "oem_timed_run", "oem_step_index", "oem_read_monitor",
-           "oem_read_field", "oem_write_field", "oem_host_factorize")
+           "oem_read_field", "oem_write_field", "oem_set_chunk", "oem_set_energy_tables",
+           "oem_energy", "oem_set_flux_weights", "oem_monitor_flux", "oem_pool_create",
+           "oem_pool_destroy", "oem_pool_ctx_create", "oem_pool_attach", "oem_pool_launch",
+           "oem_pool_ctx_done", "oem_pool_wait", "oem_pool_stats", "oem_host_factorize")
 
 
 class OpenEMLibraryError(RuntimeError):
@@ -75,6 +78,19 @@ _SIGS = {
     "oem_read_monitor": ([_P, C.c_int32, _P, _P, C.c_size_t], C.c_int),
     "oem_read_field": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "oem_write_field": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "oem_set_chunk": ([_P, C.c_int32], C.c_int),
+    "oem_set_energy_tables": ([_P] + [_P] * 7 + [C.c_int32], C.c_int),
+    "oem_energy": ([_P, _P], C.c_int),
+    "oem_set_flux_weights": ([_P, C.c_int32, _P, _P], C.c_int),
+    "oem_monitor_flux": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "oem_pool_create": ([C.c_int32, C.c_uint64, C.c_int32, _P], C.c_int),
+    "oem_pool_destroy": ([_P], C.c_int),
+    "oem_pool_ctx_create": ([_P, _P, _P], C.c_int),
+    "oem_pool_attach": ([_P, _P], C.c_int),
+    "oem_pool_launch": ([_P, C.c_int32], C.c_int),
+    "oem_pool_ctx_done": ([_P, _P, C.c_int32, _P], C.c_int),
+    "oem_pool_wait": ([_P, C.c_int32], C.c_int),
+    "oem_pool_stats": ([_P, _P, _P, _P, _P], C.c_int),
     "oem_host_factorize": ([C.c_int64, C.c_int32, _P, _P, _P, _P, C.c_int64, _P], C.c_int),
 }
 

@@ -9,8 +9,8 @@ import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-SRC = PKG / "csrc" / "openem_hip.hip"
-HOST_SRC = PKG / "csrc" / "oem_host.cpp"
+SOURCES = [PKG / "csrc" / f for f in ("openem_hip.hip", "oem_pool.hip", "oem_reduce.hip",
+                                                 "oem_host.cpp")]
 OUT = PKG / "libopenem_hip.so"
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
@@ -27,7 +27,7 @@ def hipcc() -> str:
 
 def build_library(verbose: bool = False) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, "-o", str(tmp), str(SRC), str(HOST_SRC)]
+    cmd = [hipcc(), *FLAGS, "-o", str(tmp), *map(str, SOURCES)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr[-4000:]}")

@@ -1,0 +1,158 @@
+// Host-side internals shared by the single-scene library (openem_hip.hip) and the serving pool
+// (oem_pool.hip): error reporting, the resident-scene context, and the arena hooks a pool
+// context allocates through.
+
+#pragma once
+
+#include "../../include/openem_hip.h"
+#include "oem_kernels.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace oemi {
+
+// One message per host thread; the C ABI returns a code and oem_last_error reads this.
+inline thread_local std::string g_err;
+
+inline int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+struct Monitor {
+    MonDev d;
+    size_t count;     // ncomp*nf*npts
+};
+
+// One grouped point list (dipoles or TF/SF face corrections) and which field it updates.
+struct Points {
+    PointsDev d;
+    int magnetic;       // 1: applied to H after update_h; 0: to E after update_e
+};
+
+// Arena of a serving pool (oem_pool.hip).
+int pool_arena_alloc(oem_pool *pool, size_t bytes, void **out);
+void pool_arena_free(oem_pool *pool, void *p);
+
+}  // namespace oemi
+
+struct oem_reduce_state;            // energy / flux reduction tables (oem_reduce.hip)
+namespace oemi {
+void reduce_state_free(oem_reduce_state *r);
+}
+
+using oemi::fail;
+using oemi::Monitor;
+using oemi::Points;
+
+#define HIP_OK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? OEM_ENOMEM : OEM_EHIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);               \
+    } while (0)
+
+#define TRY(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_) return rc_;       \
+    } while (0)
+
+struct oem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    oem_dims dims{};
+    size_t ncell = 0;
+    bool cplx = false;          // Bloch: complex fields, (re, im) interleaved
+    size_t esz = 1;             // floats per field element
+    float *E = nullptr, *H = nullptr;
+    PsiDev psi{};
+    GridDev g{};
+    bool axis_set[3] = {false, false, false};
+    MatDev mat{};
+    bool mat_set = false;
+    std::vector<char> class_flagged;   // LUT classes that read the ADE history term
+    DispDev disp{};
+    bool has_disp = false;
+    PlaneSrcDev pw{};
+    std::vector<Points> points;
+    std::vector<Monitor> mons;
+    int *d_step = nullptr;      // [0] steps taken, [1] steps flushed into the DFT accumulators
+    int64_t step = 0;
+    int pending = 0;
+    bool pre_ready = false;     // ADE: hist/P already advanced by the previous fused step
+    bool prepared = false;
+    int chunk_req = 0;          // x-planes per work-group requested (0: heuristic)
+    hipGraphExec_t graph = nullptr;
+    std::vector<void *> allocs;
+    oem_reduce_state *red = nullptr;
+
+    // serving pool membership (oem_pool.hip); pool == nullptr for a standalone context
+    oem_pool *pool = nullptr;
+    bool attached = false;
+    int64_t done_block = -1;    // pool block in which the last step runs (-1: not scheduled)
+    int64_t retire_after = -1;  // memory is reclaimed once this pool block has completed
+
+    ~oem_ctx() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (red) oemi::reduce_state_free(red);
+        for (void *p : allocs) {
+            if (pool) oemi::pool_arena_free(pool, p);
+            else (void)hipFree(p);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    int alloc(T **out, size_t n, bool zero) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        if (pool) {
+            TRY(oemi::pool_arena_alloc(pool, n * sizeof(T), &p));
+        } else {
+            HIP_OK(hipMalloc(&p, n * sizeof(T)));
+        }
+        allocs.push_back(p);
+        if (zero) HIP_OK(hipMemsetAsync(p, 0, n * sizeof(T), stream));
+        *out = static_cast<T *>(p);
+        return OEM_OK;
+    }
+
+    template <typename T>
+    int upload(T **out, const T *host, size_t n) {
+        int rc = alloc(out, n, false);
+        if (rc) return rc;
+        HIP_OK(hipMemcpyAsync(*out, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return OEM_OK;
+    }
+};
+
+#define CHECK_CTX(c)                                                           \
+    do {                                                                       \
+        if (!(c)) return fail(OEM_EINVAL, "null context");                     \
+        HIP_OK(hipSetDevice((c)->device));                                     \
+    } while (0)
+
+// Shared by oem_ctx_create and oem_pool_ctx_create: validate dims and allocate the fields,
+// psi slabs and counters of a context (through the pool's arena when ctx->pool is set).
+int ctx_init(oem_ctx *c, const oem_dims *d, int32_t device);
+
+// Pool hooks used by the single-scene entry points (oem_pool.hip).
+namespace oemi {
+int pool_retire(oem_ctx *ctx);                  // oem_ctx_destroy of a pool context
+int pool_wait_done(oem_ctx *ctx);               // before reading a pool context's results
+}

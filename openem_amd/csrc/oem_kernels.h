@@ -162,8 +162,8 @@ struct TileCoord {
     bool ok;
 };
 
-__device__ __forceinline__ TileCoord tile_coord(const GridDev &g) {
-    const int lb = xcd_remap(blockIdx.x, g.nblocks);
+// Logical block lb of a scene -> (j, k) of this thread and the i-range of the x march.
+__device__ __forceinline__ TileCoord tile_coord(const GridDev &g, int lb) {
     const int nkt = (g.nz + OEM_TK - 1) / OEM_TK;
     const int tile = lb % g.tiles, chunk = lb / g.tiles;
     const int jt = tile / nkt, kt = tile - jt * nkt;
@@ -189,13 +189,12 @@ __device__ __forceinline__ F stretch_v(F &psi, F d, float a, float b, float invk
 // correction (`openem/kernels/source_dft.cu:40-55`).
 // ------------------------------------------------------------------------------------------
 template <typename F>
-__global__ void __launch_bounds__(OEM_TK * OEM_TJ)
-k_update_h(F *__restrict__ H, const F *__restrict__ E, PsiDev ps, GridDev g, float ch,
-           PlaneSrcDev pw, const int *__restrict__ stp)
+__device__ __forceinline__ void update_h_tile(F *__restrict__ H, const F *__restrict__ E,
+                                              const PsiDev &ps, const GridDev &g, float ch,
+                                              const PlaneSrcDev &pw, const int *__restrict__ stp,
+                                              const TileCoord &tc)
 {
     typedef FT<F> T;
-    const TileCoord tc = tile_coord(g);
-    if (!tc.ok) return;
     const int j = tc.j, k = tc.k;
     const int nz = g.nz, ny = g.ny, nx = g.nx;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
@@ -293,6 +292,16 @@ k_update_h(F *__restrict__ H, const F *__restrict__ E, PsiDev ps, GridDev g, flo
     }
 }
 
+template <typename F>
+__global__ void __launch_bounds__(OEM_TK * OEM_TJ)
+k_update_h(F *__restrict__ H, const F *__restrict__ E, PsiDev ps, GridDev g, float ch,
+           PlaneSrcDev pw, const int *__restrict__ stp)
+{
+    const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, g.nblocks));
+    if (!tc.ok) return;
+    update_h_tile<F>(H, E, ps, g, ch, pw, stp, tc);
+}
+
 // ------------------------------------------------------------------------------------------
 // E whole step (reference update_e / update_e_lut, `openem/kernels/yee.cu:154-337`; complex
 // update_e_c, `openem/kernels/bloch.cu:128-221`) fused with the E side of the plane-wave TF/SF
@@ -300,22 +309,25 @@ k_update_h(F *__restrict__ H, const F *__restrict__ E, PsiDev ps, GridDev g, flo
 // next plane's class index is prefetched, so the per-cell coefficient gather never waits on a
 // global load issued in the same iteration.
 // ------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool lut_in_lds(const MatDev &mat) {
+    return mat.mode == 0 && mat.n_classes <= OEM_LDS_CLASSES;
+}
+
+// Stage the class LUT (2 float4 per class) in LDS; the caller brackets it with barriers.
+__device__ __forceinline__ void stage_lut(float4 *s_lut, const MatDev &mat) {
+    const int tid = threadIdx.y * OEM_TK + threadIdx.x;
+    const float4 *src = reinterpret_cast<const float4 *>(mat.lut);
+    for (int q = tid; q < 2 * mat.n_classes; q += OEM_TK * OEM_TJ) s_lut[q] = src[q];
+}
+
 template <typename F>
-__global__ void __launch_bounds__(OEM_TK * OEM_TJ)
-k_update_e(F *__restrict__ E, const F *__restrict__ H, PsiDev ps, GridDev g, MatDev mat,
-           float cw, PlaneSrcDev pw, const int *__restrict__ stp)
+__device__ __forceinline__ void update_e_tile(F *__restrict__ E, const F *__restrict__ H,
+                                              const PsiDev &ps, const GridDev &g,
+                                              const MatDev &mat, float cw, const PlaneSrcDev &pw,
+                                              const int *__restrict__ stp, const TileCoord &tc,
+                                              const float4 *s_lut, bool lds_lut)
 {
     typedef FT<F> T;
-    __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
-    const bool lds_lut = mat.mode == 0 && mat.n_classes <= OEM_LDS_CLASSES;
-    if (lds_lut) {
-        const int tid = threadIdx.y * OEM_TK + threadIdx.x;
-        const float4 *src = reinterpret_cast<const float4 *>(mat.lut);
-        for (int q = tid; q < 2 * mat.n_classes; q += OEM_TK * OEM_TJ) s_lut[q] = src[q];
-        __syncthreads();
-    }
-    const TileCoord tc = tile_coord(g);
-    if (!tc.ok) return;
     const int j = tc.j, k = tc.k;
     const int nz = g.nz, ny = g.ny, nx = g.nx;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
@@ -457,6 +469,22 @@ k_update_e(F *__restrict__ E, const F *__restrict__ H, PsiDev ps, GridDev g, Mat
     }
 }
 
+template <typename F>
+__global__ void __launch_bounds__(OEM_TK * OEM_TJ)
+k_update_e(F *__restrict__ E, const F *__restrict__ H, PsiDev ps, GridDev g, MatDev mat,
+           float cw, PlaneSrcDev pw, const int *__restrict__ stp)
+{
+    __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
+    const bool lds_lut = lut_in_lds(mat);
+    if (lds_lut) {
+        stage_lut(s_lut, mat);
+        __syncthreads();
+    }
+    const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, g.nblocks));
+    if (!tc.ok) return;
+    update_e_tile<F>(E, H, ps, g, mat, cw, pw, stp, tc, s_lut, lds_lut);
+}
+
 // ------------------------------------------------------------------------------------------
 // Grouped point corrections: one thread per distinct (component, cell), its points applied in
 // input order (no atomics, deterministic).  Used for
@@ -480,15 +508,13 @@ struct PointsDev {
                                             // TF/SF: nsteps1*ncol complex (re, im)
 };
 
+// Group t of a point list at step n.
 template <typename F>
-__global__ void __launch_bounds__(256)
-k_points(F *__restrict__ Fld, PointsDev d, MatDev mat, float ch, size_t ncell,
-         const int *__restrict__ stp)
+__device__ __forceinline__ void point_group(F *__restrict__ Fld, const PointsDev &d,
+                                            const MatDev &mat, float ch, size_t ncell, int n,
+                                            int t)
 {
     typedef FT<F> T;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.n_groups) return;
-    const int n = step_now(stp);
     if (n >= d.nsteps1) return;
     const int c = d.g_comp[t], id = d.g_cell[t];
     F *Fp = Fld + (size_t)c * ncell;
@@ -519,6 +545,16 @@ k_points(F *__restrict__ Fld, PointsDev d, MatDev mat, float ch, size_t ncell,
         v = T::addc(v, scale * d.coef[q], ar, ai);
     }
     Fp[id] = v;
+}
+
+template <typename F>
+__global__ void __launch_bounds__(256)
+k_points(F *__restrict__ Fld, PointsDev d, MatDev mat, float ch, size_t ncell,
+         const int *__restrict__ stp)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.n_groups) return;
+    point_group<F>(Fld, d, mat, ch, ncell, step_now(stp), t);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -570,12 +606,11 @@ __device__ __forceinline__ void hist_store(void *h, size_t q, float w, float) {
     static_cast<float *>(h)[q] = w;
 }
 
+// Entry e: mode 0 = pre, 1 = post, 2 = fused post + pre.
 template <typename F>
-__global__ void __launch_bounds__(256)
-k_disp(DispDev d, const F *__restrict__ E, size_t ncell, int ny, int nz, int mode)
+__device__ __forceinline__ void disp_entry(const DispDev &d, const F *__restrict__ E,
+                                           size_t ncell, int ny, int nz, int mode, int e)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= d.n_entry) return;
     const int c = d.comp[e], id = d.cell[e];
     const F en = E[(size_t)c * ncell + id];
     if constexpr (sizeof(F) == 4) {
@@ -591,6 +626,15 @@ k_disp(DispDev d, const F *__restrict__ E, size_t ncell, int ny, int nz, int mod
         }
         if (mode != 1) static_cast<cf2 *>(d.hist)[hist_index(d, c, id, ny, nz)] = cf2{wr, wi};
     }
+}
+
+template <typename F>
+__global__ void __launch_bounds__(256)
+k_disp(DispDev d, const F *__restrict__ E, size_t ncell, int ny, int nz, int mode)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.n_entry) return;
+    disp_entry<F>(d, E, ncell, ny, nz, mode, e);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -611,14 +655,12 @@ struct MonDev {
     const double *phase;     // n_steps * nf * 4
 };
 
+// Point q of a monitor into ring row `slot`.
 template <typename F>
-__global__ void __launch_bounds__(256)
-k_dft_snap(MonDev m, const F *__restrict__ E, const F *__restrict__ H, size_t ncell, int ny,
-           int nz, const int *__restrict__ stp)
+__device__ __forceinline__ void snap_point(const MonDev &m, const F *__restrict__ E,
+                                           const F *__restrict__ H, size_t ncell, int ny, int nz,
+                                           int slot, int q)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m.npts) return;
-    const int slot = step_now(stp) % m.T;
     F *row = static_cast<F *>(m.snap) + (size_t)slot * m.ncomp * m.npts;
     if (m.kind == 0) {
         const int a = q / m.n2, b = q - a * m.n2;
@@ -644,6 +686,16 @@ k_dft_snap(MonDev m, const F *__restrict__ E, const F *__restrict__ H, size_t nc
     }
 }
 
+template <typename F>
+__global__ void __launch_bounds__(256)
+k_dft_snap(MonDev m, const F *__restrict__ E, const F *__restrict__ H, size_t ncell, int ny,
+           int nz, const int *__restrict__ stp)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.npts) return;
+    snap_point<F>(m, E, H, ncell, ny, nz, step_now(stp) % m.T, q);
+}
+
 __device__ __forceinline__ void dft_acc(double &sr, double &si, float v, double c, double s) {
     sr += (double)v * c;
     si += (double)v * s;
@@ -653,18 +705,11 @@ __device__ __forceinline__ void dft_acc(double &sr, double &si, cf2 v, double c,
     si += (double)v.r * s + (double)v.i * c;
 }
 
-// grid: x over points, y over frequencies.  Launched after the step counter advanced past the
-// last snapshot: the cnt snapshots ending at step *stp - 1 are replayed in step order; NC
+// Replay the snapshots of steps first..last (in step order) for point q at frequency f; NC
 // components of which the first NC/2 are E (whole-step phase).
 template <typename F, int NC>
-__global__ void __launch_bounds__(256)
-k_dft_flush(MonDev m, int cnt, const int *__restrict__ stp)
+__device__ __forceinline__ void flush_point(const MonDev &m, int first, int last, int f, int q)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int f = blockIdx.y;
-    if (q >= m.npts || f >= m.nf) return;
-    const int last = step_now(stp) - 1;
-    const int first = last - cnt + 1;
     double sre[NC], sim[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) { sre[c] = 0.0; sim[c] = 0.0; }
@@ -686,6 +731,15 @@ k_dft_flush(MonDev m, int cnt, const int *__restrict__ stp)
     }
 }
 
-__global__ void k_step_advance(int *stp) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *stp += 1;
+// grid: x over points, y over frequencies.  Launched after the step counter advanced past the
+// last snapshot: the cnt snapshots ending at step *stp - 1 are replayed.
+template <typename F, int NC>
+__global__ void __launch_bounds__(256)
+k_dft_flush(MonDev m, int cnt, const int *__restrict__ stp)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (q >= m.npts || f >= m.nf) return;
+    const int last = step_now(stp) - 1;
+    flush_point<F, NC>(m, last - cnt + 1, last, f, q);
 }

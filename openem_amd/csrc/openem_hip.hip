@@ -12,124 +12,17 @@
 // are captured once into a hipGraph and replayed; the device-side step counter lets one graph
 // serve every block (the reference does the same with a CUDA graph, `openem/solver.py:1165-1182`).
 
-#include "../../include/openem_hip.h"
-#include "oem_kernels.h"
+#include "oem_internal.h"
 
-#include <algorithm>
-#include <cstdarg>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <numeric>
-#include <string>
-#include <vector>
-
-namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const char *fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
+__global__ void k_step_advance(int *stp) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *stp += 1;
 }
-
-#define HIP_OK(expr)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            return fail(e_ == hipErrorOutOfMemory ? OEM_ENOMEM : OEM_EHIP, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);               \
-    } while (0)
-
-struct Monitor {
-    MonDev d;
-    size_t count;     // ncomp*nf*npts
-};
-
-// One grouped point list (dipoles or TF/SF face corrections) and which field it updates.
-struct Points {
-    PointsDev d;
-    int magnetic;       // 1: applied to H after update_h; 0: to E after update_e
-};
-
-}  // namespace
-
-struct oem_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    oem_dims dims{};
-    size_t ncell = 0;
-    bool cplx = false;          // Bloch: complex fields, (re, im) interleaved
-    size_t esz = 1;             // floats per field element
-    float *E = nullptr, *H = nullptr;
-    PsiDev psi{};
-    GridDev g{};
-    bool axis_set[3] = {false, false, false};
-    MatDev mat{};
-    bool mat_set = false;
-    std::vector<char> class_flagged;   // LUT classes that read the ADE history term
-    DispDev disp{};
-    bool has_disp = false;
-    PlaneSrcDev pw{};
-    std::vector<Points> points;
-    std::vector<Monitor> mons;
-    int *d_step = nullptr;
-    int64_t step = 0;
-    int pending = 0;
-    bool pre_ready = false;     // ADE: hist/P already advanced by the previous fused step
-    bool prepared = false;
-    hipGraphExec_t graph = nullptr;
-    std::vector<void *> allocs;
-
-    ~oem_ctx() {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        for (void *p : allocs) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    template <typename T>
-    int alloc(T **out, size_t n, bool zero) {
-        void *p = nullptr;
-        if (n == 0) n = 1;
-        HIP_OK(hipMalloc(&p, n * sizeof(T)));
-        allocs.push_back(p);
-        if (zero) HIP_OK(hipMemsetAsync(p, 0, n * sizeof(T), stream));
-        *out = static_cast<T *>(p);
-        return OEM_OK;
-    }
-
-    template <typename T>
-    int upload(T **out, const T *host, size_t n) {
-        int rc = alloc(out, n, false);
-        if (rc) return rc;
-        HIP_OK(hipMemcpyAsync(*out, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
-        HIP_OK(hipStreamSynchronize(stream));
-        return OEM_OK;
-    }
-};
-
-#define CHECK_CTX(c)                                                           \
-    do {                                                                       \
-        if (!(c)) return fail(OEM_EINVAL, "null context");                     \
-        HIP_OK(hipSetDevice((c)->device));                                     \
-    } while (0)
-
-#define TRY(expr)                  \
-    do {                           \
-        int rc_ = (expr);          \
-        if (rc_) return rc_;       \
-    } while (0)
 
 extern "C" int oem_version(void) { return OEM_ABI_VERSION; }
 
 extern "C" int oem_last_error(char *buf, size_t n) {
     if (!buf || n == 0) return OEM_EINVAL;
-    std::snprintf(buf, n, "%s", g_err.c_str());
+    std::snprintf(buf, n, "%s", oemi::g_err.c_str());
     return OEM_OK;
 }
 
@@ -141,9 +34,7 @@ extern "C" int oem_device_count(int32_t *n) {
     return OEM_OK;
 }
 
-extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) {
-    if (!d || !out) return fail(OEM_EINVAL, "null argument");
-    *out = nullptr;
+int ctx_init(oem_ctx *c, const oem_dims *d, int32_t device) {
     if (d->nx < 1 || d->ny < 1 || d->nz < 1)
         return fail(OEM_EINVAL, "grid %d x %d x %d", d->nx, d->ny, d->nz);
     if ((int64_t)d->ny * d->nz > (int64_t)1 << 30)
@@ -161,23 +52,17 @@ extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) 
         return fail(OEM_EHIP, "no HIP device visible");
     if (device < 0 || device >= ndev) return fail(OEM_EINVAL, "device %d of %d", device, ndev);
     HIP_OK(hipSetDevice(device));
-    oem_ctx *c = new oem_ctx();
     c->device = device;
     c->dims = *d;
     c->ncell = (size_t)d->nx * d->ny * d->nz;
     c->cplx = d->complex_fields != 0;
     c->esz = c->cplx ? 2 : 1;
-    int rc = OEM_OK;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(OEM_EHIP, "hipStreamCreate failed");
-    }
     const size_t nc = c->ncell;
-    if (nc * 4 * c->esz > 0xffffffffull) {
-        delete c;
+    if (nc * 4 * c->esz > 0xffffffffull)
         return fail(OEM_ENOTSUP, "%zu cells: one field component exceeds 4 GiB (limit 2^%d cells)",
                     nc, c->cplx ? 29 : 30);
-    }
     const size_t slab[3] = {(size_t)(d->pml_lo[0] + d->pml_hi[0]) * d->ny * d->nz,
                             (size_t)d->nx * (d->pml_lo[1] + d->pml_hi[1]) * d->nz,
                             (size_t)d->nx * d->ny * (d->pml_lo[2] + d->pml_hi[2])};
@@ -187,20 +72,25 @@ extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) 
             c->psi.off[2 * a + m] = (uint32_t)total;
             total += slab[a];
         }
-    if (total * 4 * c->esz > 0xffffffffull) {
-        delete c;
-        return fail(OEM_ENOTSUP, "CPML slabs exceed 4 GiB per side");
-    }
+    if (total * 4 * c->esz > 0xffffffffull) return fail(OEM_ENOTSUP, "CPML slabs exceed 4 GiB per side");
     c->psi.total = (uint32_t)(total ? total : 1);
-    rc = c->alloc(&c->E, 3 * nc * c->esz, true);
-    if (!rc) rc = c->alloc(&c->H, 3 * nc * c->esz, true);
+    TRY(c->alloc(&c->E, 3 * nc * c->esz, true));
+    TRY(c->alloc(&c->H, 3 * nc * c->esz, true));
     float *ph = nullptr, *pe = nullptr;
-    if (!rc) rc = c->alloc(&ph, (size_t)c->psi.total * c->esz, true);
-    if (!rc) rc = c->alloc(&pe, (size_t)c->psi.total * c->esz, true);
+    TRY(c->alloc(&ph, (size_t)c->psi.total * c->esz, true));
+    TRY(c->alloc(&pe, (size_t)c->psi.total * c->esz, true));
     c->psi.h = ph;
     c->psi.e = pe;
-    if (!rc) rc = c->alloc(&c->d_step, 1, true);
-    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OEM_EHIP, "sync failed");
+    TRY(c->alloc(&c->d_step, 2, true));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return OEM_OK;
+}
+
+extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) {
+    if (!d || !out) return fail(OEM_EINVAL, "null argument");
+    *out = nullptr;
+    oem_ctx *c = new oem_ctx();
+    const int rc = ctx_init(c, d, device);
     if (rc) {
         delete c;
         return rc;
@@ -212,8 +102,17 @@ extern "C" int oem_ctx_create(const oem_dims *d, int32_t device, oem_ctx **out) 
 extern "C" int oem_ctx_destroy(oem_ctx *ctx) {
     if (!ctx) return OEM_OK;
     (void)hipSetDevice(ctx->device);
+    if (ctx->pool) return oemi::pool_retire(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     delete ctx;
+    return OEM_OK;
+}
+
+extern "C" int oem_set_chunk(oem_ctx *ctx, int32_t chunk) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (chunk < 0) return fail(OEM_EINVAL, "chunk %d", chunk);
+    ctx->chunk_req = chunk;
     return OEM_OK;
 }
 
@@ -585,8 +484,13 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     g.tiles = ((g.nz + OEM_TK - 1) / OEM_TK) * ((g.ny + OEM_TJ - 1) / OEM_TJ);
     int chunk;
     const char *env = std::getenv("OEM_CHUNK");
-    if (env && std::atoi(env) > 0) {
+    if (ctx->chunk_req > 0) {
+        chunk = ctx->chunk_req;
+    } else if (env && std::atoi(env) > 0) {
         chunk = std::atoi(env);
+    } else if (ctx->pool) {
+        // a pool work-group marches 32 planes: the batch of scenes supplies the parallelism
+        chunk = 32;
     } else {
         const int want_blocks = 4096;
         const int chunks_needed = std::max(1, (want_blocks + g.tiles - 1) / g.tiles);
@@ -715,6 +619,7 @@ int launch_step(oem_ctx *c, int flush_after, Prof *pr, bool last) {
 
 static int check_run(oem_ctx *ctx, int64_t n) {
     if (!ctx->prepared) return fail(OEM_ESTATE, "oem_prepare not called");
+    if (ctx->attached) return fail(OEM_ESTATE, "context is stepped by its pool");
     if (n < 0) return fail(OEM_EINVAL, "negative step count");
     if (ctx->step + n > ctx->dims.n_steps)
         return fail(OEM_EINVAL, "step %lld + %lld exceeds the %d-step tables",
@@ -852,6 +757,7 @@ extern "C" int oem_read_monitor(oem_ctx *ctx, int32_t mon_id, double *re, double
     const Monitor &m = ctx->mons[mon_id];
     if (count != m.count || !re || !im)
         return fail(OEM_EINVAL, "monitor %d holds %zu values, asked %zu", mon_id, m.count, count);
+    if (ctx->pool) TRY(oemi::pool_wait_done(ctx));
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipMemcpy(re, m.d.re, count * sizeof(double), hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(im, m.d.im, count * sizeof(double), hipMemcpyDeviceToHost));
@@ -871,6 +777,7 @@ extern "C" int oem_read_field(oem_ctx *ctx, int32_t comp, float *out, size_t cou
     float *p;
     TRY(field_ptr(ctx, comp, count, &p));
     if (!out) return fail(OEM_EINVAL, "null output");
+    if (ctx->pool) TRY(oemi::pool_wait_done(ctx));
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipMemcpy(out, p, count * sizeof(float), hipMemcpyDeviceToHost));
     return OEM_OK;
@@ -881,6 +788,7 @@ extern "C" int oem_write_field(oem_ctx *ctx, int32_t comp, const float *in, size
     float *p;
     TRY(field_ptr(ctx, comp, count, &p));
     if (!in) return fail(OEM_EINVAL, "null input");
+    if (ctx->attached) return fail(OEM_ESTATE, "context is stepped by its pool");
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipMemcpy(p, in, count * sizeof(float), hipMemcpyHostToDevice));
     return OEM_OK;

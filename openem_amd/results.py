@@ -24,3 +24,5 @@ class Result:
     fields: np.ndarray | None = None
     shutoff_triggered: bool = False
     stop_reason: str = "pinned step count"
+    decay_trace: list = field(default_factory=list)     # (step, field-decay ratio) per check
+    energy_trace: list = field(default_factory=list)    # (step, total energy) per check

@@ -212,8 +212,13 @@ def _staircase_dispersion(mat: np.ndarray, materials: list, dt: float) -> Disper
     mid3 = np.tile(mids, 3)
     counts = npole[mid3]
     ofs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
-    am1 = np.concatenate([coeffs[m][0] for m in mid3]) if mid3.size else np.zeros(0, complex)
-    b = np.concatenate([coeffs[m][1] for m in mid3]) if mid3.size else np.zeros(0, complex)
+    # pole t of entry e is pole (t - ofs[e]) of the entry's material
+    cat_am1 = np.concatenate([c[0] for c in coeffs])
+    cat_b = np.concatenate([c[1] for c in coeffs])
+    first = np.concatenate([[0], np.cumsum(npole)[:-1]])
+    src = (np.repeat(first[mid3], counts) + np.arange(int(ofs[-1]))
+           - np.repeat(ofs[:-1].astype(np.int64), counts))
+    am1, b = cat_am1[src], cat_b[src]
     gm = np.array([2.0 * c[1].real.sum() for c in coeffs])
     return Dispersion(comp=comp, cell=cell, pole_ofs=ofs, am1=am1, b=b, g=gm[mid3])
 
@@ -420,3 +425,114 @@ def tfsf_box_scene(shape=(26, 24, 44), dl: float = 20e-9, n_pml: int = 6, num_st
     return Scene(grid=grid, dt=dt, num_time_steps=num_steps, eps_ex=eps, eps_ey=eps.copy(),
                  eps_ez=eps.copy(), pml=pml_faces(dt, (dl, dl, dl), sides, n_pml),
                  tfsf_sources=[src], flux_monitors=mons)
+
+
+# ---------------------------------------------------------------------------------------------
+# Serving stream: seeded requests of three scene kinds on cubic grids
+# ---------------------------------------------------------------------------------------------
+
+SERVING_KINDS = ("slab", "soup", "sphere")
+
+
+def serving_requests(n: int = 256, rate: float = 20.0, seed: int = 0, side_lo: int = 64,
+                     side_hi: int = 320, steps_lo: int = 2000, steps_hi: int = 8000) -> list:
+    """``n`` request specs arriving as a Poisson process of ``rate`` per second: cubic side
+    log-uniform in [side_lo, side_hi], step count uniform in [steps_lo, steps_hi], 1-3 flux
+    monitors of 5-21 frequencies, kind uniform over :data:`SERVING_KINDS`."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    out = []
+    for q in range(n):
+        side = int(round(np.exp(rng.uniform(np.log(side_lo), np.log(side_hi)))))
+        out.append(dict(id=q, arrival=float(t[q]), kind=SERVING_KINDS[int(rng.integers(3))],
+                        side=side, n_steps=int(rng.integers(steps_lo, steps_hi + 1)),
+                        n_mon=int(rng.integers(1, 4)), nf=int(rng.integers(5, 22)),
+                        seed=int(rng.integers(2 ** 31))))
+    return out
+
+
+def _flux_planes(shape, n_mon: int, n_pml: int, freqs) -> list:
+    """Up to three flux planes just inside the high-side PML of z, x, y (or of z alone when
+    x and y are periodic)."""
+    out = []
+    for q in range(n_mon):
+        axis = (2, 0, 1)[q]
+        km = shape[axis] - n_pml - 3 - 2 * q
+        out.append(FluxMonitor(f"flux{q}", axis, km, +1, freqs))
+    return out
+
+
+def serving_scene(spec: dict, dl: float = 20e-9, n_pml: int = 12) -> Scene:
+    """The scene of one serving request (see :func:`serving_requests`).
+
+    * ``slab``: periodic x/y, PML z, a z-normal slab of eps in [2, 12] (thickness 10-30% of
+      the interior) driven by a +z Ex plane wave (f0 = 2.998e14 Hz, fwidth 0.2 f0); flux planes
+      normal to z.
+    * ``soup``: the seeded box soup (32 boxes, plain eps or 2-pole Lorentz) with PML on all
+      faces and a central Ez dipole (f0 = 2e14 Hz, fwidth 0.5 f0); flux planes normal to z, x, y.
+    * ``sphere``: an Ez dipole at the centre of a Lorentz sphere (eps_inf in [1, 4], one or two
+      poles) of radius 15-35% of the interior, vacuum elsewhere, PML on all faces.
+    """
+    kind, side, ns = spec["kind"], int(spec["side"]), int(spec["n_steps"])
+    rng = np.random.default_rng(spec["seed"])
+    if kind == "slab":
+        size = (side * dl, side * dl, (side - 2 * n_pml) * dl)
+        thick = rng.uniform(0.1, 0.3) * size[2]
+        f0 = 2.998e14
+        freqs = np.linspace(0.9 * f0, 1.1 * f0, spec["nf"])
+        nz_in = side - 2 * n_pml
+        src_z = -size[2] / 2 + 6 * dl
+        sc = slab_scene(dl=dl, size=size, eps_slab=float(rng.uniform(2.0, 12.0)), thick=thick,
+                        f0=f0, n_pml=n_pml, src_z=src_z, t_z=size[2] / 2 - 4 * dl, r_z=None,
+                        nf=spec["nf"], num_steps=ns)
+        mons = []
+        for q in range(spec["n_mon"]):
+            km = n_pml + nz_in - 4 - 3 * q
+            mons.append(FluxMonitor(f"flux{q}", 2, km, +1, freqs))
+        return _replace_monitors(sc, mons, [])
+    if kind == "soup":
+        f0, fw = 2e14, 0.5
+        sc = box_soup_scene(n=side, dl=dl, n_pml=n_pml, seed=int(spec["seed"]) % (2 ** 31),
+                            f0=f0, fwidth_rel=fw, num_steps=ns, nf=spec["nf"])
+        freqs = np.linspace(f0 * (1 - fw), f0 * (1 + fw), spec["nf"])
+        return _replace_monitors(sc, _flux_planes(sc.shape, spec["n_mon"], n_pml, freqs), [])
+    if kind == "sphere":
+        return _sphere_scene(side, ns, spec, rng, dl, n_pml)
+    raise ValueError(f"unknown serving scene kind {kind!r}")
+
+
+def _replace_monitors(sc: Scene, flux: list, field_mons: list) -> Scene:
+    import dataclasses
+    return dataclasses.replace(sc, flux_monitors=flux, field_monitors=field_mons)
+
+
+def _sphere_scene(side: int, ns: int, spec: dict, rng, dl: float, n_pml: int) -> Scene:
+    axes = [uniform_axis(side, dl, 0.0, "PML") for _ in range(3)]
+    grid = Grid(*axes)
+    dt = cpml.courant_dt((dl, dl, dl))
+    interior = side - 2 * n_pml
+    rad = rng.uniform(0.15, 0.35) * interior
+    c = (side - 1) / 2.0
+    i = np.arange(side, dtype=np.float64) - c
+    r2 = i[:, None, None] ** 2 + i[None, :, None] ** 2 + i[None, None, :] ** 2
+    inside = r2 <= rad * rad
+    eps_inf = rng.uniform(1.0, 4.0)
+    poles = []
+    for _p in range(int(rng.integers(1, 3))):
+        poles += lorentz_poles(rng.uniform(0.5, 3.0), rng.uniform(1e14, 8e14), 1e13)
+    eps = np.where(inside, eps_inf, 1.0)
+    mat = np.where(inside, 0, -1).astype(np.int32)
+    disp = _staircase_dispersion(mat, [poles], dt)
+    if disp.n_entry == 0:
+        disp = None
+    f0, fw = 2e14, 0.5
+    pulse = waveform.GaussianPulse(freq0=f0, fwidth=fw * f0)
+    wf = waveform.sample(pulse, dt, ns)
+    centre = [0.5 * (a.edges[0] + a.edges[-1]) for a in axes]
+    idx, coef = dipole_stencil(grid, 2, centre)
+    dip = PointDipole(component=2, indices=idx, coef=coef * 1e-20, waveform=wf)
+    freqs = np.linspace(f0 * (1 - fw), f0 * (1 + fw), spec["nf"])
+    sides_all = [(a, s) for a in range(3) for s in ("lo", "hi")]
+    return Scene(grid=grid, dt=dt, num_time_steps=ns, eps_ex=eps, eps_ey=eps, eps_ez=eps,
+                 pml=pml_faces(dt, (dl, dl, dl), sides_all, n_pml), dispersion=disp,
+                 dipoles=[dip], flux_monitors=_flux_planes(grid.shape, spec["n_mon"], n_pml, freqs))

@@ -14,7 +14,7 @@ import time
 
 import numpy as np
 
-from . import _lib, tables as tables_mod
+from . import _lib, cpml, flux as flux_mod, shutoff as shutoff_mod, tables as tables_mod
 from .model import Scene
 from .results import Phasors, Result
 
@@ -32,11 +32,13 @@ def _f64(a) -> np.ndarray:
 
 
 class Stepper:
-    """One scene resident on the GPU (real fields)."""
+    """One scene resident on the GPU.  With ``pool`` (a :class:`openem_amd.serving.Pool`) its
+    device memory comes from the pool's arena and the pool steps it once attached."""
 
     def __init__(self, tabs: tables_mod.StepTables, device: int = 0, dft_batch: int = 16,
-                 dense_materials: bool = False):
+                 dense_materials: bool = False, pool=None, chunk: int = 0):
         self.lib = lib = _lib.load()
+        self.pool = pool
         self.cplx = bool(tabs.bloch)
         self.tabs = tabs
         nx, ny, nz = tabs.shape
@@ -51,9 +53,13 @@ class Stepper:
         d.ch = np.float32(tabs.ch)
         d.cw = np.float32(tabs.cw)
         self.ctx = C.c_void_p()
-        _lib.check(lib, lib.oem_ctx_create(C.byref(d), int(device), C.byref(self.ctx)))
-        self._keep = []
+        if pool is None:
+            _lib.check(lib, lib.oem_ctx_create(C.byref(d), int(device), C.byref(self.ctx)))
+        else:
+            _lib.check(lib, lib.oem_pool_ctx_create(pool.handle, C.byref(d), C.byref(self.ctx)))
         try:
+            if chunk:
+                _lib.check(lib, lib.oem_set_chunk(self.ctx, int(chunk)))
             self._upload(dense_materials)
             _lib.check(lib, lib.oem_prepare(self.ctx))
         except Exception:
@@ -180,6 +186,62 @@ class Stepper:
                                                        _lib.ptr(im), re.size))
         return re + 1j * im
 
+    # -- float64 reductions on the device ----------------------------------------------------
+    def setup_energy(self) -> None:
+        """Upload the energy weights: per-axis primal and dual lengths and eps per class (or
+        dense eps when the scene has no class LUT)."""
+        sc = self.tabs.scene
+        ls = [np.ascontiguousarray(a.dl, dtype=np.float64) for a in sc.grid.axes]
+        ds = [np.ascontiguousarray(a.dl_dual, dtype=np.float64) for a in sc.grid.axes]
+        if self.n_classes:
+            eps, per_class = np.ascontiguousarray(self.tabs.class_eps(), dtype=np.float32), 1
+        else:
+            eps = np.ascontiguousarray(np.stack([np.broadcast_to(e, sc.shape) for e in
+                                                 (sc.eps_ex, sc.eps_ey, sc.eps_ez)]),
+                                       dtype=np.float32)
+            per_class = 0
+        self._energy_keep = ls + ds + [eps]
+        _lib.check(self.lib, self.lib.oem_set_energy_tables(
+            self.ctx, *[_lib.ptr(a) for a in ls + ds + [eps]], per_class))
+
+    def energy(self) -> np.ndarray:
+        """``[sum V|E|^2, sum V eps|E|^2, sum V|H|^2]`` (float64, reduced on the device)."""
+        out = np.zeros(3, np.float64)
+        _lib.check(self.lib, self.lib.oem_energy(self.ctx, _lib.ptr(out)))
+        return out
+
+    def setup_flux(self, transverse: bool = True) -> None:
+        """Upload the face weights of every plane monitor (``transverse``: apply the flux
+        monitor's transverse window, as :func:`openem_amd.flux.monitor_flux` does)."""
+        sc = self.tabs.scene
+        for i, m in enumerate(self.tabs.monitors):
+            if m.kind != "plane":
+                continue
+            w1, w2 = flux_mod.area_weights(sc.grid, m.axis)
+            fm = sc.flux_monitors[i]
+            if transverse and fm.transverse is not None:
+                mask = np.zeros(w1.shape)
+                (j0, j1), (k0, k1) = fm.transverse
+                mask[j0:j1 + 1, k0:k1 + 1] = 1.0
+                w1, w2 = w1 * mask, w2 * mask
+            w1 = np.ascontiguousarray(w1, dtype=np.float64)
+            w2 = np.ascontiguousarray(w2, dtype=np.float64)
+            _lib.check(self.lib, self.lib.oem_set_flux_weights(self.ctx, self.mon_ids[i],
+                                                               _lib.ptr(w1), _lib.ptr(w2)))
+
+    def flux_terms(self, i: int) -> np.ndarray:
+        """``(nf, 2)``: the two face sums of plane monitor ``i`` (flux = (t1 - t2) / 2)."""
+        m = self.tabs.monitors[i]
+        out = np.zeros((m.nf, 2), np.float64)
+        _lib.check(self.lib, self.lib.oem_monitor_flux(self.ctx, self.mon_ids[i], _lib.ptr(out),
+                                                       out.size))
+        return out
+
+    def plane_flux(self, i: int) -> np.ndarray:
+        """``plane_flux`` of monitor ``i`` (no axis sign), reduced on the device."""
+        t = self.flux_terms(i)
+        return 0.5 * (t[:, 0] - t[:, 1])
+
     def close(self) -> None:
         if getattr(self, "ctx", None) and self.ctx.value:
             self.lib.oem_ctx_destroy(self.ctx)
@@ -195,18 +257,48 @@ class Stepper:
 def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kernels=None,
         verbose: bool = True, return_fields: bool = False, policy=None,
         force_complex: bool = False, device: int = 0) -> Result:
-    """Run one simulation; see the module docstring. ``use_shutoff``/``policy`` are accepted
-    for signature parity: this stepper always runs the pinned step count (no early stop), and
-    ``kernels`` is ignored (the library is loaded once per process)."""
+    """Run one simulation; see the module docstring.  With ``use_shutoff`` the run checks the
+    ``policy`` (default: :func:`openem_amd.shutoff.default_policy` at the scene's shutoff
+    threshold) every ``policy.interval`` steps and may stop early once the sources are done
+    (reference ``openem/solver.py:1165-1213``); ``kernels`` is ignored (the library is loaded
+    once per process)."""
     if force_complex and not sc.any_bloch:
         raise NotImplementedError("force_complex needs a Bloch scene (set bloch_k)")
     n_total = int(num_steps if num_steps is not None else sc.num_time_steps)
     t0 = time.time()
     tabs = tables_mod.build(sc, n_total)
     st = Stepper(tabs, device=device)
+    triggered, reason = False, "pinned step count"
+    decay, energy_tr = [], []
     try:
         t1 = time.time()
-        st.run(n_total)
+        if use_shutoff:
+            pol = policy or shutoff_mod.default_policy(sc.shutoff)
+            src_end = shutoff_mod.source_end_step(sc)
+            st.setup_energy()
+            st.setup_flux(transverse=False)
+            plane_ids = [i for i, m in enumerate(tabs.monitors) if m.kind == "plane"]
+            step = 0
+            while step < n_total:
+                k = min(pol.interval - step % pol.interval, n_total - step)
+                st.run(k)
+                step += k
+                if step % pol.interval:
+                    continue
+                e = st.energy()
+                u = 0.5 * (cpml.EPSILON_0 * e[1] + cpml.MU_0 * e[2])
+                fl = (np.concatenate([st.plane_flux(i) for i in plane_ids]) if plane_ids
+                      else np.zeros(0))
+                stop = pol.check(step, e[0], u, fl) and step > src_end
+                decay.append((step, pol.field_decay.ratio if pol.field_decay else float("nan")))
+                energy_tr.append((step, u))
+                if stop:
+                    triggered, reason = True, pol.reason
+                    break
+            steps_run = step
+        else:
+            st.run(n_total)
+            steps_run = n_total
         wall = time.time() - t1
         phasors, fphas = {}, {}
         for i, m in enumerate(tabs.monitors):
@@ -221,11 +313,12 @@ def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kerne
     finally:
         st.close()
     if verbose:
-        print(f"[openem_amd] {sc.shape} x {n_total} steps: setup {t1 - t0:.2f}s, "
-              f"stepping {wall:.2f}s")
+        print(f"[openem_amd] {sc.shape} x {steps_run} steps: setup {t1 - t0:.2f}s, "
+              f"stepping {wall:.2f}s" + (f", stopped: {reason}" if triggered else ""))
     _warn_nonfinite(phasors, fphas)
-    return Result(phasors=phasors, field_phasors=fphas, steps_run=n_total,
-                  wall_seconds=wall, fields=fields)
+    return Result(phasors=phasors, field_phasors=fphas, steps_run=steps_run,
+                  wall_seconds=wall, fields=fields, shutoff_triggered=triggered,
+                  stop_reason=reason, decay_trace=decay, energy_trace=energy_tr)
 
 
 def _warn_nonfinite(ph, fph) -> None:

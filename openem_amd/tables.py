@@ -130,6 +130,7 @@ class StepTables:
     _flag: np.ndarray | None = None
     _cls: np.ndarray | None = None
     _lut: np.ndarray | None = None
+    _eps_cls: np.ndarray | None = None
 
     @property
     def ca(self) -> np.ndarray:
@@ -155,10 +156,15 @@ class StepTables:
         """``(cls (nx,ny,nz) uint16, lut (n_classes, 8) float32)``, or ``None`` when more than
         65535 distinct classes exist (the caller then uploads dense ca/cb)."""
         if self._cls is None:
-            self._cls, self._lut = _material_classes(self.scene)
+            self._cls, self._lut, self._eps_cls = _material_classes(self.scene)
         if self._cls.size == 0:
             return None
         return self._cls, self._lut
+
+    def class_eps(self) -> np.ndarray | None:
+        """``(n_classes, 4)`` float32: eps_x, eps_y, eps_z of each material class (the energy
+        weights of the shutoff), or ``None`` without classes."""
+        return None if self.material_classes() is None else self._eps_cls
 
 
 def _disp_flags(sc: Scene) -> np.ndarray:
@@ -259,13 +265,15 @@ def _material_classes(sc: Scene):
     flags): the float64 ca/cb of each distinct input row follow :func:`e_coeffs`' formula term
     by term, so every class row equals the float32 cast of ``e_coeffs`` at its cells.  Classes
     are numbered in order of first appearance (native hash factorization)."""
-    empty = (np.zeros(0, np.uint16), np.zeros((0, LUT_STRIDE), np.float32))
+    empty = (np.zeros(0, np.uint16), np.zeros((0, LUT_STRIDE), np.float32),
+             np.zeros((0, 4), np.float32))
     shape = sc.shape
     gg = sc.dispersion.g_dense(shape) if sc.any_dispersion else None
     eps = (sc.eps_ex, sc.eps_ey, sc.eps_ez)
     sig = (sc.sigma_ex, sc.sigma_ey, sc.sigma_ez)
-    cols, layout = [], []
+    cols, layout, eps_col = [], [], []
     for c in range(3):
+        eps_col.append(len(cols))          # every column so far is float64: one w64 column each
         cols.append(np.asarray(eps[c], dtype=np.float64))
         has_s, has_g = sig[c] is not None, gg is not None
         if has_s:
@@ -305,14 +313,19 @@ def _material_classes(sc: Scene):
         lut[:, 2 * c] = ca
         lut[:, 2 * c + 1] = cb
     lut.view(np.uint32)[:, 6] = fl & np.uint32(7)
-    # distinct input rows can share one coefficient row: deduplicate the LUT itself
-    r2 = _lib.factorize([lut[:, i].copy() for i in range(7)], MAX_CLASSES)
+    eps32 = np.stack([w64[:, q] for q in eps_col], axis=1).astype(np.float32)
+    # distinct input rows can share one coefficient row: deduplicate (LUT row, float32 eps)
+    r2 = _lib.factorize([lut[:, i].copy() for i in range(7)]
+                        + [eps32[:, c].copy() for c in range(3)], MAX_CLASSES)
     if r2 is None:
         return empty
     inv2, rows2 = r2
     out = np.zeros((rows2.shape[0], LUT_STRIDE), np.uint32)
-    out[:, :7] = rows2
-    return inv2[inv].reshape(shape).astype(np.uint16), out.view(np.float32)
+    out[:, :7] = rows2[:, :7]
+    eps_cls = np.zeros((rows2.shape[0], 4), np.uint32)
+    eps_cls[:, :3] = rows2[:, 7:]
+    return (inv2[inv].reshape(shape).astype(np.uint16), out.view(np.float32),
+            eps_cls.view(np.float32))
 
 
 def disp_tables(sc: Scene) -> DispTables | None:

@@ -21,7 +21,10 @@ It restates the reference's step term by term, on the float64 master tables that
   ``H -= ch*coef*amp_int[n]``);
 - running DFT: ``source_dft.cu:216-271`` (box, native Yee positions) and ``:371-430`` (plane,
   H colocated as ``(H[k]+H[k-1])/2``), E at ``(n+1)dt``, H at ``(n+1/2)dt``;
-- launch order of ``openem/solver.py:1145-1162``.
+- launch order of ``openem/solver.py:1145-1162``;
+- shutoff sums (:func:`energy_sums`): ``openem/solver.py:107-122`` and ``:162-177`` with the
+  float32 dual volumes and eps of ``openem/device_tables.py:320-343``; plane flux
+  (:func:`plane_flux_terms`): ``openem/flux.py:75-110``.
 
 Pinned by (see tests/test_oracle.py): discrete div(curl)=0, the Yee dispersion relation, the
 CPML reflection bound (``tests/test_pml_reflection.py:57-60`` of the reference), TF/SF
@@ -272,3 +275,28 @@ class Oracle64:
         for _ in range(int(n_steps)):
             self.step()
         return self
+
+
+def energy_sums(sc, E, H) -> np.ndarray:
+    """``[sum V|E|^2, sum V eps|E|^2, sum V|H|^2]`` in float64 (reference ``_sum2`` over the
+    three components, ``openem/solver.py:107-122``, ``:172-176``): V is each E component's Yee
+    dual volume rounded to float32, eps the component's permittivity rounded to float32, and H
+    is weighted by the same E-component volumes."""
+    vols = [v.astype(np.float32) for v in sc.grid.yee_dual_volumes()]
+    eps = [np.asarray(e, dtype=np.float64).astype(np.float32) * np.ones(sc.shape, np.float32)
+           for e in (sc.eps_ex, sc.eps_ey, sc.eps_ez)]
+    out = np.zeros(3)
+    for c in range(3):
+        e2 = np.abs(E[c]) ** 2
+        out[0] += float(np.sum(vols[c].astype(np.float64) * e2))
+        out[1] += float(np.sum((vols[c] * eps[c]).astype(np.float64) * e2))
+        out[2] += float(np.sum(vols[c].astype(np.float64) * np.abs(H[c]) ** 2))
+    return out
+
+
+def plane_flux_terms(data, w1, w2) -> np.ndarray:
+    """``(nf, 2)``: ``sum w1 Re(E_t1 conj H_t2)`` and ``sum w2 Re(E_t2 conj H_t1)`` of a plane
+    monitor's phasors ``(4, nf, n1, n2)`` (reference ``openem/flux.py:75-110``)."""
+    t1 = (np.real(data[0] * np.conj(data[3])) * w1[None]).sum(axis=(1, 2))
+    t2 = (np.real(data[1] * np.conj(data[2])) * w2[None]).sum(axis=(1, 2))
+    return np.stack([t1, t2], axis=1)
