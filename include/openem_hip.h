@@ -39,7 +39,8 @@ enum oem_status {
 enum oem_kernel_id {
     OEM_K_UPDATE_H = 0, OEM_K_UPDATE_E = 1, OEM_K_DIPOLE_E = 2, OEM_K_DIPOLE_H = 3,
     OEM_K_DISP_PRE = 4, OEM_K_DISP_POST = 5, OEM_K_DFT_SNAP = 6, OEM_K_DFT_FLUSH = 7,
-    OEM_K_STEP = 8, OEM_K_TFSF_H = 9, OEM_K_TFSF_E = 10, OEM_K_COUNT = 11
+    OEM_K_STEP = 8, OEM_K_TFSF_H = 9, OEM_K_TFSF_E = 10, OEM_K_MODE_H = 11, OEM_K_MODE_E = 12,
+    OEM_K_COUNT = 13
 };
 
 typedef struct oem_ctx oem_ctx;
@@ -132,6 +133,18 @@ int oem_add_tfsf_points(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const int
                         const int32_t *cell, const float *coef, const int32_t *col,
                         const float *phase, int32_t ncol, const double *table);
 
+/* Mode source points on one plane: H side (magnetic = 1, after the TF/SF H corrections)
+ * H[comp][cell] += ch_coef * Re(phase * amp[n]) with amp the whole-step amplitude table, E side
+ * (magnetic = 0, after the TF/SF E corrections) E[comp][cell] += cb * coef * Re(phase * amp[n])
+ * with the half-step table; phase (re, im float pairs) is the mode profile value at the point
+ * and coef carries the face sign and 1/dl (H: also dt/mu0).  amp: n_steps+1 complex (re, im
+ * doubles).  Real fields take the real part of a complex profile.  Replaces inject_mode_h /
+ * inject_mode_e and _launch_mode_inject (`openem/kernels/source_dft.cu:84-128`,
+ * `openem/solver.py:230-268`). */
+int oem_add_mode_points(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const int32_t *comp,
+                        const int32_t *cell, const float *coef, const float *phase,
+                        const double *amp);
+
 /* Plane DFT monitor (normal `axis`, E plane index km >= 1): accumulates
  * (E_t1, E_t2, (H_t1[km]+H_t1[km-1])/2, (H_t2..)/2) for nf frequencies in float64.
  * phase is n_steps*nf*4 float64: cos/sin at t_e=(n+1)dt then at t_h=(n+1/2)dt, windowed.
@@ -201,6 +214,17 @@ int oem_set_flux_weights(oem_ctx *ctx, int32_t mon_id, const double *w1, const d
  * (out[2f] - out[2f+1]) / 2.  count = 2 * nf.  Replaces the host reduction of flux.plane_flux
  * (`openem/flux.py:75-110`) on the shutoff and readout paths. */
 int oem_monitor_flux(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
+
+/* Overlap weights of a plane monitor: n_sets (1..16) sets of 4 * nf * npts complex values
+ * (re, im doubles), layout (set, component, f, point) over the monitor's (E_t1, E_t2, H_t1,
+ * H_t2) accumulators. */
+int oem_set_overlap_weights(oem_ctx *ctx, int32_t mon_id, int32_t n_sets, const double *g);
+
+/* out[(set * nf + f) * 2 + {0, 1}] = sum over points and components of accumulator * weight
+ * (complex, float64, deterministic).  With the conjugated mode profiles as weights these are
+ * the mode overlaps.  Replaces the mode-overlap readout of nb/backend._mode_data
+ * (`openem/nb/backend.py:1422`), done there on the host. */
+int oem_monitor_overlap(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
 
 /* ---- Serving pool: many resident scenes advanced by the same launches ------------------
  * No counterpart in the reference, which runs one scene per openem.solver.run call

@@ -17,7 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 OEM_OK, OEM_EINVAL, OEM_ENOMEM, OEM_EHIP, OEM_ESTATE, OEM_ENOTSUP = range(6)
 KERNELS = ("update_h", "update_e", "dipole_e", "dipole_h", "disp_pre", "disp_post",
-           "dft_snap", "dft_flush", "step_advance", "tfsf_h", "tfsf_e")
+           "dft_snap", "dft_flush", "step_advance", "tfsf_h", "tfsf_e", "mode_h", "mode_e")
 K_COUNT = len(KERNELS)
 
 #: every symbol include/openem_hip.h and include/openem_host.h declare (checked by the CPU
@@ -25,10 +25,12 @@ K_COUNT = len(KERNELS)
 EXPORTS = ("oem_version", "oem_last_error", "oem_device_count", "oem_ctx_create",
            "oem_ctx_destroy", "oem_set_axis", "oem_set_materials_lut",
            "oem_set_materials_dense", "oem_set_dispersion", "oem_add_plane_source",
-           "oem_add_dipoles", "oem_add_tfsf_points", "oem_add_flux_plane", "oem_add_field_box", "oem_prepare",
+           "oem_add_dipoles", "oem_add_tfsf_points", "oem_add_mode_points", "oem_add_flux_plane",
+           "oem_add_field_box", "oem_prepare",
            "oem_run", "oem_profile_run", "oem_timed_run", "oem_step_index", "oem_read_monitor",
            "oem_read_field", "oem_write_field", "oem_set_chunk", "oem_set_energy_tables",
-           "oem_energy", "oem_set_flux_weights", "oem_monitor_flux", "oem_pool_create",
+           "oem_energy", "oem_set_flux_weights", "oem_monitor_flux", "oem_set_overlap_weights",
+           "oem_monitor_overlap", "oem_pool_create",
            "oem_pool_destroy", "oem_pool_ctx_create", "oem_pool_attach", "oem_pool_launch",
            "oem_pool_ctx_done", "oem_pool_wait", "oem_pool_stats", "oem_host_factorize")
 
@@ -68,6 +70,7 @@ _SIGS = {
                         C.c_int),
     "oem_add_tfsf_points": ([_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, C.c_int32, _P],
                             C.c_int),
+    "oem_add_mode_points": ([_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P], C.c_int),
     "oem_add_flux_plane": ([_P, C.c_int32, C.c_int32, C.c_int32, _P, _P], C.c_int),
     "oem_add_field_box": ([_P, _P, _P, C.c_int32, _P, _P], C.c_int),
     "oem_prepare": ([_P], C.c_int),
@@ -83,6 +86,8 @@ _SIGS = {
     "oem_energy": ([_P, _P], C.c_int),
     "oem_set_flux_weights": ([_P, C.c_int32, _P, _P], C.c_int),
     "oem_monitor_flux": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "oem_set_overlap_weights": ([_P, C.c_int32, C.c_int32, _P], C.c_int),
+    "oem_monitor_overlap": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "oem_pool_create": ([C.c_int32, C.c_uint64, C.c_int32, _P], C.c_int),
     "oem_pool_destroy": ([_P], C.c_int),
     "oem_pool_ctx_create": ([_P, _P, _P], C.c_int),

@@ -37,10 +37,27 @@ struct Monitor {
 };
 
 // One grouped point list (dipoles or TF/SF face corrections) and which field it updates.
+enum { PT_DIPOLE = 0, PT_TFSF = 1, PT_MODE = 2 };
+
 struct Points {
     PointsDev d;
     int magnetic;       // 1: applied to H after update_h; 0: to E after update_e
+    int kind;           // PT_*: launch order H: dipole, TF/SF, mode; E: TF/SF, mode, dipole
 };
+
+// Kernel family of a point list (profiling ids of include/openem_hip.h).
+inline int points_kid(const Points &p) {
+    if (p.kind == PT_MODE) return p.magnetic ? OEM_K_MODE_H : OEM_K_MODE_E;
+    if (p.kind == PT_TFSF) return p.magnetic ? OEM_K_TFSF_H : OEM_K_TFSF_E;
+    return p.magnetic ? OEM_K_DIPOLE_H : OEM_K_DIPOLE_E;
+}
+
+// Position of a point list in the step (0..5): H side dipole, TF/SF, mode; E side TF/SF,
+// mode, dipole -- the reference's order (`openem/solver.py:876-968`).
+inline int points_slot(const Points &p) {
+    static const int h[3] = {0, 1, 2}, e[3] = {5, 3, 4};
+    return p.magnetic ? h[p.kind] : e[p.kind];
+}
 
 // Arena of a serving pool (oem_pool.hip).
 int pool_arena_alloc(oem_pool *pool, size_t bytes, void **out);
@@ -56,6 +73,9 @@ void reduce_state_free(oem_reduce_state *r);
 using oemi::fail;
 using oemi::Monitor;
 using oemi::Points;
+using oemi::PT_DIPOLE;
+using oemi::PT_MODE;
+using oemi::PT_TFSF;
 
 #define HIP_OK(expr)                                                                     \
     do {                                                                                 \

@@ -28,7 +28,7 @@
 
 #define OEM_POOL_MAX 64             // attached scenes
 #define OEM_POOL_MAX_MON 256        // monitors over all attached scenes
-#define OEM_POOL_SLOTS 4            // point kinds: magnetic dipoles, TF/SF H, TF/SF E, dipoles E
+#define OEM_POOL_SLOTS 6            // point lists in step order (points_slot)
 #define OEM_POOL_STAGES 4           // pinned staging buffers for descriptor uploads
 #define OEM_ARENA_ALIGN 256
 
@@ -346,7 +346,7 @@ void reap(oem_pool *pool) {
 int build_desc(oem_pool *pool, PoolDev *h) {
     std::memset(h, 0, sizeof(PoolDev));
     int ns = 0, items = 0, dsp = 0, nm = 0, snap = 0, flush = 0;
-    int npt[OEM_POOL_SLOTS] = {0, 0, 0, 0}, pto[OEM_POOL_SLOTS] = {0, 0, 0, 0};
+    int npt[OEM_POOL_SLOTS] = {}, pto[OEM_POOL_SLOTS] = {};
     for (oem_ctx *c : pool->members) {
         if (c->step >= c->dims.n_steps) continue;
         if (ns == OEM_POOL_MAX) return fail(OEM_ENOTSUP, "more than %d running scenes", OEM_POOL_MAX);
@@ -370,7 +370,7 @@ int build_desc(oem_pool *pool, PoolDev *h) {
         h->disp_ofs[ns] = dsp;
         dsp += c->has_disp ? c->disp.n_entry : 0;
         for (auto &pt : c->points) {
-            const int slot = pt.magnetic ? (pt.d.tfsf ? 1 : 0) : (pt.d.tfsf ? 2 : 3);
+            const int slot = points_slot(pt);
             const int q = npt[slot]++;
             h->pt[slot][q].scene = ns;
             h->pt[slot][q].d = pt.d;
@@ -410,12 +410,12 @@ int capture_graph(oem_pool *pool) {
     const dim3 tb(OEM_TK, OEM_TJ), ab(256);
     for (int s = 0; s < pool->T; ++s) {
         hipLaunchKernelGGL(k_pool_update_h, dim3(pool->grid_h), tb, 0, pool->stream, P);
-        hipLaunchKernelGGL(k_pool_points, dim3(pool->grid_aux), ab, 0, pool->stream, P, 0);
-        hipLaunchKernelGGL(k_pool_points, dim3(pool->grid_aux), ab, 0, pool->stream, P, 1);
+        for (int slot = 0; slot < 3; ++slot)
+            hipLaunchKernelGGL(k_pool_points, dim3(pool->grid_aux), ab, 0, pool->stream, P, slot);
         hipLaunchKernelGGL(k_pool_disp, dim3(pool->grid_aux), ab, 0, pool->stream, P, 0);
         hipLaunchKernelGGL(k_pool_update_e, dim3(pool->grid_e), tb, 0, pool->stream, P);
-        hipLaunchKernelGGL(k_pool_points, dim3(pool->grid_aux), ab, 0, pool->stream, P, 2);
-        hipLaunchKernelGGL(k_pool_points, dim3(pool->grid_aux), ab, 0, pool->stream, P, 3);
+        for (int slot = 3; slot < 6; ++slot)
+            hipLaunchKernelGGL(k_pool_points, dim3(pool->grid_aux), ab, 0, pool->stream, P, slot);
         hipLaunchKernelGGL(k_pool_disp, dim3(pool->grid_aux), ab, 0, pool->stream, P, 1);
         hipLaunchKernelGGL(k_pool_snap, dim3(pool->grid_aux), ab, 0, pool->stream, P);
         hipLaunchKernelGGL(k_pool_advance, dim3(1), dim3(OEM_POOL_MAX), 0, pool->stream, P);
@@ -559,11 +559,9 @@ extern "C" int oem_pool_attach(oem_pool *pool, oem_ctx *ctx) {
     if (!ctx->prepared) return fail(OEM_ESTATE, "oem_prepare not called");
     if (ctx->attached) return fail(OEM_ESTATE, "already attached");
     if (ctx->step != 0) return fail(OEM_ESTATE, "context already stepped %lld times", (long long)ctx->step);
-    int per[OEM_POOL_SLOTS] = {0, 0, 0, 0};
-    for (auto &pt : ctx->points) {
-        const int slot = pt.magnetic ? (pt.d.tfsf ? 1 : 0) : (pt.d.tfsf ? 2 : 3);
-        if (++per[slot] > 1) return fail(OEM_ENOTSUP, "more than one point list of one kind");
-    }
+    int per[OEM_POOL_SLOTS] = {};
+    for (auto &pt : ctx->points)
+        if (++per[points_slot(pt)] > 1) return fail(OEM_ENOTSUP, "more than one point list of one kind");
     HIP_OK(hipStreamSynchronize(ctx->stream));    // its uploads have landed
     std::lock_guard<std::mutex> lk(pool->mu);
     int running = 0, mons = 0;

@@ -43,11 +43,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
         }
 }
 
-__device__ __forceinline__ float sq(float x) { return x * x; }
-__device__ __forceinline__ float sq(cf2 x) { return x.r * x.r + x.i * x.i; }
+// Terms rounded as separate float32 operations (no fused multiply-add), so they are the exact
+// float32 values w * a * a and w * (re * re + im * im).
 template <typename F> __device__ __forceinline__ float wsq(float w, F x);
-template <> __device__ __forceinline__ float wsq<float>(float w, float x) { return w * x * x; }
-template <> __device__ __forceinline__ float wsq<cf2>(float w, cf2 x) { return w * sq(x); }
+template <> __device__ __forceinline__ float wsq<float>(float w, float x) {
+    return __fmul_rn(__fmul_rn(w, x), x);
+}
+template <> __device__ __forceinline__ float wsq<cf2>(float w, cf2 x) {
+    return __fmul_rn(w, __fadd_rn(__fmul_rn(x.r, x.r), __fmul_rn(x.i, x.i)));
+}
 
 // partial[3 * block + q]: q = 0 sum V|E|^2, 1 sum V eps |E|^2, 2 sum V |H|^2, with V the E
 // component's Yee dual volume rounded to float32 and each term formed in float32.
@@ -77,7 +81,7 @@ k_energy(const F *__restrict__ E, const F *__restrict__ H, EnergyDev d, int nx, 
         for (int c = 0; c < 3; ++c) {
             const F e = E[(size_t)c * ncell + id], h = H[(size_t)c * ncell + id];
             acc[0] += (double)wsq<F>(v[c], e);
-            acc[1] += (double)wsq<F>(v[c] * ep[c], e);
+            acc[1] += (double)wsq<F>(__fmul_rn(v[c], ep[c]), e);
             acc[2] += (double)wsq<F>(v[c], h);
         }
     }
@@ -134,6 +138,33 @@ __global__ void k_sum_flux(const double *__restrict__ partial, int nblk, int nf,
     out[t] = s;
 }
 
+// Weighted complex sums of a plane monitor's phasors: for set s and frequency f (grid.y =
+// s * nf + f), sum over points q and components c of D_c[f][q] * G[s][c][f][q] (complex), with
+// D the monitor's float64 accumulators and G (re, im) pairs; partial[(y * nblk + b) * 2 + {re, im}].
+__global__ void __launch_bounds__(OEM_RED_THREADS)
+k_overlap_partial(const double *__restrict__ re, const double *__restrict__ im,
+                  const double *__restrict__ g, int nf, int npts, double *__restrict__ partial)
+{
+    const int y = blockIdx.y, f = y % nf, s = y / nf;
+    const size_t plane = (size_t)nf * npts;
+    double acc[2] = {0.0, 0.0};
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npts; q += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const size_t o = c * plane + (size_t)f * npts + q;
+            const double dr = re[o], di = im[o];
+            const double *w = g + 2 * (((size_t)s * 4 + c) * plane + (size_t)f * npts + q);
+            acc[0] += dr * w[0] - di * w[1];
+            acc[1] += dr * w[1] + di * w[0];
+        }
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        partial[((size_t)y * gridDim.x + blockIdx.x) * 2] = acc[0];
+        partial[((size_t)y * gridDim.x + blockIdx.x) * 2 + 1] = acc[1];
+    }
+}
+
 }  // namespace
 
 // Reduction state kept on the context: tables for the energies, flux weights per monitor and
@@ -142,6 +173,7 @@ struct oem_reduce_state {
     EnergyDev ed{};
     bool energy_ready = false;
     std::vector<std::pair<double *, double *>> flux_w;   // per monitor (w1, w2) or nulls
+    std::vector<std::pair<double *, int>> over_w;        // per monitor (G, n_sets) or null
     double *partial = nullptr;                           // 3 * OEM_RED_BLOCKS doubles at least
     size_t partial_n = 0;
     double *out = nullptr;                               // small result buffer
@@ -256,6 +288,46 @@ extern "C" int oem_monitor_flux(oem_ctx *ctx, int32_t mon_id, double *out, size_
                        m.npts, r->partial);
     hipLaunchKernelGGL(k_sum_flux, dim3((2 * m.nf + 255) / 256), dim3(256), 0, ctx->stream,
                        r->partial, nblk, m.nf, r->out);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, r->out, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return OEM_OK;
+}
+
+extern "C" int oem_set_overlap_weights(oem_ctx *ctx, int32_t mon_id, int32_t n_sets,
+                                       const double *g) {
+    CHECK_CTX(ctx);
+    if (mon_id < 0 || mon_id >= (int)ctx->mons.size()) return fail(OEM_EINVAL, "monitor %d", mon_id);
+    const MonDev &m = ctx->mons[mon_id].d;
+    if (m.kind != 0) return fail(OEM_EINVAL, "monitor %d is not a plane", mon_id);
+    if (!g || n_sets < 1 || n_sets > 16) return fail(OEM_EINVAL, "bad overlap weights (n_sets %d)", n_sets);
+    oem_reduce_state *r;
+    TRY(red_state(ctx, &r));
+    if (r->over_w.size() < ctx->mons.size()) r->over_w.resize(ctx->mons.size(), {nullptr, 0});
+    double *p;
+    TRY(ctx->upload(&p, g, (size_t)n_sets * 4 * m.nf * m.npts * 2));
+    r->over_w[mon_id] = {p, n_sets};
+    return OEM_OK;
+}
+
+extern "C" int oem_monitor_overlap(oem_ctx *ctx, int32_t mon_id, double *out, size_t count) {
+    CHECK_CTX(ctx);
+    if (mon_id < 0 || mon_id >= (int)ctx->mons.size()) return fail(OEM_EINVAL, "monitor %d", mon_id);
+    const MonDev &m = ctx->mons[mon_id].d;
+    oem_reduce_state *r = ctx->red;
+    if (!r || (int)r->over_w.size() <= mon_id || !r->over_w[mon_id].first)
+        return fail(OEM_ESTATE, "overlap weights of monitor %d not set", mon_id);
+    const int ns = r->over_w[mon_id].second;
+    if (!out || count != 2 * (size_t)ns * m.nf)
+        return fail(OEM_EINVAL, "monitor %d: output holds 2*n_sets*nf = %d doubles", mon_id, 2 * ns * m.nf);
+    if (ctx->pool) TRY(oemi::pool_wait_done(ctx));
+    const int nblk = std::max(1, std::min(64, (m.npts + OEM_RED_THREADS - 1) / OEM_RED_THREADS));
+    const int ny = ns * m.nf;
+    TRY(red_scratch(ctx, r, std::max((size_t)3 * OEM_RED_BLOCKS, (size_t)2 * nblk * ny)));
+    hipLaunchKernelGGL(k_overlap_partial, dim3(nblk, ny), dim3(OEM_RED_THREADS), 0, ctx->stream,
+                       m.re, m.im, r->over_w[mon_id].first, m.nf, m.npts, r->partial);
+    hipLaunchKernelGGL(k_sum_flux, dim3((2 * ny + 255) / 256), dim3(256), 0, ctx->stream,
+                       r->partial, nblk, ny, r->out);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(out, r->out, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));

@@ -296,7 +296,7 @@ extern "C" int oem_add_plane_source(oem_ctx *ctx, int32_t pol, int32_t kh, int32
 }
 
 // Group points by (comp, cell), keeping input order inside a group, upload, and append.
-static int add_points(oem_ctx *ctx, int magnetic, int tfsf, int cplx_amp, int32_t n_pts,
+static int add_points(oem_ctx *ctx, int magnetic, int kind, int tfsf, int cplx_amp, int32_t n_pts,
                       const int32_t *comp, const int32_t *cell, const float *coef,
                       const int32_t *col, const float *phase, int32_t ncol, const double *tab,
                       size_t tab_doubles) {
@@ -324,6 +324,7 @@ static int add_points(oem_ctx *ctx, int magnetic, int tfsf, int cplx_amp, int32_
     go.push_back(n_pts);
     Points P{};
     P.magnetic = magnetic ? 1 : 0;
+    P.kind = kind;
     PointsDev &d = P.d;
     d.n_groups = (int)gc.size();
     d.nsteps1 = ctx->dims.n_steps + 1;
@@ -366,8 +367,8 @@ extern "C" int oem_add_dipoles(oem_ctx *ctx, int32_t magnetic, int32_t n_pts,
     if (n_src < 1 || !amp) return fail(OEM_EINVAL, "bad dipole amplitude table");
     TRY(check_points(ctx, n_pts, comp, cell, coef, src_of, n_src));
     const size_t nd = (size_t)n_src * (ctx->dims.n_steps + 1) * (amp_complex ? 2 : 1);
-    return add_points(ctx, magnetic, 0, amp_complex ? 1 : 0, n_pts, comp, cell, coef, src_of,
-                      nullptr, n_src, amp, nd);
+    return add_points(ctx, magnetic, PT_DIPOLE, 0, amp_complex ? 1 : 0, n_pts, comp, cell, coef,
+                      src_of, nullptr, n_src, amp, nd);
 }
 
 extern "C" int oem_add_tfsf_points(oem_ctx *ctx, int32_t magnetic, int32_t n_pts,
@@ -383,7 +384,21 @@ extern "C" int oem_add_tfsf_points(oem_ctx *ctx, int32_t magnetic, int32_t n_pts
             if (phase[2 * q + 1] != 0.f)
                 return fail(OEM_EINVAL, "point %d: complex phase needs complex fields", q);
     const size_t nd = (size_t)(ctx->dims.n_steps + 1) * ncol * 2;
-    return add_points(ctx, magnetic, 1, 1, n_pts, comp, cell, coef, col, phase, ncol, table, nd);
+    return add_points(ctx, magnetic, PT_TFSF, 1, 1, n_pts, comp, cell, coef, col, phase, ncol,
+                      table, nd);
+}
+
+extern "C" int oem_add_mode_points(oem_ctx *ctx, int32_t magnetic, int32_t n_pts,
+                                   const int32_t *comp, const int32_t *cell, const float *coef,
+                                   const float *phase, const double *amp) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (!phase || !amp) return fail(OEM_EINVAL, "bad mode tables");
+    std::vector<int32_t> col(n_pts > 0 ? n_pts : 0, 0);
+    TRY(check_points(ctx, n_pts, comp, cell, coef, col.data(), 1));
+    const size_t nd = (size_t)(ctx->dims.n_steps + 1) * 2;
+    return add_points(ctx, magnetic, PT_MODE, 1, 1, n_pts, comp, cell, coef, col.data(), phase, 1,
+                      amp, nd);
 }
 
 static int add_monitor(oem_ctx *ctx, MonDev m, const double *phase, int32_t *mon_id) {
@@ -555,13 +570,14 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
     hipLaunchKernelGGL((k_update_h<F>), yg, yb, 0, c->stream, H, (const F *)E, c->psi, c->g,
                        c->dims.ch, c->pw, c->d_step);
     TRY(mark(OEM_K_UPDATE_H, false));
-    for (auto &pt : c->points) {
-        if (!pt.magnetic) continue;
-        TRY(mark(pt.d.tfsf ? OEM_K_TFSF_H : OEM_K_DIPOLE_H, true));
-        hipLaunchKernelGGL((k_points<F>), dim3(blocks_for(pt.d.n_groups, 256)), dim3(256), 0,
-                           c->stream, H, pt.d, c->mat, c->dims.ch, nc, c->d_step);
-        TRY(mark(pt.d.tfsf ? OEM_K_TFSF_H : OEM_K_DIPOLE_H, false));
-    }
+    for (int slot = 0; slot < 3; ++slot)
+        for (auto &pt : c->points) {
+            if (points_slot(pt) != slot) continue;
+            TRY(mark(points_kid(pt), true));
+            hipLaunchKernelGGL((k_points<F>), dim3(blocks_for(pt.d.n_groups, 256)), dim3(256), 0,
+                               c->stream, H, pt.d, c->mat, c->dims.ch, nc, c->d_step);
+            TRY(mark(points_kid(pt), false));
+        }
     const dim3 db(256), dg(c->has_disp ? blocks_for(c->disp.n_entry, 256) : 1);
     if (c->has_disp && !c->pre_ready) {
         TRY(mark(OEM_K_DISP_PRE, true));
@@ -573,17 +589,16 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
     hipLaunchKernelGGL((k_update_e<F>), yg, yb, 0, c->stream, E, (const F *)H, c->psi, c->g,
                        c->mat, c->dims.cw, c->pw, c->d_step);
     TRY(mark(OEM_K_UPDATE_E, false));
-    // E-side corrections: TF/SF faces first, then electric dipoles (reference order,
-    // `openem/solver.py:960-970`)
-    for (int pass = 0; pass < 2; ++pass) {
+    // E-side corrections: TF/SF faces, mode sources, then electric dipoles (reference order,
+    // `openem/solver.py:945-968`)
+    for (int slot = 3; slot < 6; ++slot)
         for (auto &pt : c->points) {
-            if (pt.magnetic || (pass == 0) != (pt.d.tfsf == 1)) continue;
-            TRY(mark(pt.d.tfsf ? OEM_K_TFSF_E : OEM_K_DIPOLE_E, true));
+            if (points_slot(pt) != slot) continue;
+            TRY(mark(points_kid(pt), true));
             hipLaunchKernelGGL((k_points<F>), dim3(blocks_for(pt.d.n_groups, 256)), dim3(256), 0,
                                c->stream, E, pt.d, c->mat, c->dims.ch, nc, c->d_step);
-            TRY(mark(pt.d.tfsf ? OEM_K_TFSF_E : OEM_K_DIPOLE_E, false));
+            TRY(mark(points_kid(pt), false));
         }
-    }
     if (c->has_disp) {
         TRY(mark(OEM_K_DISP_POST, true));
         hipLaunchKernelGGL((k_disp<F>), dg, db, 0, c->stream, c->disp, (const F *)E, nc,

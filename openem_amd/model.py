@@ -125,6 +125,20 @@ class FluxMonitor:
 
 
 @dataclass(frozen=True)
+class ModeMonitor:
+    """Guided-mode amplitudes on a plane (the reference reads tidy3d ModeMonitors from the
+    plane phasors in ``nb/backend._mode_data``, ``openem/nb/backend.py:1422``): the phasors are
+    projected on the ``mode_index``-th mode solved at every frequency."""
+
+    name: str
+    axis: int
+    plane_index: int
+    freqs: np.ndarray
+    mode_index: int = 0
+    n_guess: float | None = None
+
+
+@dataclass(frozen=True)
 class Dispersion:
     """Complex-pole ADE data in CSR form (reference ``openem/model.py:462-527``).
 
@@ -204,6 +218,7 @@ class Scene:
     mode_sources: list = field(default_factory=list)
     flux_monitors: list = field(default_factory=list)
     field_monitors: list = field(default_factory=list)
+    mode_monitors: list = field(default_factory=list)
 
     @property
     def shape(self) -> tuple[int, int, int]:

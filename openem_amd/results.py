@@ -19,6 +19,7 @@ class Phasors:
 class Result:
     phasors: dict = field(default_factory=dict)
     field_phasors: dict = field(default_factory=dict)
+    mode_amps: dict = field(default_factory=dict)      # name -> a_plus, a_minus, freqs, n_eff
     steps_run: int = 0
     wall_seconds: float = 0.0
     fields: np.ndarray | None = None

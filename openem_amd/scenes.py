@@ -19,7 +19,8 @@ import numpy as np
 
 from . import cpml, waveform
 from .grid import Axis, Grid, nearest_edge_index, uniform_axis
-from .model import (Dispersion, FieldMonitor, FluxMonitor, PlaneWaveSource, PointDipole, Scene,
+from .model import (Dispersion, FieldMonitor, FluxMonitor, ModeMonitor, PlaneWaveSource,
+                    PointDipole, Scene,
                     lorentz_poles, trap_coeffs)
 
 UM = 1e-6
@@ -536,3 +537,44 @@ def _sphere_scene(side: int, ns: int, spec: dict, rng, dl: float, n_pml: int) ->
     return Scene(grid=grid, dt=dt, num_time_steps=ns, eps_ex=eps, eps_ey=eps, eps_ez=eps,
                  pml=pml_faces(dt, (dl, dl, dl), sides_all, n_pml), dispersion=disp,
                  dipoles=[dip], flux_monitors=_flux_planes(grid.shape, spec["n_mon"], n_pml, freqs))
+
+
+# ---------------------------------------------------------------------------------------------
+# Waveguides: mode source + mode monitors
+# ---------------------------------------------------------------------------------------------
+
+def waveguide_scene(shape=(60, 40, 30), dl: float = 40e-9, n_pml: int = 8,
+                    core_eps: float = 12.1, clad_eps: float = 2.1, width_cells: int = 12,
+                    thick_cells: int = 6, f0: float = 1.934e14, fwidth_rel: float = 0.1,
+                    src_index: int | None = None, num_steps: int = 1200, nf: int = 3,
+                    direction: int = +1, mode_index: int = 0) -> Scene:
+    """A strip waveguide along x (core ``width_cells`` x ``thick_cells`` of ``core_eps`` in
+    ``clad_eps``), PML on all faces, the fundamental mode launched from ``src_index`` along
+    ``direction`` (GaussianPulse f0, fwidth ``fwidth_rel`` f0), and mode + flux monitors on a
+    plane ahead of the source ("fwd") and one behind it ("back")."""
+    import dataclasses
+    from . import modes
+    nx, ny, nz = shape
+    axes = [uniform_axis(m, dl, 0.0, "PML") for m in shape]
+    grid = Grid(*axes)
+    dt = cpml.courant_dt((dl, dl, dl))
+    eps = np.full(shape, clad_eps)
+    yc, zc = ny // 2, nz // 2
+    eps[:, yc - width_cells // 2: yc - width_cells // 2 + width_cells,
+        zc - thick_cells // 2: zc - thick_cells // 2 + thick_cells] = core_eps
+    ks = src_index if src_index is not None else (n_pml + 6 if direction > 0 else nx - n_pml - 7)
+    k_fwd = ks + direction * (nx - 2 * n_pml - 14) // 1
+    k_fwd = min(max(k_fwd, n_pml + 2), nx - n_pml - 3)
+    k_back = ks - direction * 3
+    freqs = np.linspace(f0 * (1 - 0.05), f0 * (1 + 0.05), nf) if nf > 1 else np.array([f0])
+    sides_all = [(a, s) for a in range(3) for s in ("lo", "hi")]
+    base = Scene(grid=grid, dt=dt, num_time_steps=num_steps, eps_ex=eps, eps_ey=eps, eps_ez=eps,
+                 pml=pml_faces(dt, (dl, dl, dl), sides_all, n_pml))
+    m = modes.solve_mode(base, 0, ks, f0, mode_index=mode_index)
+    pulse = waveform.GaussianPulse(freq0=f0, fwidth=fwidth_rel * f0)
+    src = modes.mode_source(base, m, pulse, direction, num_steps)
+    mons = [ModeMonitor("fwd", 0, k_fwd, freqs, mode_index),
+            ModeMonitor("back", 0, k_back, freqs, mode_index)]
+    flux = [FluxMonitor("fwd_flux", 0, k_fwd, +1, freqs),
+            FluxMonitor("back_flux", 0, k_back, +1, freqs)]
+    return dataclasses.replace(base, mode_sources=[src], mode_monitors=mons, flux_monitors=flux)

@@ -125,6 +125,17 @@ class Stepper:
                 _lib.check(lib, lib.oem_add_tfsf_points(
                     ctx, magnetic, comp.size, *[_lib.ptr(x) for x in arrs[:5]], ncol,
                     _lib.ptr(arrs[5])))
+        for mt in t.modes:
+            for magnetic, (comp, cell, coef, ph, amp) in (
+                    (1, (mt.h_comp, mt.h_cell, mt.h_coef, mt.h_phase, mt.amp_e)),
+                    (0, (mt.e_comp, mt.e_cell, mt.e_coef, mt.e_phase, mt.amp_h))):
+                if comp.size == 0:
+                    continue
+                phase = np.ascontiguousarray(ph, dtype=np.complex64).view(np.float32)
+                table = np.ascontiguousarray(amp, dtype=np.complex128).view(np.float64)
+                arrs = [_i32(comp), _i32(cell), _f32(coef), phase, table]
+                _lib.check(lib, lib.oem_add_mode_points(ctx, magnetic, comp.size,
+                                                        *[_lib.ptr(x) for x in arrs]))
         self.mon_ids = []
         for m in t.monitors:
             ph = _f64(m.phase)
@@ -242,6 +253,28 @@ class Stepper:
         t = self.flux_terms(i)
         return 0.5 * (t[:, 0] - t[:, 1])
 
+    def setup_modes(self) -> None:
+        """Upload the overlap weights of every mode monitor."""
+        from . import modes as modes_mod
+        for i, m in enumerate(self.tabs.monitors):
+            if not m.modes:
+                continue
+            g = modes_mod.overlap_weight_sets(m.modes, self.tabs.scene.grid)
+            g = np.ascontiguousarray(g, dtype=np.complex128).view(np.float64)
+            _lib.check(self.lib, self.lib.oem_set_overlap_weights(self.ctx, self.mon_ids[i], 2,
+                                                                  _lib.ptr(g)))
+
+    def mode_amplitudes(self, i: int):
+        """``(a_plus, a_minus)`` ``(nf,)`` complex of mode monitor ``i`` (overlaps reduced on
+        the device from the float64 accumulators; the mode carries unit power)."""
+        from . import modes as modes_mod
+        m = self.tabs.monitors[i]
+        out = np.zeros((2, m.nf, 2), np.float64)
+        _lib.check(self.lib, self.lib.oem_monitor_overlap(self.ctx, self.mon_ids[i], _lib.ptr(out),
+                                                          out.size))
+        s = modes_mod.orientation(m.axis) * (out[..., 0] + 1j * out[..., 1])
+        return 0.25 * (s[0] + s[1]), 0.25 * (s[0] - s[1])
+
     def close(self) -> None:
         if getattr(self, "ctx", None) and self.ctx.value:
             self.lib.oem_ctx_destroy(self.ctx)
@@ -300,13 +333,19 @@ def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kerne
             st.run(n_total)
             steps_run = n_total
         wall = time.time() - t1
-        phasors, fphas = {}, {}
+        phasors, fphas, mamps = {}, {}, {}
+        if any(m.modes for m in tabs.monitors):
+            st.setup_modes()
         for i, m in enumerate(tabs.monitors):
             data = st.read_monitor(i)
             if m.kind == "plane":
                 phasors[m.name] = Phasors(data=data, freqs=m.freqs)
             else:
                 fphas[m.name] = data
+            if m.modes:
+                ap, am = st.mode_amplitudes(i)
+                mamps[m.name] = dict(a_plus=ap, a_minus=am, freqs=m.freqs,
+                                     n_eff=np.array([x.n_eff for x in m.modes]))
         fields = None
         if return_fields:
             fields = np.stack([st.read_field(c) for c in range(6)])
@@ -316,7 +355,7 @@ def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kerne
         print(f"[openem_amd] {sc.shape} x {steps_run} steps: setup {t1 - t0:.2f}s, "
               f"stepping {wall:.2f}s" + (f", stopped: {reason}" if triggered else ""))
     _warn_nonfinite(phasors, fphas)
-    return Result(phasors=phasors, field_phasors=fphas, steps_run=steps_run,
+    return Result(phasors=phasors, field_phasors=fphas, mode_amps=mamps, steps_run=steps_run,
                   wall_seconds=wall, fields=fields, shutoff_triggered=triggered,
                   stop_reason=reason, decay_trace=decay, energy_trace=energy_tr)
 

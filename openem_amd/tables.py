@@ -93,6 +93,7 @@ class MonitorTables:
     n2: int = 0
     origin: tuple = (0, 0, 0)
     box: tuple = (0, 0, 0)
+    modes: list | None = None   # mode monitors: one solved Mode per frequency
 
 
 @dataclass
@@ -109,6 +110,24 @@ class DispTables:
 
 
 @dataclass
+class ModeTables:
+    """Point lists of one mode source (H side after the TF/SF H corrections, E side after the
+    TF/SF E corrections) and its two amplitude tables (reference ``_mode_source_entry``,
+    ``openem/sources_setup.py:376-416``)."""
+
+    h_comp: np.ndarray
+    h_cell: np.ndarray
+    h_coef: np.ndarray            # float64: face sign * dt/mu0/dl
+    h_phase: np.ndarray           # complex128 profile value
+    e_comp: np.ndarray
+    e_cell: np.ndarray
+    e_coef: np.ndarray            # float64: face sign / dl_dual (times the cell's cb on device)
+    e_phase: np.ndarray
+    amp_e: np.ndarray             # (n_total+1,) complex128, amp_time(n dt)
+    amp_h: np.ndarray             # (n_total+1,) complex128, amp_time((n+1/2) dt)
+
+
+@dataclass
 class StepTables:
     shape: tuple
     n_total: int
@@ -122,6 +141,7 @@ class StepTables:
     dipoles: list = field(default_factory=list)
     monitors: list = field(default_factory=list)
     tfsf: list = field(default_factory=list)
+    modes: list = field(default_factory=list)
     bloch: bool = False
     coords: list | None = None   # per axis (edge coordinates, centre coordinates)
 
@@ -469,6 +489,64 @@ def monitor_tables(sc: Scene, n_total: int) -> list[MonitorTables]:
         out.append(MonitorTables(
             name=m.name, kind="box", nf=len(m.freqs), freqs=np.asarray(m.freqs, np.float64),
             phase=phase_table(m.freqs, sc.dt, n_total, m.apodization), origin=o, box=b))
+    if sc.mode_monitors:
+        from . import modes as modes_mod
+    for m in sc.mode_monitors:
+        t1, t2 = transverse_axes(m.axis)
+        if not (1 <= m.plane_index < shape[m.axis]):
+            raise ValueError(f"mode monitor {m.name}: plane {m.plane_index} needs a cell below it")
+        sol = [modes_mod.solve_mode(sc, m.axis, m.plane_index, float(f), m.mode_index, m.n_guess)
+               for f in np.asarray(m.freqs, dtype=np.float64)]
+        out.append(MonitorTables(
+            name=m.name, kind="plane", nf=len(m.freqs), freqs=np.asarray(m.freqs, np.float64),
+            phase=phase_table(m.freqs, sc.dt, n_total, None), axis=int(m.axis),
+            km=int(m.plane_index), n1=shape[t1], n2=shape[t2], modes=sol))
+    return out
+
+
+def mode_tables(sc: Scene, n_total: int) -> list[ModeTables]:
+    """Point lists of every mode source (zero profile entries dropped)."""
+    from . import modes as modes_mod
+    out = []
+    for ms in sc.mode_sources:
+        a = int(ms.axis)
+        u, v = modes_mod.cyclic(a)
+        if ms.direction not in (+1, -1):
+            raise ValueError(f"mode source direction {ms.direction}")
+        ks = int(ms.plane_index)
+        kh = ks - 1 if ms.direction > 0 else ks
+        if not (1 <= ks < sc.shape[a] - 1):
+            raise ValueError(f"mode source plane {ks} needs a cell on both sides")
+        shp = (sc.shape[u], sc.shape[v])
+        for arr in (ms.ey_inc, ms.ez_inc, ms.hy_inc, ms.hz_inc):
+            if np.shape(arr) != shp:
+                raise ValueError(f"mode profile {np.shape(arr)} != plane {shp}")
+        if ms.amp_e.size < n_total + 1 or ms.amp_h.size < n_total + 1:
+            raise ValueError("mode source amplitude tables are shorter than the run")
+        s_hu, s_hv, s_eu, s_ev = modes_mod.MODE_SIGNS[ms.direction]
+        ax = sc.grid.axes[a]
+        ch = sc.dt / cpml.MU_0 / float(ax.dl[kh])
+        ce = 1.0 / float(ax.dl_dual[ks])
+        cells_h = modes_mod.plane_cells(sc.shape, a, kh).ravel()
+        cells_e = modes_mod.plane_cells(sc.shape, a, ks).ravel()
+        lists = {"h": ([], [], [], []), "e": ([], [], [], [])}
+        for side, comp, prof, coef, cells in (("h", u, ms.ez_inc, s_hu * ch, cells_h),
+                                              ("h", v, ms.ey_inc, s_hv * ch, cells_h),
+                                              ("e", u, ms.hz_inc, s_eu * ce, cells_e),
+                                              ("e", v, ms.hy_inc, s_ev * ce, cells_e)):
+            pr = np.asarray(prof, dtype=np.complex128).ravel()
+            keep = np.flatnonzero(pr != 0)
+            L = lists[side]
+            L[0].append(np.full(keep.size, comp, np.int32))
+            L[1].append(cells[keep].astype(np.int32))
+            L[2].append(np.full(keep.size, coef))
+            L[3].append(pr[keep])
+        h = [np.concatenate(x) for x in lists["h"]]
+        e = [np.concatenate(x) for x in lists["e"]]
+        out.append(ModeTables(h_comp=h[0], h_cell=h[1], h_coef=h[2], h_phase=h[3],
+                              e_comp=e[0], e_cell=e[1], e_coef=e[2], e_phase=e[3],
+                              amp_e=np.ascontiguousarray(ms.amp_e[:n_total + 1], np.complex128),
+                              amp_h=np.ascontiguousarray(ms.amp_h[:n_total + 1], np.complex128)))
     return out
 
 
@@ -489,5 +567,6 @@ def build(sc: Scene, n_total: int | None = None, force_complex: bool = False) ->
         cw=cpml.EPSILON_0 / sc.dt, axes=axis_tables(sc), scene=sc, disp=disp_tables(sc), plane_srcs=plane_source_tables(sc),
         dipoles=dipole_tables(sc, n_total), monitors=monitor_tables(sc, n_total),
         tfsf=[tfsf_mod.tables(sc, s, n_total) for s in sc.tfsf_sources],
+        modes=mode_tables(sc, n_total),
         bloch=sc.any_bloch or force_complex,
         coords=[(a.edges[:-1], a.centers) for a in sc.grid.axes])

@@ -21,6 +21,9 @@ It restates the reference's step term by term, on the float64 master tables that
   ``H -= ch*coef*amp_int[n]``);
 - running DFT: ``source_dft.cu:216-271`` (box, native Yee positions) and ``:371-430`` (plane,
   H colocated as ``(H[k]+H[k-1])/2``), E at ``(n+1)dt``, H at ``(n+1/2)dt``;
+- mode sources (:meth:`Oracle64._mode`): ``inject_mode_h`` / ``inject_mode_e`` of
+  ``openem/kernels/source_dft.cu:84-128`` on plane slices (H side after the TF/SF H
+  corrections, E side after the TF/SF E corrections, ``openem/solver.py:907-966``);
 - launch order of ``openem/solver.py:1145-1162``;
 - shutoff sums (:func:`energy_sums`): ``openem/solver.py:107-122`` and ``:162-177`` with the
   float32 dual volumes and eps of ``openem/device_tables.py:320-343``; plane flux
@@ -64,6 +67,7 @@ class Oracle64:
 
     def __init__(self, tabs):
         self.t = tabs
+        self.sc = tabs.scene
         shp = tabs.shape
         dt = np.complex128 if tabs.bloch else np.float64
         self.E = np.zeros((3,) + shp, dtype=dt)
@@ -141,6 +145,8 @@ class Oracle64:
         # ---- E phase ----
         for tf in t.tfsf:
             self._tfsf(tf, n, magnetic=True)
+        for ms in self.sc.mode_sources:
+            self._mode(ms, n, magnetic=True)
         d = t.disp
         if d is not None:
             en = E[self._idx4]
@@ -171,6 +177,8 @@ class Oracle64:
             E[ps.pol][:, :, ps.ks] += cb[ps.pol][:, :, ps.ks] * ps.coef_e * ps.hy_inc[n]
         for tf in t.tfsf:
             self._tfsf(tf, n, magnetic=False)
+        for ms in self.sc.mode_sources:
+            self._mode(ms, n, magnetic=False)
         for dp in t.dipoles:
             if not dp.magnetic:
                 for c in range(3):
@@ -185,6 +193,43 @@ class Oracle64:
                 self.Q += d.b * en.imag[self._pole_entry]
         self._dft(n)
         self.n += 1
+
+    def _mode(self, ms, n, magnetic):
+        """Mode-source plane corrections: H side ``H_u += s_hu ch/dl Re(E_v amp_e[n])``,
+        ``H_v += s_hv ch/dl Re(E_u amp_e[n])`` on the H plane; E side
+        ``E_u += cb s_eu/dl_dual Re(H_v amp_h[n])``, ``E_v += cb s_ev/dl_dual Re(H_u amp_h[n])``
+        on the source plane; (s_hu, s_hv, s_eu, s_ev) the TF/SF face signs, (-, +, +, -) for a
+        forward and (+, -, -, +) for a backward source whose profiles carry the incident H."""
+        t = self.t
+        a = int(ms.axis)
+        u, v = (a + 1) % 3, (a + 2) % 3
+        ks = int(ms.plane_index)
+        fwd = ms.direction > 0
+        kh = ks - 1 if fwd else ks
+        s_hu, s_hv, s_eu, s_ev = (-1.0, 1.0, 1.0, -1.0) if fwd else (1.0, -1.0, -1.0, 1.0)
+
+        def plane(F, k, val):                      # F[..] on plane k += val (cyclic layout)
+            idx = [slice(None)] * 3
+            idx[a] = k
+            F[tuple(idx)] += val if u < v else val.T
+
+        if magnetic:
+            amp = ms.amp_e[n]
+            ch = t.ch * t.axes[a].idl_p[kh]
+            take = (lambda z: z) if t.bloch else np.real
+            plane(self.H[u], kh, s_hu * ch * take(ms.ez_inc * amp))
+            plane(self.H[v], kh, s_hv * ch * take(ms.ey_inc * amp))
+        else:
+            amp = ms.amp_h[n]
+            ce = t.axes[a].idl_d[ks]
+            idx = [slice(None)] * 3
+            idx[a] = ks
+            cbu, cbv = t.cb[u][tuple(idx)], t.cb[v][tuple(idx)]
+            if u > v:
+                cbu, cbv = cbu.T, cbv.T
+            take = (lambda z: z) if t.bloch else np.real
+            plane(self.E[u], ks, cbu * s_eu * ce * take(ms.hz_inc * amp))
+            plane(self.E[v], ks, cbv * s_ev * ce * take(ms.hy_inc * amp))
 
     def _tfsf(self, tf, n, magnetic):
         """TF/SF face corrections of ``openem/kernels/tfsf.cu:49-83`` restated per face with
@@ -278,19 +323,30 @@ class Oracle64:
 
 
 def energy_sums(sc, E, H) -> np.ndarray:
-    """``[sum V|E|^2, sum V eps|E|^2, sum V|H|^2]`` in float64 (reference ``_sum2`` over the
-    three components, ``openem/solver.py:107-122``, ``:172-176``): V is each E component's Yee
-    dual volume rounded to float32, eps the component's permittivity rounded to float32, and H
-    is weighted by the same E-component volumes."""
+    """``[sum V|E|^2, sum V eps|E|^2, sum V|H|^2]`` accumulated in float64 (reference ``_sum2``
+    over the three components, ``openem/solver.py:107-122``, ``:172-176``): V is each E
+    component's Yee dual volume rounded to float32, eps the component's permittivity rounded to
+    float32, and H is weighted by the same E-component volumes.  Each term is formed in the
+    fields' precision, as the reference forms ``w * a * a`` on its float32 device arrays; with
+    float64 fields the terms are float64."""
     vols = [v.astype(np.float32) for v in sc.grid.yee_dual_volumes()]
-    eps = [np.asarray(e, dtype=np.float64).astype(np.float32) * np.ones(sc.shape, np.float32)
+    eps = [np.broadcast_to(np.asarray(e, dtype=np.float64).astype(np.float32), sc.shape)
            for e in (sc.eps_ex, sc.eps_ey, sc.eps_ez)]
+    single = E.dtype in (np.float32, np.complex64)
+    rt = np.float32 if single else np.float64
+
+    def term(w, a):
+        w = w.astype(rt)
+        if np.iscomplexobj(a):
+            return (w * (a.real * a.real + a.imag * a.imag)).astype(np.float64)
+        return (w * a * a).astype(np.float64)
+
     out = np.zeros(3)
     for c in range(3):
-        e2 = np.abs(E[c]) ** 2
-        out[0] += float(np.sum(vols[c].astype(np.float64) * e2))
-        out[1] += float(np.sum((vols[c] * eps[c]).astype(np.float64) * e2))
-        out[2] += float(np.sum(vols[c].astype(np.float64) * np.abs(H[c]) ** 2))
+        ve = vols[c] * eps[c]                      # float32 product, as on the device
+        out[0] += float(np.sum(term(vols[c], E[c])))
+        out[1] += float(np.sum(term(ve, E[c])))
+        out[2] += float(np.sum(term(vols[c], H[c])))
     return out
 
 

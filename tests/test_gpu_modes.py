@@ -1,0 +1,65 @@
+"""GPU tests of mode sources and mode monitors.
+
+Tolerances: fields and monitor phasors of the fp32 stepper against the float64 oracle on the
+same tables, 1e-4 of their peak; the device overlap reduction against the host overlap of the
+same float64 phasors, 1e-12 relative.
+"""
+
+import numpy as np
+import pytest
+
+from openem_amd import modes, scenes, solver, tables
+from openem_amd.solver import Stepper
+from oracle.yee64 import Oracle64
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("direction", [+1, -1])
+def test_waveguide_mode_source_matches_oracle(gpu_lib, direction):
+    n = 600
+    sc = scenes.waveguide_scene(shape=(44, 36, 32), n_pml=8, width_cells=10, thick_cells=6,
+                                fwidth_rel=0.2, num_steps=n, nf=2, direction=direction)
+    t = tables.build(sc)
+    print(f"waveguide direction {direction} shape {sc.shape} steps {n}", flush=True)
+    st = Stepper(t)
+    try:
+        st.setup_modes()
+        st.run(n)
+        f = np.stack([st.read_field(c) for c in range(6)]).astype(np.float64)
+        acc = [st.read_monitor(i) for i in range(len(t.monitors))]
+        amps = {i: st.mode_amplitudes(i) for i, m in enumerate(t.monitors) if m.modes}
+    finally:
+        st.close()
+    o = Oracle64(t).run(n)
+    ref = np.concatenate([o.E, o.H])
+    for c in range(6):
+        assert _rel(f[c], ref[c]) < 1e-4, c
+    for i in range(len(t.monitors)):
+        assert _rel(acc[i], o.acc[i]) < 1e-4, t.monitors[i].name
+    for i, (ap, am) in amps.items():
+        m = t.monitors[i]
+        for fi in range(m.nf):
+            hp, hm = modes.amplitudes(acc[i][:, fi:fi + 1], m.modes[fi], sc.grid,
+                                      modes.orientation(m.axis))
+            assert abs(ap[fi] - hp[0]) <= 1e-12 * max(abs(hp[0]), abs(hm[0]))
+            assert abs(am[fi] - hm[0]) <= 1e-12 * max(abs(hp[0]), abs(hm[0]))
+
+
+def test_run_reports_mode_amplitudes(gpu_lib):
+    n = 600
+    sc = scenes.waveguide_scene(shape=(44, 36, 32), n_pml=8, width_cells=10, thick_cells=6,
+                                fwidth_rel=0.2, num_steps=n, nf=1)
+    res = solver.run(sc, use_shutoff=False, verbose=False)
+    ms = sc.mode_sources[0]
+    f0 = res.mode_amps["fwd"]["freqs"][0]
+    k = np.arange(n)
+    S = abs(np.sum(ms.amp_e[1:n + 1] * np.exp(2j * np.pi * f0 * (k + 1) * sc.dt)))
+    ap = abs(res.mode_amps["fwd"]["a_plus"][0]) / S
+    am = abs(res.mode_amps["back"]["a_minus"][0]) / S
+    print(f"|a+|/|S| = {ap:.5f}, backward {am:.2e}", flush=True)
+    assert abs(ap - 0.5) < 5e-3 and am < 1e-3

@@ -30,8 +30,8 @@ def test_energy_and_flux_reductions(gpu_lib, dense):
         st.setup_flux()
         st.run(180)
         e_gpu = st.energy()
-        E = np.stack([st.read_field(c) for c in range(3)]).astype(np.float64)
-        H = np.stack([st.read_field(c) for c in range(3, 6)]).astype(np.float64)
+        E = np.stack([st.read_field(c) for c in range(3)])          # float32, as stepped
+        H = np.stack([st.read_field(c) for c in range(3, 6)])
         terms = {i: st.flux_terms(i) for i, m in enumerate(t.monitors) if m.kind == "plane"}
         data = {i: st.read_monitor(i) for i in terms}
     finally:
