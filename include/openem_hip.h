@@ -40,7 +40,7 @@ enum oem_kernel_id {
     OEM_K_UPDATE_H = 0, OEM_K_UPDATE_E = 1, OEM_K_DIPOLE_E = 2, OEM_K_DIPOLE_H = 3,
     OEM_K_DISP_PRE = 4, OEM_K_DISP_POST = 5, OEM_K_DFT_SNAP = 6, OEM_K_DFT_FLUSH = 7,
     OEM_K_STEP = 8, OEM_K_TFSF_H = 9, OEM_K_TFSF_E = 10, OEM_K_MODE_H = 11, OEM_K_MODE_E = 12,
-    OEM_K_COUNT = 13
+    OEM_K_UPDATE_HE = 13, OEM_K_COUNT = 14
 };
 
 typedef struct oem_ctx oem_ctx;
@@ -134,10 +134,11 @@ int oem_add_tfsf_points(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const int
                         const float *phase, int32_t ncol, const double *table);
 
 /* Mode source points on one plane: H side (magnetic = 1, after the TF/SF H corrections)
- * H[comp][cell] += ch_coef * Re(phase * amp[n]) with amp the whole-step amplitude table, E side
- * (magnetic = 0, after the TF/SF E corrections) E[comp][cell] += cb * coef * Re(phase * amp[n])
- * with the half-step table; phase (re, im float pairs) is the mode profile value at the point
- * and coef carries the face sign and 1/dl (H: also dt/mu0).  amp: n_steps+1 complex (re, im
+ * H[comp][cell] += ch * coef * Re(phase * amp[n]) with ch = dt/mu0 and amp the whole-step
+ * amplitude table, E side (magnetic = 0, after the TF/SF E corrections)
+ * E[comp][cell] += cb * coef * Re(phase * amp[n]) with the cell's cb and the half-step table;
+ * phase (re, im float pairs) is the mode profile value at the point and coef carries the face
+ * sign and 1/dl.  amp: n_steps+1 complex (re, im
  * doubles).  Real fields take the real part of a complex profile.  Replaces inject_mode_h /
  * inject_mode_e and _launch_mode_inject (`openem/kernels/source_dft.cu:84-128`,
  * `openem/solver.py:230-268`). */

@@ -17,7 +17,8 @@ LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 OEM_OK, OEM_EINVAL, OEM_ENOMEM, OEM_EHIP, OEM_ESTATE, OEM_ENOTSUP = range(6)
 KERNELS = ("update_h", "update_e", "dipole_e", "dipole_h", "disp_pre", "disp_post",
-           "dft_snap", "dft_flush", "step_advance", "tfsf_h", "tfsf_e", "mode_h", "mode_e")
+           "dft_snap", "dft_flush", "step_advance", "tfsf_h", "tfsf_e", "mode_h", "mode_e",
+           "update_he")
 K_COUNT = len(KERNELS)
 
 #: every symbol include/openem_hip.h and include/openem_host.h declare (checked by the CPU

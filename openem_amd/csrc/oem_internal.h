@@ -100,7 +100,14 @@ struct oem_ctx {
     size_t esz = 1;             // floats per field element
     float *E = nullptr, *H = nullptr;
     PsiDev psi{};
+    // fused H+E step: a second set of fields and psi; after s steps the state is in set s % 2
+    bool fused = false;
+    float *E2 = nullptr, *H2 = nullptr;
+    PsiDev psi2{};
+    float *cur_E() const { return (fused && (step & 1)) ? E2 : E; }
+    float *cur_H() const { return (fused && (step & 1)) ? H2 : H; }
     GridDev g{};
+    GridDev fg{};               // launch geometry of the fused H+E step
     bool axis_set[3] = {false, false, false};
     MatDev mat{};
     bool mat_set = false;

@@ -239,12 +239,12 @@ extern "C" int oem_energy(oem_ctx *ctx, double *out) {
     TRY(red_scratch(ctx, r, 3 * (size_t)OEM_RED_BLOCKS));
     const dim3 g(OEM_RED_BLOCKS), b(OEM_RED_THREADS);
     if (ctx->cplx)
-        hipLaunchKernelGGL((k_energy<cf2>), g, b, 0, ctx->stream, (const cf2 *)ctx->E,
-                           (const cf2 *)ctx->H, r->ed, ctx->dims.nx, ctx->dims.ny, ctx->dims.nz,
+        hipLaunchKernelGGL((k_energy<cf2>), g, b, 0, ctx->stream, (const cf2 *)ctx->cur_E(),
+                           (const cf2 *)ctx->cur_H(), r->ed, ctx->dims.nx, ctx->dims.ny, ctx->dims.nz,
                            r->partial);
     else
-        hipLaunchKernelGGL((k_energy<float>), g, b, 0, ctx->stream, (const float *)ctx->E,
-                           (const float *)ctx->H, r->ed, ctx->dims.nx, ctx->dims.ny, ctx->dims.nz,
+        hipLaunchKernelGGL((k_energy<float>), g, b, 0, ctx->stream, (const float *)ctx->cur_E(),
+                           (const float *)ctx->cur_H(), r->ed, ctx->dims.nx, ctx->dims.ny, ctx->dims.nz,
                            r->partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, ctx->stream, r->partial,
                        OEM_RED_BLOCKS, 3, r->out);

@@ -515,6 +515,39 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     const long long nb = (long long)g.tiles * ((g.nx + g.chunk - 1) / g.chunk);
     if (nb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
     g.nblocks = (int)nb;
+    // fused H+E step unless a point list corrects H between the sweeps (or it is disabled)
+    const char *nf = std::getenv("OEM_NO_FUSE");
+    bool fuse = !ctx->pool && !(nf && nf[0] == '1');
+    for (auto &pt : ctx->points) fuse &= !pt.magnetic;
+    if (fuse) {
+        const size_t nc = ctx->ncell;
+        TRY(ctx->alloc(&ctx->E2, 3 * nc * ctx->esz, true));
+        TRY(ctx->alloc(&ctx->H2, 3 * nc * ctx->esz, true));
+        float *ph = nullptr, *pe = nullptr;
+        TRY(ctx->alloc(&ph, (size_t)ctx->psi.total * ctx->esz, true));
+        TRY(ctx->alloc(&pe, (size_t)ctx->psi.total * ctx->esz, true));
+        ctx->psi2 = ctx->psi;
+        ctx->psi2.h = ph;
+        ctx->psi2.e = pe;
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        ctx->fg = g;
+        ctx->fg.tiles = ((g.nz + OEM_FTK - 1) / OEM_FTK) * ((g.ny + OEM_FTJ - 1) / OEM_FTJ);
+        int fchunk;
+        if (ctx->chunk_req > 0) {
+            fchunk = ctx->chunk_req;
+        } else if (env && std::atoi(env) > 0) {
+            fchunk = std::atoi(env);
+        } else {
+            const int want = 2048;
+            const int need = std::max(1, (want + ctx->fg.tiles - 1) / ctx->fg.tiles);
+            fchunk = std::max(8, (g.nx + need - 1) / need);
+        }
+        ctx->fg.chunk = std::min(fchunk, g.nx);
+        const long long fb = (long long)ctx->fg.tiles * ((g.nx + ctx->fg.chunk - 1) / ctx->fg.chunk);
+        if (fb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
+        ctx->fg.nblocks = (int)fb;
+        ctx->fused = true;
+    }
     ctx->prepared = true;
     return OEM_OK;
 }
@@ -551,7 +584,10 @@ int launch_flush(oem_ctx *c, int cnt) {
 // the post is fused with the next step's pre.  With `pr`, every launch is bracketed by events.
 template <typename F>
 int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
-    F *E = reinterpret_cast<F *>(c->E), *H = reinterpret_cast<F *>(c->H);
+    // the fields this step produces: the other set on the fused path
+    const bool odd = c->fused && (c->step & 1);
+    F *E = reinterpret_cast<F *>(c->fused && !odd ? c->E2 : c->E);
+    F *H = reinterpret_cast<F *>(c->fused && !odd ? c->H2 : c->H);
     const size_t nc = c->ncell;
     auto mark = [&](int kid, bool begin) -> int {
         if (!pr) return OEM_OK;
@@ -566,6 +602,23 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
         return OEM_OK;
     };
     const dim3 yb(OEM_TK, OEM_TJ), yg(c->g.nblocks);
+    const dim3 db(256), dg(c->has_disp ? blocks_for(c->disp.n_entry, 256) : 1);
+    if (c->fused) {
+        // ADE pre on the old E, then H and E in one march from the old set into the new one
+        F *Eo = reinterpret_cast<F *>(odd ? c->E2 : c->E), *Ho = reinterpret_cast<F *>(odd ? c->H2 : c->H);
+        const PsiDev &po = odd ? c->psi2 : c->psi, &pn = odd ? c->psi : c->psi2;
+        if (c->has_disp && !c->pre_ready) {
+            TRY(mark(OEM_K_DISP_PRE, true));
+            hipLaunchKernelGGL((k_disp<F>), dg, db, 0, c->stream, c->disp, (const F *)Eo, nc,
+                               c->dims.ny, c->dims.nz, 0);
+            TRY(mark(OEM_K_DISP_PRE, false));
+        }
+        TRY(mark(OEM_K_UPDATE_HE, true));
+        hipLaunchKernelGGL((k_update_he<F>), dim3(c->fg.nblocks), dim3(OEM_FTK, OEM_FTJ + 1), 0,
+                           c->stream, (const F *)Eo, (const F *)Ho, E, H, po, pn, c->fg, c->mat,
+                           c->dims.ch, c->dims.cw, c->pw, c->d_step);
+        TRY(mark(OEM_K_UPDATE_HE, false));
+    } else {
     TRY(mark(OEM_K_UPDATE_H, true));
     hipLaunchKernelGGL((k_update_h<F>), yg, yb, 0, c->stream, H, (const F *)E, c->psi, c->g,
                        c->dims.ch, c->pw, c->d_step);
@@ -578,7 +631,6 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
                                c->stream, H, pt.d, c->mat, c->dims.ch, nc, c->d_step);
             TRY(mark(points_kid(pt), false));
         }
-    const dim3 db(256), dg(c->has_disp ? blocks_for(c->disp.n_entry, 256) : 1);
     if (c->has_disp && !c->pre_ready) {
         TRY(mark(OEM_K_DISP_PRE, true));
         hipLaunchKernelGGL((k_disp<F>), dg, db, 0, c->stream, c->disp, (const F *)E, nc,
@@ -589,6 +641,7 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
     hipLaunchKernelGGL((k_update_e<F>), yg, yb, 0, c->stream, E, (const F *)H, c->psi, c->g,
                        c->mat, c->dims.cw, c->pw, c->d_step);
     TRY(mark(OEM_K_UPDATE_E, false));
+    }
     // E-side corrections: TF/SF faces, mode sources, then electric dipoles (reference order,
     // `openem/solver.py:945-968`)
     for (int slot = 3; slot < 6; ++slot)
@@ -650,7 +703,8 @@ static int run_steps(oem_ctx *ctx, int64_t n) {
     int64_t left = n;
     while (left > 0) {
         const bool disp_ok = !ctx->has_disp || ctx->pre_ready;
-        if (use_graph && ctx->pending == 0 && ctx->step % T == 0 && left > T && disp_ok) {
+        if (use_graph && ctx->pending == 0 && ctx->step % T == 0 && left > T && disp_ok &&
+            !(ctx->fused && (T & 1))) {
             if (!ctx->graph) {
                 hipGraph_t gr = nullptr;
                 HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -783,7 +837,7 @@ static int field_ptr(oem_ctx *ctx, int32_t comp, size_t count, float **p) {
     if (comp < 0 || comp > 5) return fail(OEM_EINVAL, "component %d", comp);
     if (count != ctx->ncell * ctx->esz)
         return fail(OEM_EINVAL, "field holds %zu floats, got %zu", ctx->ncell * ctx->esz, count);
-    *p = (comp < 3 ? ctx->E : ctx->H) + (size_t)(comp % 3) * ctx->ncell * ctx->esz;
+    *p = (comp < 3 ? ctx->cur_E() : ctx->cur_H()) + (size_t)(comp % 3) * ctx->ncell * ctx->esz;
     return OEM_OK;
 }
 

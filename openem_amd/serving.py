@@ -154,10 +154,17 @@ def _client(specs, t0, outq, build_ahead: int, stop):
     th.join()
 
 
+#: cells stepped together before a new request waits for room (FIFO): enough to fill the GPU
+#: with small scenes, few enough that large ones are not time-shared many ways
+BATCH_CELLS = 48_000_000
+
+
 def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers: int = 6,
           max_in_flight: int = 2, build_ahead: int = 8, device: int = 0,
-          on_admit=None) -> dict:
+          batch_cells: int = BATCH_CELLS, on_admit=None) -> dict:
     """Continuous batching of ``specs`` (from :func:`openem_amd.scenes.serving_requests`).
+    Ready scenes join the running batch in arrival order while the batch's cells stay within
+    ``batch_cells`` (a scene that alone exceeds it runs when the batch is empty).
     Returns ``dict(requests=[Request...], wall=seconds, peak_running=int)``."""
     pool = Pool(arena_bytes, dft_batch, device)
     t0 = time.perf_counter()
@@ -227,13 +234,14 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
     reader.start()
 
     running: list = []
+    waiting: list = []              # ready, not yet admitted (arrival order)
     finished_setups = 0
     peak = 0
     try:
         while True:
-            # admit everything that is ready (wait for something when idle and more may come)
+            # collect what is ready (wait for something when idle and more may come)
             while finished_setups < workers:
-                idle = not running
+                idle = not running and not waiting
                 try:
                     item = ready.get(block=idle, timeout=0.5 if idle else None)
                 except queue.Empty:
@@ -242,16 +250,23 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
                     break
                 if item is None:
                     finished_setups += 1
-                    if finished_setups == workers:
-                        break
                     continue
-                req, st, need = item
+                waiting.append(item)
+            waiting.sort(key=lambda it: it[0].spec["id"])
+            cells = sum(int(np.prod(st.tabs.shape)) for _, st, _ in running)
+            while waiting:
+                req, st, need = waiting[0]
+                c = int(np.prod(st.tabs.shape))
+                if running and cells + c > batch_cells:
+                    break
+                waiting.pop(0)
                 pool.attach(st)
                 if on_admit is not None:
                     on_admit(req, st)
                 req.t_admit = time.perf_counter() - t0
                 running.append((req, st, need))
-            if not running and finished_setups == workers:
+                cells += c
+            if not running and not waiting and finished_setups == workers:
                 break
             peak = max(peak, len(running))
             if running:

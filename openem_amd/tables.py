@@ -117,7 +117,7 @@ class ModeTables:
 
     h_comp: np.ndarray
     h_cell: np.ndarray
-    h_coef: np.ndarray            # float64: face sign * dt/mu0/dl
+    h_coef: np.ndarray            # float64: face sign / dl (times dt/mu0 on device)
     h_phase: np.ndarray           # complex128 profile value
     e_comp: np.ndarray
     e_cell: np.ndarray
@@ -525,8 +525,8 @@ def mode_tables(sc: Scene, n_total: int) -> list[ModeTables]:
             raise ValueError("mode source amplitude tables are shorter than the run")
         s_hu, s_hv, s_eu, s_ev = modes_mod.MODE_SIGNS[ms.direction]
         ax = sc.grid.axes[a]
-        ch = sc.dt / cpml.MU_0 / float(ax.dl[kh])
-        ce = 1.0 / float(ax.dl_dual[ks])
+        ch = 1.0 / float(ax.dl[kh])          # the device multiplies H-side points by dt/mu0
+        ce = 1.0 / float(ax.dl_dual[ks])     # and E-side points by the cell's cb
         cells_h = modes_mod.plane_cells(sc.shape, a, kh).ravel()
         cells_e = modes_mod.plane_cells(sc.shape, a, ks).ravel()
         lists = {"h": ([], [], [], []), "e": ([], [], [], [])}

@@ -182,3 +182,21 @@ def test_tfsf_box_matches_oracle(gpu_lib, kw):
     for i, m in enumerate(tabs.monitors):
         err = np.abs(acc[i] - o.acc[i]).max() / scale
         assert err < 1e-4, (m.name, err)
+
+
+@pytest.mark.parametrize("name", ["all_features", "ey_minus_z_nonuniform", "pec_walls"])
+def test_fused_step_equals_the_two_sweeps(gpu_lib, monkeypatch, name):
+    """The fused H+E march (ping-pong buffers, halo recompute) gives the fields and phasors of
+    the separate H and E sweeps; the arithmetic per cell is the same, so they agree to float32
+    rounding (1e-6 of the peak)."""
+    kw = CASES[name]
+    n = 180
+    tabs = tables.build(scenes.mini_scene(num_steps=n, **kw))
+    print(f"fused vs two sweeps: {name}", flush=True)
+    f_fused, acc_fused = _gpu_state(tabs, n)
+    monkeypatch.setenv("OEM_NO_FUSE", "1")
+    f_two, acc_two = _gpu_state(tabs, n)
+    for c in range(6):
+        assert _rel(f_fused[c], f_two[c]) <= 1e-6 if np.abs(f_two[c]).max() else True, c
+    for a, b in zip(acc_fused, acc_two):
+        assert _rel(a, b) <= 1e-6

@@ -99,3 +99,31 @@ def test_mode_source_is_one_directional_in_the_oracle(direction):
     # the plane's whole flux also counts the little power radiated where the mode's evanescent
     # tail meets the transverse PML, which the plain-Yee mode operator does not model
     assert abs(direction * amp["fwd_flux"] / amp["fwd"][fwd] ** 2 - 1.0) < 2e-2, amp
+
+
+def test_mode_point_tables_equal_the_oracle_plane_corrections():
+    """The device applies each point as F += scale * coef * Re(phase * amp[n]) with scale =
+    dt/mu0 on the H side and the cell's cb on the E side (k_points); applying the tables that
+    way reproduces the oracle's plane restatement of inject_mode_h / inject_mode_e."""
+    sc = scenes.waveguide_scene(shape=(30, 28, 26), n_pml=6, width_cells=8, thick_cells=4,
+                                num_steps=300, nf=1)
+    t = tables.build(sc)
+    o = Oracle64(t)
+    mt = t.modes[0]
+    for n in (0, 137):
+        for magnetic in (True, False):
+            F = o.H if magnetic else o.E
+            F[:] = 0.0
+            o._mode(sc.mode_sources[0], n, magnetic)
+            want = F.copy()
+            got = np.zeros_like(F)
+            comp, cell, coef, ph, amp = ((mt.h_comp, mt.h_cell, mt.h_coef, mt.h_phase, mt.amp_e)
+                                         if magnetic else
+                                         (mt.e_comp, mt.e_cell, mt.e_coef, mt.e_phase, mt.amp_h))
+            for c in range(3):
+                sel = comp == c
+                scale = t.ch if magnetic else t.cb[c].reshape(-1)[cell[sel]]
+                np.add.at(got[c].reshape(-1), cell[sel],
+                          scale * coef[sel] * np.real(ph[sel] * amp[n]))
+            assert np.abs(want).max() > 0
+            assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
