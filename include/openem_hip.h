@@ -227,6 +227,34 @@ int oem_set_overlap_weights(oem_ctx *ctx, int32_t mon_id, int32_t n_sets, const 
  * (`openem/nb/backend.py:1422`), done there on the host. */
 int oem_monitor_overlap(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
 
+/* ---- Discrete adjoint (real, non-dispersive scenes) ---------------------------------
+ * The reference hands differentiation to tidy3d's adjoint (`openem/nb/autograd_hook.py:1-28`,
+ * gradients assembled in `autograd_hook._vjp`); here the exact transpose of the time step runs
+ * on the device.  Use: checkpoint the forward run every K steps, then per segment from the last:
+ * oem_checkpoint_load, oem_adjoint_record (recompute it, E recorded at the design cells),
+ * oem_adjoint_back (the transposed steps; the objective's dJ/dfield and the eps gradient are
+ * applied on the way). */
+int oem_checkpoint_alloc(oem_ctx *ctx, int32_t n_slots);
+/* E, H, psi and the step index (between DFT batches, i.e. after an oem_run call). */
+int oem_checkpoint_save(oem_ctx *ctx, int32_t slot);
+int oem_checkpoint_load(oem_ctx *ctx, int32_t slot);
+/* Design entries (component 0..2, flat cell) with d ca/d eps, d cb/d eps, ca, cb (float32 each)
+ * and the longest segment to be recorded. */
+int oem_adjoint_setup(oem_ctx *ctx, int32_t n_design, const int32_t *comp, const int32_t *cell,
+                      const float *dca, const float *dcb, const float *ca, const float *cb,
+                      int32_t max_seg);
+/* The objective's gradient with respect to the field samples of forward step n:
+ * lambda[comp][cell] += Re(z * tab[n]) (z: (re, im) floats per point, tab: n_steps complex
+ * doubles); magnetic = 1 for H^{n+1/2}, 0 for E^{n+1}. */
+int oem_adjoint_objective(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const int32_t *comp,
+                          const int32_t *cell, const float *z, const double *tab);
+/* Recompute n_steps forward from the loaded checkpoint (no DFT), recording E at the design. */
+int oem_adjoint_record(oem_ctx *ctx, int64_t n_steps);
+/* The transposed steps over the recorded segment, last step first. */
+int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps);
+/* dJ/d eps of every design entry (float64). */
+int oem_adjoint_gradient(oem_ctx *ctx, double *out, size_t n);
+
 /* ---- Serving pool: many resident scenes advanced by the same launches ------------------
  * No counterpart in the reference, which runs one scene per openem.solver.run call
  * (`openem/solver.py:403-511`) with its own allocations.  A pool owns one device arena of

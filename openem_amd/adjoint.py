@@ -1,0 +1,132 @@
+"""Gradients by the discrete adjoint on the device (real, non-dispersive scenes).
+
+The reference computes inverse-design gradients through tidy3d's adjoint
+(``openem/nb/autograd_hook.py:1-28``).  Here the exact transpose of the time step runs on the
+MI355X (``csrc/oem_adjoint.hip``) with segmented checkpoint recomputation:
+
+1. forward run, the state checkpointed every ``checkpoint_every`` steps;
+2. the objective from the device mode overlap: ``J = |a+|^2`` of a mode monitor;
+3. for each segment from the last: reload its checkpoint, recompute it with E recorded at the
+   design cells, and step the adjoint back over it, accumulating
+   ``dJ/d eps = sum_n lambda_E^{n+1} (ca' E^n + cb' S^n)``.
+
+The gradient is of the discrete objective the stepper computes, so it agrees with central
+finite differences of the same stepper to their truncation error.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib, cpml, modes as modes_mod, tables as tables_mod
+from .grid import transverse_axes
+from .solver import Stepper
+
+
+@dataclass
+class Gradient:
+    J: float                   # |a+|^2
+    a_plus: complex
+    grad: np.ndarray           # dJ/d eps per design entry
+    steps: int
+    segments: int
+
+
+def design_coefficients(tabs, comp: np.ndarray, cell: np.ndarray):
+    """``(dca, dcb, ca, cb)`` float32 per design entry: the E-update coefficients and their eps
+    derivatives, ``ca = (eps - s)/(eps + s)``, ``cb = (dt/eps0)/(eps + s)``."""
+    sc = tabs.scene
+    eps = (sc.eps_ex, sc.eps_ey, sc.eps_ez)
+    sig = (sc.sigma_ex, sc.sigma_ey, sc.sigma_ez)
+    out = [np.zeros(comp.size) for _ in range(4)]
+    for c in range(3):
+        sel = comp == c
+        if not sel.any():
+            continue
+        e = np.broadcast_to(np.asarray(eps[c], dtype=np.float64), sc.shape).reshape(-1)[cell[sel]]
+        s = (0.0 if sig[c] is None else
+             np.broadcast_to(np.asarray(sig[c]), sc.shape).reshape(-1)[cell[sel]]
+             * sc.dt / (2.0 * cpml.EPSILON_0))
+        den = e + s
+        out[0][sel] = 2.0 * s / den ** 2
+        out[1][sel] = -(sc.dt / cpml.EPSILON_0) / den ** 2
+        out[2][sel] = tabs.ca[c].reshape(-1)[cell[sel]]
+        out[3][sel] = tabs.cb[c].reshape(-1)[cell[sel]]
+    return [np.ascontiguousarray(x, dtype=np.float32) for x in out]
+
+
+def _plane_cells(shape, axis: int, k: int) -> np.ndarray:
+    """Flat cells ``(n1, n2)`` of plane ``k`` in the monitors' ascending transverse layout."""
+    t1, t2 = transverse_axes(axis)
+    idx = [None, None, None]
+    idx[axis] = np.full((shape[t1], shape[t2]), k)
+    idx[t1], idx[t2] = np.meshgrid(np.arange(shape[t1]), np.arange(shape[t2]), indexing="ij")
+    return ((idx[0] * shape[1] + idx[1]) * shape[2] + idx[2]).ravel()
+
+
+def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
+                        checkpoint_every: int = 500, n_total: int | None = None,
+                        device: int = 0) -> Gradient:
+    """``J = |a+|^2`` of mode monitor ``monitor`` at frequency ``f_index`` and ``dJ/d eps`` at the
+    design entries (``comp`` 0..2, flat ``cell``)."""
+    tabs = tables_mod.build(sc, n_total)
+    names = [m.name for m in tabs.monitors]
+    mi = names.index(monitor)
+    m = tabs.monitors[mi]
+    if not m.modes:
+        raise ValueError(f"{monitor} is not a mode monitor")
+    comp = np.ascontiguousarray(comp, dtype=np.int32)
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    n = tabs.n_total
+    K = max(1, int(checkpoint_every))
+    n_seg = (n + K - 1) // K
+    st = Stepper(tabs, device=device)
+    lib, ctx = st.lib, st.ctx
+    try:
+        _lib.check(lib, lib.oem_checkpoint_alloc(ctx, n_seg))
+        for s in range(n_seg):
+            _lib.check(lib, lib.oem_checkpoint_save(ctx, s))
+            st.run(min(K, n - s * K))
+        st.setup_modes()
+        ap, _ = st.mode_amplitudes(mi)
+        a = complex(ap[f_index])
+        # dJ/d(sample) = 2 Re(conj(a) K_c e^{i w t}); K the summed overlap weights
+        g = modes_mod.overlap_weight_sets(m.modes, sc.grid)
+        Kw = 0.25 * modes_mod.orientation(m.axis) * (g[0, :, f_index] + g[1, :, f_index])
+        t1, t2 = transverse_axes(m.axis)
+        ph = m.phase[:, f_index]
+        tab_e = np.ascontiguousarray(ph[:, 0] + 1j * ph[:, 1], dtype=np.complex128).view(np.float64)
+        tab_h = np.ascontiguousarray(ph[:, 2] + 1j * ph[:, 3], dtype=np.complex128).view(np.float64)
+        z = 2.0 * np.conj(a) * Kw                         # (4, n1, n2)
+        cells_k = _plane_cells(tabs.shape, m.axis, m.km)
+        cells_km = _plane_cells(tabs.shape, m.axis, m.km - 1)
+        dca, dcb, ca, cb = design_coefficients(tabs, comp, cell)
+        _lib.check(lib, lib.oem_adjoint_setup(ctx, comp.size, _lib.ptr(comp), _lib.ptr(cell),
+                                              _lib.ptr(dca), _lib.ptr(dcb), _lib.ptr(ca),
+                                              _lib.ptr(cb), K))
+        for magnetic, slot, fcomp, planes, tab, w in (
+                (0, 0, t1, (cells_k,), tab_e, 1.0), (0, 1, t2, (cells_k,), tab_e, 1.0),
+                (1, 2, t1, (cells_k, cells_km), tab_h, 0.5), (1, 3, t2, (cells_k, cells_km), tab_h, 0.5)):
+            zc = (w * z[slot]).ravel()
+            keep = np.flatnonzero(zc != 0)
+            if keep.size == 0:
+                continue
+            for cells in planes:
+                pc = np.full(keep.size, fcomp, np.int32)
+                pcell = np.ascontiguousarray(cells[keep], dtype=np.int32)
+                zz = np.ascontiguousarray(zc[keep], dtype=np.complex64).view(np.float32)
+                _lib.check(lib, lib.oem_adjoint_objective(ctx, magnetic, keep.size, _lib.ptr(pc),
+                                                          _lib.ptr(pcell), _lib.ptr(zz),
+                                                          _lib.ptr(tab)))
+        for s in range(n_seg - 1, -1, -1):
+            ln = min(K, n - s * K)
+            _lib.check(lib, lib.oem_checkpoint_load(ctx, s))
+            _lib.check(lib, lib.oem_adjoint_record(ctx, ln))
+            _lib.check(lib, lib.oem_adjoint_back(ctx, ln))
+        out = np.zeros(comp.size, np.float64)
+        _lib.check(lib, lib.oem_adjoint_gradient(ctx, _lib.ptr(out), out.size))
+    finally:
+        st.close()
+    return Gradient(J=abs(a) ** 2, a_plus=a, grad=out, steps=n, segments=n_seg)

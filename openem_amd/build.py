@@ -10,7 +10,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 SOURCES = [PKG / "csrc" / f for f in ("openem_hip.hip", "oem_pool.hip", "oem_reduce.hip",
-                                                 "oem_host.cpp")]
+                                                 "oem_adjoint.hip", "oem_host.cpp")]
 OUT = PKG / "libopenem_hip.so"
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",

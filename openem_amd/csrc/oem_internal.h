@@ -66,8 +66,10 @@ void pool_arena_free(oem_pool *pool, void *p);
 }  // namespace oemi
 
 struct oem_reduce_state;            // energy / flux reduction tables (oem_reduce.hip)
+struct oem_adjoint_state;           // checkpoints, adjoint fields, design gradient (oem_adjoint.hip)
 namespace oemi {
 void reduce_state_free(oem_reduce_state *r);
+void adjoint_state_free(oem_adjoint_state *a);
 }
 
 using oemi::fail;
@@ -123,6 +125,8 @@ struct oem_ctx {
     bool pre_ready = false;     // ADE: hist/P already advanced by the previous fused step
     bool prepared = false;
     int chunk_req = 0;          // x-planes per work-group requested (0: heuristic)
+    bool mons_off = false;      // adjoint recomputation: no DFT snapshots or flushes
+    struct oem_adjoint_state *adj = nullptr;
     hipGraphExec_t graph = nullptr;
     std::vector<void *> allocs;
     oem_reduce_state *red = nullptr;
@@ -136,6 +140,7 @@ struct oem_ctx {
     ~oem_ctx() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (red) oemi::reduce_state_free(red);
+        if (adj) oemi::adjoint_state_free(adj);
         for (void *p : allocs) {
             if (pool) oemi::pool_arena_free(pool, p);
             else (void)hipFree(p);
@@ -178,8 +183,14 @@ struct oem_ctx {
 // psi slabs and counters of a context (through the pool's arena when ctx->pool is set).
 int ctx_init(oem_ctx *c, const oem_dims *d, int32_t device);
 
-// Pool hooks used by the single-scene entry points (oem_pool.hip).
+// Pool hooks used by the single-scene entry points (oem_pool.hip), and the point-list builder
+// and step launcher shared with the adjoint (openem_hip.hip).
 namespace oemi {
+int build_points(oem_ctx *ctx, int magnetic, int kind, int tfsf, int cplx_amp, int32_t n_pts,
+                 const int32_t *comp, const int32_t *cell, const float *coef, const int32_t *col,
+                 const float *phase, int32_t ncol, const double *tab, size_t tab_doubles,
+                 Points *out);
+int launch_one_step(oem_ctx *ctx, bool last);       // one eager step, no flush, no profiling
 int pool_retire(oem_ctx *ctx);                  // oem_ctx_destroy of a pool context
 int pool_wait_done(oem_ctx *ctx);               // before reading a pool context's results
 }

@@ -295,11 +295,11 @@ extern "C" int oem_add_plane_source(oem_ctx *ctx, int32_t pol, int32_t kh, int32
     return OEM_OK;
 }
 
-// Group points by (comp, cell), keeping input order inside a group, upload, and append.
-static int add_points(oem_ctx *ctx, int magnetic, int kind, int tfsf, int cplx_amp, int32_t n_pts,
-                      const int32_t *comp, const int32_t *cell, const float *coef,
-                      const int32_t *col, const float *phase, int32_t ncol, const double *tab,
-                      size_t tab_doubles) {
+// Group points by (comp, cell), keeping input order inside a group, and upload them.
+int oemi::build_points(oem_ctx *ctx, int magnetic, int kind, int tfsf, int cplx_amp,
+                       int32_t n_pts, const int32_t *comp, const int32_t *cell, const float *coef,
+                       const int32_t *col, const float *phase, int32_t ncol, const double *tab,
+                       size_t tab_doubles, Points *out) {
     std::vector<int> order(n_pts);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -343,6 +343,17 @@ static int add_points(oem_ctx *ctx, int magnetic, int kind, int tfsf, int cplx_a
     if (phase) TRY(ctx->upload(&pp, ph.data(), ph.size()));
     TRY(ctx->upload(&pt, tab, tab_doubles));
     d.g_comp = pa; d.g_cell = pb; d.g_ofs = po; d.coef = pf; d.col = pc; d.phase = pp; d.tab = pt;
+    *out = P;
+    return OEM_OK;
+}
+
+static int add_points(oem_ctx *ctx, int magnetic, int kind, int tfsf, int cplx_amp, int32_t n_pts,
+                      const int32_t *comp, const int32_t *cell, const float *coef,
+                      const int32_t *col, const float *phase, int32_t ncol, const double *tab,
+                      size_t tab_doubles) {
+    Points P{};
+    TRY(oemi::build_points(ctx, magnetic, kind, tfsf, cplx_amp, n_pts, comp, cell, coef, col,
+                           phase, ncol, tab, tab_doubles, &P));
     ctx->points.push_back(P);
     return OEM_OK;
 }
@@ -660,6 +671,7 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
         c->pre_ready = !last;
     }
     for (auto &m : c->mons) {
+        if (c->mons_off) break;
         TRY(mark(OEM_K_DFT_SNAP, true));
         hipLaunchKernelGGL((k_dft_snap<F>), dim3(blocks_for(m.d.npts, 256)), dim3(256), 0,
                            c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
@@ -669,7 +681,7 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
     TRY(mark(OEM_K_STEP, true));
     hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, c->stream, c->d_step);
     TRY(mark(OEM_K_STEP, false));
-    if (flush_after > 0 && !c->mons.empty()) {
+    if (flush_after > 0 && !c->mons.empty() && !c->mons_off) {
         TRY(mark(OEM_K_DFT_FLUSH, true));
         TRY(launch_flush(c, flush_after));
         TRY(mark(OEM_K_DFT_FLUSH, false));
@@ -684,6 +696,12 @@ int launch_step(oem_ctx *c, int flush_after, Prof *pr, bool last) {
 }
 
 }  // namespace
+
+int oemi::launch_one_step(oem_ctx *ctx, bool last) {
+    TRY(launch_step(ctx, 0, nullptr, last));
+    ctx->step++;
+    return OEM_OK;
+}
 
 static int check_run(oem_ctx *ctx, int64_t n) {
     if (!ctx->prepared) return fail(OEM_ESTATE, "oem_prepare not called");

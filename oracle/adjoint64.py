@@ -1,0 +1,193 @@
+"""Float64 discrete adjoint of the oracle's time step.  TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/`` may import this module (it checks the device adjoint).
+
+The forward step of :class:`oracle.yee64.Oracle64` on a real, non-dispersive scene is linear in
+the state ``(E, H, psi_H, psi_E)`` plus additive source terms, with the E update
+``E' = pec (ca E + cb S)`` depending on eps through ``ca`` and ``cb``
+(reference ``openem/kernels/yee.cu:154-240``, coefficients ``openem/coeffs.py:98-136``).  For an
+objective ``J`` of the DFT phasors the exact gradient of the discrete run is
+
+    dJ/deps = sum_n lambda_E^{n+1} . (d ca/d eps E^n + d cb/d eps S^n),
+    S^n = (E^{n+1} - ca E^n) / cb,
+
+where ``lambda`` runs backwards through the transposed step: the objective's gradient with
+respect to E^{n+1} and H^{n+1/2} is added, then the transpose of the E update (its backward
+differences become forward differences of ``cb pec lambda_E``, the CPML recursion transposes
+onto the psi adjoints), then the transpose of the H update.  This is the "discrete adjoint"
+that tidy3d's ``grad`` stands for in the reference (``openem/nb/autograd_hook.py:1-28``); the
+reference hands it to tidy3d, here it is written out.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .yee64 import Oracle64, _PSI, _bc
+
+# Terms of each update as (source field component, derivative axis, psi key), first with +,
+# second with - (oracle.yee64.Oracle64.step):
+#   H_c -= ch (s_first - s_second),  s = stretch(psi_H[key], dfwd(E_src, axis))
+#   E_c  = ca E_c + cb (s_first - s_second),  s = stretch(psi_E[key], dbwd(H_src, axis))
+_TERMS = {0: ((2, 1, "hx_y"), (1, 2, "hx_z")),
+          1: ((0, 2, "hy_z"), (2, 0, "hy_x")),
+          2: ((1, 0, "hz_x"), (0, 1, "hz_y"))}
+_H_TERMS = _E_TERMS = _TERMS
+
+
+class Adjoint64:
+    """Backward sweep over an :class:`Oracle64` run (real fields, no dispersion)."""
+
+    def __init__(self, tabs):
+        if tabs.bloch or tabs.disp is not None:
+            raise NotImplementedError("the adjoint covers real, non-dispersive scenes")
+        self.t = tabs
+        self.o = Oracle64(tabs)                      # for the difference operators and masks
+        shp = tabs.shape
+        self.lE = np.zeros((3,) + shp)
+        self.lH = np.zeros((3,) + shp)
+        self.lpe = {k: np.zeros(shp) for k in _PSI}
+        self.lph = {k: np.zeros(shp) for k in _PSI}
+
+    # transposes of the oracle's one-sided differences along axis a
+    def _dfwd_T(self, g, a):
+        m = self.o._m[a]
+        idl = _bc(self.t.axes[a].idl_p, a)
+        w = g * idl
+        out = -w
+        src = np.moveaxis(w * _bc(m["mnx"], a), a, 0)
+        acc = np.moveaxis(out, a, 0)
+        np.add.at(acc, m["nxt"], src)
+        return np.moveaxis(acc, 0, a)
+
+    def _dbwd_T(self, g, a):
+        m = self.o._m[a]
+        idl = _bc(self.t.axes[a].idl_d, a)
+        w = g * idl
+        out = w.copy()
+        src = np.moveaxis(-w * _bc(m["mpv"], a), a, 0)
+        acc = np.moveaxis(out, a, 0)
+        np.add.at(acc, m["prv"], src)
+        return np.moveaxis(acc, 0, a)
+
+    def _coef(self, a, side):
+        m = self.o._m[a]
+        return m["a" + side], m["b" + side], m["k" + side]
+
+    def step_back(self, gE=None, gH=None):
+        """Adjoint of one forward step: ``gE``/``gH`` are dJ/dE^{n+1}, dJ/dH^{n+1/2} of this
+        step's samples (added first); leaves lambda for the state before the step."""
+        t, m = self.t, self.o._m
+        lE, lH = self.lE, self.lH
+        if gE is not None:
+            lE += gE
+        if gH is not None:
+            lH += gH
+        # E update transpose
+        pec = [m[1]["pec"] * m[2]["pec"], m[2]["pec"] * m[0]["pec"], m[0]["pec"] * m[1]["pec"]]
+        nu = [t.cb[c] * pec[c] * lE[c] for c in range(3)]
+        dH = np.zeros_like(lH)
+        new_pe = {}
+        for c in range(3):
+            for sgn, (b, a, key) in zip((1.0, -1.0), _E_TERMS[c]):
+                ac, bc, kc = self._coef(a, "E")
+                f = sgn * nu[c]
+                q = (kc + ac) * f + ac * self.lpe[key]
+                dH[b] += self._dbwd_T(q, a)
+                new_pe[key] = bc * (f + self.lpe[key])
+        for c in range(3):
+            lE[c] = t.ca[c] * pec[c] * lE[c]
+        self.lpe = new_pe
+        lH += dH
+        # H update transpose: H_c' = H_c - ch (s1 - s2)
+        dE = np.zeros_like(lE)
+        new_ph = {}
+        for c in range(3):
+            for sgn, (b, a, key) in zip((1.0, -1.0), _H_TERMS[c]):
+                ac, bc, kc = self._coef(a, "H")
+                f = -t.ch * sgn * lH[c]
+                q = (kc + ac) * f + ac * self.lph[key]
+                dE[b] += self._dfwd_T(q, a)
+                new_ph[key] = bc * (f + self.lph[key])
+        self.lph = new_ph
+        lE += dE
+
+
+def eps_derivatives(tabs, sc):
+    """Per component (d ca/d eps, d cb/d eps) of ``ca = (eps - s)/(eps + s + G)``,
+    ``cb = (dt/eps0)/(eps + s + G)``."""
+    from openem_amd import cpml
+    out = []
+    sig = (sc.sigma_ex, sc.sigma_ey, sc.sigma_ez)
+    eps = (sc.eps_ex, sc.eps_ey, sc.eps_ez)
+    for c in range(3):
+        e = np.broadcast_to(np.asarray(eps[c], dtype=np.float64), sc.shape)
+        s = 0.0 if sig[c] is None else np.asarray(sig[c]) * sc.dt / (2 * cpml.EPSILON_0)
+        den = e + s
+        out.append(((2 * s) / den ** 2 + 0 * e, -(sc.dt / cpml.EPSILON_0) / den ** 2 + 0 * e))
+    return out
+
+
+def mode_objective_weights(tabs, mon_index: int, f_index: int = 0):
+    """``K`` (4, n1, n2) complex: a+ = sum_n sum_c K_c . sample_c^n e^{i w t_c(n)} for the mode
+    monitor ``mon_index`` at frequency ``f_index`` (the set sum of the overlap weights)."""
+    from openem_amd import modes
+    m = tabs.monitors[mon_index]
+    g = modes.overlap_weight_sets(m.modes, tabs.scene.grid)
+    sign = modes.orientation(m.axis)
+    return 0.25 * sign * (g[0, :, f_index] + g[1, :, f_index])
+
+
+def gradient_mode_power(sc, tabs, mon_index: int, design: np.ndarray, f_index: int = 0):
+    """J = |a+|^2 of a mode monitor after ``tabs.n_total`` steps and dJ/d eps at ``design``
+    ((n, 2) int: component, flat cell), by a stored-trajectory float64 adjoint."""
+    from openem_amd.grid import transverse_axes
+    o = Oracle64(tabs)
+    n_total = tabs.n_total
+    traj = [o.E.copy()]
+    for _ in range(n_total):
+        o.step()
+        traj.append(o.E.copy())
+    m = tabs.monitors[mon_index]
+    K = mode_objective_weights(tabs, mon_index, f_index)
+    ph = m.phase[:, f_index]                              # (n_total, 4): cos/sin at t_e, t_h
+    data = o.acc[mon_index][:, f_index]
+    a = np.sum(K * data)
+    J = abs(a) ** 2
+    t1, t2 = transverse_axes(m.axis)
+    adj = Adjoint64(tabs)
+    dca_dcb = eps_derivatives(tabs, sc)
+    grad = np.zeros(len(design))
+    dc, dcell = design[:, 0], design[:, 1]
+    for n in range(n_total - 1, -1, -1):
+        we = ph[n, 0] + 1j * ph[n, 1]
+        wh = ph[n, 2] + 1j * ph[n, 3]
+        gE = np.zeros((3,) + tabs.shape)
+        gH = np.zeros((3,) + tabs.shape)
+        for slot, comp, w in ((0, t1, we), (1, t2, we)):
+            gE[comp] += _on_plane(tabs.shape, m.axis, m.km, 2.0 * np.real(np.conj(a) * K[slot] * w))
+        for slot, comp in ((2, t1), (3, t2)):
+            v = 2.0 * np.real(np.conj(a) * K[slot] * wh) * 0.5
+            gH[comp] += _on_plane(tabs.shape, m.axis, m.km, v)
+            gH[comp] += _on_plane(tabs.shape, m.axis, m.km - 1, v)
+        adj.lE += gE
+        adj.lH += gH
+        E0, E1 = traj[n], traj[n + 1]
+        for q in range(len(design)):
+            c, cell = int(dc[q]), int(dcell[q])
+            ca = tabs.ca[c].reshape(-1)[cell]
+            cb = tabs.cb[c].reshape(-1)[cell]
+            e0, e1 = E0[c].reshape(-1)[cell], E1[c].reshape(-1)[cell]
+            dca = dca_dcb[c][0].reshape(-1)[cell]
+            dcb = dca_dcb[c][1].reshape(-1)[cell]
+            grad[q] += adj.lE[c].reshape(-1)[cell] * (dca * e0 + dcb * (e1 - ca * e0) / cb)
+        adj.step_back()
+    return J, grad
+
+
+def _on_plane(shape, axis, k, val):
+    out = np.zeros(shape)
+    idx = [slice(None)] * 3
+    idx[axis] = k
+    out[tuple(idx)] = val
+    return out
