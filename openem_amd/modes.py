@@ -106,10 +106,39 @@ def plane_eps(sc: Scene, axis: int, k: int):
     return out
 
 
+_CACHE: dict = {}
+_CACHE_MAX = 64
+
+
 def solve_mode(sc: Scene, axis: int, plane_index: int, freq: float, mode_index: int = 0,
                n_guess: float | None = None, n_modes: int = 4) -> Mode:
     """The ``mode_index``-th guided mode (by decreasing effective index) of the cross-section
-    of ``sc`` at ``plane_index`` along ``axis``."""
+    of ``sc`` at ``plane_index`` along ``axis``.  Solutions are cached on everything they
+    depend on (the plane's eps, the transverse axes, dt, the frequency and the selection)."""
+    import copy
+    import hashlib
+    u, v = cyclic(axis)
+    h = hashlib.sha1()
+    for arr in plane_eps(sc, axis, plane_index):
+        h.update(np.ascontiguousarray(arr).tobytes())
+    for ax in (sc.grid.axes[u], sc.grid.axes[v]):
+        h.update(np.ascontiguousarray(ax.edges).tobytes())
+        h.update(f"{ax.boundary_lo}/{ax.boundary_hi}".encode())
+    key = (h.hexdigest(), axis, float(sc.grid.axes[axis].dl[plane_index]), float(sc.dt),
+           float(freq), int(mode_index), n_guess, int(n_modes))
+    hit = _CACHE.get(key)
+    if hit is None:
+        hit = _solve(sc, axis, plane_index, freq, mode_index, n_guess, n_modes)
+        if len(_CACHE) >= _CACHE_MAX:
+            _CACHE.pop(next(iter(_CACHE)))
+        _CACHE[key] = hit
+    m = copy.copy(hit)
+    m.plane_index = plane_index
+    return m
+
+
+def _solve(sc: Scene, axis: int, plane_index: int, freq: float, mode_index: int,
+           n_guess: float | None, n_modes: int) -> Mode:
     g = sc.grid
     u, v = cyclic(axis)
     au, av, aa = g.axes[u], g.axes[v], g.axes[axis]

@@ -578,3 +578,59 @@ def waveguide_scene(shape=(60, 40, 30), dl: float = 40e-9, n_pml: int = 8,
     flux = [FluxMonitor("fwd_flux", 0, k_fwd, +1, freqs),
             FluxMonitor("back_flux", 0, k_back, +1, freqs)]
     return dataclasses.replace(base, mode_sources=[src], mode_monitors=mons, flux_monitors=flux)
+
+
+def bend_design(n_pix: int = 160, seed: int = 0, sigma: float = 4.0, eps_lo: float = 2.1,
+                eps_hi: float = 12.1) -> np.ndarray:
+    """A seeded Gaussian-smoothed random field mapped into [eps_lo, eps_hi] (``(n_pix, n_pix)``),
+    the inverse-design starting point."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    f = gaussian_filter(rng.standard_normal((n_pix, n_pix)), sigma, mode="reflect")
+    f = (f - f.min()) / max(float(np.ptp(f)), 1e-30)
+    return eps_lo + (eps_hi - eps_lo) * f
+
+
+def bend_scene(design: np.ndarray | None = None, n: int = 400, nz: int = 48, dl: float = 20e-9,
+               n_pml: int = 12, thick: float = 0.22e-6, width: float = 0.5e-6,
+               core_eps: float = 12.1, clad_eps: float = 2.1, f0: float = 1.934e14,
+               fwidth_rel: float = 0.1, num_steps: int = 6000, seed: int = 0):
+    """Inverse-design 90-degree bend: an x-directed input guide and a y-directed output guide
+    (``width`` x ``thick`` strips of ``core_eps`` in ``clad_eps``) meet a square design region
+    of ``design.shape`` pixels (one cell each, uniform through the slab thickness, all three eps
+    components); PML on all faces; the fundamental mode launched along +x, a mode monitor on the
+    output guide along +y.  Returns ``(scene, design_cells)`` where ``design_cells`` is
+    ``(n_pix, n_pix, n_layers)`` flat indices of the design cells.  (The two mode solves do not
+    depend on the design and are cached across designs.)"""
+    import dataclasses
+    from . import modes as modes_mod
+    if design is None:
+        design = bend_design(seed=seed)
+    npx = design.shape[0]
+    axes = [uniform_axis(n, dl, 0.0, "PML"), uniform_axis(n, dl, 0.0, "PML"),
+            uniform_axis(nz, dl, 0.0, "PML")]
+    grid = Grid(*axes)
+    dt = cpml.courant_dt((dl, dl, dl))
+    eps = np.full((n, n, nz), clad_eps)
+    nt = int(round(thick / dl))
+    k0 = nz // 2 - nt // 2
+    nw = int(round(width / dl))
+    c = n // 2
+    d0 = c - npx // 2                                  # design region [d0, d0 + npx) in x and y
+    w0 = c - nw // 2
+    eps[:d0, w0:w0 + nw, k0:k0 + nt] = core_eps        # input guide along x, from the left
+    eps[w0:w0 + nw, d0 + npx:, k0:k0 + nt] = core_eps  # output guide along y, to the top
+    eps[d0:d0 + npx, d0:d0 + npx, k0:k0 + nt] = design[:, :, None]
+    ii, jj, kk = np.meshgrid(np.arange(d0, d0 + npx), np.arange(d0, d0 + npx),
+                             np.arange(k0, k0 + nt), indexing="ij")
+    cells = (ii * n + jj) * nz + kk
+    sides_all = [(a, s) for a in range(3) for s in ("lo", "hi")]
+    base = Scene(grid=grid, dt=dt, num_time_steps=num_steps, eps_ex=eps, eps_ey=eps, eps_ez=eps,
+                 pml=pml_faces(dt, (dl, dl, dl), sides_all, n_pml))
+    ks = n_pml + 10
+    km = n - n_pml - 10
+    m_src = modes_mod.solve_mode(base, 0, ks, f0)
+    pulse = waveform.GaussianPulse(freq0=f0, fwidth=fwidth_rel * f0)
+    src = modes_mod.mode_source(base, m_src, pulse, +1, num_steps)
+    mon = ModeMonitor("out", 1, km, np.array([f0]))
+    return dataclasses.replace(base, mode_sources=[src], mode_monitors=[mon]), cells

@@ -10,7 +10,10 @@ with the same launches (persistent batched kernels in a hipGraph of ``dft_batch`
   (admission against the HBM budget), create it in the arena and upload it;
 * the stepping thread attaches ready scenes at the next block boundary, keeps a bounded number
   of blocks in flight and hands finished scenes to the readout, which reads the phasors,
-  computes the fluxes and releases the scene's memory.
+  computes the fluxes and releases the scene's memory;
+* scenes of at least ``solo_cells`` cells fill the GPU by themselves: they run in a second lane,
+  one at a time as standalone contexts (fused H+E step, hipGraph blocks) on their own stream,
+  concurrently with the batched small scenes.
 
 :func:`serve_serial` is the plain loop the batched form is measured against: one
 :func:`openem_amd.solver.run` per request, in arrival order.
@@ -159,9 +162,13 @@ def _client(specs, t0, outq, build_ahead: int, stop):
 BATCH_CELLS = 48_000_000
 
 
+#: scenes at least this large run standalone in the solo lane
+SOLO_CELLS = 8_000_000
+
+
 def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers: int = 6,
           max_in_flight: int = 2, build_ahead: int = 8, device: int = 0,
-          batch_cells: int = BATCH_CELLS, on_admit=None) -> dict:
+          batch_cells: int = BATCH_CELLS, solo_cells: int = SOLO_CELLS, on_admit=None) -> dict:
     """Continuous batching of ``specs`` (from :func:`openem_amd.scenes.serving_requests`).
     Ready scenes join the running batch in arrival order while the batch's cells stay within
     ``batch_cells`` (a scene that alone exceeds it runs when the batch is empty).
@@ -170,6 +177,7 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
     t0 = time.perf_counter()
     arrivals: queue.Queue = queue.Queue()
     ready: queue.Queue = queue.Queue()
+    solo: queue.Queue = queue.Queue()
     done_q: queue.Queue = queue.Queue()
     stop = threading.Event()
     mem = threading.Condition()
@@ -186,6 +194,9 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
             req, sc = item
             try:
                 tabs = tables_mod.build(sc)
+                if int(np.prod(tabs.shape)) >= solo_cells:
+                    solo.put((req, tabs))
+                    continue
                 need = device_bytes(tabs, dft_batch)
                 with mem:
                     while budget["free"] < need and not stop.is_set():
@@ -224,9 +235,30 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
                 req.t_done = time.perf_counter() - t0
             results.append(req)
 
+    def solo_worker():
+        while True:
+            item = solo.get()
+            if item is None:
+                return
+            req, tabs = item
+            req.t_admit = time.perf_counter() - t0
+            try:
+                st = solver.Stepper(tabs, device=device, dft_batch=dft_batch)
+                try:
+                    st.run(tabs.n_total)
+                    req.flux = _flux_of(st)
+                finally:
+                    st.close()
+            except Exception as e:  # noqa: BLE001
+                req.error = repr(e)
+            req.t_done = time.perf_counter() - t0
+            results.append(req)
+
     client = threading.Thread(target=_client, args=(specs, t0, arrivals, build_ahead, stop),
                               daemon=True)
     setups = [threading.Thread(target=setup_worker, daemon=True) for _ in range(workers)]
+    solo_th = threading.Thread(target=solo_worker, daemon=True)
+    solo_th.start()
     reader = threading.Thread(target=readout_worker, daemon=True)
     client.start()
     for th in setups:
@@ -281,6 +313,8 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
             running = still
     finally:
         stop.set()
+        solo.put(None)
+        solo_th.join()
         done_q.put(None)
         reader.join()
         client.join(timeout=5)
