@@ -726,9 +726,15 @@ static int run_steps(oem_ctx *ctx, int64_t n) {
             if (!ctx->graph) {
                 hipGraph_t gr = nullptr;
                 HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                // each captured step sees its own step index: on the fused path its parity picks
+                // which field set is old and which is new
                 int rc = OEM_OK;
-                for (int s = 0; s < T && !rc; ++s)
+                const int64_t s0 = ctx->step;
+                for (int s = 0; s < T && !rc; ++s) {
+                    ctx->step = s0 + s;
                     rc = launch_step(ctx, s == T - 1 ? T : 0, nullptr, false);
+                }
+                ctx->step = s0;
                 hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
                 if (rc) {
                     if (gr) (void)hipGraphDestroy(gr);
