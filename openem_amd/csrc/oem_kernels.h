@@ -489,18 +489,24 @@ k_update_e(F *__restrict__ E, const F *__restrict__ H, PsiDev ps, GridDev g, Mat
 // Fused H + E step: one x march computes H^{n+1/2} and then E^{n+1} of every plane, so each
 // field is read once and written once per step (about 50 B per cell instead of 74 for the two
 // sweeps).  Old and new fields live in separate buffers (ping-pong between steps): every value
-// a block reads is an old one, so neighbouring blocks never race.  The E update needs the new
-// H of its j-1 and k-1 neighbours; a block owns 8 rows x 64 k of a plane and a ninth wave
-// recomputes the new H of the row Y.prv[j0] and the column Z.prv[k0] (from the same old values,
-// so bit-identical to what the owning block computes).  The new H plane of the tile goes through
-// double-buffered LDS, one barrier per plane.  The x neighbour H(i-1) is carried in registers;
-// a prologue computes it for the first plane of the chunk.  Valid when no point correction
-// acts on H between the sweeps (the host checks); the plane-wave corrections are applied in
-// the march as in the separate sweeps.
+// a wave reads is an old one, so waves never race and need no barrier.
+//
+// Each wave works alone on a strip of OEM_WR rows (j) x 63 cells (k) and marches along x.
+// Lane 0 is a halo lane at k0 - 1 (Z.prv[k0]); lanes 1..63 own k0 .. k0 + 62.  In each plane
+// the wave first computes the new H of the halo row Y.prv[j0] (not stored), then row by row
+// the new H (stored) and at once the new E of the same row, which needs the new H of
+//   j - 1: the previous row, carried in registers,
+//   k - 1: the lane to the left (a cross-lane shift),
+//   i - 1: the same row one plane back, carried in registers (a prologue computes it for the
+//          first plane of the chunk).
+// Halo values are recomputed from the same old values with the same arithmetic, so they are
+// bit-identical to what the owning wave stores.  Valid when no point correction acts on H
+// between the sweeps (the host checks); the plane-wave corrections are applied in the march as
+// in the separate sweeps.
 // ------------------------------------------------------------------------------------------
-#define OEM_FTK 64
-#define OEM_FTJ 8
-#define OEM_FT_THREADS (OEM_FTK * (OEM_FTJ + 1))
+#define OEM_WK 63           // owned k per wave (lane 0 is the k - 1 halo)
+#define OEM_WR 8            // owned rows per wave
+#define OEM_WW 4            // independent waves per work-group
 
 struct FusedIO {
     rsrc_t ex, ey, ez, hx, hy, hz;          // old fields
@@ -508,76 +514,112 @@ struct FusedIO {
     rsrc_t ph_o, ph_n, pe_o, pe_n;          // psi: H side old / new, E side old / new
 };
 
-// H^{n+1/2} at (i, j, k) from the old buffers.  e0 = old (Ex, Ey, Ez) at (i, j, k); eyn/ezn =
-// old Ey/Ez at (X.nxt[i], j, k).  With `store` the new H and H-side psi are written.
+// Everything the new H at one cell reads: old E at the cell, at X.nxt[i] (ey, ez), at Y.nxt[j]
+// (ex, ez) and Z.nxt[k] (ex, ey), old H, and the H-side psi of the slabs holding the cell.
 template <typename F>
-__device__ __forceinline__ void fused_h(const FusedIO &io, const PsiDev &ps, const GridDev &g,
-                                        int i, int j, int k, F ex, F ey, F ez, F eyn, F ezn,
-                                        float ch, float inj_hx, float inj_hy, bool store,
-                                        F &hx, F &hy, F &hz)
-{
+struct HIn {
+    F ex, ey, ez, eyn, ezn, ex_j, ez_j, ex_k, ey_k, hx, hy, hz;
+    F px0, px1, py0, py1, pz0, pz1;
+};
+
+// Offsets of the psi elements of cell (i, j, k) in the x / y / z slabs (ps.off order), and the
+// slab indices (-1 outside).
+struct PsiAt {
+    int sx, sy, sz;
+    uint32_t ox, oy, oz;        // per-lane byte offsets
+    uint32_t sx0, sx1, sy0, sy1, sz0, sz1;  // per-plane byte offsets
+};
+
+template <typename F>
+__device__ __forceinline__ PsiAt psi_at(const PsiDev &ps, const GridDev &g, int i, int j, int k) {
     typedef FT<F> T;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
-    const int nz = g.nz, ny = g.ny, nx = g.nx;
-    const uint32_t planeB = (uint32_t)g.plane * T::B, so = (uint32_t)i * planeB;
+    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+    PsiAt q;
+    q.sx = slab_of(i, g.nx, X.lo, X.hi);
+    q.sy = slab_of(j, g.ny, Y.lo, Y.hi);
+    q.sz = slab_of(k, g.nz, Z.lo, Z.hi);
+    q.ox = (uint32_t)(j * g.nz + k) * T::B;
+    q.oy = (uint32_t)(max(q.sy, 0) * g.nz + k) * T::B;
+    q.oz = (uint32_t)(j * wz + max(q.sz, 0)) * T::B;
+    q.sx0 = (ps.off[0] + (uint32_t)max(q.sx, 0) * (uint32_t)g.plane) * T::B;
+    q.sx1 = (ps.off[1] + (uint32_t)max(q.sx, 0) * (uint32_t)g.plane) * T::B;
+    q.sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
+    q.sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
+    q.sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
+    q.sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
+    return q;
+}
+
+// Issue every load of the new H at (i, j, k).
+template <typename F>
+__device__ __forceinline__ void fused_h_ld(const FusedIO &io, const GridDev &g, const PsiAt &q,
+                                           int i, int j, int k, HIn<F> &v) {
+    typedef FT<F> T;
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int nz = g.nz;
+    const uint32_t planeB = (uint32_t)g.plane * T::B;
+    const uint32_t so = (uint32_t)i * planeB, sn = (uint32_t)X.nxt[i] * planeB;
     const uint32_t op = (uint32_t)(j * nz + k) * T::B;
     const uint32_t oj = (uint32_t)(Y.nxt[j] * nz + k) * T::B;
     const uint32_t ok = (uint32_t)(j * nz + Z.nxt[k]) * T::B;
-    const F ex_j = T::ld(io.ex, oj, so), ex_k = T::ld(io.ex, ok, so);
-    const F ey_k = T::ld(io.ey, ok, so), ez_j = T::ld(io.ez, oj, so);
-    hx = T::ld(io.hx, op, so);
-    hy = T::ld(io.hy, op, so);
-    hz = T::ld(io.hz, op, so);
+    v.ex = T::ld(io.ex, op, so); v.ey = T::ld(io.ey, op, so); v.ez = T::ld(io.ez, op, so);
+    v.eyn = T::ld(io.ey, op, sn); v.ezn = T::ld(io.ez, op, sn);
+    v.ex_j = T::ld(io.ex, oj, so); v.ez_j = T::ld(io.ez, oj, so);
+    v.ex_k = T::ld(io.ex, ok, so); v.ey_k = T::ld(io.ey, ok, so);
+    v.hx = T::ld(io.hx, op, so); v.hy = T::ld(io.hy, op, so); v.hz = T::ld(io.hz, op, so);
+    v.px0 = v.px1 = v.py0 = v.py1 = v.pz0 = v.pz1 = T::zero();
+    if (q.sx >= 0) { v.px0 = T::ld(io.ph_o, q.ox, q.sx0); v.px1 = T::ld(io.ph_o, q.ox, q.sx1); }
+    if (q.sy >= 0) { v.py0 = T::ld(io.ph_o, q.oy, q.sy0); v.py1 = T::ld(io.ph_o, q.oy, q.sy1); }
+    if (q.sz >= 0) { v.pz0 = T::ld(io.ph_o, q.oz, q.sz0); v.pz1 = T::ld(io.ph_o, q.oz, q.sz1); }
+}
+
+// H^{n+1/2} at (i, j, k) from loaded inputs (the arithmetic of update_h_tile); with `store` the
+// new H and H-side psi are written.
+template <typename F>
+__device__ __forceinline__ void fused_h_do(const FusedIO &io, const GridDev &g, const PsiAt &q,
+                                           int i, int j, int k, HIn<F> &v, float ch,
+                                           float inj_hx, float inj_hy, bool store,
+                                           F &hx, F &hy, F &hz) {
+    typedef FT<F> T;
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const uint32_t so = (uint32_t)i * ((uint32_t)g.plane * T::B);
     const float mj = axv(Y, j, AX_MNX), mk = axv(Z, k, AX_MNX);
     const float mji = axv(Y, j, AX_MNXI), mki = axv(Z, k, AX_MNXI);
     const float idy = axv(Y, j, AX_IDLP), idz = axv(Z, k, AX_IDLP);
     const float mi = axv(X, i, AX_MNX), mii = axv(X, i, AX_MNXI), idx = axv(X, i, AX_IDLP);
-    const F dzy = (T::mmul(ez_j, mj, mji) - ez) * idy;
-    const F dyz = (T::mmul(ey_k, mk, mki) - ey) * idz;
-    const F dxz = (T::mmul(ex_k, mk, mki) - ex) * idz;
-    const F dzx = (T::mmul(ezn, mi, mii) - ez) * idx;
-    const F dyx = (T::mmul(eyn, mi, mii) - ey) * idx;
-    const F dxy = (T::mmul(ex_j, mj, mji) - ex) * idy;
+    const F dzy = (T::mmul(v.ez_j, mj, mji) - v.ez) * idy;
+    const F dyz = (T::mmul(v.ey_k, mk, mki) - v.ey) * idz;
+    const F dxz = (T::mmul(v.ex_k, mk, mki) - v.ex) * idz;
+    const F dzx = (T::mmul(v.ezn, mi, mii) - v.ez) * idx;
+    const F dyx = (T::mmul(v.eyn, mi, mii) - v.ey) * idx;
+    const F dxy = (T::mmul(v.ex_j, mj, mji) - v.ex) * idy;
     F s_hx_y = dzy, s_hz_y = dxy, s_hx_z = dyz, s_hy_z = dxz, s_hy_x = dzx, s_hz_x = dyx;
-    const int sx = slab_of(i, nx, X.lo, X.hi), sy = slab_of(j, ny, Y.lo, Y.hi),
-              sz = slab_of(k, nz, Z.lo, Z.hi);
-    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
-    if (sx >= 0) {
-        const uint32_t o0 = (ps.off[0] + (uint32_t)sx * (uint32_t)g.plane) * T::B;
-        const uint32_t o1 = (ps.off[1] + (uint32_t)sx * (uint32_t)g.plane) * T::B;
-        F p0 = T::ld(io.ph_o, op, o0), p1 = T::ld(io.ph_o, op, o1);
+    if (q.sx >= 0) {
         const float a_ = axv(X, i, AX_AH), b_ = axv(X, i, AX_BH), k_ = axv(X, i, AX_KH);
-        s_hy_x = stretch_v(p0, dzx, a_, b_, k_);
-        s_hz_x = stretch_v(p1, dyx, a_, b_, k_);
-        if (store) { T::st(io.ph_n, p0, op, o0); T::st(io.ph_n, p1, op, o1); }
+        s_hy_x = stretch_v(v.px0, dzx, a_, b_, k_);
+        s_hz_x = stretch_v(v.px1, dyx, a_, b_, k_);
+        if (store) { T::st(io.ph_n, v.px0, q.ox, q.sx0); T::st(io.ph_n, v.px1, q.ox, q.sx1); }
     }
-    if (sy >= 0) {
-        const uint32_t oy = (uint32_t)(sy * nz + k) * T::B;
-        const uint32_t o0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
-        const uint32_t o1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
-        F p0 = T::ld(io.ph_o, oy, o0), p1 = T::ld(io.ph_o, oy, o1);
+    if (q.sy >= 0) {
         const float a_ = axv(Y, j, AX_AH), b_ = axv(Y, j, AX_BH), k_ = axv(Y, j, AX_KH);
-        s_hx_y = stretch_v(p0, dzy, a_, b_, k_);
-        s_hz_y = stretch_v(p1, dxy, a_, b_, k_);
-        if (store) { T::st(io.ph_n, p0, oy, o0); T::st(io.ph_n, p1, oy, o1); }
+        s_hx_y = stretch_v(v.py0, dzy, a_, b_, k_);
+        s_hz_y = stretch_v(v.py1, dxy, a_, b_, k_);
+        if (store) { T::st(io.ph_n, v.py0, q.oy, q.sy0); T::st(io.ph_n, v.py1, q.oy, q.sy1); }
     }
-    if (sz >= 0) {
-        const uint32_t oz = (uint32_t)(j * wz + sz) * T::B;
-        const uint32_t o0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
-        const uint32_t o1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
-        F p0 = T::ld(io.ph_o, oz, o0), p1 = T::ld(io.ph_o, oz, o1);
+    if (q.sz >= 0) {
         const float a_ = axv(Z, k, AX_AH), b_ = axv(Z, k, AX_BH), k_ = axv(Z, k, AX_KH);
-        s_hx_z = stretch_v(p0, dyz, a_, b_, k_);
-        s_hy_z = stretch_v(p1, dxz, a_, b_, k_);
-        if (store) { T::st(io.ph_n, p0, oz, o0); T::st(io.ph_n, p1, oz, o1); }
+        s_hx_z = stretch_v(v.pz0, dyz, a_, b_, k_);
+        s_hy_z = stretch_v(v.pz1, dxz, a_, b_, k_);
+        if (store) { T::st(io.ph_n, v.pz0, q.oz, q.sz0); T::st(io.ph_n, v.pz1, q.oz, q.sz1); }
     }
-    hx = T::addre(hx - ch * (s_hx_y - s_hx_z), inj_hx);
-    hy = T::addre(hy - ch * (s_hy_z - s_hy_x), inj_hy);
-    hz = hz - ch * (s_hz_x - s_hz_y);
+    hx = T::addre(v.hx - ch * (s_hx_y - s_hx_z), inj_hx);
+    hy = T::addre(v.hy - ch * (s_hy_z - s_hy_x), inj_hy);
+    hz = v.hz - ch * (s_hz_x - s_hz_y);
     if (store) {
-        T::st(io.hxn, hx, op, so);
-        T::st(io.hyn, hy, op, so);
-        T::st(io.hzn, hz, op, so);
+        T::st(io.hxn, hx, q.ox, so);
+        T::st(io.hyn, hy, q.ox, so);
+        T::st(io.hzn, hz, q.ox, so);
     }
 }
 
@@ -595,8 +637,23 @@ __device__ __forceinline__ void pw_h_inj(const PlaneSrcDev &pw, int k, const int
         }
 }
 
+// Value of lane - 1 (lane 0 keeps its own); every lane of the wave must be active.
+__device__ __forceinline__ float lane_prev(float x) {
+    const int src = ((int)threadIdx.x - 1) & 63;
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+}
+template <typename F> __device__ __forceinline__ F lane_prev_f(F x);
+template <> __device__ __forceinline__ float lane_prev_f<float>(float x) { return lane_prev(x); }
+template <> __device__ __forceinline__ cf2 lane_prev_f<cf2>(cf2 x) {
+    return {lane_prev(x.r), lane_prev(x.i)};
+}
+
+#ifndef OEM_HE_WAVES_PER_EU
+#define OEM_HE_WAVES_PER_EU 4
+#endif
 template <typename F>
-__global__ void __launch_bounds__(OEM_FT_THREADS)
+__global__ void __launch_bounds__(64 * OEM_WW)
+__attribute__((amdgpu_waves_per_eu(OEM_HE_WAVES_PER_EU)))
 k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ En,
             F *__restrict__ Hn, PsiDev ps, PsiDev pn, GridDev g, MatDev mat, float ch, float cw,
             PlaneSrcDev pw, const int *__restrict__ stp)
@@ -614,53 +671,47 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
         io.pe_o = mk_rsrc(ps.e, pb); io.pe_n = mk_rsrc(pn.e, pb);
     }
     __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
-    __shared__ F sH[2][3][OEM_FTJ + 1][OEM_FTK + 1];
     const bool lds_lut = lut_in_lds(mat);
     if (lds_lut) {
-        const int tid = threadIdx.y * OEM_FTK + threadIdx.x;
+        const int tid = threadIdx.y * 64 + threadIdx.x;
         const float4 *src = reinterpret_cast<const float4 *>(mat.lut);
-        for (int q = tid; q < 2 * mat.n_classes; q += OEM_FT_THREADS) s_lut[q] = src[q];
+        for (int q = tid; q < 2 * mat.n_classes; q += 64 * OEM_WW) s_lut[q] = src[q];
+        __syncthreads();
     }
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int lb = xcd_remap(blockIdx.x, g.nblocks);
-    const int nkt = (nz + OEM_FTK - 1) / OEM_FTK;
-    const int tile = lb % g.tiles, chunk = lb / g.tiles;
-    const int j0 = (tile / nkt) * OEM_FTJ, k0 = (tile % nkt) * OEM_FTK;
+    // work item of this wave: (chunk, row group, k strip)
+    const int nks = (nz + OEM_WK - 1) / OEM_WK;
+    const int n_items = g.tiles * ((nx + g.chunk - 1) / g.chunk);
+    const int item = xcd_remap(blockIdx.x, g.nblocks) * OEM_WW + (int)threadIdx.y;
+    if (item >= n_items) return;
+    const int tile = item % g.tiles, chunk = item / g.tiles;
+    const int j0 = (tile / nks) * OEM_WR, k0 = (tile % nks) * OEM_WK;
     const int i0 = chunk * g.chunk, i1 = min(i0 + g.chunk, nx);
-    const int tx = threadIdx.x, w = threadIdx.y;
-    const bool halo = w == OEM_FTJ;
-    // owner: (j, k) of the tile; halo wave: row cell (jr, kr) and, lanes < 8, column cell (jc, kc)
-    const int j = halo ? Y.prv[j0] : j0 + w, k = k0 + tx;
-    const bool act = j < ny && k < nz;
-    const int jc = j0 + tx, kc = Z.prv[k0];
-    const bool actc = halo && tx < OEM_FTJ && jc < ny;
-    const uint32_t planeB = (uint32_t)g.plane * T::B;
+    const int nr = min(OEM_WR, ny - j0);
+    const int jh = Y.prv[j0];
+    const int lane = threadIdx.x;
+    const bool act = lane == 0 || k0 + lane - 1 < nz;       // computes H
+    const bool own = lane > 0 && k0 + lane - 1 < nz;        // stores H and E
+    const int k = lane == 0 ? Z.prv[k0] : (act ? k0 + lane - 1 : 0);
+    const uint32_t planeB = (uint32_t)g.plane * T::B, plane2 = (uint32_t)g.plane * 2u;
     const size_t ncell = (size_t)g.plane * nx;
 
-    // E-side per-(j, k) quantities (owners)
-    const uint32_t op = (uint32_t)(j * nz + k) * T::B;
-    const float mjE = act ? axv(Y, j, AX_MPV) : 0.f, mkE = act ? axv(Z, k, AX_MPV) : 0.f;
-    const float mjEi = act ? axv(Y, j, AX_MPVI) : 0.f, mkEi = act ? axv(Z, k, AX_MPVI) : 0.f;
-    const float idyE = act ? axv(Y, j, AX_IDLD) : 0.f, idzE = act ? axv(Z, k, AX_IDLD) : 0.f;
-    const float pecy = act ? axv(Y, j, AX_PEC) : 0.f, pecz = act ? axv(Z, k, AX_PEC) : 0.f;
-    const int sy = act ? slab_of(j, ny, Y.lo, Y.hi) : -1, sz = act ? slab_of(k, nz, Z.lo, Z.hi) : -1;
-    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
-    float ay = 0.f, by = 1.f, ky = 1.f, az = 0.f, bz = 1.f, kz = 1.f;
-    if (sy >= 0) { ay = axv(Y, j, AX_AE); by = axv(Y, j, AX_BE); ky = axv(Y, j, AX_KE); }
-    if (sz >= 0) { az = axv(Z, k, AX_AE); bz = axv(Z, k, AX_BE); kz = axv(Z, k, AX_KE); }
-    const uint32_t oy = (uint32_t)(max(sy, 0) * nz + k) * T::B;
-    const uint32_t oz = (uint32_t)(j * wz + max(sz, 0)) * T::B;
+    // E-side per-k quantities (owners)
+    const float mkE = own ? axv(Z, k, AX_MPV) : 0.f, mkEi = own ? axv(Z, k, AX_MPVI) : 0.f;
+    const float idzE = own ? axv(Z, k, AX_IDLD) : 0.f, pecz = own ? axv(Z, k, AX_PEC) : 0.f;
+    float az = 0.f, bz = 1.f, kz = 1.f;
+    if (own && slab_of(k, nz, Z.lo, Z.hi) >= 0) {
+        az = axv(Z, k, AX_AE); bz = axv(Z, k, AX_BE); kz = axv(Z, k, AX_KE);
+    }
     const rsrc_t rC = mk_rsrc(mat.cls, (uint32_t)(ncell * 2));
-    const uint32_t oc = (uint32_t)(j * nz + k) * 2u, plane2 = (uint32_t)g.plane * 2u;
     const int hbx = mat.hb[1] - mat.hb[0], hby = mat.hb[3] - mat.hb[2], hbz = mat.hb[5] - mat.hb[4];
-    const bool hrow = mat.hist && j >= mat.hb[2] && j < mat.hb[3] && k >= mat.hb[4] && k < mat.hb[5];
     const uint32_t hcomp = (uint32_t)(hbx * hby * hbz);
     const rsrc_t rHist = mk_rsrc(mat.hist, 3u * hcomp * T::B);
-    const uint32_t oh = hrow ? (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * T::B : 0u;
+    const bool hcol = mat.hist && own && k >= mat.hb[4] && k < mat.hb[5];
     float inj_ex = 0.f, inj_ey = 0.f;
     bool inj_x = false, inj_y = false;
-    if (pw.n && act && !halo) {
+    if (pw.n && own) {
         const int n = step_now(stp);
         for (int s = 0; s < pw.n; ++s)
             if (k == pw.ks[s]) {
@@ -668,144 +719,154 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
                 if (pw.pol[s] == 0) { inj_ex += v; inj_x = true; } else { inj_ey += v; inj_y = true; }
             }
     }
-    float ihx = 0.f, ihy = 0.f, ihxc = 0.f, ihyc = 0.f;
+    float ihx = 0.f, ihy = 0.f;
     if (act) pw_h_inj(pw, k, stp, ihx, ihy);
-    if (actc) pw_h_inj(pw, kc, stp, ihxc, ihyc);
 
-    // carried state: old E at the current plane (row / owner cell and halo column cell), new H
-    // of the previous plane (owners)
-    F ex = T::zero(), ey = T::zero(), ez = T::zero();
-    F exc = T::zero(), eyc = T::zero(), ezc = T::zero();
-    F hy_m = T::zero(), hz_m = T::zero();
-    const uint32_t opc = (uint32_t)(jc * nz + kc) * T::B;
-    if (act) {
-        if (!halo) {
-            // prologue: new H at X.prv[i0] (own cell only), the x neighbour of E(i0)
-            const int ip = X.prv[i0];
-            const uint32_t sp = (uint32_t)ip * planeB, sn = (uint32_t)X.nxt[ip] * planeB;
-            const F pex = T::ld(io.ex, op, sp), pey = T::ld(io.ey, op, sp), pez = T::ld(io.ez, op, sp);
-            const F peyn = T::ld(io.ey, op, sn), pezn = T::ld(io.ez, op, sn);
-            F hx_, hy_, hz_;
-            fused_h<F>(io, ps, g, ip, j, k, pex, pey, pez, peyn, pezn, ch, ihx, ihy, false, hx_, hy_, hz_);
-            hy_m = hy_;
-            hz_m = hz_;
+    // new H of each row one plane back (the x neighbour of E), as a queue that rotates by one
+    // entry per row step: with the row loop rolled the compiler keeps one row's registers
+    F hy_q[OEM_WR], hz_q[OEM_WR];
+#pragma unroll
+    for (int r = 0; r < OEM_WR; ++r) {
+        hy_q[r] = T::zero();
+        hz_q[r] = T::zero();
+    }
+#pragma unroll 1
+    for (int r = 0; r < OEM_WR; ++r) {
+        F hy_ = T::zero(), hz_ = T::zero();
+        if (r < nr && own) {
+            const int ip = X.prv[i0], j = j0 + r;
+            const PsiAt q = psi_at<F>(ps, g, ip, j, k);
+            HIn<F> v;
+            F hx_;
+            fused_h_ld<F>(io, g, q, ip, j, k, v);
+            fused_h_do<F>(io, g, q, ip, j, k, v, ch, ihx, ihy, false, hx_, hy_, hz_);
         }
-        const uint32_t s0 = (uint32_t)i0 * planeB;
-        ex = T::ld(io.ex, op, s0); ey = T::ld(io.ey, op, s0); ez = T::ld(io.ez, op, s0);
+#pragma unroll
+        for (int q = 0; q + 1 < OEM_WR; ++q) {
+            hy_q[q] = hy_q[q + 1];
+            hz_q[q] = hz_q[q + 1];
+        }
+        hy_q[OEM_WR - 1] = hy_;
+        hz_q[OEM_WR - 1] = hz_;
     }
-    if (actc) {
-        const uint32_t s0 = (uint32_t)i0 * planeB;
-        exc = T::ld(io.ex, opc, s0); eyc = T::ld(io.ey, opc, s0); ezc = T::ld(io.ez, opc, s0);
-    }
-    uint32_t cls_next = (act && !halo && mat.mode == 0) ? bld_u16(rC, oc, (uint32_t)i0 * plane2) : 0u;
-    __syncthreads();                                   // LUT staged
-    int b = 0;
+
     for (int i = i0; i < i1; ++i) {
-        const uint32_t so = (uint32_t)i * planeB, sn = (uint32_t)X.nxt[i] * planeB;
-        F hx = T::zero(), hy = T::zero(), hz = T::zero();
-        F exn = T::zero(), eyn = T::zero(), ezn = T::zero();
+        const uint32_t so = (uint32_t)i * planeB;
+        const float mi = axv(X, i, AX_MPV), mii = axv(X, i, AX_MPVI);
+        const float idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
+        const bool hplane = i >= mat.hb[0] && i < mat.hb[1];
+
+        // new H of the halo row (its Hx, Hz are the j - 1 neighbours of row 0)
+        F hxp = T::zero(), hzp = T::zero();
         if (act) {
-            exn = T::ld(io.ex, op, sn); eyn = T::ld(io.ey, op, sn); ezn = T::ld(io.ez, op, sn);
-            fused_h<F>(io, ps, g, i, j, k, ex, ey, ez, eyn, ezn, ch, ihx, ihy, !halo, hx, hy, hz);
-            const int r = halo ? 0 : w + 1;
-            sH[b][0][r][tx + 1] = hx;
-            sH[b][1][r][tx + 1] = hy;
-            sH[b][2][r][tx + 1] = hz;
+            const PsiAt q = psi_at<F>(ps, g, i, jh, k);
+            HIn<F> v;
+            F hy_;
+            fused_h_ld<F>(io, g, q, i, jh, k, v);
+            fused_h_do<F>(io, g, q, i, jh, k, v, ch, ihx, ihy, false, hxp, hy_, hzp);
         }
-        F excn = T::zero(), eycn = T::zero(), ezcn = T::zero();
-        if (actc) {
-            excn = T::ld(io.ex, opc, sn); eycn = T::ld(io.ey, opc, sn); ezcn = T::ld(io.ez, opc, sn);
-            F cx, cy, cz;
-            fused_h<F>(io, ps, g, i, jc, kc, exc, eyc, ezc, eycn, ezcn, ch, ihxc, ihyc, false, cx, cy, cz);
-            sH[b][0][tx + 1][0] = cx;
-            sH[b][1][tx + 1][0] = cy;
-            sH[b][2][tx + 1][0] = cz;
-        }
-        __syncthreads();
-        if (act && !halo) {
-            const uint32_t cls = cls_next;
-            if (mat.mode == 0 && i + 1 < i1) cls_next = bld_u16(rC, oc, (uint32_t)(i + 1) * plane2);
-            // new H neighbours: (j-1) from the row above in LDS, (k-1) from the lane to the left
-            const F hx_j = sH[b][0][w][tx + 1], hz_j = sH[b][2][w][tx + 1];
-            const F hx_k = sH[b][0][w + 1][tx], hy_k = sH[b][1][w + 1][tx];
-            const int sx = slab_of(i, nx, X.lo, X.hi);
-            F psx0 = T::zero(), psx1 = T::zero(), psy0 = T::zero(), psy1 = T::zero(),
-              psz0 = T::zero(), psz1 = T::zero();
-            const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
-            const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
-            const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
-            const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
-            const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
-            const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
-            if (sx >= 0) { psx0 = T::ld(io.pe_o, op, sx0); psx1 = T::ld(io.pe_o, op, sx1); }
-            if (sy >= 0) { psy0 = T::ld(io.pe_o, oy, sy0); psy1 = T::ld(io.pe_o, oy, sy1); }
-            if (sz >= 0) { psz0 = T::ld(io.pe_o, oz, sz0); psz1 = T::ld(io.pe_o, oz, sz1); }
-            float cax, cbx, cay, cby, caz, cbz;
-            uint32_t fl;
-            if (lds_lut) {
-                const float4 r0 = s_lut[2 * cls], r1 = s_lut[2 * cls + 1];
-                cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
-                fl = __float_as_uint(r1.z);
-            } else if (mat.mode == 0) {
-                const float4 *row = reinterpret_cast<const float4 *>(mat.lut + 8u * cls);
-                const float4 r0 = row[0], r1 = row[1];
-                cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
-                fl = __float_as_uint(r1.z);
-            } else {
-                const size_t id = (size_t)i * g.plane + (size_t)j * nz + k;
-                cax = mat.ca[id]; cay = mat.ca[ncell + id]; caz = mat.ca[2 * ncell + id];
-                cbx = mat.cb[id]; cby = mat.cb[ncell + id]; cbz = mat.cb[2 * ncell + id];
-                fl = (hrow && i >= mat.hb[0] && i < mat.hb[1]) ? 7u : 0u;
-            }
+#pragma unroll 1
+        for (int r = 0; r < OEM_WR; ++r) {
+            const int j = j0 + min(r, nr - 1);     // rows past the last are rotated, not computed
+            const bool row = r < nr;
+            const PsiAt q = psi_at<F>(ps, g, i, j, k);
+            F hx = T::zero(), hy = T::zero(), hz = T::zero();
+            HIn<F> v;
+            // every load of the row before its first store: H inputs, class, E-side psi and
+            // the ADE history (by box geometry; the class flags pick the components later)
+            uint32_t cls = 0u;
+            F pex0 = T::zero(), pex1 = T::zero(), pey0 = T::zero(), pey1 = T::zero(),
+              pez0 = T::zero(), pez1 = T::zero();
             F phx = T::zero(), phy = T::zero(), phz = T::zero();
-            if (fl) {
-                const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * T::B;
-                if (fl & 1u) phx = T::ld(rHist, oh, sh);
-                if (fl & 2u) phy = T::ld(rHist, oh, sh + hcomp * T::B);
-                if (fl & 4u) phz = T::ld(rHist, oh, sh + 2u * hcomp * T::B);
+            const bool hin = hcol && hplane && j >= mat.hb[2] && j < mat.hb[3];
+            if (act && row) {
+                fused_h_ld<F>(io, g, q, i, j, k, v);
+                if (own) {
+                    if (mat.mode == 0) cls = bld_u16(rC, (uint32_t)(j * nz + k) * 2u, (uint32_t)i * plane2);
+                    if (q.sx >= 0) { pex0 = T::ld(io.pe_o, q.ox, q.sx0); pex1 = T::ld(io.pe_o, q.ox, q.sx1); }
+                    if (q.sy >= 0) { pey0 = T::ld(io.pe_o, q.oy, q.sy0); pey1 = T::ld(io.pe_o, q.oy, q.sy1); }
+                    if (q.sz >= 0) { pez0 = T::ld(io.pe_o, q.oz, q.sz0); pez1 = T::ld(io.pe_o, q.oz, q.sz1); }
+                    if (hin) {
+                        const uint32_t oh = (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * T::B;
+                        const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * T::B;
+                        phx = T::ld(rHist, oh, sh);
+                        phy = T::ld(rHist, oh, sh + hcomp * T::B);
+                        phz = T::ld(rHist, oh, sh + 2u * hcomp * T::B);
+                    }
+                }
+                fused_h_do<F>(io, g, q, i, j, k, v, ch, ihx, ihy, own, hx, hy, hz);
             }
-            const float mi = axv(X, i, AX_MPV), mii = axv(X, i, AX_MPVI);
-            const float idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
-            const F dzy = (hz - T::mmul(hz_j, mjE, mjEi)) * idyE;
-            const F dyz = (hy - T::mmul(hy_k, mkE, mkEi)) * idzE;
-            const F dxz = (hx - T::mmul(hx_k, mkE, mkEi)) * idzE;
-            const F dzx = (hz - T::mmul(hz_m, mi, mii)) * idx;
-            const F dyx = (hy - T::mmul(hy_m, mi, mii)) * idx;
-            const F dxy = (hx - T::mmul(hx_j, mjE, mjEi)) * idyE;
-            F s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
-            if (sx >= 0) {
-                const float ax_ = axv(X, i, AX_AE), bx_ = axv(X, i, AX_BE), kx_ = axv(X, i, AX_KE);
-                s_ey_x = stretch_v(psx0, dzx, ax_, bx_, kx_);
-                s_ez_x = stretch_v(psx1, dyx, ax_, bx_, kx_);
-                T::st(io.pe_n, psx0, op, sx0);
-                T::st(io.pe_n, psx1, op, sx1);
+            // new H at k - 1 from the lane to the left (all lanes active here)
+            const F hx_k = lane_prev_f<F>(hx), hy_k = lane_prev_f<F>(hy);
+            if (own && row) {
+                const float mjE = axv(Y, j, AX_MPV), mjEi = axv(Y, j, AX_MPVI);
+                const float idyE = axv(Y, j, AX_IDLD), pecy = axv(Y, j, AX_PEC);
+                float cax, cbx, cay, cby, caz, cbz;
+                uint32_t fl;
+                if (lds_lut) {
+                    const float4 r0 = s_lut[2 * cls], r1 = s_lut[2 * cls + 1];
+                    cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
+                    fl = __float_as_uint(r1.z);
+                } else if (mat.mode == 0) {
+                    const float4 *rw = reinterpret_cast<const float4 *>(mat.lut + 8u * cls);
+                    const float4 r0 = rw[0], r1 = rw[1];
+                    cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
+                    fl = __float_as_uint(r1.z);
+                } else {
+                    const size_t id = (size_t)i * g.plane + (size_t)j * nz + k;
+                    cax = mat.ca[id]; cay = mat.ca[ncell + id]; caz = mat.ca[2 * ncell + id];
+                    cbx = mat.cb[id]; cby = mat.cb[ncell + id]; cbz = mat.cb[2 * ncell + id];
+                    fl = hin ? 7u : 0u;
+                }
+                if (!(fl & 1u)) phx = T::zero();
+                if (!(fl & 2u)) phy = T::zero();
+                if (!(fl & 4u)) phz = T::zero();
+                const F dzy = (hz - T::mmul(hzp, mjE, mjEi)) * idyE;
+                const F dyz = (hy - T::mmul(hy_k, mkE, mkEi)) * idzE;
+                const F dxz = (hx - T::mmul(hx_k, mkE, mkEi)) * idzE;
+                const F dzx = (hz - T::mmul(hz_q[0], mi, mii)) * idx;
+                const F dyx = (hy - T::mmul(hy_q[0], mi, mii)) * idx;
+                const F dxy = (hx - T::mmul(hxp, mjE, mjEi)) * idyE;
+                F s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
+                if (q.sx >= 0) {
+                    const float a_ = axv(X, i, AX_AE), b_ = axv(X, i, AX_BE), k_ = axv(X, i, AX_KE);
+                    s_ey_x = stretch_v(pex0, dzx, a_, b_, k_);
+                    s_ez_x = stretch_v(pex1, dyx, a_, b_, k_);
+                    T::st(io.pe_n, pex0, q.ox, q.sx0);
+                    T::st(io.pe_n, pex1, q.ox, q.sx1);
+                }
+                if (q.sy >= 0) {
+                    const float a_ = axv(Y, j, AX_AE), b_ = axv(Y, j, AX_BE), k_ = axv(Y, j, AX_KE);
+                    s_ex_y = stretch_v(pey0, dzy, a_, b_, k_);
+                    s_ez_y = stretch_v(pey1, dxy, a_, b_, k_);
+                    T::st(io.pe_n, pey0, q.oy, q.sy0);
+                    T::st(io.pe_n, pey1, q.oy, q.sy1);
+                }
+                if (q.sz >= 0) {
+                    s_ex_z = stretch_v(pez0, dyz, az, bz, kz);
+                    s_ey_z = stretch_v(pez1, dxz, az, bz, kz);
+                    T::st(io.pe_n, pez0, q.oz, q.sz0);
+                    T::st(io.pe_n, pez1, q.oz, q.sz1);
+                }
+                F nex = (cax * v.ex + cbx * (s_ex_y - s_ex_z - cw * phx)) * pecy * pecz;
+                F ney = (cay * v.ey + cby * (s_ey_z - s_ey_x - cw * phy)) * pecz * pecx;
+                F nez = (caz * v.ez + cbz * (s_ez_x - s_ez_y - cw * phz)) * pecx * pecy;
+                if (inj_x) nex = T::addre(nex, cbx * inj_ex);
+                if (inj_y) ney = T::addre(ney, cby * inj_ey);
+                T::st(io.exn, nex, q.ox, so);
+                T::st(io.eyn, ney, q.ox, so);
+                T::st(io.ezn, nez, q.ox, so);
             }
-            if (sy >= 0) {
-                s_ex_y = stretch_v(psy0, dzy, ay, by, ky);
-                s_ez_y = stretch_v(psy1, dxy, ay, by, ky);
-                T::st(io.pe_n, psy0, oy, sy0);
-                T::st(io.pe_n, psy1, oy, sy1);
+            hxp = hx;
+            hzp = hz;
+#pragma unroll
+            for (int qq = 0; qq + 1 < OEM_WR; ++qq) {
+                hy_q[qq] = hy_q[qq + 1];
+                hz_q[qq] = hz_q[qq + 1];
             }
-            if (sz >= 0) {
-                s_ex_z = stretch_v(psz0, dyz, az, bz, kz);
-                s_ey_z = stretch_v(psz1, dxz, az, bz, kz);
-                T::st(io.pe_n, psz0, oz, sz0);
-                T::st(io.pe_n, psz1, oz, sz1);
-            }
-            F nex = (cax * ex + cbx * (s_ex_y - s_ex_z - cw * phx)) * pecy * pecz;
-            F ney = (cay * ey + cby * (s_ey_z - s_ey_x - cw * phy)) * pecz * pecx;
-            F nez = (caz * ez + cbz * (s_ez_x - s_ez_y - cw * phz)) * pecx * pecy;
-            if (inj_x) nex = T::addre(nex, cbx * inj_ex);
-            if (inj_y) ney = T::addre(ney, cby * inj_ey);
-            T::st(io.exn, nex, op, so);
-            T::st(io.eyn, ney, op, so);
-            T::st(io.ezn, nez, op, so);
-            hy_m = hy;
-            hz_m = hz;
+            hy_q[OEM_WR - 1] = hy;
+            hz_q[OEM_WR - 1] = hz;
         }
-        ex = exn; ey = eyn; ez = ezn;
-        exc = excn; eyc = eycn; ezc = ezcn;
-        b ^= 1;
     }
 }
 

@@ -526,9 +526,9 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     const long long nb = (long long)g.tiles * ((g.nx + g.chunk - 1) / g.chunk);
     if (nb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
     g.nblocks = (int)nb;
-    // fused H+E step unless a point list corrects H between the sweeps (or it is disabled)
-    const char *nf = std::getenv("OEM_NO_FUSE");
-    bool fuse = !ctx->pool && !(nf && nf[0] == '1');
+    // fused H+E step on request (OEM_FUSE=1), unless a point list corrects H between the sweeps
+    const char *fe = std::getenv("OEM_FUSE");
+    bool fuse = !ctx->pool && fe && fe[0] == '1';
     for (auto &pt : ctx->points) fuse &= !pt.magnetic;
     if (fuse) {
         const size_t nc = ctx->ncell;
@@ -542,20 +542,23 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
         ctx->psi2.e = pe;
         HIP_OK(hipStreamSynchronize(ctx->stream));
         ctx->fg = g;
-        ctx->fg.tiles = ((g.nz + OEM_FTK - 1) / OEM_FTK) * ((g.ny + OEM_FTJ - 1) / OEM_FTJ);
+        // one wave per (OEM_WR rows x OEM_WK k) strip and x chunk, OEM_WW waves per work-group
+        ctx->fg.tiles = ((g.nz + OEM_WK - 1) / OEM_WK) * ((g.ny + OEM_WR - 1) / OEM_WR);
         int fchunk;
         if (ctx->chunk_req > 0) {
             fchunk = ctx->chunk_req;
         } else if (env && std::atoi(env) > 0) {
             fchunk = std::atoi(env);
         } else {
-            const int want = 2048;
+            // about 8 waves per SIMD over the whole chip, chunks of at least 16 planes
+            const int want = 8192;
             const int need = std::max(1, (want + ctx->fg.tiles - 1) / ctx->fg.tiles);
-            fchunk = std::max(8, (g.nx + need - 1) / need);
+            fchunk = std::max(16, (g.nx + need - 1) / need);
         }
         ctx->fg.chunk = std::min(fchunk, g.nx);
-        const long long fb = (long long)ctx->fg.tiles * ((g.nx + ctx->fg.chunk - 1) / ctx->fg.chunk);
-        if (fb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
+        const long long items = (long long)ctx->fg.tiles * ((g.nx + ctx->fg.chunk - 1) / ctx->fg.chunk);
+        const long long fb = (items + OEM_WW - 1) / OEM_WW;
+        if (items > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
         ctx->fg.nblocks = (int)fb;
         ctx->fused = true;
     }
@@ -625,7 +628,7 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
             TRY(mark(OEM_K_DISP_PRE, false));
         }
         TRY(mark(OEM_K_UPDATE_HE, true));
-        hipLaunchKernelGGL((k_update_he<F>), dim3(c->fg.nblocks), dim3(OEM_FTK, OEM_FTJ + 1), 0,
+        hipLaunchKernelGGL((k_update_he<F>), dim3(c->fg.nblocks), dim3(64, OEM_WW), 0,
                            c->stream, (const F *)Eo, (const F *)Ho, E, H, po, pn, c->fg, c->mat,
                            c->dims.ch, c->dims.cw, c->pw, c->d_step);
         TRY(mark(OEM_K_UPDATE_HE, false));

@@ -122,6 +122,9 @@ class FluxMonitor:
     freqs: np.ndarray
     transverse: tuple[tuple[int, int], tuple[int, int]] | None = None
     apodization: tuple | None = None
+    # physical bounds ((lo, hi) along t1, (lo, hi) along t2) in metres; with them the flux is
+    # read out in the colocated convention (reference ``openem/model.py`` FluxMonitor.t_bounds)
+    t_bounds: tuple[tuple[float, float], tuple[float, float]] | None = None
 
 
 @dataclass(frozen=True)

@@ -118,9 +118,12 @@ def slab_scene(dl: float = 0.02 * UM, size=(0.6 * UM, 0.6 * UM, 3.0 * UM), eps_s
     src = plane_wave(grid, ex if pol_axis == 0 else ey, dt, n_steps, src_z, direction,
                      pol_axis, pulse)
     freqs = np.linspace(f0 * 0.9, f0 * 1.1, nf)
-    mons = [FluxMonitor("T", 2, nearest_edge_index(az.edges, t_z), +1, freqs)]
+    # infinite monitors: their physical bounds are the whole periodic cross-section
+    tb = ((float(ax.edges[0]), float(ax.edges[-1])), (float(ay.edges[0]), float(ay.edges[-1])))
+    mons = [FluxMonitor("T", 2, nearest_edge_index(az.edges, t_z), +1, freqs, t_bounds=tb)]
     if r_z is not None:
-        mons.append(FluxMonitor("R", 2, nearest_edge_index(az.edges, r_z), -1, freqs))
+        mons.append(FluxMonitor("R", 2, nearest_edge_index(az.edges, r_z), -1, freqs,
+                                t_bounds=tb))
     return Scene(grid=grid, dt=dt, num_time_steps=n_steps, eps_ex=ex, eps_ey=ey, eps_ez=ez,
                  pml=pml_faces(dt, (dl, dl, dl), [(2, "lo"), (2, "hi")], n_pml),
                  sources=[src], flux_monitors=mons)

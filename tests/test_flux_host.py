@@ -1,0 +1,122 @@
+"""CPU tests of the plane-flux readout conventions (in-place Yee and colocated)."""
+
+import numpy as np
+import pytest
+
+from openem_amd import flux
+from openem_amd.grid import Axis, Grid, uniform_axis
+from openem_amd.model import FluxMonitor
+
+
+def _axis(rng, n, lo, bnd="PML"):
+    dl = rng.uniform(0.6, 1.6, n) * 1e-8
+    return Axis(lo + np.concatenate([[0.0], np.cumsum(dl)]), bnd, bnd)
+
+
+def test_axis_weights_equal_clipped_dual_cells():
+    """Brute force: the weight of edge i is the length of [cell_lo_i, cell_hi_i] inside
+    [lo, hi], the cell running between the midpoints next to the edge (the grid ends close the
+    outer cells); the returned run is the contiguous set of non-zero weights."""
+    rng = np.random.default_rng(4)
+    for trial in range(40):
+        ax = _axis(rng, int(rng.integers(3, 12)), -3e-8)
+        e = ax.edges
+        span = e[-1] - e[0]
+        lo = e[0] + rng.uniform(-0.2, 0.5) * span
+        hi = max(lo, e[0]) + rng.uniform(0.05, 1.0) * span
+        want = []
+        for i in range(e.size):
+            c_lo = e[0] if i == 0 else 0.5 * (e[i - 1] + e[i])
+            c_hi = e[-1] if i == e.size - 1 else 0.5 * (e[i] + e[i + 1])
+            want.append(max(0.0, min(c_hi, hi) - max(c_lo, lo)))
+        want = np.array(want)
+        nz = [i for i in range(e.size) if want[i] > 0]
+        i0, i1, w = flux.colocated_axis_weights(e, lo, hi)
+        assert (i0, i1) == (nz[0], nz[-1])
+        assert np.allclose(w, want[i0:i1 + 1], rtol=0, atol=1e-24)
+        assert abs(w.sum() - (min(hi, e[-1]) - max(lo, e[0]))) < 1e-22
+    with pytest.raises(ValueError):
+        flux.colocated_axis_weights(e, e[-1] + 1e-9, e[-1] + 2e-9)
+
+
+def _linear(c, x, y):
+    return c[0] + c[1] * x + c[2] * y
+
+
+def test_colocated_flux_is_exact_for_linear_fields():
+    """Fields linear in (x1, x2) sampled at their Yee positions: centre -> edge interpolation of
+    a linear function is exact, so the colocated flux equals the trapezoid sum of the exact
+    product at the edge samples.  A swapped interpolation axis or weight fails this."""
+    rng = np.random.default_rng(7)
+    a0, a1, a2 = _axis(rng, 6, 0.0), _axis(rng, 9, -4e-8), _axis(rng, 11, -5e-8)
+    g = Grid(a0, a1, a2)
+    e1, e2 = a1.edges, a2.edges
+    c1, c2 = a1.centers, a2.centers
+    bounds = ((e1[1] + 0.3 * (e1[2] - e1[1]), e1[-2] - 0.2 * (e1[-1] - e1[-2])),
+              (e2[2] - 0.4 * (e2[2] - e2[1]), e2[-3] + 0.1 * (e2[-2] - e2[-3])))
+    coef = [rng.normal(size=3) + 1j * rng.normal(size=3) for _ in range(4)]
+    nf = 2
+    data = np.zeros((4, nf, a1.n, a2.n), dtype=np.complex128)
+    pos = {0: (c1, e2[:-1]), 3: (c1, e2[:-1]), 1: (e1[:-1], c2), 2: (e1[:-1], c2)}
+    for comp, (x, y) in pos.items():
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        for f in range(nf):
+            data[comp, f] = (f + 1) * _linear(coef[comp], X / 1e-8, Y / 1e-8)
+    mon = FluxMonitor(name="m", axis=0, plane_index=3, normal_dir=1, freqs=np.ones(nf),
+                      t_bounds=bounds)
+    got = flux.monitor_flux_colocated(data, g, mon)
+    i0, i1, w1 = flux.colocated_axis_weights(e1, *bounds[0])
+    j0, j1, w2 = flux.colocated_axis_weights(e2, *bounds[1])
+    assert i0 >= 1 and j0 >= 1 and i1 <= a1.n - 1 and j1 <= a2.n - 1
+    X, Y = np.meshgrid(e1[i0:i1 + 1] / 1e-8, e2[j0:j1 + 1] / 1e-8, indexing="ij")
+    for f in range(nf):
+        s = (f + 1) ** 2
+        eu, ev = _linear(coef[0], X, Y), _linear(coef[1], X, Y)
+        hu, hv = _linear(coef[2], X, Y), _linear(coef[3], X, Y)
+        want = 0.5 * s * np.real(np.sum((eu * np.conj(hv) - ev * np.conj(hu))
+                                        * np.outer(w1, w2)))
+        assert abs(got[f] - want) < 1e-12 * abs(want), (got[f], want)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_full_periodic_span_equals_the_yee_sum_for_uniform_fields(axis):
+    """A monitor spanning periodic transverse axes whole: the wrapped colocation integrates
+    exactly one period, so uniform fields give the in-place Yee flux (including the -S_y sign
+    convention of the y normal)."""
+    axes = [uniform_axis(8, 2e-8, 0.0, "PML")] * 3
+    for t in range(3):
+        if t != axis:
+            axes[t] = uniform_axis(6 + t, 2e-8, -6e-8, "Periodic")
+    g = Grid(*axes)
+    from openem_amd.grid import transverse_axes
+    t1, t2 = transverse_axes(axis)
+    n1, n2 = g.axes[t1].n, g.axes[t2].n
+    v = np.array([1.0 + 0.5j, -0.3 + 0.2j, 0.7 - 0.1j, 0.4 + 0.9j])
+    data = np.broadcast_to(v[:, None, None, None], (4, 1, n1, n2)).copy()
+    tb = tuple((float(g.axes[t].edges[0]), float(g.axes[t].edges[-1])) for t in (t1, t2))
+    mon = FluxMonitor(name="m", axis=axis, plane_index=4, normal_dir=1, freqs=np.ones(1),
+                      t_bounds=tb)
+    col = flux.monitor_flux_colocated(data, g, mon)
+    yee = (-1.0 if axis == 1 else 1.0) * flux.plane_flux(data, g, axis=axis)
+    assert abs(col[0] - yee[0]) < 1e-12 * abs(yee[0])
+    assert abs(flux.monitor_flux(data, g, mon)[0] - yee[0]) < 1e-12 * abs(yee[0])
+
+
+def test_refused_colocation_falls_back_to_the_yee_sum(monkeypatch, capsys):
+    """Samples on a PEC grid end cannot be dropped: the readout falls back to the in-place Yee
+    sum with a message, as the reference does; OPENEM_FLUX_COLOCATED=0 selects it outright."""
+    rng = np.random.default_rng(1)
+    g = Grid(uniform_axis(5, 2e-8, 0.0, "PML"), uniform_axis(6, 2e-8, 0.0, "PECBoundary"),
+             uniform_axis(7, 2e-8, 0.0, "PML"))
+    data = rng.normal(size=(4, 2, 6, 7)) + 1j * rng.normal(size=(4, 2, 6, 7))
+    mon = FluxMonitor(name="m", axis=0, plane_index=2, normal_dir=1, freqs=np.ones(2),
+                      t_bounds=((0.0, 12e-8), (2e-8, 10e-8)))
+    with pytest.raises(ValueError):
+        flux.monitor_flux_colocated(data, g, mon)
+    yee = flux.plane_flux(data, g, axis=0)
+    assert np.array_equal(flux.monitor_flux(data, g, mon), yee)
+    assert "refused" in capsys.readouterr().out
+    monkeypatch.setenv("OPENEM_FLUX_COLOCATED", "0")
+    ok = FluxMonitor(name="m", axis=0, plane_index=2, normal_dir=1, freqs=np.ones(2),
+                     t_bounds=((2e-8, 10e-8), (2e-8, 10e-8)))
+    assert np.array_equal(flux.monitor_flux(data, g, ok), yee)

@@ -184,17 +184,32 @@ def test_tfsf_box_matches_oracle(gpu_lib, kw):
         assert err < 1e-4, (m.name, err)
 
 
-@pytest.mark.parametrize("name", ["all_features", "ey_minus_z_nonuniform", "pec_walls"])
+FUSED_CASES = {
+    "all_features": lambda n: scenes.mini_scene(num_steps=n),
+    "ey_minus_z_nonuniform": lambda n: scenes.mini_scene(num_steps=n, **CASES["ey_minus_z_nonuniform"]),
+    "pec_walls": lambda n: scenes.mini_scene(num_steps=n, **CASES["pec_walls"]),
+    # three k strips (63 + 63 + 24), row groups of 8, 8 and 5, CPML on all faces, ADE boxes
+    "soup_24x21x150": lambda n: scenes.box_soup_scene(shape=(24, 21, 150), n_pml=6, n_boxes=8,
+                                                      num_steps=n, seed=3, moment=1e-18),
+}
+# the complex field path at k = 0 on the all-features scene
+FUSED_COMPLEX = {"all_features_complex": "all_features"}
+
+
+@pytest.mark.parametrize("name", sorted(FUSED_CASES) + sorted(FUSED_COMPLEX))
 def test_fused_step_equals_the_two_sweeps(gpu_lib, monkeypatch, name):
     """The fused H+E march (ping-pong buffers, halo recompute) gives the fields and phasors of
     the separate H and E sweeps; the arithmetic per cell is the same, so they agree to float32
     rounding (1e-6 of the peak)."""
-    kw = CASES[name]
     n = 180
-    tabs = tables.build(scenes.mini_scene(num_steps=n, **kw))
-    print(f"fused vs two sweeps: {name}", flush=True)
+    if name in FUSED_COMPLEX:
+        tabs = tables.build(FUSED_CASES[FUSED_COMPLEX[name]](n), force_complex=True)
+    else:
+        tabs = tables.build(FUSED_CASES[name](n))
+    print(f"fused vs two sweeps: {name} {tabs.shape}", flush=True)
+    monkeypatch.setenv("OEM_FUSE", "1")
     f_fused, acc_fused = _gpu_state(tabs, n)
-    monkeypatch.setenv("OEM_NO_FUSE", "1")
+    monkeypatch.delenv("OEM_FUSE")
     f_two, acc_two = _gpu_state(tabs, n)
     for c in range(6):
         assert _rel(f_fused[c], f_two[c]) <= 1e-6 if np.abs(f_two[c]).max() else True, c

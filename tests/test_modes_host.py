@@ -95,6 +95,12 @@ def test_mode_source_is_one_directional_in_the_oracle(direction):
             amp[m.name] = float(flux.monitor_flux(o.acc[i], sc.grid, sc.flux_monitors[i])[0]) / S ** 2
     fwd, back = (0, 1) if direction > 0 else (1, 0)
     assert abs(amp["fwd"][fwd] - 0.5) < 5e-3, amp                # Re(profile amp): half at +f0
+    # tidy3d-convention normalisation: the unit-power mode source launches unit amplitude
+    from openem_amd import normalize
+    m0 = next(i for i, m in enumerate(t.monitors) if m.name == "fwd")
+    a_fwd = modes.amplitudes(o.acc[m0], t.monitors[m0].modes[0], sc.grid,
+                             modes.orientation(t.monitors[m0].axis))[fwd][0]
+    assert abs(abs(a_fwd / normalize.field_norm(sc, [f0])[0]) - 1.0) < 1e-2
     assert amp["fwd"][back] < 1e-3 and amp["back"][back] < 1e-3, amp
     # the plane's whole flux also counts the little power radiated where the mode's evanescent
     # tail meets the transverse PML, which the plain-Yee mode operator does not model

@@ -117,6 +117,25 @@ def test_slab_transmission_matches_reference(slab_runs):
     assert abs(R[5] - 0.1689) < 2e-4                      # docs/tutorial.md:102
 
 
+def test_normalized_single_run_transmission(slab_runs):
+    """The tidy3d-convention normalisation (analytic plane-wave divisor |D|^2 n A / (2 eta0)
+    cos(k dz_h / 2)) turns ONE run's flux into T: the homogeneous run reads 1 and the slab run
+    the reference's published T = 0.8310044973694679 (README.md:62), R = 0.1689
+    (docs/tutorial.md:102)."""
+    from openem_amd import normalize
+    from openem_amd.results import Phasors, Result
+    for hom, want_t in ((True, 1.0), (False, 0.8310044973694679)):
+        sc, _, acc = slab_runs[hom]
+        res = Result(phasors={m.name: Phasors(a, m.freqs) for a, m in zip(acc, sc.flux_monitors)})
+        T = normalize.normalized_flux(sc, res, "T")
+        assert abs(T[5] - want_t) < 2e-6, (hom, T[5])
+        if hom:
+            assert np.all(np.abs(T - 1.0) < 1e-6)
+        else:
+            R = normalize.normalized_flux(sc, res, "R")
+            assert abs(R[5] - 0.1689) < 2e-4 and np.all(np.abs(T + R - 1.0) < 1e-4)
+
+
 def test_tfsf_injection_is_one_sided(slab_runs):
     """Homogeneous run: the flux upstream of the source plane (scattered-field side) is the
     leakage of the TF/SF injection plus the PML reflection, < 1e-8 of the injected flux."""
