@@ -35,6 +35,9 @@ class PlaneWaveSource:
     hy_inc: np.ndarray
     angle_theta: float = 0.0
     num_freqs: int = 1
+    # the user's source-plane coordinate before snapping to the grid (nan: unknown); only the
+    # readout normalisation uses it
+    center: float = float("nan")
 
 
 @dataclass(frozen=True)

@@ -1,0 +1,169 @@
+"""CPU tests of the ``scene.npz`` reader/writer against a file the reference's own
+``openem/serialize.py`` wrote (``tests/golden/scene_v10_small.npz``, made by
+``tests/golden/make_scene_npz.py``) and by round trips of this package's scenes."""
+
+import dataclasses
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from openem_amd import scene_io, scenes, tables
+from oracle.yee64 import Oracle64
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "scene_v10_small.npz"
+
+
+def _raw():
+    with np.load(GOLDEN, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_reference_written_file_decodes_key_for_key():
+    """Every array the reference wrote lands in the matching Scene field."""
+    z = _raw()
+    sc = scene_io.load(GOLDEN)
+    assert int(z["format_version"]) == 10
+    for n, ax in zip("xyz", sc.grid.axes):
+        assert np.array_equal(ax.edges, z[f"edges_{n}"])
+        assert (ax.boundary_lo, ax.boundary_hi) == tuple(str(v) for v in z[f"bnd_{n}"])
+    for c in ("ex", "ey", "ez"):
+        assert np.array_equal(getattr(sc, f"eps_{c}"), z[f"eps_{c}"])
+        assert np.array_equal(getattr(sc, f"sigma_{c}"), z[f"sigma_{c}"])
+        assert np.array_equal(getattr(sc, f"pec_{c}"), z[f"pec_{c}"])
+    assert (sc.dt, sc.num_time_steps, sc.shutoff) == (float(z["dt"]), int(z["num_time_steps"]),
+                                                      float(z["shutoff"]))
+    assert sorted(sc.pml) == [(2, "hi"), (2, "lo")]
+    for (a, s), c in sc.pml.items():
+        for f in scene_io._PML_FIELDS:
+            assert np.array_equal(getattr(c, f), z[f"pml_{a}_{s}_{f}"])
+    src = sc.sources[0]
+    assert [src.axis, src.plane_index, src.direction, src.pol_axis, src.num_freqs] == \
+        list(z["src0_meta"])
+    assert src.center == float(z["src0_center"])
+    assert np.array_equal(src.ex_inc, z["src0_ex_inc"]) and np.array_equal(src.hy_inc, z["src0_hy_inc"])
+    assert np.array_equal(src.waveform.amp_half, z["src0_amp_half"])
+    d0, d1 = sc.dipoles
+    assert np.array_equal(d0.waveform.amp_int_complex, z["dip0_amp_int"] + 1j * z["dip0_amp_int_im"])
+    assert np.array_equal(d0.waveform.amp_half_complex, z["dip0_amp_half"] + 1j * z["dip0_amp_half_im"])
+    assert np.array_equal(d1.waveform.amp_half_complex, z["dip1_amp_half"])   # no imaginary part stored
+    assert (d0.magnetic, d1.magnetic, d0.component, d1.component) == (False, True, 2, 0)
+    assert np.array_equal(d0.indices, z["dip0_indices"]) and np.array_equal(d0.coef, z["dip0_coef"])
+    ms = sc.mode_sources[0]
+    assert [ms.plane_index, ms.direction, ms.axis] == list(z["msrc0_meta"])
+    assert ms.n_eff == complex(*z["msrc0_n_eff"])
+    for f in scene_io._MODE_FIELDS:
+        assert np.array_equal(getattr(ms, f), z[f"msrc0_{f}"])
+    for f in ("comp", "cell", "pole_ofs", "am1", "b", "g"):
+        assert np.array_equal(getattr(sc.dispersion, f), z[f"disp_{f}"])
+    t, r = sc.flux_monitors
+    assert (t.name, t.axis, t.plane_index, t.normal_dir) == ("T", *map(int, z["mon0_meta"]))
+    assert t.transverse == tuple(map(tuple, z["mon0_transverse"].tolist()))
+    assert t.t_bounds == tuple(map(tuple, z["mon0_tbounds"].tolist()))
+    assert t.apodization[1] is None and t.apodization[0] == z["mon0_apod"][0]
+    assert r.transverse is None and r.t_bounds is None and r.normal_dir == -1
+    box, plane = sc.field_monitors
+    assert box.origin + box.box == tuple(int(v) for v in z["fmon0_meta"][:6])
+    # the mode monitor names its planar field monitor: normal = the thin box axis, the plane one
+    # spare cell above the box origin
+    (mm,) = sc.mode_monitors
+    assert (mm.name, mm.axis, mm.plane_index) == ("mm", 2, plane.origin[2] + plane.box[2] - 1)
+    assert np.array_equal(mm.freqs, plane.freqs)
+
+
+def test_save_writes_the_reference_layout():
+    """Re-saving the decoded reference scene reproduces every key the reference wrote with the
+    same value (the mode monitor's plane name aside: this package stores the plane itself)."""
+    z = _raw()
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        p = scene_io.save(scene_io.load(GOLDEN), Path(td) / "s.npz")
+        with np.load(p, allow_pickle=False) as y:
+            mine = {k: y[k] for k in y.files}
+    for k, v in z.items():
+        if k == "modemon_planes":
+            continue
+        assert k in mine, k
+        a, b = np.asarray(mine[k]), np.asarray(v)
+        if a.dtype.kind in "US":
+            assert a.tolist() == b.tolist(), k
+        else:
+            assert a.shape == b.shape and np.array_equal(a, b, equal_nan=a.dtype.kind in "fc"), k
+
+
+def _same(a, b, path="scene"):
+    if dataclasses.is_dataclass(a):
+        assert type(a) is type(b), path
+        for f in dataclasses.fields(a):
+            _same(getattr(a, f.name), getattr(b, f.name), f"{path}.{f.name}")
+    elif isinstance(a, np.ndarray) or isinstance(b, np.ndarray):
+        a, b = np.asarray(a), np.asarray(b)
+        assert a.shape == b.shape and np.array_equal(a, b, equal_nan=a.dtype.kind in "fc"), path
+    elif isinstance(a, dict):
+        assert sorted(a) == sorted(b), path
+        for k in a:
+            _same(a[k], b[k], f"{path}[{k}]")
+    elif isinstance(a, (list, tuple)):
+        assert len(a) == len(b), path
+        for i, (x, y) in enumerate(zip(a, b)):
+            _same(x, y, f"{path}[{i}]")
+    elif hasattr(a, "edges") and hasattr(a, "boundary_lo"):
+        _same(a.edges, b.edges, path + ".edges")
+        assert (a.boundary_lo, a.boundary_hi) == (b.boundary_lo, b.boundary_hi), path
+    elif hasattr(a, "axes"):
+        _same(list(a.axes), list(b.axes), path + ".axes")
+    elif isinstance(a, float) and np.isnan(a):
+        assert isinstance(b, float) and np.isnan(b), path
+    else:
+        assert a == b, (path, a, b)
+
+
+SCENES = {
+    "mini_all": lambda: scenes.mini_scene(num_steps=30),
+    "mini_pml_sigma_pec_hdip": lambda: scenes.mini_scene(num_steps=30, pml_xy=True, sigma_on=True,
+                                                         pec_on=True, magnetic_dipole=True),
+    "slab": lambda: scenes.slab_scene(size=(0.04e-6, 0.04e-6, 3.0e-6), num_steps=30),
+    "soup": lambda: scenes.box_soup_scene(shape=(20, 22, 24), n_pml=6, n_boxes=5, num_steps=30,
+                                          seed=2),
+    "waveguide": lambda: scenes.waveguide_scene(shape=(30, 28, 26), n_pml=6, width_cells=8,
+                                                thick_cells=4, num_steps=30, nf=1),
+    "tfsf_box": lambda: scenes.tfsf_box_scene(num_steps=30, scatterer=True),
+    "bloch": lambda: scenes.bloch_cell_scene(nxy=8, nz=40, n_pml=6, num_steps=30, nf=2),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_round_trip_keeps_every_field_and_the_step(tmp_path, name):
+    sc = SCENES[name]()
+    back = scene_io.load(scene_io.save(sc, tmp_path / f"{name}.npz"))
+    _same(sc, back)
+    t1, t2 = tables.build(sc), tables.build(back)
+    o1, o2 = Oracle64(t1).run(12), Oracle64(t2).run(12)
+    assert np.array_equal(o1.E, o2.E) and np.array_equal(o1.H, o2.H)
+
+
+@pytest.mark.parametrize("extra, what", [
+    ({"mod_comp": np.array([0])}, "time modulation"),
+    ({"dmix_comp": np.array([0])}, "mixed-interface"),
+    ({"tensor_comp": np.array([0])}, "tensor"),
+    ({"n_absorbers": np.array(1)}, "absorber"),
+    ({"symmetry": np.array([1, 0, 0])}, "symmetry"),
+    ({"tmon_names": np.array(["t"])}, "field-time"),
+    ({"flat_y": np.array(True)}, "flat"),
+    ({"msrc0_bb_ey": np.zeros(2)}, "broadband"),
+])
+def test_entities_this_stepper_does_not_model_fail_closed(tmp_path, extra, what):
+    z = _raw()
+    z.update(extra)
+    p = tmp_path / "x.npz"
+    np.savez(p, **z)
+    with pytest.raises(NotImplementedError, match=what):
+        scene_io.load(p)
+
+
+def test_unknown_format_version_is_refused(tmp_path):
+    z = _raw()
+    z["format_version"] = np.array(11)
+    np.savez(tmp_path / "v.npz", **z)
+    with pytest.raises(ValueError, match="format version 11"):
+        scene_io.load(tmp_path / "v.npz")
