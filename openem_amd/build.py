@@ -15,7 +15,10 @@ OUT = PKG / "libopenem_hip.so"
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
          # no packed-FP32 instructions: hipcc may form v_pk_* from scalar float code
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # no multiply-add contraction: every kernel rounds an expression the way it is written,
+         # so the fused step, the two sweeps and the pool kernels agree bit for bit
+         "-ffp-contract=off"]
 
 
 def hipcc() -> str:

@@ -1,7 +1,7 @@
 // Device kernels of the MI355X (gfx950, CDNA4) FDTD stepper.
 //
 // The Yee update is HBM-bound (about 1 flop per byte), so the design goal is: every field
-// byte crosses HBM once per sweep.  Each work-group owns a 64 (k) x 4 (j) tile of the (ny, nz)
+// byte crosses HBM once per sweep.  Each work-group owns a 64 (k) x 8 (j) tile of the (ny, nz)
 // plane and marches along x (the slowest axis) over a chunk of i-planes.  The x neighbour plane
 // the curl needs is carried in registers from one iteration to the next; the y/z neighbours are
 // in the same or an adjacent tile and are served by L1/L2.  The linear block id is remapped so
@@ -25,7 +25,7 @@
 #define OEM_MAX_PW 2
 #define OEM_TK 64
 #ifndef OEM_TJ
-#define OEM_TJ 4
+#define OEM_TJ 8
 #endif
 #define OEM_LDS_CLASSES 512
 
@@ -508,10 +508,13 @@ k_update_e(F *__restrict__ E, const F *__restrict__ H, PsiDev ps, GridDev g, Mat
 #define OEM_WR 8            // owned rows per wave
 #define OEM_WW 4            // independent waves per work-group
 
+// One descriptor per field set (three components back to back, the component picked by the
+// scalar offset: `cb` bytes per component) keeps the march's SGPRs below the spill point; the
+// host limits the fused step to sets under 4 GiB.
 struct FusedIO {
-    rsrc_t ex, ey, ez, hx, hy, hz;          // old fields
-    rsrc_t hxn, hyn, hzn, exn, eyn, ezn;    // new fields
+    rsrc_t eo, ho, en, hn;                  // old / new E and H
     rsrc_t ph_o, ph_n, pe_o, pe_n;          // psi: H side old / new, E side old / new
+    uint32_t cb;                            // bytes per component
 };
 
 // Everything the new H at one cell reads: old E at the cell, at X.nxt[i] (ey, ez), at Y.nxt[j]
@@ -522,6 +525,15 @@ struct HIn {
     F px0, px1, py0, py1, pz0, pz1;
 };
 
+// Everything one row task reads: the H inputs and, for owned rows, the class index, the E-side
+// psi and the ADE history.
+template <typename F>
+struct RowIn {
+    HIn<F> v;
+    uint32_t cls;
+    F pe[6], ph[3];
+};
+
 // Offsets of the psi elements of cell (i, j, k) in the x / y / z slabs (ps.off order), and the
 // slab indices (-1 outside).
 struct PsiAt {
@@ -530,15 +542,43 @@ struct PsiAt {
     uint32_t sx0, sx1, sy0, sy1, sz0, sz1;  // per-plane byte offsets
 };
 
+// Per-lane values of the H update at k, loaded once per kernel: inside the march the compiler
+// must assume the buffer stores may alias the axis tables and would reload them every row.
+struct KConst {
+    int k, kn, sz;              // cell, Z.nxt[k], z slab index (-1 outside)
+    float mk, mki, idz;         // forward mask (re, im) and 1/dl
+    float a, b, ik;             // H-side CPML of the z slab
+};
+
+__device__ __forceinline__ KConst k_const(const GridDev &g, int k) {
+    const AxisDev &Z = g.ax[2];
+    KConst c;
+    c.k = k;
+    c.kn = Z.nxt[k];
+    c.sz = slab_of(k, g.nz, Z.lo, Z.hi);
+    c.mk = axv(Z, k, AX_MNX);
+    c.mki = axv(Z, k, AX_MNXI);
+    c.idz = axv(Z, k, AX_IDLP);
+    c.a = 0.f; c.b = 1.f; c.ik = 1.f;
+    if (c.sz >= 0) { c.a = axv(Z, k, AX_AH); c.b = axv(Z, k, AX_BH); c.ik = axv(Z, k, AX_KH); }
+    return c;
+}
+
+// Next index along an axis: interior neighbours are i + 1 (the host checks), only the last
+// cell's neighbour comes from the table (`last` = A.nxt[n - 1], loaded once).
+__device__ __forceinline__ int next_of(int i, int n, int last) { return i + 1 < n ? i + 1 : last; }
+
 template <typename F>
-__device__ __forceinline__ PsiAt psi_at(const PsiDev &ps, const GridDev &g, int i, int j, int k) {
+__device__ __forceinline__ PsiAt psi_at(const PsiDev &ps, const GridDev &g, int i, int j,
+                                        const KConst &kc) {
     typedef FT<F> T;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+    const int k = kc.k;
     PsiAt q;
     q.sx = slab_of(i, g.nx, X.lo, X.hi);
     q.sy = slab_of(j, g.ny, Y.lo, Y.hi);
-    q.sz = slab_of(k, g.nz, Z.lo, Z.hi);
+    q.sz = kc.sz;
     q.ox = (uint32_t)(j * g.nz + k) * T::B;
     q.oy = (uint32_t)(max(q.sy, 0) * g.nz + k) * T::B;
     q.oz = (uint32_t)(j * wz + max(q.sz, 0)) * T::B;
@@ -551,42 +591,43 @@ __device__ __forceinline__ PsiAt psi_at(const PsiDev &ps, const GridDev &g, int 
     return q;
 }
 
-// Issue every load of the new H at (i, j, k).
+// Issue every load of the new H at (i, j, k); `in_` / `jn` are the x / y neighbours.
 template <typename F>
 __device__ __forceinline__ void fused_h_ld(const FusedIO &io, const GridDev &g, const PsiAt &q,
-                                           int i, int j, int k, HIn<F> &v) {
+                                           int i, int in_, int j, int jn, const KConst &kc,
+                                           HIn<F> &v) {
     typedef FT<F> T;
-    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
-    const int nz = g.nz;
+    const int nz = g.nz, k = kc.k;
     const uint32_t planeB = (uint32_t)g.plane * T::B;
-    const uint32_t so = (uint32_t)i * planeB, sn = (uint32_t)X.nxt[i] * planeB;
+    const uint32_t so = (uint32_t)i * planeB, sn = (uint32_t)in_ * planeB;
     const uint32_t op = (uint32_t)(j * nz + k) * T::B;
-    const uint32_t oj = (uint32_t)(Y.nxt[j] * nz + k) * T::B;
-    const uint32_t ok = (uint32_t)(j * nz + Z.nxt[k]) * T::B;
-    v.ex = T::ld(io.ex, op, so); v.ey = T::ld(io.ey, op, so); v.ez = T::ld(io.ez, op, so);
-    v.eyn = T::ld(io.ey, op, sn); v.ezn = T::ld(io.ez, op, sn);
-    v.ex_j = T::ld(io.ex, oj, so); v.ez_j = T::ld(io.ez, oj, so);
-    v.ex_k = T::ld(io.ex, ok, so); v.ey_k = T::ld(io.ey, ok, so);
-    v.hx = T::ld(io.hx, op, so); v.hy = T::ld(io.hy, op, so); v.hz = T::ld(io.hz, op, so);
+    const uint32_t oj = (uint32_t)(jn * nz + k) * T::B;
+    const uint32_t ok = (uint32_t)(j * nz + kc.kn) * T::B;
+    const uint32_t c1 = io.cb, c2 = 2u * io.cb;
+    v.ex = T::ld(io.eo, op + so, 0u); v.ey = T::ld(io.eo, op + so + c1, 0u); v.ez = T::ld(io.eo, op + so + c2, 0u);
+    v.eyn = T::ld(io.eo, op + sn + c1, 0u); v.ezn = T::ld(io.eo, op + sn + c2, 0u);
+    v.ex_j = T::ld(io.eo, oj + so, 0u); v.ez_j = T::ld(io.eo, oj + so + c2, 0u);
+    v.ex_k = T::ld(io.eo, ok + so, 0u); v.ey_k = T::ld(io.eo, ok + so + c1, 0u);
+    v.hx = T::ld(io.ho, op + so, 0u); v.hy = T::ld(io.ho, op + so + c1, 0u); v.hz = T::ld(io.ho, op + so + c2, 0u);
     v.px0 = v.px1 = v.py0 = v.py1 = v.pz0 = v.pz1 = T::zero();
-    if (q.sx >= 0) { v.px0 = T::ld(io.ph_o, q.ox, q.sx0); v.px1 = T::ld(io.ph_o, q.ox, q.sx1); }
-    if (q.sy >= 0) { v.py0 = T::ld(io.ph_o, q.oy, q.sy0); v.py1 = T::ld(io.ph_o, q.oy, q.sy1); }
-    if (q.sz >= 0) { v.pz0 = T::ld(io.ph_o, q.oz, q.sz0); v.pz1 = T::ld(io.ph_o, q.oz, q.sz1); }
+    if (q.sx >= 0) { v.px0 = T::ld(io.ph_o, q.ox + q.sx0, 0u); v.px1 = T::ld(io.ph_o, q.ox + q.sx1, 0u); }
+    if (q.sy >= 0) { v.py0 = T::ld(io.ph_o, q.oy + q.sy0, 0u); v.py1 = T::ld(io.ph_o, q.oy + q.sy1, 0u); }
+    if (q.sz >= 0) { v.pz0 = T::ld(io.ph_o, q.oz + q.sz0, 0u); v.pz1 = T::ld(io.ph_o, q.oz + q.sz1, 0u); }
 }
 
 // H^{n+1/2} at (i, j, k) from loaded inputs (the arithmetic of update_h_tile); with `store` the
 // new H and H-side psi are written.
 template <typename F>
 __device__ __forceinline__ void fused_h_do(const FusedIO &io, const GridDev &g, const PsiAt &q,
-                                           int i, int j, int k, HIn<F> &v, float ch,
+                                           int i, int j, const KConst &kc, HIn<F> &v, float ch,
                                            float inj_hx, float inj_hy, bool store,
                                            F &hx, F &hy, F &hz) {
     typedef FT<F> T;
-    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const AxisDev &X = g.ax[0], &Y = g.ax[1];
     const uint32_t so = (uint32_t)i * ((uint32_t)g.plane * T::B);
-    const float mj = axv(Y, j, AX_MNX), mk = axv(Z, k, AX_MNX);
-    const float mji = axv(Y, j, AX_MNXI), mki = axv(Z, k, AX_MNXI);
-    const float idy = axv(Y, j, AX_IDLP), idz = axv(Z, k, AX_IDLP);
+    const float mj = axv(Y, j, AX_MNX), mk = kc.mk;
+    const float mji = axv(Y, j, AX_MNXI), mki = kc.mki;
+    const float idy = axv(Y, j, AX_IDLP), idz = kc.idz;
     const float mi = axv(X, i, AX_MNX), mii = axv(X, i, AX_MNXI), idx = axv(X, i, AX_IDLP);
     const F dzy = (T::mmul(v.ez_j, mj, mji) - v.ez) * idy;
     const F dyz = (T::mmul(v.ey_k, mk, mki) - v.ey) * idz;
@@ -599,27 +640,26 @@ __device__ __forceinline__ void fused_h_do(const FusedIO &io, const GridDev &g, 
         const float a_ = axv(X, i, AX_AH), b_ = axv(X, i, AX_BH), k_ = axv(X, i, AX_KH);
         s_hy_x = stretch_v(v.px0, dzx, a_, b_, k_);
         s_hz_x = stretch_v(v.px1, dyx, a_, b_, k_);
-        if (store) { T::st(io.ph_n, v.px0, q.ox, q.sx0); T::st(io.ph_n, v.px1, q.ox, q.sx1); }
+        if (store) { T::st(io.ph_n, v.px0, q.ox + q.sx0, 0u); T::st(io.ph_n, v.px1, q.ox + q.sx1, 0u); }
     }
     if (q.sy >= 0) {
         const float a_ = axv(Y, j, AX_AH), b_ = axv(Y, j, AX_BH), k_ = axv(Y, j, AX_KH);
         s_hx_y = stretch_v(v.py0, dzy, a_, b_, k_);
         s_hz_y = stretch_v(v.py1, dxy, a_, b_, k_);
-        if (store) { T::st(io.ph_n, v.py0, q.oy, q.sy0); T::st(io.ph_n, v.py1, q.oy, q.sy1); }
+        if (store) { T::st(io.ph_n, v.py0, q.oy + q.sy0, 0u); T::st(io.ph_n, v.py1, q.oy + q.sy1, 0u); }
     }
     if (q.sz >= 0) {
-        const float a_ = axv(Z, k, AX_AH), b_ = axv(Z, k, AX_BH), k_ = axv(Z, k, AX_KH);
-        s_hx_z = stretch_v(v.pz0, dyz, a_, b_, k_);
-        s_hy_z = stretch_v(v.pz1, dxz, a_, b_, k_);
-        if (store) { T::st(io.ph_n, v.pz0, q.oz, q.sz0); T::st(io.ph_n, v.pz1, q.oz, q.sz1); }
+        s_hx_z = stretch_v(v.pz0, dyz, kc.a, kc.b, kc.ik);
+        s_hy_z = stretch_v(v.pz1, dxz, kc.a, kc.b, kc.ik);
+        if (store) { T::st(io.ph_n, v.pz0, q.oz + q.sz0, 0u); T::st(io.ph_n, v.pz1, q.oz + q.sz1, 0u); }
     }
     hx = T::addre(v.hx - ch * (s_hx_y - s_hx_z), inj_hx);
     hy = T::addre(v.hy - ch * (s_hy_z - s_hy_x), inj_hy);
     hz = v.hz - ch * (s_hz_x - s_hz_y);
     if (store) {
-        T::st(io.hxn, hx, q.ox, so);
-        T::st(io.hyn, hy, q.ox, so);
-        T::st(io.hzn, hz, q.ox, so);
+        T::st(io.hn, hx, q.ox + so, 0u);
+        T::st(io.hn, hy, q.ox + so + io.cb, 0u);
+        T::st(io.hn, hz, q.ox + so + 2u * io.cb, 0u);
     }
 }
 
@@ -648,12 +688,13 @@ template <> __device__ __forceinline__ cf2 lane_prev_f<cf2>(cf2 x) {
     return {lane_prev(x.r), lane_prev(x.i)};
 }
 
-#ifndef OEM_HE_WAVES_PER_EU
-#define OEM_HE_WAVES_PER_EU 4
+#ifdef OEM_HE_WAVES_PER_EU
+#define OEM_HE_ATTR __attribute__((amdgpu_waves_per_eu(OEM_HE_WAVES_PER_EU)))
+#else
+#define OEM_HE_ATTR
 #endif
 template <typename F>
-__global__ void __launch_bounds__(64 * OEM_WW)
-__attribute__((amdgpu_waves_per_eu(OEM_HE_WAVES_PER_EU)))
+__global__ void __launch_bounds__(64 * OEM_WW) OEM_HE_ATTR
 k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ En,
             F *__restrict__ Hn, PsiDev ps, PsiDev pn, GridDev g, MatDev mat, float ch, float cw,
             PlaneSrcDev pw, const int *__restrict__ stp)
@@ -662,11 +703,11 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
     FusedIO io;
     {
         const size_t nc = (size_t)g.plane * g.nx;
-        const uint32_t cb = (uint32_t)(nc * T::B), pb = ps.total * T::B;
-        io.ex = mk_rsrc(Eo, cb); io.ey = mk_rsrc(Eo + nc, cb); io.ez = mk_rsrc(Eo + 2 * nc, cb);
-        io.hx = mk_rsrc(Ho, cb); io.hy = mk_rsrc(Ho + nc, cb); io.hz = mk_rsrc(Ho + 2 * nc, cb);
-        io.exn = mk_rsrc(En, cb); io.eyn = mk_rsrc(En + nc, cb); io.ezn = mk_rsrc(En + 2 * nc, cb);
-        io.hxn = mk_rsrc(Hn, cb); io.hyn = mk_rsrc(Hn + nc, cb); io.hzn = mk_rsrc(Hn + 2 * nc, cb);
+        const uint32_t pb = ps.total * T::B;
+        io.cb = (uint32_t)(nc * T::B);
+        const uint32_t sb = 3u * io.cb;
+        io.eo = mk_rsrc(Eo, sb); io.ho = mk_rsrc(Ho, sb);
+        io.en = mk_rsrc(En, sb); io.hn = mk_rsrc(Hn, sb);
         io.ph_o = mk_rsrc(ps.h, pb); io.ph_n = mk_rsrc(pn.h, pb);
         io.pe_o = mk_rsrc(ps.e, pb); io.pe_n = mk_rsrc(pn.e, pb);
     }
@@ -683,17 +724,30 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
     // work item of this wave: (chunk, row group, k strip)
     const int nks = (nz + OEM_WK - 1) / OEM_WK;
     const int n_items = g.tiles * ((nx + g.chunk - 1) / g.chunk);
-    const int item = xcd_remap(blockIdx.x, g.nblocks) * OEM_WW + (int)threadIdx.y;
+    // the wave index through readfirstlane: the compiler then knows every per-wave quantity
+    // (item, plane i, rows j) is uniform, keeps it in SGPRs, gives the buffer ops scalar
+    // offsets and reads the axis tables with scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int item = xcd_remap(blockIdx.x, g.nblocks) * OEM_WW + wave;
     if (item >= n_items) return;
-    const int tile = item % g.tiles, chunk = item / g.tiles;
-    const int j0 = (tile / nks) * OEM_WR, k0 = (tile % nks) * OEM_WK;
+    // the divisions run on the vector ALU: their wave-uniform results go back to SGPRs
+    const int tile = __builtin_amdgcn_readfirstlane(item % g.tiles);
+    const int chunk = __builtin_amdgcn_readfirstlane(item / g.tiles);
+    const int j0 = __builtin_amdgcn_readfirstlane((tile / nks) * OEM_WR);
+    const int k0 = __builtin_amdgcn_readfirstlane((tile % nks) * OEM_WK);
     const int i0 = chunk * g.chunk, i1 = min(i0 + g.chunk, nx);
     const int nr = min(OEM_WR, ny - j0);
-    const int jh = Y.prv[j0];
+    // neighbour indices read from the tables are wave-uniform: say so, or every buffer access
+    // offset by them becomes a per-lane loop
+    const int jh = __builtin_amdgcn_readfirstlane(Y.prv[j0]);
     const int lane = threadIdx.x;
     const bool act = lane == 0 || k0 + lane - 1 < nz;       // computes H
     const bool own = lane > 0 && k0 + lane - 1 < nz;        // stores H and E
     const int k = lane == 0 ? Z.prv[k0] : (act ? k0 + lane - 1 : 0);
+    const KConst kc = k_const(g, k);
+    const int xn_last = __builtin_amdgcn_readfirstlane(X.nxt[nx - 1]);
+    const int yn_last = __builtin_amdgcn_readfirstlane(Y.nxt[ny - 1]);
+    const int ip = __builtin_amdgcn_readfirstlane(X.prv[i0]);
     const uint32_t planeB = (uint32_t)g.plane * T::B, plane2 = (uint32_t)g.plane * 2u;
     const size_t ncell = (size_t)g.plane * nx;
 
@@ -734,12 +788,12 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
     for (int r = 0; r < OEM_WR; ++r) {
         F hy_ = T::zero(), hz_ = T::zero();
         if (r < nr && own) {
-            const int ip = X.prv[i0], j = j0 + r;
-            const PsiAt q = psi_at<F>(ps, g, ip, j, k);
+            const int j = j0 + r;
+            const PsiAt q = psi_at<F>(ps, g, ip, j, kc);
             HIn<F> v;
             F hx_;
-            fused_h_ld<F>(io, g, q, ip, j, k, v);
-            fused_h_do<F>(io, g, q, ip, j, k, v, ch, ihx, ihy, false, hx_, hy_, hz_);
+            fused_h_ld<F>(io, g, q, ip, next_of(ip, nx, xn_last), j, next_of(j, ny, yn_last), kc, v);
+            fused_h_do<F>(io, g, q, ip, j, kc, v, ch, ihx, ihy, false, hx_, hy_, hz_);
         }
 #pragma unroll
         for (int q = 0; q + 1 < OEM_WR; ++q) {
@@ -750,65 +804,74 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
         hz_q[OEM_WR - 1] = hz_;
     }
 
-    for (int i = i0; i < i1; ++i) {
-        const uint32_t so = (uint32_t)i * planeB;
-        const float mi = axv(X, i, AX_MPV), mii = axv(X, i, AX_MPVI);
-        const float idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
-        const bool hplane = i >= mat.hb[0] && i < mat.hb[1];
-
-        // new H of the halo row (its Hx, Hz are the j - 1 neighbours of row 0)
-        F hxp = T::zero(), hzp = T::zero();
-        if (act) {
-            const PsiAt q = psi_at<F>(ps, g, i, jh, k);
-            HIn<F> v;
-            F hy_;
-            fused_h_ld<F>(io, g, q, i, jh, k, v);
-            fused_h_do<F>(io, g, q, i, jh, k, v, ch, ihx, ihy, false, hxp, hy_, hzp);
+    const int jhn = next_of(jh, ny, yn_last);
+    // The march as a flat list of tasks, OEM_WR + 1 per plane: the halo row, then the owned rows
+    // (rows past the last of a partial group are empty tasks that only rotate the queue).  The
+    // loads of task t + 1 are issued before the stores of task t: the memory counter is in
+    // order, so a load issued after a store could not be waited for without the store.
+    const int per_plane = OEM_WR + 1;
+    const int n_tasks = (i1 - i0) * per_plane;
+    RowIn<F> cur, nxt;
+    auto task_load = [&](int t, RowIn<F> &in) {
+        const int i = i0 + t / per_plane, r = t % per_plane - 1;
+        const bool halo = r < 0, valid = halo || r < nr;
+        const int j = halo ? jh : j0 + min(r, nr - 1);
+        const int jn = halo ? jhn : next_of(j, ny, yn_last);
+        const PsiAt q = psi_at<F>(ps, g, i, j, kc);
+        in.cls = 0u;
+        in.pe[0] = in.pe[1] = in.pe[2] = in.pe[3] = in.pe[4] = in.pe[5] = T::zero();
+        in.ph[0] = in.ph[1] = in.ph[2] = T::zero();
+        if (!(act && valid)) return;
+        fused_h_ld<F>(io, g, q, i, next_of(i, nx, xn_last), j, jn, kc, in.v);
+        if (!own || halo) return;
+        if (mat.mode == 0) in.cls = bld_u16(rC, (uint32_t)(j * nz + k) * 2u + (uint32_t)i * plane2, 0u);
+        if (q.sx >= 0) { in.pe[0] = T::ld(io.pe_o, q.ox + q.sx0, 0u); in.pe[1] = T::ld(io.pe_o, q.ox + q.sx1, 0u); }
+        if (q.sy >= 0) { in.pe[2] = T::ld(io.pe_o, q.oy + q.sy0, 0u); in.pe[3] = T::ld(io.pe_o, q.oy + q.sy1, 0u); }
+        if (q.sz >= 0) { in.pe[4] = T::ld(io.pe_o, q.oz + q.sz0, 0u); in.pe[5] = T::ld(io.pe_o, q.oz + q.sz1, 0u); }
+        const bool hin = hcol && i >= mat.hb[0] && i < mat.hb[1] && j >= mat.hb[2] && j < mat.hb[3];
+        if (hin) {
+            const uint32_t oh = (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * T::B;
+            const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * T::B;
+            in.ph[0] = T::ld(rHist, oh + sh, 0u);
+            in.ph[1] = T::ld(rHist, oh + sh + hcomp * T::B, 0u);
+            in.ph[2] = T::ld(rHist, oh + sh + 2u * hcomp * T::B, 0u);
         }
+    };
+    F hxp = T::zero(), hzp = T::zero();
+    if (n_tasks > 0) task_load(0, cur);
 #pragma unroll 1
-        for (int r = 0; r < OEM_WR; ++r) {
-            const int j = j0 + min(r, nr - 1);     // rows past the last are rotated, not computed
-            const bool row = r < nr;
-            const PsiAt q = psi_at<F>(ps, g, i, j, k);
-            F hx = T::zero(), hy = T::zero(), hz = T::zero();
-            HIn<F> v;
-            // every load of the row before its first store: H inputs, class, E-side psi and
-            // the ADE history (by box geometry; the class flags pick the components later)
-            uint32_t cls = 0u;
-            F pex0 = T::zero(), pex1 = T::zero(), pey0 = T::zero(), pey1 = T::zero(),
-              pez0 = T::zero(), pez1 = T::zero();
-            F phx = T::zero(), phy = T::zero(), phz = T::zero();
-            const bool hin = hcol && hplane && j >= mat.hb[2] && j < mat.hb[3];
-            if (act && row) {
-                fused_h_ld<F>(io, g, q, i, j, k, v);
-                if (own) {
-                    if (mat.mode == 0) cls = bld_u16(rC, (uint32_t)(j * nz + k) * 2u, (uint32_t)i * plane2);
-                    if (q.sx >= 0) { pex0 = T::ld(io.pe_o, q.ox, q.sx0); pex1 = T::ld(io.pe_o, q.ox, q.sx1); }
-                    if (q.sy >= 0) { pey0 = T::ld(io.pe_o, q.oy, q.sy0); pey1 = T::ld(io.pe_o, q.oy, q.sy1); }
-                    if (q.sz >= 0) { pez0 = T::ld(io.pe_o, q.oz, q.sz0); pez1 = T::ld(io.pe_o, q.oz, q.sz1); }
-                    if (hin) {
-                        const uint32_t oh = (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * T::B;
-                        const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * T::B;
-                        phx = T::ld(rHist, oh, sh);
-                        phy = T::ld(rHist, oh, sh + hcomp * T::B);
-                        phz = T::ld(rHist, oh, sh + 2u * hcomp * T::B);
-                    }
-                }
-                fused_h_do<F>(io, g, q, i, j, k, v, ch, ihx, ihy, own, hx, hy, hz);
-            }
+    for (int t = 0; t < n_tasks; ++t) {
+        if (t + 1 < n_tasks) task_load(t + 1, nxt);
+        const int i = i0 + t / per_plane, r = t % per_plane - 1;
+        const bool halo = r < 0, valid = halo || r < nr;
+        const int j = halo ? jh : j0 + min(r, nr - 1);
+        const PsiAt q = psi_at<F>(ps, g, i, j, kc);
+        const uint32_t so = (uint32_t)i * planeB;
+        F hx = T::zero(), hy = T::zero(), hz = T::zero();
+        if (act && valid)
+            fused_h_do<F>(io, g, q, i, j, kc, cur.v, ch, ihx, ihy, own && !halo, hx, hy, hz);
+        if (halo) {
+            // the halo row's Hx, Hz are the j - 1 neighbours of row 0
+            hxp = hx;
+            hzp = hz;
+        } else {
             // new H at k - 1 from the lane to the left (all lanes active here)
             const F hx_k = lane_prev_f<F>(hx), hy_k = lane_prev_f<F>(hy);
-            if (own && row) {
+            if (own && valid) {
+                const float mi = axv(X, i, AX_MPV), mii = axv(X, i, AX_MPVI);
+                const float idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
                 const float mjE = axv(Y, j, AX_MPV), mjEi = axv(Y, j, AX_MPVI);
                 const float idyE = axv(Y, j, AX_IDLD), pecy = axv(Y, j, AX_PEC);
+                const bool hin = hcol && i >= mat.hb[0] && i < mat.hb[1] && j >= mat.hb[2] &&
+                                 j < mat.hb[3];
                 float cax, cbx, cay, cby, caz, cbz;
                 uint32_t fl;
                 if (lds_lut) {
-                    const float4 r0 = s_lut[2 * cls], r1 = s_lut[2 * cls + 1];
+                    const float4 r0 = s_lut[2 * cur.cls], r1 = s_lut[2 * cur.cls + 1];
                     cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
                     fl = __float_as_uint(r1.z);
                 } else if (mat.mode == 0) {
-                    const float4 *rw = reinterpret_cast<const float4 *>(mat.lut + 8u * cls);
+                    const float4 *rw = reinterpret_cast<const float4 *>(mat.lut + 8u * cur.cls);
                     const float4 r0 = rw[0], r1 = rw[1];
                     cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
                     fl = __float_as_uint(r1.z);
@@ -818,9 +881,9 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
                     cbx = mat.cb[id]; cby = mat.cb[ncell + id]; cbz = mat.cb[2 * ncell + id];
                     fl = hin ? 7u : 0u;
                 }
-                if (!(fl & 1u)) phx = T::zero();
-                if (!(fl & 2u)) phy = T::zero();
-                if (!(fl & 4u)) phz = T::zero();
+                const F phx = (fl & 1u) ? cur.ph[0] : T::zero();
+                const F phy = (fl & 2u) ? cur.ph[1] : T::zero();
+                const F phz = (fl & 4u) ? cur.ph[2] : T::zero();
                 const F dzy = (hz - T::mmul(hzp, mjE, mjEi)) * idyE;
                 const F dyz = (hy - T::mmul(hy_k, mkE, mkEi)) * idzE;
                 const F dxz = (hx - T::mmul(hx_k, mkE, mkEi)) * idzE;
@@ -830,32 +893,32 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
                 F s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
                 if (q.sx >= 0) {
                     const float a_ = axv(X, i, AX_AE), b_ = axv(X, i, AX_BE), k_ = axv(X, i, AX_KE);
-                    s_ey_x = stretch_v(pex0, dzx, a_, b_, k_);
-                    s_ez_x = stretch_v(pex1, dyx, a_, b_, k_);
-                    T::st(io.pe_n, pex0, q.ox, q.sx0);
-                    T::st(io.pe_n, pex1, q.ox, q.sx1);
+                    s_ey_x = stretch_v(cur.pe[0], dzx, a_, b_, k_);
+                    s_ez_x = stretch_v(cur.pe[1], dyx, a_, b_, k_);
+                    T::st(io.pe_n, cur.pe[0], q.ox + q.sx0, 0u);
+                    T::st(io.pe_n, cur.pe[1], q.ox + q.sx1, 0u);
                 }
                 if (q.sy >= 0) {
                     const float a_ = axv(Y, j, AX_AE), b_ = axv(Y, j, AX_BE), k_ = axv(Y, j, AX_KE);
-                    s_ex_y = stretch_v(pey0, dzy, a_, b_, k_);
-                    s_ez_y = stretch_v(pey1, dxy, a_, b_, k_);
-                    T::st(io.pe_n, pey0, q.oy, q.sy0);
-                    T::st(io.pe_n, pey1, q.oy, q.sy1);
+                    s_ex_y = stretch_v(cur.pe[2], dzy, a_, b_, k_);
+                    s_ez_y = stretch_v(cur.pe[3], dxy, a_, b_, k_);
+                    T::st(io.pe_n, cur.pe[2], q.oy + q.sy0, 0u);
+                    T::st(io.pe_n, cur.pe[3], q.oy + q.sy1, 0u);
                 }
                 if (q.sz >= 0) {
-                    s_ex_z = stretch_v(pez0, dyz, az, bz, kz);
-                    s_ey_z = stretch_v(pez1, dxz, az, bz, kz);
-                    T::st(io.pe_n, pez0, q.oz, q.sz0);
-                    T::st(io.pe_n, pez1, q.oz, q.sz1);
+                    s_ex_z = stretch_v(cur.pe[4], dyz, az, bz, kz);
+                    s_ey_z = stretch_v(cur.pe[5], dxz, az, bz, kz);
+                    T::st(io.pe_n, cur.pe[4], q.oz + q.sz0, 0u);
+                    T::st(io.pe_n, cur.pe[5], q.oz + q.sz1, 0u);
                 }
-                F nex = (cax * v.ex + cbx * (s_ex_y - s_ex_z - cw * phx)) * pecy * pecz;
-                F ney = (cay * v.ey + cby * (s_ey_z - s_ey_x - cw * phy)) * pecz * pecx;
-                F nez = (caz * v.ez + cbz * (s_ez_x - s_ez_y - cw * phz)) * pecx * pecy;
+                F nex = (cax * cur.v.ex + cbx * (s_ex_y - s_ex_z - cw * phx)) * pecy * pecz;
+                F ney = (cay * cur.v.ey + cby * (s_ey_z - s_ey_x - cw * phy)) * pecz * pecx;
+                F nez = (caz * cur.v.ez + cbz * (s_ez_x - s_ez_y - cw * phz)) * pecx * pecy;
                 if (inj_x) nex = T::addre(nex, cbx * inj_ex);
                 if (inj_y) ney = T::addre(ney, cby * inj_ey);
-                T::st(io.exn, nex, q.ox, so);
-                T::st(io.eyn, ney, q.ox, so);
-                T::st(io.ezn, nez, q.ox, so);
+                T::st(io.en, nex, q.ox + so, 0u);
+                T::st(io.en, ney, q.ox + so + io.cb, 0u);
+                T::st(io.en, nez, q.ox + so + 2u * io.cb, 0u);
             }
             hxp = hx;
             hzp = hz;
@@ -867,6 +930,7 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
             hy_q[OEM_WR - 1] = hy;
             hz_q[OEM_WR - 1] = hz;
         }
+        cur = nxt;
     }
 }
 

@@ -518,7 +518,9 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
         // a pool work-group marches 32 planes: the batch of scenes supplies the parallelism
         chunk = 32;
     } else {
-        const int want_blocks = 4096;
+        // 64 x 8 tiles marching 32 planes at 512^3: measured best of tiles 4/8/16 rows and
+        // chunks 32/64/128 planes
+        const int want_blocks = 8192;
         const int chunks_needed = std::max(1, (want_blocks + g.tiles - 1) / g.tiles);
         chunk = std::max(4, (g.nx + chunks_needed - 1) / chunks_needed);
     }
@@ -529,6 +531,8 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     // fused H+E step on request (OEM_FUSE=1), unless a point list corrects H between the sweeps
     const char *fe = std::getenv("OEM_FUSE");
     bool fuse = !ctx->pool && fe && fe[0] == '1';
+    // the fused march addresses a whole field set (three components) with 32-bit offsets
+    fuse &= 3ull * ctx->ncell * 4ull * (uint64_t)ctx->esz < (1ull << 32);
     for (auto &pt : ctx->points) fuse &= !pt.magnetic;
     if (fuse) {
         const size_t nc = ctx->ncell;
@@ -550,10 +554,10 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
         } else if (env && std::atoi(env) > 0) {
             fchunk = std::atoi(env);
         } else {
-            // about 8 waves per SIMD over the whole chip, chunks of at least 16 planes
-            const int want = 8192;
+            // short chunks (many waves per SIMD) measured best: 8 planes at 512^3
+            const int want = 32768;
             const int need = std::max(1, (want + ctx->fg.tiles - 1) / ctx->fg.tiles);
-            fchunk = std::max(16, (g.nx + need - 1) / need);
+            fchunk = std::max(8, (g.nx + need - 1) / need);
         }
         ctx->fg.chunk = std::min(fchunk, g.nx);
         const long long items = (long long)ctx->fg.tiles * ((g.nx + ctx->fg.chunk - 1) / ctx->fg.chunk);
