@@ -12,7 +12,8 @@ NumPy oracle on a bounded sample of the same workload.  ``serving`` is the batch
 half of the metric: the first ``--serve-requests`` requests of the seeded serving stream (Poisson
 arrivals at 20 / s, cubic sides log-uniform in [64, 320], 2000-8000 steps, 1-3 flux monitors of
 5-21 frequencies, slab / box-soup / Lorentz-sphere scenes) served by continuous batching and by
-the serial plain loop on the same GPU: sims/s and p50/p99 time-to-result of each.
+the serial plain loop on the same GPU: sims/s and p50/p99 time-to-result of each.  ``bloch``
+times the complex field path on the full-size Bloch / oblique unit cell (256 x 256 x 600).
 """
 
 from __future__ import annotations
@@ -34,7 +35,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(tabs) -> dict:
-    """Bytes each sweep must move per step (float32 fields, uint16 material class):
+    """Bytes each sweep must move per step (float32 fields, twice that for the complex fields
+    of a Bloch scene; uint16 material class):
 
     H sweep: read E (12 B) + read/write H (24 B) per cell, + 16 B per cell per CPML slab axis
     (2 psi read+write); E sweep: read H (12 B) + read/write E (24 B) + class index (2 B) per
@@ -49,11 +51,12 @@ def algorithmic_bytes(tabs) -> dict:
         w = ax.pml_lo + ax.pml_hi
         slab_cells += n // tabs.shape[a] * w
     disp_comps = 0 if tabs.disp is None else tabs.disp.comp.size
-    h = 36 * n + 16 * slab_cells
-    e = 38 * n + 16 * slab_cells + 4 * disp_comps
+    c = 2 if tabs.bloch else 1          # complex (Bloch) fields: every field value is 8 B
+    h = c * 36 * n + c * 16 * slab_cells
+    e = c * 36 * n + 2 * n + c * 16 * slab_cells + c * 4 * disp_comps
     # fused H+E march: E and H each read once and written once, the class index, both psi
     # pairs, the history term
-    he = 50 * n + 32 * slab_cells + 4 * disp_comps
+    he = c * 48 * n + 2 * n + c * 32 * slab_cells + c * 4 * disp_comps
     return {"update_h": h, "update_e": e, "update_he": he, "cells": n, "slab_cells": slab_cells}
 
 
@@ -159,6 +162,44 @@ def adjoint_bench(n_designs: int, steps: int, check_pixels: int, n: int = 400) -
             "fd_check": check, "setup_s": round(time.perf_counter() - t0 - sum(per), 1)}
 
 
+def bloch_bench(steps: int, warmup: int = 10, profile_steps: int = 10) -> dict:
+    """The Bloch / oblique unit cell at its full size (256 x 256 x 600 complex cells, Bloch x/y,
+    CPML z, oblique TF/SF box, Drude + Lorentz metal cylinders, four 21-frequency flux planes):
+    the complex field path's throughput and the roofline of its dominant sweep."""
+    from openem_amd import scenes, tables
+    from openem_amd.solver import Stepper
+    n_total = warmup + steps + profile_steps + 8
+    t0 = time.time()
+    sc = scenes.bloch_cell_scene(num_steps=max(n_total, 400))
+    tabs = tables.build(sc, n_total)
+    st = Stepper(tabs)
+    setup = time.time() - t0
+    try:
+        st.run(warmup)
+        ms = st.timed_run(steps)
+        prof = st.profile_run(profile_steps)
+    finally:
+        st.close()
+    cells = int(np.prod(sc.shape))
+    ab = algorithmic_bytes(tabs)
+    k_dom = max(("update_h", "update_e"), key=lambda k: prof[k][0] / max(prof[k][1], 1))
+    avg = prof[k_dom][0] / prof[k_dom][1]
+    achieved = ab[k_dom] / (avg * 1e-3) / 1e9
+    kern = {k: round(v[0] / v[1], 5) for k, v in prof.items() if v[1]}
+    print(f"[bench] bloch cell {sc.shape}: setup {setup:.1f}s, per-kernel avg ms {kern}",
+          file=sys.stderr, flush=True)
+    return {"workload": "Bloch unit cell 256x256x600 complex (Bloch x/y, CPML z, oblique TF/SF, "
+                        "Drude+Lorentz metal, 4 flux planes x 21 freqs)",
+            "cells": cells, "steps": steps, "ms_per_step": round(ms / steps, 4),
+            "mcell_steps_per_s": round(cells / (ms / steps * 1e-3) / 1e6, 1),
+            "dtype": "complex f32",
+            "roofline": {"bound": "hbm", "kernel": k_dom, "achieved": round(achieved, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes_per_launch": ab[k_dom],
+                         "avg_launch_ms": round(avg, 5)}}
+
+
 def _mode_power(sc) -> float:
     from openem_amd import solver
     r = solver.run(sc, use_shutoff=False, verbose=False)
@@ -183,6 +224,8 @@ def main() -> int:
     ap.add_argument("--adjoint-designs", type=int, default=2)
     ap.add_argument("--adjoint-steps", type=int, default=6000)
     ap.add_argument("--adjoint-check", type=int, default=0)
+    ap.add_argument("--bloch-steps", type=int, default=60,
+                    help="steps of the full-size Bloch unit cell to time (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -233,6 +276,7 @@ def main() -> int:
     cpu = None if args.no_cpu else cpu_baseline(args.n)
     srv = (serving_bench(args.serve_requests, args.arena_gib, args.serve_workers)
            if args.serve_requests > 0 else None)
+    blo = bloch_bench(args.bloch_steps) if args.bloch_steps > 0 else None
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -250,6 +294,7 @@ def main() -> int:
         "step_hbm_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "cpu_baseline": cpu,
         "serving": srv,
+        "bloch": blo,
     }
     print(json.dumps(out), flush=True)
     return 0
