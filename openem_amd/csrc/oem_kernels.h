@@ -157,6 +157,12 @@ __device__ __forceinline__ int slab_of(int i, int n, int lo, int hi) {
 
 __device__ __forceinline__ int step_now(const int *stp) { return *stp; }
 
+// Next index along an axis: interior neighbours are i + 1 (the host checks), only the last
+// cell's neighbour comes from the table (`last` = A.nxt[n - 1], loaded once and marked
+// wave-uniform).  A table load inside the x march would make the next plane's offset a
+// per-lane value, and every load offset by it a per-lane loop behind a full memory wait.
+__device__ __forceinline__ int next_of(int i, int n, int last) { return i + 1 < n ? i + 1 : last; }
+
 struct TileCoord {
     int j, k, i0, i1;
     bool ok;
@@ -231,11 +237,12 @@ __device__ __forceinline__ void update_h_tile(F *__restrict__ H, const F *__rest
     }
 
     const uint32_t planeB = (uint32_t)g.plane * T::B;
+    const int xn_last = __builtin_amdgcn_readfirstlane(X.nxt[nx - 1]);
     F ey = T::ld(rEy, op, (uint32_t)tc.i0 * planeB);
     F ez = T::ld(rEz, op, (uint32_t)tc.i0 * planeB);
     for (int i = tc.i0; i < tc.i1; ++i) {
         const uint32_t so = (uint32_t)i * planeB;
-        const uint32_t sn = (uint32_t)X.nxt[i] * planeB;
+        const uint32_t sn = (uint32_t)next_of(i, nx, xn_last) * planeB;
         const F ex = T::ld(rEx, op, so), ex_j = T::ld(rEx, oj, so), ex_k = T::ld(rEx, ok, so);
         const F ey_k = T::ld(rEy, ok, so), ez_j = T::ld(rEz, oj, so);
         const F ey_n = T::ld(rEy, op, sn), ez_n = T::ld(rEz, op, sn);
@@ -378,7 +385,7 @@ __device__ __forceinline__ void update_e_tile(F *__restrict__ E, const F *__rest
     const uint32_t oc = (uint32_t)(j * nz + k) * 2u;               // uint16 class index
 
     const uint32_t planeB = (uint32_t)g.plane * T::B, plane2 = (uint32_t)g.plane * 2u;
-    const uint32_t sp0 = (uint32_t)X.prv[tc.i0] * planeB;
+    const uint32_t sp0 = (uint32_t)__builtin_amdgcn_readfirstlane(X.prv[tc.i0]) * planeB;
     F hy_m = T::ld(rHy, op, sp0), hz_m = T::ld(rHz, op, sp0);
     uint32_t cls_next = mat.mode == 0 ? bld_u16(rC, oc, (uint32_t)tc.i0 * plane2) : 0u;
     for (int i = tc.i0; i < tc.i1; ++i) {
@@ -564,9 +571,6 @@ __device__ __forceinline__ KConst k_const(const GridDev &g, int k) {
     return c;
 }
 
-// Next index along an axis: interior neighbours are i + 1 (the host checks), only the last
-// cell's neighbour comes from the table (`last` = A.nxt[n - 1], loaded once).
-__device__ __forceinline__ int next_of(int i, int n, int last) { return i + 1 < n ? i + 1 : last; }
 
 template <typename F>
 __device__ __forceinline__ PsiAt psi_at(const PsiDev &ps, const GridDev &g, int i, int j,

@@ -515,8 +515,10 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     } else if (env && std::atoi(env) > 0) {
         chunk = std::atoi(env);
     } else if (ctx->pool) {
-        // a pool work-group marches 32 planes: the batch of scenes supplies the parallelism
-        chunk = 32;
+        // a pool work-group marches a fixed number of planes: the batch of scenes supplies the
+        // parallelism
+        const char *pe = std::getenv("OEM_POOL_CHUNK");
+        chunk = (pe && std::atoi(pe) > 0) ? std::atoi(pe) : 32;
     } else {
         // 64 x 8 tiles marching 32 planes at 512^3: measured best of tiles 4/8/16 rows and
         // chunks 32/64/128 planes
