@@ -2,7 +2,7 @@
 
 The reference runs one scene per ``openem.solver.run`` call (``openem/solver.py:403-511``).
 Here a :class:`Pool` keeps many scenes resident in one device arena and advances all of them
-with the same launches (persistent batched kernels in a hipGraph of ``dft_batch`` steps; see
+with the same launches (batched kernels over every resident scene in a hipGraph of ``dft_batch`` steps; see
 ``csrc/oem_pool.hip``).  :func:`serve` is the request loop:
 
 * a client thread releases requests at their arrival times (building each Scene just ahead);
