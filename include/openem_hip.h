@@ -160,6 +160,20 @@ int oem_add_flux_plane(oem_ctx *ctx, int32_t axis, int32_t km, int32_t nf, const
 int oem_add_field_box(oem_ctx *ctx, const int32_t *origin, const int32_t *box, int32_t nf,
                       const double *phase, int32_t *mon_id);
 
+/* Time-domain field monitor: components comps[0..n_comps) (0..5 = Ex..Hz, at most 6) in the
+ * index box origin/box, sampled at steps m = step_begin, step_begin + interval, ... (m <
+ * step_end, at most n_slots): E^m and (H^{m-1/2} + H^{m+1/2}) / 2.  Buffer layout (n_comps,
+ * n_slots, ni, nj, nk), zero-initialised.  Replaces monitors_setup._tmon_entry
+ * (`openem/monitors_setup.py:258-286`) and sample_time_box (`openem/kernels/source_dft.cu:272-316`). */
+int oem_add_time_monitor(oem_ctx *ctx, int32_t n_comps, const int32_t *comps,
+                         const int32_t *origin, const int32_t *box, int32_t step_begin,
+                         int32_t step_end, int32_t interval, int32_t n_slots, int32_t *mon_id);
+
+/* The whole time-monitor buffer: count = n_comps * n_slots * cells floats (twice that, (re,
+ * im) pairs, on a complex context).  Replaces readout._readout's time_samples
+ * (`openem/readout.py:183-210`). */
+int oem_read_time_monitor(oem_ctx *ctx, int32_t mon_id, float *out, size_t count);
+
 /* Launch shape: x-planes each work-group marches (0 = the library's choice).  Before
  * oem_prepare.  No counterpart in the reference (its launch shapes are fixed in the kernels'
  * callers, `openem/solver.py:876-1162`); it is the knob the launch autotuner sets. */

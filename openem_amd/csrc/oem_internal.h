@@ -36,6 +36,11 @@ struct Monitor {
     size_t count;     // ncomp*nf*npts
 };
 
+struct TimeMonitor {
+    TimeMonDev d;
+    size_t count;     // nc*slots*cells elements of the field type
+};
+
 // One grouped point list (dipoles or TF/SF face corrections) and which field it updates.
 enum { PT_DIPOLE = 0, PT_TFSF = 1, PT_MODE = 2 };
 
@@ -74,6 +79,7 @@ void adjoint_state_free(oem_adjoint_state *a);
 
 using oemi::fail;
 using oemi::Monitor;
+using oemi::TimeMonitor;
 using oemi::Points;
 using oemi::PT_DIPOLE;
 using oemi::PT_MODE;
@@ -119,6 +125,7 @@ struct oem_ctx {
     PlaneSrcDev pw{};
     std::vector<Points> points;
     std::vector<Monitor> mons;
+    std::vector<TimeMonitor> tmons;
     int *d_step = nullptr;      // [0] steps taken, [1] steps flushed into the DFT accumulators
     int64_t step = 0;
     int pending = 0;

@@ -1091,6 +1091,46 @@ k_disp(DispDev d, const F *__restrict__ E, size_t ncell, int ny, int nz, int mod
 }
 
 // ------------------------------------------------------------------------------------------
+// Time-domain field sampling (reference sample_time_box, `openem/kernels/source_dft.cu:272-316`;
+// complex sample_time_box_c, `openem/kernels/bloch.cu:278-300`).  At the end of step n (E^{n+1}
+// and H^{n+1/2} in hand) shot 0 writes slot(m = n + 1) := E^m + H^{m-1/2} / 2 and shot 1 adds
+// H^{m+1/2} / 2 into slot(m = n): H is reported as the average of the two half steps around
+// m dt.  Layout (nc, slots, ni, nj, nk) of F at native Yee positions.  One thread per box cell.
+// ------------------------------------------------------------------------------------------
+struct TimeMonDev {
+    int nc, comps[6];
+    int o[3], b[3];
+    int beg, end, iv, slots;
+    int cells;
+    void *buf;
+};
+
+template <typename F>
+__global__ void __launch_bounds__(256)
+k_time_sample(TimeMonDev d, const F *__restrict__ E, const F *__restrict__ H, size_t ncell,
+              int ny, int nz, const int *__restrict__ stp, int shot)
+{
+    const int m = step_now(stp) + 1 - shot;
+    const int off = m - d.beg;
+    if (off < 0 || m >= d.end || off % d.iv) return;
+    const int slot = off / d.iv;
+    if (slot >= d.slots) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.cells) return;
+    const int a = q / (d.b[1] * d.b[2]), r = q - a * d.b[1] * d.b[2];
+    const int b = r / d.b[2], c = r - b * d.b[2];
+    const size_t id = ((size_t)(d.o[0] + a) * ny + (d.o[1] + b)) * nz + (d.o[2] + c);
+    F *out = static_cast<F *>(d.buf);
+    for (int t = 0; t < d.nc; ++t) {
+        const int comp = d.comps[t];
+        if (shot == 1 && comp < 3) continue;
+        const F v = comp < 3 ? E[(size_t)comp * ncell + id] : 0.5f * H[(size_t)(comp - 3) * ncell + id];
+        F &o = out[((size_t)t * d.slots + slot) * d.cells + q];
+        o = shot ? o + v : v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Running DFT, batched: every step a snapshot of the monitored values goes into a ring of T
 // rows; every T steps one flush replays them in step order against the float64 phase table and
 // adds into the float64 accumulators (reference accumulate_dft / accumulate_dft_box and the

@@ -474,6 +474,60 @@ extern "C" int oem_add_field_box(oem_ctx *ctx, const int32_t *origin, const int3
     return add_monitor(ctx, m, phase, mon_id);
 }
 
+extern "C" int oem_add_time_monitor(oem_ctx *ctx, int32_t n_comps, const int32_t *comps,
+                                    const int32_t *origin, const int32_t *box,
+                                    int32_t step_begin, int32_t step_end, int32_t interval,
+                                    int32_t n_slots, int32_t *mon_id) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (!comps || !origin || !box || !mon_id) return fail(OEM_EINVAL, "null argument");
+    if (n_comps < 1 || n_comps > 6) return fail(OEM_EINVAL, "%d components", n_comps);
+    if (interval < 1 || n_slots < 1 || step_begin < 0 || step_end < step_begin)
+        return fail(OEM_EINVAL, "sampling window [%d, %d) every %d, %d slots", step_begin,
+                    step_end, interval, n_slots);
+    const int n[3] = {ctx->dims.nx, ctx->dims.ny, ctx->dims.nz};
+    TimeMonDev d{};
+    d.nc = n_comps;
+    for (int t = 0; t < n_comps; ++t) {
+        if (comps[t] < 0 || comps[t] > 5) return fail(OEM_EINVAL, "component %d", comps[t]);
+        d.comps[t] = comps[t];
+    }
+    size_t cells = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (origin[a] < 0 || box[a] < 1 || origin[a] + box[a] > n[a])
+            return fail(OEM_EINVAL, "box outside the grid on axis %d", a);
+        d.o[a] = origin[a];
+        d.b[a] = box[a];
+        cells *= (size_t)box[a];
+    }
+    const size_t count = (size_t)n_comps * n_slots * cells;
+    if (cells > ((size_t)1 << 30) || count > ((size_t)1 << 34))
+        return fail(OEM_EINVAL, "time monitor too large (%zu values)", count);
+    d.cells = (int)cells;
+    d.beg = step_begin;
+    d.end = step_end;
+    d.iv = interval;
+    d.slots = n_slots;
+    float *buf = nullptr;
+    TRY(ctx->alloc(&buf, count * ctx->esz, true));
+    d.buf = buf;
+    ctx->tmons.push_back(TimeMonitor{d, count});
+    *mon_id = (int32_t)ctx->tmons.size() - 1;
+    return OEM_OK;
+}
+
+extern "C" int oem_read_time_monitor(oem_ctx *ctx, int32_t mon_id, float *out, size_t count) {
+    CHECK_CTX(ctx);
+    if (mon_id < 0 || mon_id >= (int)ctx->tmons.size()) return fail(OEM_EINVAL, "time monitor %d", mon_id);
+    const TimeMonitor &m = ctx->tmons[mon_id];
+    if (!out || count != m.count * ctx->esz)
+        return fail(OEM_EINVAL, "time monitor %d holds %zu floats, asked %zu", mon_id,
+                    m.count * ctx->esz, count);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipMemcpy(out, m.d.buf, count * sizeof(float), hipMemcpyDeviceToHost));
+    return OEM_OK;
+}
+
 extern "C" int oem_prepare(oem_ctx *ctx) {
     CHECK_CTX(ctx);
     if (ctx->prepared) return OEM_OK;
@@ -678,6 +732,16 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
                            c->dims.ny, c->dims.nz, last ? 1 : 2);
         TRY(mark(OEM_K_DISP_POST, false));
         c->pre_ready = !last;
+    }
+    for (auto &tm : c->tmons) {
+        if (c->mons_off) break;
+        for (int shot = 0; shot < 2; ++shot) {
+            TRY(mark(OEM_K_DFT_SNAP, true));
+            hipLaunchKernelGGL((k_time_sample<F>), dim3(blocks_for(tm.d.cells, 256)), dim3(256), 0,
+                               c->stream, tm.d, (const F *)E, (const F *)H, nc, c->dims.ny,
+                               c->dims.nz, c->d_step, shot);
+            TRY(mark(OEM_K_DFT_SNAP, false));
+        }
     }
     for (auto &m : c->mons) {
         if (c->mons_off) break;

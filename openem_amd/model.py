@@ -114,6 +114,38 @@ class FieldMonitor:
 
 
 @dataclass(frozen=True)
+class FieldTimeMonitor:
+    """Time-domain field monitor (reference ``openem/model.py:276-313``): the components
+    ``comps`` (indices into :data:`COMPONENTS`) in the index box ``origin`` / ``box`` at their
+    Yee positions, sampled at steps ``m = step_begin, step_begin + interval, ...`` (``m <
+    step_end``, at most ``num_slots``) at times ``m dt``: E is ``E^m`` and H the average
+    ``(H^{m-1/2} + H^{m+1/2}) / 2`` of the two half steps around ``m dt``."""
+
+    name: str
+    comps: tuple[int, ...]
+    origin: tuple[int, int, int]
+    box: tuple[int, int, int]
+    step_begin: int
+    step_end: int
+    interval: int
+    num_slots: int
+
+    def slot_of(self, m: int) -> int | None:
+        """Slot written by time step ``m``, or None when ``m`` is not sampled."""
+        off = m - self.step_begin
+        if off < 0 or m >= self.step_end or off % self.interval:
+            return None
+        s = off // self.interval
+        return s if s < self.num_slots else None
+
+    def filled(self, steps_run: int) -> int:
+        """Slots complete after ``steps_run`` steps: the H average of step ``m`` needs step ``m``
+        itself (reference ``openem/readout.py:183-197``)."""
+        off = min(steps_run - 1, self.step_end - 1) - self.step_begin
+        return 0 if off < 0 else min(self.num_slots, off // self.interval + 1)
+
+
+@dataclass(frozen=True)
 class FluxMonitor:
     """Planar flux monitor (reference ``openem/model.py:387-418``): tangential phasors
     (E_t1, E_t2, H_t1, H_t2) with H colocated onto the E plane."""
@@ -225,6 +257,7 @@ class Scene:
     flux_monitors: list = field(default_factory=list)
     field_monitors: list = field(default_factory=list)
     mode_monitors: list = field(default_factory=list)
+    field_time_monitors: list = field(default_factory=list)
 
     @property
     def shape(self) -> tuple[int, int, int]:

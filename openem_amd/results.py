@@ -17,13 +17,20 @@ class Phasors:
 
 @dataclass
 class Result:
+    """The reference's ``Result`` fields (``openem/results.py:50-84``) plus ``mode_amps``."""
+
     phasors: dict = field(default_factory=dict)
-    field_phasors: dict = field(default_factory=dict)
-    mode_amps: dict = field(default_factory=dict)      # name -> a_plus, a_minus, freqs, n_eff
+    field_phasors: dict = field(default_factory=dict)    # name -> (6, nf, ni, nj, nk), Yee positions
+    mode_amps: dict = field(default_factory=dict)        # name -> a_plus, a_minus, freqs, n_eff
     steps_run: int = 0
     wall_seconds: float = 0.0
-    fields: np.ndarray | None = None
+    fields: dict = field(default_factory=dict)           # "Ex".."Hz" -> (nx, ny, nz), with return_fields
     shutoff_triggered: bool = False
-    stop_reason: str = "pinned step count"
-    decay_trace: list = field(default_factory=list)     # (step, field-decay ratio) per check
-    energy_trace: list = field(default_factory=list)    # (step, total energy) per check
+    stop_reason: str = ""                                # the criterion that stopped the run, or ""
+    shutoff_summary: str = ""
+    source_end: int = -1                                 # step at which every source is done
+    decay_trace: list = field(default_factory=list)      # (step, field-decay ratio) per check
+    energy_trace: list = field(default_factory=list)     # (step, total energy) per check
+    time_samples: dict = field(default_factory=dict)     # FieldTimeMonitor name -> samples
+    time_slots: dict = field(default_factory=dict)       # FieldTimeMonitor name -> slots filled
+    flux_time: dict = field(default_factory=dict)        # FluxTimeMonitor name -> flux series

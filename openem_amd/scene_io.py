@@ -8,13 +8,13 @@ one in the same key layout, so scenes move between the two in both directions.
 
 Covered one for one (key names and meanings of ``openem/serialize.py:120-420``): the grid, eps /
 sigma / PEC arrays, CPML faces, plane-wave sources, mode sources, point dipoles, the pole-residue
-dispersion CSR, flux and field monitors, ``dt``, ``num_time_steps``, ``shutoff`` and
+dispersion CSR, flux, field and field-time monitors, ``dt``, ``num_time_steps``, ``shutoff`` and
 ``bloch_k``.
 
 Differences, each failing closed rather than silently:
 
 * entities this stepper does not model (mixed-interface dispersion, time modulation, tensor
-  media, absorber slabs, symmetry planes, time-domain / permittivity / projection / box-flux
+  media, absorber slabs, symmetry planes, flux-time / permittivity / projection / box-flux
   monitors, broadband mode-source terms, strided field monitors, 2D flat axes) raise
   ``NotImplementedError`` on load;
 * TF/SF boxes: the reference stores its own 1D line (grid, eps, drive tables); this package
@@ -37,8 +37,8 @@ import numpy as np
 
 from . import cpml, waveform
 from .grid import Axis, Grid
-from .model import (Dispersion, FieldMonitor, FluxMonitor, ModeMonitor, ModeSource,
-                    PlaneWaveSource, PointDipole, Scene, TFSFSource)
+from .model import (Dispersion, FieldMonitor, FieldTimeMonitor, FluxMonitor, ModeMonitor,
+                    ModeSource, PlaneWaveSource, PointDipole, Scene, TFSFSource)
 
 FORMAT_VERSION = 10
 
@@ -46,7 +46,7 @@ _PML_FIELDS = ("a_E", "b_E", "inv_kappa_E", "a_H", "b_H", "inv_kappa_H")
 _MODE_FIELDS = ("ey_inc", "ez_inc", "hy_inc", "hz_inc", "amp_e", "amp_h")
 _UNSUPPORTED_PREFIXES = {"dmix_": "mixed-interface dispersion", "mod_": "time modulation",
                          "tensor_": "tensor media"}
-_UNSUPPORTED_LISTS = {"tmon_names": "field-time monitors", "ftmon_names": "flux-time monitors",
+_UNSUPPORTED_LISTS = {"ftmon_names": "flux-time monitors",
                       "pmon_names": "permittivity monitors",
                       "projmon_names": "projection monitors", "bfmon_names": "box flux monitors"}
 
@@ -186,6 +186,12 @@ def save(sc: Scene, path) -> pathlib.Path:
         d[f"oem_modemon{i}_meta"] = np.array([m.axis, m.plane_index, m.mode_index])
         d[f"oem_modemon{i}_freqs"] = np.asarray(m.freqs, dtype=np.float64)
         d[f"oem_modemon{i}_n_guess"] = np.array(np.nan if m.n_guess is None else float(m.n_guess))
+    d["tmon_names"] = np.array([m.name for m in sc.field_time_monitors])
+    for i, m in enumerate(sc.field_time_monitors):
+        d[f"tmon{i}_meta"] = np.array([m.step_begin, m.step_end, m.interval, m.num_slots])
+        d[f"tmon{i}_comps"] = np.array(m.comps)
+        d[f"tmon{i}_origin"] = np.array(m.origin)
+        d[f"tmon{i}_box"] = np.array(m.box)
     for key in _UNSUPPORTED_LISTS:
         d[key] = np.array([], dtype="<U1")
     d["n_dipoles"] = np.array(len(sc.dipoles))
@@ -325,6 +331,14 @@ def load(path) -> Scene:
             mode_mons.append(ModeMonitor(name=name, axis=axis,
                                          plane_index=fm.origin[axis] + fm.box[axis] - 1,
                                          freqs=fm.freqs))
+        time_mons = []
+        for i, name in enumerate(_names(z, "tmon_names")):
+            beg, end, iv, slots = (int(v) for v in z[f"tmon{i}_meta"])
+            time_mons.append(FieldTimeMonitor(
+                name=name, comps=tuple(int(v) for v in z[f"tmon{i}_comps"]),
+                origin=tuple(int(v) for v in z[f"tmon{i}_origin"]),
+                box=tuple(int(v) for v in z[f"tmon{i}_box"]), step_begin=beg, step_end=end,
+                interval=iv, num_slots=slots))
         dipoles = []
         for i in range(_count(z, "n_dipoles")):
             dipoles.append(PointDipole(
@@ -344,4 +358,5 @@ def load(path) -> Scene:
             bloch_k=(tuple(float(v) for v in z["bloch_k"]) if "bloch_k" in z.files
                      else (0.0, 0.0, 0.0)),
             sources=sources, tfsf_sources=tfsf, dipoles=dipoles, mode_sources=mode_sources,
-            flux_monitors=flux_mons, field_monitors=field_mons, mode_monitors=mode_mons)
+            flux_monitors=flux_mons, field_monitors=field_mons, mode_monitors=mode_mons,
+            field_time_monitors=time_mons)

@@ -15,7 +15,7 @@ import time
 import numpy as np
 
 from . import _lib, cpml, flux as flux_mod, shutoff as shutoff_mod, tables as tables_mod
-from .model import Scene
+from .model import COMPONENTS, Scene
 from .results import Phasors, Result
 
 
@@ -148,6 +148,14 @@ class Stepper:
                 _lib.check(lib, lib.oem_add_field_box(ctx, _lib.ptr(o), _lib.ptr(b), m.nf,
                                                       _lib.ptr(ph), C.byref(mid)))
             self.mon_ids.append(int(mid.value))
+        self.tmon_ids = []
+        for m in t.scene.field_time_monitors:
+            mid = C.c_int32(-1)
+            comps, o, b = _i32(m.comps), _i32(m.origin), _i32(m.box)
+            _lib.check(lib, lib.oem_add_time_monitor(ctx, comps.size, _lib.ptr(comps), _lib.ptr(o),
+                                                     _lib.ptr(b), m.step_begin, m.step_end,
+                                                     m.interval, m.num_slots, C.byref(mid)))
+            self.tmon_ids.append(int(mid.value))
 
     # -- stepping ---------------------------------------------------------------------------
     def run(self, n_steps: int) -> None:
@@ -196,6 +204,17 @@ class Stepper:
         _lib.check(self.lib, self.lib.oem_read_monitor(self.ctx, self.mon_ids[i], _lib.ptr(re),
                                                        _lib.ptr(im), re.size))
         return re + 1j * im
+
+    def read_time_monitor(self, i: int) -> np.ndarray:
+        """The whole buffer of field-time monitor ``i``: ``(n_comps, n_slots, ni, nj, nk)``
+        float32, or complex64 on a Bloch scene."""
+        m = self.tabs.scene.field_time_monitors[i]
+        out = np.empty((len(m.comps), m.num_slots) + tuple(m.box),
+                       np.complex64 if self.cplx else np.float32)
+        raw = out.view(np.float32)
+        _lib.check(self.lib, self.lib.oem_read_time_monitor(self.ctx, self.tmon_ids[i],
+                                                            _lib.ptr(raw), raw.size))
+        return out
 
     # -- float64 reductions on the device ----------------------------------------------------
     def setup_energy(self) -> None:
@@ -301,13 +320,13 @@ def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kerne
     t0 = time.time()
     tabs = tables_mod.build(sc, n_total)
     st = Stepper(tabs, device=device)
-    triggered, reason = False, "pinned step count"
+    triggered, reason = False, ""
     decay, energy_tr = [], []
+    src_end = shutoff_mod.source_end_step(sc)
     try:
         t1 = time.time()
         if use_shutoff:
             pol = policy or shutoff_mod.default_policy(sc.shutoff)
-            src_end = shutoff_mod.source_end_step(sc)
             st.setup_energy()
             st.setup_flux(transverse=False)
             plane_ids = [i for i, m in enumerate(tabs.monitors) if m.kind == "plane"]
@@ -346,18 +365,30 @@ def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kerne
                 ap, am = st.mode_amplitudes(i)
                 mamps[m.name] = dict(a_plus=ap, a_minus=am, freqs=m.freqs,
                                      n_eff=np.array([x.n_eff for x in m.modes]))
-        fields = None
+        tsamp, tslots = {}, {}
+        for i, m in enumerate(sc.field_time_monitors):
+            filled = m.filled(steps_run)
+            tsamp[m.name] = st.read_time_monitor(i)[:, :filled]
+            tslots[m.name] = filled
+        fields = {}
         if return_fields:
-            fields = np.stack([st.read_field(c) for c in range(6)])
+            fields = {name: st.read_field(c) for c, name in enumerate(COMPONENTS)}
     finally:
         st.close()
     if verbose:
         print(f"[openem_amd] {sc.shape} x {steps_run} steps: setup {t1 - t0:.2f}s, "
               f"stepping {wall:.2f}s" + (f", stopped: {reason}" if triggered else ""))
     _warn_nonfinite(phasors, fphas)
+    summary = ""
+    if use_shutoff:
+        summary = (f"stopped at step {steps_run}: {reason}" if triggered else
+                   f"ran all {steps_run} steps; last decay ratio "
+                   f"{decay[-1][1] if decay else float('nan'):.3g}")
     return Result(phasors=phasors, field_phasors=fphas, mode_amps=mamps, steps_run=steps_run,
                   wall_seconds=wall, fields=fields, shutoff_triggered=triggered,
-                  stop_reason=reason, decay_trace=decay, energy_trace=energy_tr)
+                  stop_reason=reason, shutoff_summary=summary, source_end=src_end,
+                  decay_trace=decay, energy_trace=energy_tr, time_samples=tsamp,
+                  time_slots=tslots)
 
 
 def _warn_nonfinite(ph, fph) -> None:

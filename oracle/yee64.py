@@ -97,6 +97,8 @@ class Oracle64:
                 self.acc.append(np.zeros((4, m.nf, m.n1, m.n2), dtype=np.complex128))
             else:
                 self.acc.append(np.zeros((6, m.nf) + tuple(m.box), dtype=np.complex128))
+        self.tsamp = [np.zeros((len(m.comps), m.num_slots) + tuple(m.box), dtype=dt)
+                      for m in self.sc.field_time_monitors]
         self.n = 0
 
     # -- derivatives ------------------------------------------------------------------------
@@ -192,6 +194,7 @@ class Oracle64:
             if t.bloch:
                 self.Q += d.b * en.imag[self._pole_entry]
         self._dft(n)
+        self._sample_time(n)
         self.n += 1
 
     def _mode(self, ms, n, magnetic):
@@ -315,6 +318,22 @@ class Oracle64:
                 for c in range(6):
                     F = E[c] if c < 3 else H[c - 3]
                     acc[c] += F[sl][None] * (we[..., None] if c < 3 else wh[..., None])
+
+    def _sample_time(self, n):
+        """Field-time monitors at the end of step ``n`` (E^{n+1}, H^{n+1/2} in hand; reference
+        ``openem/kernels/source_dft.cu:272-316``): the first shot of ``m = n + 1`` writes
+        ``E^m + H^{m-1/2} / 2``, the second shot of ``m = n`` adds ``H^{m+1/2} / 2``."""
+        E, H = self.E, self.H
+        for m, buf in zip(self.sc.field_time_monitors, self.tsamp):
+            (i0, j0, k0), (ni, nj, nk) = m.origin, m.box
+            sl = (slice(i0, i0 + ni), slice(j0, j0 + nj), slice(k0, k0 + nk))
+            s0, s1 = m.slot_of(n + 1), m.slot_of(n)
+            for q, c in enumerate(m.comps):
+                F = E[c] if c < 3 else H[c - 3]
+                if s0 is not None:
+                    buf[q, s0] = F[sl] if c < 3 else 0.5 * F[sl]
+                if s1 is not None and c >= 3:
+                    buf[q, s1] += 0.5 * F[sl]
 
     def run(self, n_steps: int):
         for _ in range(int(n_steps)):

@@ -90,5 +90,8 @@ def test_run_stops_where_the_oracle_stops(gpu_lib):
 def test_run_without_shutoff_runs_every_step(gpu_lib):
     sc = scenes.mini_scene(shape=(10, 12, 40), pml_xy=True, plane_wave_on=False,
                            lorentz_on=False, num_steps=400)
-    res = solver.run(sc, use_shutoff=False, verbose=False)
-    assert res.steps_run == 400 and not res.shutoff_triggered
+    res = solver.run(sc, use_shutoff=False, verbose=False, return_fields=True)
+    assert res.steps_run == 400 and not res.shutoff_triggered and res.stop_reason == ""
+    # the reference's return_fields layout: {"Ex": (nx, ny, nz), ...}
+    assert sorted(res.fields) == sorted(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))
+    assert all(v.shape == sc.shape for v in res.fields.values())

@@ -3,12 +3,14 @@
 ``tests/golden/make_scene_npz.py``) and by round trips of this package's scenes."""
 
 import dataclasses
+from dataclasses import replace
 from pathlib import Path
 
 import numpy as np
 import pytest
 
 from openem_amd import scene_io, scenes, tables
+from openem_amd.model import FieldTimeMonitor
 from oracle.yee64 import Oracle64
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "scene_v10_small.npz"
@@ -129,6 +131,8 @@ SCENES = {
                                                 thick_cells=4, num_steps=30, nf=1),
     "tfsf_box": lambda: scenes.tfsf_box_scene(num_steps=30, scatterer=True),
     "bloch": lambda: scenes.bloch_cell_scene(nxy=8, nz=40, n_pml=6, num_steps=30, nf=2),
+    "time_monitors": lambda: replace(scenes.mini_scene(num_steps=30), field_time_monitors=[
+        FieldTimeMonitor("t", (0, 4), (1, 2, 3), (4, 5, 6), 2, 25, 3, 7)]),
 }
 
 
@@ -148,7 +152,7 @@ def test_round_trip_keeps_every_field_and_the_step(tmp_path, name):
     ({"tensor_comp": np.array([0])}, "tensor"),
     ({"n_absorbers": np.array(1)}, "absorber"),
     ({"symmetry": np.array([1, 0, 0])}, "symmetry"),
-    ({"tmon_names": np.array(["t"])}, "field-time"),
+    ({"ftmon_names": np.array(["t"])}, "flux-time"),
     ({"flat_y": np.array(True)}, "flat"),
     ({"msrc0_bb_ey": np.zeros(2)}, "broadband"),
 ])
