@@ -174,6 +174,27 @@ int oem_add_time_monitor(oem_ctx *ctx, int32_t n_comps, const int32_t *comps,
  * (`openem/readout.py:183-210`). */
 int oem_read_time_monitor(oem_ctx *ctx, int32_t mon_id, float *out, size_t count);
 
+/* Time-domain flux monitor (real fields): one float64 scalar per slot, sum_p w_p sum dS (E_u
+ * Hbar_v - E_v Hbar_u) with the sampling window of a time monitor.  Geometry from the host:
+ * per transverse axis (ascending t1, t2) n samples with their edge index e, lower / upper
+ * cell l / h, centre -> edge weight w of the lower cell and trapezoid weight tr; per normal
+ * plane (1 or 2) the weight plane_w, the E layer ke, the lower H cell kh and the coordinate
+ * weights wh_lo / wh_hi of H at kh / kh+1; sign = normal_dir (times -1 for a y normal).
+ * Replaces monitors_setup._ftmon_entry (`openem/monitors_setup.py:36-71`) and
+ * readout._sample_flux_time (`openem/readout.py:20-73`). */
+int oem_add_flux_time_monitor(oem_ctx *ctx, int32_t axis, int32_t n1, const int32_t *e1,
+                              const int32_t *l1, const int32_t *h1, const float *w1,
+                              const double *tr1, int32_t n2, const int32_t *e2,
+                              const int32_t *l2, const int32_t *h2, const float *w2,
+                              const double *tr2, int32_t n_planes, const double *plane_w,
+                              const int32_t *ke, const int32_t *kh, const float *wh_lo,
+                              const float *wh_hi, double sign, int32_t step_begin,
+                              int32_t step_end, int32_t interval, int32_t n_slots,
+                              int32_t *mon_id);
+
+/* The flux series of a flux-time monitor: count = n_slots doubles. */
+int oem_read_flux_time_monitor(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
+
 /* Launch shape: x-planes each work-group marches (0 = the library's choice).  Before
  * oem_prepare.  No counterpart in the reference (its launch shapes are fixed in the kernels'
  * callers, `openem/solver.py:876-1162`); it is the knob the launch autotuner sets. */

@@ -27,7 +27,8 @@ EXPORTS = ("oem_version", "oem_last_error", "oem_device_count", "oem_ctx_create"
            "oem_ctx_destroy", "oem_set_axis", "oem_set_materials_lut",
            "oem_set_materials_dense", "oem_set_dispersion", "oem_add_plane_source",
            "oem_add_dipoles", "oem_add_tfsf_points", "oem_add_mode_points", "oem_add_flux_plane",
-           "oem_add_field_box", "oem_add_time_monitor", "oem_read_time_monitor", "oem_prepare",
+           "oem_add_field_box", "oem_add_time_monitor", "oem_read_time_monitor",
+           "oem_add_flux_time_monitor", "oem_read_flux_time_monitor", "oem_prepare",
            "oem_run", "oem_profile_run", "oem_timed_run", "oem_step_index", "oem_read_monitor",
            "oem_read_field", "oem_write_field", "oem_set_chunk", "oem_set_energy_tables",
            "oem_energy", "oem_set_flux_weights", "oem_monitor_flux", "oem_set_overlap_weights",
@@ -79,6 +80,10 @@ _SIGS = {
     "oem_add_time_monitor": ([_P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32,
                               C.c_int32, _P], C.c_int),
     "oem_read_time_monitor": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "oem_add_flux_time_monitor": ([_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, C.c_int32, _P, _P,
+                                   _P, _P, _P, C.c_int32, _P, _P, _P, _P, _P, C.c_double,
+                                   C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P], C.c_int),
+    "oem_read_flux_time_monitor": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "oem_prepare": ([_P], C.c_int),
     "oem_run": ([_P, C.c_int64], C.c_int),
     "oem_profile_run": ([_P, C.c_int64, _P, _P], C.c_int),

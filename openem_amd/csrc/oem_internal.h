@@ -41,6 +41,10 @@ struct TimeMonitor {
     size_t count;     // nc*slots*cells elements of the field type
 };
 
+struct FluxTimeMonitor {
+    FluxTimeDev d;
+};
+
 // One grouped point list (dipoles or TF/SF face corrections) and which field it updates.
 enum { PT_DIPOLE = 0, PT_TFSF = 1, PT_MODE = 2 };
 
@@ -80,6 +84,7 @@ void adjoint_state_free(oem_adjoint_state *a);
 using oemi::fail;
 using oemi::Monitor;
 using oemi::TimeMonitor;
+using oemi::FluxTimeMonitor;
 using oemi::Points;
 using oemi::PT_DIPOLE;
 using oemi::PT_MODE;
@@ -126,6 +131,7 @@ struct oem_ctx {
     std::vector<Points> points;
     std::vector<Monitor> mons;
     std::vector<TimeMonitor> tmons;
+    std::vector<FluxTimeMonitor> ftmons;
     int *d_step = nullptr;      // [0] steps taken, [1] steps flushed into the DFT accumulators
     int64_t step = 0;
     int pending = 0;

@@ -1131,6 +1131,141 @@ k_time_sample(TimeMonDev d, const F *__restrict__ E, const F *__restrict__ H, si
 }
 
 // ------------------------------------------------------------------------------------------
+// Time-domain flux (reference _sample_flux_time, `openem/readout.py:20-73`, geometry
+// `openem/flux.py:334-513`): one scalar per sampled step, sum_p w_p sum dS (E_u Hbar_v - E_v
+// Hbar_u) over the colocated edge samples of the monitor plane(s).  At the end of step n the
+// second shot of m = n adds the H^{n+1/2} half with the E planes kept by the previous step, and
+// the first shot of m = n + 1 keeps the colocated planes of E^{n+1} and writes the H^{n+1/2}
+// half.  Terms are formed in float32 like the fields, summed in float64: per-block partials
+// (a fixed tree) and then one block summing the partials in block order, so the result does
+// not depend on scheduling.
+// ------------------------------------------------------------------------------------------
+#define OEM_FT_BLK 256
+struct FluxTimeDev {
+    int axis, t1, t2, n1, n2, n_planes;
+    const int *e1, *l1, *h1, *e2, *l2, *h2;     // edge sample and its lower / upper cell
+    const float *w1, *w2;                       // centre -> edge weight of the lower cell
+    const double *tr1, *tr2;                    // trapezoid weights
+    double plane_w[2];
+    int ke[2], kh[2];
+    float wh_lo[2], wh_hi[2];
+    double sign;
+    int beg, end, iv, slots;
+    float *ep;                                  // kept E planes: (plane, {u, v}, n1 * n2)
+    double *partial;                            // (2, n_blocks): first / second shot
+    double *buf;                                // (slots,)
+    int n_blocks;
+};
+
+__device__ __forceinline__ int ft_slot(const FluxTimeDev &d, int m) {
+    const int off = m - d.beg;
+    if (off < 0 || m >= d.end || off % d.iv) return -1;
+    const int s = off / d.iv;
+    return s < d.slots ? s : -1;
+}
+
+__device__ __forceinline__ size_t ft_cell(const FluxTimeDev &d, int k, int i1, int i2, int ny,
+                                          int nz) {
+    int c[3];
+    c[d.axis] = k;
+    c[d.t1] = i1;
+    c[d.t2] = i2;
+    return ((size_t)c[0] * ny + c[1]) * nz + c[2];
+}
+
+// Sum of a double over the block (fixed tree), valid in thread 0.
+__device__ __forceinline__ double ft_block_sum(double v, double *sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = OEM_FT_BLK / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+template <typename F>
+__global__ void __launch_bounds__(OEM_FT_BLK)
+k_flux_time_partial(FluxTimeDev d, const F *__restrict__ E, const F *__restrict__ H,
+                    size_t ncell, int ny, int nz, const int *__restrict__ stp)
+{
+    static_assert(sizeof(F) == 4, "flux-time monitors sample real fields");
+    const int n = step_now(stp);
+    const int s1 = ft_slot(d, n), s0 = ft_slot(d, n + 1);
+    if (s1 < 0 && s0 < 0) return;                       // uniform over the grid
+    __shared__ double sh[OEM_FT_BLK];
+    const int np = d.n1 * d.n2;
+    const int q = blockIdx.x * OEM_FT_BLK + threadIdx.x;
+    double v0 = 0.0, v1 = 0.0;
+    if (q < np) {
+        const int a = q / d.n2, b = q - a * d.n2;
+        const float w1 = d.w1[a], w2 = d.w2[b];
+        const double dS = d.sign * d.tr1[a] * d.tr2[b];
+        const float *Eu = E + (size_t)d.t1 * ncell, *Ev = E + (size_t)d.t2 * ncell;
+        const float *Hu = H + (size_t)d.t1 * ncell, *Hv = H + (size_t)d.t2 * ncell;
+        for (int p = 0; p < d.n_planes; ++p) {
+            const int ke = d.ke[p], kh = d.kh[p];
+            const float lo = d.wh_lo[p], hi = d.wh_hi[p];
+            // H_u: on the edge along t1, centre -> edge along t2, coordinate weights along the normal
+            const float hu_l = lo * Hu[ft_cell(d, kh, d.e1[a], d.l2[b], ny, nz)] +
+                               hi * Hu[ft_cell(d, kh + 1, d.e1[a], d.l2[b], ny, nz)];
+            const float hu_h = lo * Hu[ft_cell(d, kh, d.e1[a], d.h2[b], ny, nz)] +
+                               hi * Hu[ft_cell(d, kh + 1, d.e1[a], d.h2[b], ny, nz)];
+            const float hu = w2 * hu_l + (1.0f - w2) * hu_h;
+            // H_v: centre -> edge along t1, on the edge along t2
+            const float hv_l = lo * Hv[ft_cell(d, kh, d.l1[a], d.e2[b], ny, nz)] +
+                               hi * Hv[ft_cell(d, kh + 1, d.l1[a], d.e2[b], ny, nz)];
+            const float hv_h = lo * Hv[ft_cell(d, kh, d.h1[a], d.e2[b], ny, nz)] +
+                               hi * Hv[ft_cell(d, kh + 1, d.h1[a], d.e2[b], ny, nz)];
+            const float hv = w1 * hv_l + (1.0f - w1) * hv_h;
+            float *keep = d.ep + (size_t)(2 * p) * np;
+            const double wp = 0.5 * d.plane_w[p] * dS;
+            if (s1 >= 0)
+                v1 += wp * ((double)(keep[q] * hv) - (double)(keep[np + q] * hu));
+            if (s0 >= 0) {
+                const float eu = w1 * Eu[ft_cell(d, ke, d.l1[a], d.e2[b], ny, nz)] +
+                                 (1.0f - w1) * Eu[ft_cell(d, ke, d.h1[a], d.e2[b], ny, nz)];
+                const float ev = w2 * Ev[ft_cell(d, ke, d.e1[a], d.l2[b], ny, nz)] +
+                                 (1.0f - w2) * Ev[ft_cell(d, ke, d.e1[a], d.h2[b], ny, nz)];
+                v0 += wp * ((double)(eu * hv) - (double)(ev * hu));
+                keep[q] = eu;
+                keep[np + q] = ev;
+            }
+        }
+    }
+    const double t0 = ft_block_sum(v0, sh);
+    __syncthreads();
+    const double t1 = ft_block_sum(v1, sh);
+    if (threadIdx.x == 0) {
+        d.partial[blockIdx.x] = t0;
+        d.partial[d.n_blocks + blockIdx.x] = t1;
+    }
+}
+
+template <typename F>
+__global__ void __launch_bounds__(OEM_FT_BLK)
+k_flux_time_sum(FluxTimeDev d, const int *__restrict__ stp)
+{
+    const int n = step_now(stp);
+    const int s1 = ft_slot(d, n), s0 = ft_slot(d, n + 1);
+    if (s1 < 0 && s0 < 0) return;
+    __shared__ double sh[OEM_FT_BLK];
+    double a0 = 0.0, a1 = 0.0;
+    for (int b = threadIdx.x; b < d.n_blocks; b += OEM_FT_BLK) {
+        a0 += d.partial[b];
+        a1 += d.partial[d.n_blocks + b];
+    }
+    const double t0 = ft_block_sum(a0, sh);
+    __syncthreads();
+    const double t1 = ft_block_sum(a1, sh);
+    if (threadIdx.x == 0) {
+        if (s1 >= 0) d.buf[s1] += t1;
+        if (s0 >= 0) d.buf[s0] = t0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Running DFT, batched: every step a snapshot of the monitored values goes into a ring of T
 // rows; every T steps one flush replays them in step order against the float64 phase table and
 // adds into the float64 accumulators (reference accumulate_dft / accumulate_dft_box and the

@@ -575,7 +575,8 @@ extern "C" int oem_pool_attach(oem_pool *pool, oem_ctx *ctx) {
     int per[OEM_POOL_SLOTS] = {};
     for (auto &pt : ctx->points)
         if (++per[points_slot(pt)] > 1) return fail(OEM_ENOTSUP, "more than one point list of one kind");
-    if (!ctx->tmons.empty()) return fail(OEM_ENOTSUP, "time-domain monitors run standalone only");
+    if (!ctx->tmons.empty() || !ctx->ftmons.empty())
+        return fail(OEM_ENOTSUP, "time-domain monitors run standalone only");
     HIP_OK(hipStreamSynchronize(ctx->stream));    // its uploads have landed
     std::lock_guard<std::mutex> lk(pool->mu);
     int running = 0, mons = 0;

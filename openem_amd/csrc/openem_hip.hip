@@ -528,6 +528,84 @@ extern "C" int oem_read_time_monitor(oem_ctx *ctx, int32_t mon_id, float *out, s
     return OEM_OK;
 }
 
+extern "C" int oem_add_flux_time_monitor(
+    oem_ctx *ctx, int32_t axis, int32_t n1, const int32_t *e1, const int32_t *l1,
+    const int32_t *h1, const float *w1, const double *tr1, int32_t n2, const int32_t *e2,
+    const int32_t *l2, const int32_t *h2, const float *w2, const double *tr2, int32_t n_planes,
+    const double *plane_w, const int32_t *ke, const int32_t *kh, const float *wh_lo,
+    const float *wh_hi, double sign, int32_t step_begin, int32_t step_end, int32_t interval,
+    int32_t n_slots, int32_t *mon_id) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (ctx->cplx) return fail(OEM_ENOTSUP, "flux-time monitors on real fields only");
+    if (!mon_id || !plane_w || !ke || !kh || !wh_lo || !wh_hi) return fail(OEM_EINVAL, "null argument");
+    if (axis < 0 || axis > 2) return fail(OEM_EINVAL, "axis %d", axis);
+    if (n1 < 1 || n2 < 1 || (int64_t)n1 * n2 > (1 << 28)) return fail(OEM_EINVAL, "%d x %d samples", n1, n2);
+    if (n_planes < 1 || n_planes > 2) return fail(OEM_EINVAL, "%d planes", n_planes);
+    if (interval < 1 || n_slots < 1 || step_begin < 0 || step_end < step_begin)
+        return fail(OEM_EINVAL, "sampling window [%d, %d) every %d, %d slots", step_begin,
+                    step_end, interval, n_slots);
+    const int n[3] = {ctx->dims.nx, ctx->dims.ny, ctx->dims.nz};
+    const int t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
+    const int32_t *idx[2][3] = {{e1, l1, h1}, {e2, l2, h2}};
+    const int cnt[2] = {n1, n2}, lim[2] = {n[t1], n[t2]};
+    for (int t = 0; t < 2; ++t)
+        for (int f = 0; f < 3; ++f) {
+            if (!idx[t][f]) return fail(OEM_EINVAL, "null sample table");
+            for (int q = 0; q < cnt[t]; ++q)
+                if (idx[t][f][q] < 0 || idx[t][f][q] >= lim[t])
+                    return fail(OEM_EINVAL, "transverse sample index out of range");
+        }
+    if (!w1 || !w2 || !tr1 || !tr2) return fail(OEM_EINVAL, "null weights");
+    for (int p = 0; p < n_planes; ++p)
+        if (ke[p] < 0 || ke[p] >= n[axis] || kh[p] < 0 || kh[p] + 1 >= n[axis])
+            return fail(OEM_EINVAL, "normal plane %d out of range", p);
+    FluxTimeDev d{};
+    d.axis = axis; d.t1 = t1; d.t2 = t2; d.n1 = n1; d.n2 = n2; d.n_planes = n_planes;
+    d.sign = sign; d.beg = step_begin; d.end = step_end; d.iv = interval; d.slots = n_slots;
+    for (int p = 0; p < n_planes; ++p) {
+        d.plane_w[p] = plane_w[p]; d.ke[p] = ke[p]; d.kh[p] = kh[p];
+        d.wh_lo[p] = wh_lo[p]; d.wh_hi[p] = wh_hi[p];
+    }
+    int *di[2][3];
+    for (int t = 0; t < 2; ++t)
+        for (int f = 0; f < 3; ++f) TRY(ctx->upload(&di[t][f], (const int *)idx[t][f], (size_t)cnt[t]));
+    d.e1 = di[0][0]; d.l1 = di[0][1]; d.h1 = di[0][2];
+    d.e2 = di[1][0]; d.l2 = di[1][1]; d.h2 = di[1][2];
+    float *dw1, *dw2;
+    TRY(ctx->upload(&dw1, w1, (size_t)n1));
+    TRY(ctx->upload(&dw2, w2, (size_t)n2));
+    d.w1 = dw1; d.w2 = dw2;
+    double *dt1, *dt2;
+    TRY(ctx->upload(&dt1, tr1, (size_t)n1));
+    TRY(ctx->upload(&dt2, tr2, (size_t)n2));
+    d.tr1 = dt1; d.tr2 = dt2;
+    const size_t np = (size_t)n1 * n2;
+    d.n_blocks = (int)((np + OEM_FT_BLK - 1) / OEM_FT_BLK);
+    float *ep;
+    TRY(ctx->alloc(&ep, (size_t)2 * n_planes * np, true));
+    d.ep = ep;
+    double *part, *buf;
+    TRY(ctx->alloc(&part, (size_t)2 * d.n_blocks, true));
+    TRY(ctx->alloc(&buf, (size_t)n_slots, true));
+    d.partial = part;
+    d.buf = buf;
+    ctx->ftmons.push_back(FluxTimeMonitor{d});
+    *mon_id = (int32_t)ctx->ftmons.size() - 1;
+    return OEM_OK;
+}
+
+extern "C" int oem_read_flux_time_monitor(oem_ctx *ctx, int32_t mon_id, double *out, size_t count) {
+    CHECK_CTX(ctx);
+    if (mon_id < 0 || mon_id >= (int)ctx->ftmons.size()) return fail(OEM_EINVAL, "flux-time monitor %d", mon_id);
+    const FluxTimeMonitor &m = ctx->ftmons[mon_id];
+    if (!out || count != (size_t)m.d.slots)
+        return fail(OEM_EINVAL, "flux-time monitor %d holds %d values, asked %zu", mon_id, m.d.slots, count);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipMemcpy(out, m.d.buf, count * sizeof(double), hipMemcpyDeviceToHost));
+    return OEM_OK;
+}
+
 extern "C" int oem_prepare(oem_ctx *ctx) {
     CHECK_CTX(ctx);
     if (ctx->prepared) return OEM_OK;
@@ -740,6 +818,18 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
             hipLaunchKernelGGL((k_time_sample<F>), dim3(blocks_for(tm.d.cells, 256)), dim3(256), 0,
                                c->stream, tm.d, (const F *)E, (const F *)H, nc, c->dims.ny,
                                c->dims.nz, c->d_step, shot);
+            TRY(mark(OEM_K_DFT_SNAP, false));
+        }
+    }
+    if constexpr (sizeof(F) == 4) {
+        for (auto &fm : c->ftmons) {
+            if (c->mons_off) break;
+            TRY(mark(OEM_K_DFT_SNAP, true));
+            hipLaunchKernelGGL((k_flux_time_partial<float>), dim3(fm.d.n_blocks), dim3(OEM_FT_BLK), 0,
+                               c->stream, fm.d, (const float *)E, (const float *)H, nc,
+                               c->dims.ny, c->dims.nz, c->d_step);
+            hipLaunchKernelGGL((k_flux_time_sum<float>), dim3(1), dim3(OEM_FT_BLK), 0, c->stream, fm.d,
+                               c->d_step);
             TRY(mark(OEM_K_DFT_SNAP, false));
         }
     }

@@ -164,3 +164,73 @@ def monitor_flux(data: np.ndarray, grid: Grid, mon) -> np.ndarray:
                       "using the in-place Yee sum")
     sign = -1.0 if mon.axis == 1 else 1.0
     return sign * plane_flux(data, grid, transverse=mon.transverse, axis=mon.axis)
+
+
+def normal_planes(grid: Grid, mon):
+    """The one or two primal planes of a flux-time monitor (reference ``openem/flux.py:334-365``):
+    ``(w, ke, kh, wh_lo, wh_hi)`` per plane, E taken on layer ``ke`` and H interpolated by
+    coordinate onto ``edges[ke]`` from the cell centres ``kh = ke - 1`` and ``ke``."""
+    ax = grid.axes[mon.axis]
+    e, c = np.asarray(ax.edges, np.float64), np.asarray(ax.centers, np.float64)
+    km, fr = int(mon.plane_index), float(mon.frac)
+    planes = []
+    for w, k in ((1.0 - fr, km), (fr, km + 1)):
+        if w == 0.0:
+            continue
+        kh = k - 1
+        if not 0 <= kh < c.size - 1:
+            raise ValueError(f"monitor {mon.name}: plane {k} on the grid end cannot colocate H")
+        lo = (c[kh + 1] - e[k]) / (c[kh + 1] - c[kh])
+        planes.append((w, k, kh, lo, 1.0 - lo))
+    return planes
+
+
+def flux_time_geometry(grid: Grid, mon) -> dict:
+    """Everything the instantaneous colocated flux needs (reference
+    ``openem/flux.py:480-513``): per transverse axis the edge samples with their lower / upper
+    cell centres, the centre -> edge weight of the lower one and the trapezoid weights; the
+    normal planes; the sign (``normal_dir``, and -1 for a y normal's ascending-pair order)."""
+    t1, t2 = transverse_axes(mon.axis)
+    s1 = _edge_samples(grid.axes[t1], mon.t_bounds[0], mon.name, "xyz"[t1])
+    s2 = _edge_samples(grid.axes[t2], mon.t_bounds[1], mon.name, "xyz"[t2])
+    sign = float(mon.normal_dir) * (-1.0 if mon.axis == 1 else 1.0)
+    return {"axis": mon.axis, "tax": (t1, t2), "s": (s1, s2), "planes": normal_planes(grid, mon),
+            "sign": sign}
+
+
+def colocated_planes(E, H, g: dict, plane: int):
+    """``(eu, ev, hu, hv)`` on the edge samples of one normal plane from 3D fields ``E``, ``H``
+    (component-first arrays): E_u and H_v interpolated along t1, E_v and H_u along t2, H along
+    the normal by the plane's coordinate weights."""
+    a = g["axis"]
+    t1, t2 = g["tax"]
+    (e1, l1, h1, w1, _), (e2, l2, h2, w2, _) = g["s"]
+    _, ke, kh, wlo, whi = g["planes"][plane]
+
+    def layer(F, k):
+        return np.take(F, k, axis=a)          # (n_t1, n_t2): the remaining axes stay ascending
+
+    def along1(P):
+        return w1[:, None] * P[l1][:, e2] + (1.0 - w1)[:, None] * P[h1][:, e2]
+
+    def along2(P):
+        Q = P[e1]
+        return w2[None, :] * Q[:, l2] + (1.0 - w2)[None, :] * Q[:, h2]
+
+    eu = along1(layer(E[t1], ke))
+    ev = along2(layer(E[t2], ke))
+    hu = along2(wlo * layer(H[t1], kh) + whi * layer(H[t1], kh + 1))
+    hv = along1(wlo * layer(H[t2], kh) + whi * layer(H[t2], kh + 1))
+    return eu, ev, hu, hv
+
+
+def flux_time_sample(E, H, g: dict) -> float:
+    """Instantaneous colocated flux ``sum_p w_p sum dS (E_u H_v - E_v H_u)`` of fields ``E``,
+    ``H`` (reference ``openem/flux.py:566-580``)."""
+    (_, _, _, _, tr1), (_, _, _, _, tr2) = g["s"]
+    dS = g["sign"] * np.outer(tr1, tr2)
+    out = 0.0
+    for p, (w, *_rest) in enumerate(g["planes"]):
+        eu, ev, hu, hv = colocated_planes(E, H, g, p)
+        out += w * float(np.sum((eu * hv - ev * hu) * dS))
+    return out

@@ -146,6 +146,30 @@ class FieldTimeMonitor:
 
 
 @dataclass(frozen=True)
+class FluxTimeMonitor:
+    """Time-domain flux monitor (reference ``openem/model.py:316-362``): one instantaneous
+    Poynting-flux scalar per sampled step, ``s(m) = sum dS (E_u Hbar_v - E_v Hbar_u)`` over the
+    colocated plane (the colocated flux convention, no 1/2), E at ``m dt`` and ``Hbar`` the average
+    of the half steps around it; a plane between primal planes ``plane_index`` and
+    ``plane_index + 1`` combines their fluxes as ``(1 - frac) s_km + frac s_km+1``.  Sampling
+    window as :class:`FieldTimeMonitor`."""
+
+    name: str
+    axis: int
+    plane_index: int
+    normal_dir: int
+    step_begin: int
+    step_end: int
+    interval: int
+    num_slots: int
+    t_bounds: tuple[tuple[float, float], tuple[float, float]] = ((0.0, 0.0), (0.0, 0.0))
+    frac: float = 0.0
+
+    slot_of = FieldTimeMonitor.slot_of
+    filled = FieldTimeMonitor.filled
+
+
+@dataclass(frozen=True)
 class FluxMonitor:
     """Planar flux monitor (reference ``openem/model.py:387-418``): tangential phasors
     (E_t1, E_t2, H_t1, H_t2) with H colocated onto the E plane."""
@@ -258,6 +282,7 @@ class Scene:
     field_monitors: list = field(default_factory=list)
     mode_monitors: list = field(default_factory=list)
     field_time_monitors: list = field(default_factory=list)
+    flux_time_monitors: list = field(default_factory=list)
 
     @property
     def shape(self) -> tuple[int, int, int]:

@@ -8,14 +8,15 @@ one in the same key layout, so scenes move between the two in both directions.
 
 Covered one for one (key names and meanings of ``openem/serialize.py:120-420``): the grid, eps /
 sigma / PEC arrays, CPML faces, plane-wave sources, mode sources, point dipoles, the pole-residue
-dispersion CSR, flux, field and field-time monitors, ``dt``, ``num_time_steps``, ``shutoff`` and
+dispersion CSR, flux, field, field-time and flux-time monitors, ``dt``, ``num_time_steps``,
+``shutoff`` and
 ``bloch_k``.
 
 Differences, each failing closed rather than silently:
 
 * entities this stepper does not model (mixed-interface dispersion, time modulation, tensor
-  media, absorber slabs, symmetry planes, flux-time / permittivity / projection / box-flux
-  monitors, broadband mode-source terms, strided field monitors, 2D flat axes) raise
+  media, absorber slabs, symmetry planes, permittivity / projection / box-flux monitors,
+  broadband mode-source terms, strided field monitors, 2D flat axes) raise
   ``NotImplementedError`` on load;
 * TF/SF boxes: the reference stores its own 1D line (grid, eps, drive tables); this package
   builds its line from the scene (``openem_amd.tfsf``), so a loaded box keeps the geometry,
@@ -37,8 +38,8 @@ import numpy as np
 
 from . import cpml, waveform
 from .grid import Axis, Grid
-from .model import (Dispersion, FieldMonitor, FieldTimeMonitor, FluxMonitor, ModeMonitor,
-                    ModeSource, PlaneWaveSource, PointDipole, Scene, TFSFSource)
+from .model import (Dispersion, FieldMonitor, FieldTimeMonitor, FluxMonitor, FluxTimeMonitor,
+                    ModeMonitor, ModeSource, PlaneWaveSource, PointDipole, Scene, TFSFSource)
 
 FORMAT_VERSION = 10
 
@@ -46,8 +47,7 @@ _PML_FIELDS = ("a_E", "b_E", "inv_kappa_E", "a_H", "b_H", "inv_kappa_H")
 _MODE_FIELDS = ("ey_inc", "ez_inc", "hy_inc", "hz_inc", "amp_e", "amp_h")
 _UNSUPPORTED_PREFIXES = {"dmix_": "mixed-interface dispersion", "mod_": "time modulation",
                          "tensor_": "tensor media"}
-_UNSUPPORTED_LISTS = {"ftmon_names": "flux-time monitors",
-                      "pmon_names": "permittivity monitors",
+_UNSUPPORTED_LISTS = {"pmon_names": "permittivity monitors",
                       "projmon_names": "projection monitors", "bfmon_names": "box flux monitors"}
 
 
@@ -192,6 +192,12 @@ def save(sc: Scene, path) -> pathlib.Path:
         d[f"tmon{i}_comps"] = np.array(m.comps)
         d[f"tmon{i}_origin"] = np.array(m.origin)
         d[f"tmon{i}_box"] = np.array(m.box)
+    d["ftmon_names"] = np.array([m.name for m in sc.flux_time_monitors])
+    for i, m in enumerate(sc.flux_time_monitors):
+        d[f"ftmon{i}_meta"] = np.array([m.axis, m.plane_index, m.normal_dir, m.step_begin,
+                                        m.step_end, m.interval, m.num_slots])
+        d[f"ftmon{i}_frac"] = np.array(float(m.frac))
+        d[f"ftmon{i}_tbounds"] = np.array(m.t_bounds, dtype=np.float64)
     for key in _UNSUPPORTED_LISTS:
         d[key] = np.array([], dtype="<U1")
     d["n_dipoles"] = np.array(len(sc.dipoles))
@@ -339,6 +345,14 @@ def load(path) -> Scene:
                 origin=tuple(int(v) for v in z[f"tmon{i}_origin"]),
                 box=tuple(int(v) for v in z[f"tmon{i}_box"]), step_begin=beg, step_end=end,
                 interval=iv, num_slots=slots))
+        flux_time_mons = []
+        for i, name in enumerate(_names(z, "ftmon_names")):
+            axis, plane, ndir, beg, end, iv, slots = (int(v) for v in z[f"ftmon{i}_meta"])
+            flux_time_mons.append(FluxTimeMonitor(
+                name=name, axis=axis, plane_index=plane, normal_dir=ndir, step_begin=beg,
+                step_end=end, interval=iv, num_slots=slots,
+                t_bounds=tuple(tuple(float(v) for v in r) for r in z[f"ftmon{i}_tbounds"]),
+                frac=float(z[f"ftmon{i}_frac"])))
         dipoles = []
         for i in range(_count(z, "n_dipoles")):
             dipoles.append(PointDipole(
@@ -359,4 +373,4 @@ def load(path) -> Scene:
                      else (0.0, 0.0, 0.0)),
             sources=sources, tfsf_sources=tfsf, dipoles=dipoles, mode_sources=mode_sources,
             flux_monitors=flux_mons, field_monitors=field_mons, mode_monitors=mode_mons,
-            field_time_monitors=time_mons)
+            field_time_monitors=time_mons, flux_time_monitors=flux_time_mons)

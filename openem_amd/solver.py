@@ -156,6 +156,23 @@ class Stepper:
                                                      _lib.ptr(b), m.step_begin, m.step_end,
                                                      m.interval, m.num_slots, C.byref(mid)))
             self.tmon_ids.append(int(mid.value))
+        self.ftmon_ids = []
+        for m in t.scene.flux_time_monitors:
+            g = flux_mod.flux_time_geometry(t.scene.grid, m)
+            (e1, l1, h1, w1, tr1), (e2, l2, h2, w2, tr2) = g["s"]
+            pl = g["planes"]
+            arrs1 = [_i32(e1), _i32(l1), _i32(h1), _f32(w1), _f64(tr1)]
+            arrs2 = [_i32(e2), _i32(l2), _i32(h2), _f32(w2), _f64(tr2)]
+            pw = _f64([x[0] for x in pl])
+            ke, kh = _i32([x[1] for x in pl]), _i32([x[2] for x in pl])
+            wlo, whi = _f32([x[3] for x in pl]), _f32([x[4] for x in pl])
+            mid = C.c_int32(-1)
+            _lib.check(lib, lib.oem_add_flux_time_monitor(
+                ctx, m.axis, e1.size, *[_lib.ptr(x) for x in arrs1], e2.size,
+                *[_lib.ptr(x) for x in arrs2], len(pl), _lib.ptr(pw), _lib.ptr(ke), _lib.ptr(kh),
+                _lib.ptr(wlo), _lib.ptr(whi), float(g["sign"]), m.step_begin, m.step_end,
+                m.interval, m.num_slots, C.byref(mid)))
+            self.ftmon_ids.append(int(mid.value))
 
     # -- stepping ---------------------------------------------------------------------------
     def run(self, n_steps: int) -> None:
@@ -214,6 +231,14 @@ class Stepper:
         raw = out.view(np.float32)
         _lib.check(self.lib, self.lib.oem_read_time_monitor(self.ctx, self.tmon_ids[i],
                                                             _lib.ptr(raw), raw.size))
+        return out
+
+    def read_flux_time_monitor(self, i: int) -> np.ndarray:
+        """The float64 flux series (all slots) of flux-time monitor ``i``."""
+        m = self.tabs.scene.flux_time_monitors[i]
+        out = np.empty(m.num_slots, np.float64)
+        _lib.check(self.lib, self.lib.oem_read_flux_time_monitor(self.ctx, self.ftmon_ids[i],
+                                                                 _lib.ptr(out), out.size))
         return out
 
     # -- float64 reductions on the device ----------------------------------------------------
@@ -370,6 +395,8 @@ def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kerne
             filled = m.filled(steps_run)
             tsamp[m.name] = st.read_time_monitor(i)[:, :filled]
             tslots[m.name] = filled
+        ftime = {m.name: st.read_flux_time_monitor(i)[:m.filled(steps_run)]
+                 for i, m in enumerate(sc.flux_time_monitors)}
         fields = {}
         if return_fields:
             fields = {name: st.read_field(c) for c, name in enumerate(COMPONENTS)}
@@ -388,7 +415,7 @@ def run(sc: Scene, num_steps: int | None = None, use_shutoff: bool = True, kerne
                   wall_seconds=wall, fields=fields, shutoff_triggered=triggered,
                   stop_reason=reason, shutoff_summary=summary, source_end=src_end,
                   decay_trace=decay, energy_trace=energy_tr, time_samples=tsamp,
-                  time_slots=tslots)
+                  time_slots=tslots, flux_time=ftime)
 
 
 def _warn_nonfinite(ph, fph) -> None:

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from openem_amd import scene_io, scenes, tables
-from openem_amd.model import FieldTimeMonitor
+from openem_amd.model import FieldTimeMonitor, FluxTimeMonitor
 from oracle.yee64 import Oracle64
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "scene_v10_small.npz"
@@ -120,6 +120,14 @@ def _same(a, b, path="scene"):
         assert a == b, (path, a, b)
 
 
+def _with_time_monitors(sc):
+    g = sc.grid
+    span = tuple((float(g.axes[t].edges[0]), float(g.axes[t].edges[-1])) for t in (0, 1))
+    return replace(sc, field_time_monitors=[
+        FieldTimeMonitor("t", (0, 4), (1, 2, 3), (4, 5, 6), 2, 25, 3, 7)], flux_time_monitors=[
+        FluxTimeMonitor("f", 2, 20, -1, 1, 28, 2, 12, t_bounds=span, frac=0.25)])
+
+
 SCENES = {
     "mini_all": lambda: scenes.mini_scene(num_steps=30),
     "mini_pml_sigma_pec_hdip": lambda: scenes.mini_scene(num_steps=30, pml_xy=True, sigma_on=True,
@@ -131,8 +139,7 @@ SCENES = {
                                                 thick_cells=4, num_steps=30, nf=1),
     "tfsf_box": lambda: scenes.tfsf_box_scene(num_steps=30, scatterer=True),
     "bloch": lambda: scenes.bloch_cell_scene(nxy=8, nz=40, n_pml=6, num_steps=30, nf=2),
-    "time_monitors": lambda: replace(scenes.mini_scene(num_steps=30), field_time_monitors=[
-        FieldTimeMonitor("t", (0, 4), (1, 2, 3), (4, 5, 6), 2, 25, 3, 7)]),
+    "time_monitors": lambda: _with_time_monitors(scenes.mini_scene(num_steps=30)),
 }
 
 
@@ -152,7 +159,7 @@ def test_round_trip_keeps_every_field_and_the_step(tmp_path, name):
     ({"tensor_comp": np.array([0])}, "tensor"),
     ({"n_absorbers": np.array(1)}, "absorber"),
     ({"symmetry": np.array([1, 0, 0])}, "symmetry"),
-    ({"ftmon_names": np.array(["t"])}, "flux-time"),
+    ({"pmon_names": np.array(["t"])}, "permittivity"),
     ({"flat_y": np.array(True)}, "flat"),
     ({"msrc0_bb_ey": np.zeros(2)}, "broadband"),
 ])
