@@ -111,6 +111,40 @@ class FieldMonitor:
     origin: tuple[int, int, int]
     box: tuple[int, int, int]
     apodization: tuple | None = None
+    # tidy3d's spatial stride: the box is accumulated whole, the stride applies at write-back
+    interval_space: tuple[int, int, int] = (1, 1, 1)
+
+
+@dataclass(frozen=True)
+class PermittivityMonitor:
+    """eps on the Yee positions of an index box (reference ``openem/model.py:366-384``): read
+    from the scene, not stepped (:meth:`Scene.eps_box`)."""
+
+    name: str
+    freqs: np.ndarray
+    origin: tuple[int, int, int]
+    box: tuple[int, int, int]
+
+
+@dataclass(frozen=True)
+class BoxFluxMonitor:
+    """A closed-surface flux monitor as the flux monitors of its faces (reference
+    ``openem/model.py:817-833``): the net outward flux is the sum of each face's flux times its
+    ``normal_dir`` (:func:`openem_amd.normalize.box_flux`)."""
+
+    name: str
+    face_names: tuple[str, ...]
+
+
+@dataclass(frozen=True)
+class ProjectionMonitor:
+    """A far-field projection monitor as the field monitors of its surfaces (reference
+    ``openem/model.py:836-855``); the stepper fills those field monitors, the projection itself
+    is post-processing."""
+
+    name: str
+    surface_names: tuple[str, ...]
+    normal_dirs: tuple[str, ...]
 
 
 @dataclass(frozen=True)
@@ -283,10 +317,21 @@ class Scene:
     mode_monitors: list = field(default_factory=list)
     field_time_monitors: list = field(default_factory=list)
     flux_time_monitors: list = field(default_factory=list)
+    permittivity_monitors: list = field(default_factory=list)
+    box_flux_monitors: list = field(default_factory=list)
+    projection_monitors: list = field(default_factory=list)
 
     @property
     def shape(self) -> tuple[int, int, int]:
         return self.grid.shape
+
+    def eps_box(self, mon) -> tuple[np.ndarray, ...]:
+        """``(eps_xx, eps_yy, eps_zz)`` of a :class:`PermittivityMonitor`'s box at the Yee
+        positions (reference ``openem/model.py:933-943``)."""
+        (i0, j0, k0), (ni, nj, nk) = mon.origin, mon.box
+        sl = (slice(i0, i0 + ni), slice(j0, j0 + nj), slice(k0, k0 + nk))
+        return tuple(np.broadcast_to(np.asarray(e), self.shape)[sl]
+                     for e in (self.eps_ex, self.eps_ey, self.eps_ez))
 
     @property
     def any_dispersion(self) -> bool:

@@ -135,3 +135,23 @@ def normalized_flux(sc, result, name: str) -> np.ndarray:
     if sc.sources:
         return vals / plane_flux_divisor(sc, m, m.freqs)
     return vals / flux_norm(sc, m.freqs)
+
+
+def box_flux(sc, result, name: str) -> np.ndarray:
+    """Net outward flux of the :class:`~openem_amd.model.BoxFluxMonitor` ``name``: each face's
+    flux times its ``normal_dir``, summed (reference ``openem/nb/backend.py:858-878``); raw
+    units."""
+    box = next(b for b in sc.box_flux_monitors if b.name == name)
+    faces = {m.name: m for m in sc.flux_monitors}
+    out = 0.0
+    for fn in box.face_names:
+        m = faces[fn]
+        out = out + flux_mod.monitor_flux(result.phasors[fn].data, sc.grid, m) * float(m.normal_dir)
+    return np.asarray(out)
+
+
+def normalized_box_flux(sc, result, name: str) -> np.ndarray:
+    """:func:`box_flux` in the tidy3d convention (divided by :func:`flux_norm`)."""
+    box = next(b for b in sc.box_flux_monitors if b.name == name)
+    freqs = next(m.freqs for m in sc.flux_monitors if m.name == box.face_names[0])
+    return box_flux(sc, result, name) / flux_norm(sc, freqs)

@@ -34,3 +34,24 @@ class Result:
     time_samples: dict = field(default_factory=dict)     # FieldTimeMonitor name -> samples
     time_slots: dict = field(default_factory=dict)       # FieldTimeMonitor name -> slots filled
     flux_time: dict = field(default_factory=dict)        # FluxTimeMonitor name -> flux series
+
+
+def downsample_indices(n: int, k: int) -> np.ndarray:
+    """Points kept along an axis of ``n`` samples by tidy3d's ``interval_space`` stride ``k``:
+    every ``k``-th from the first, the last always kept (reference
+    ``openem/nb/backend.py:1942-1955``)."""
+    n, k = int(n), max(int(k), 1)
+    idx = np.arange(0, n, k)
+    if n > 0 and idx[-1] != n - 1:
+        idx = np.append(idx, n - 1)
+    return idx
+
+
+def strided_field_phasors(result: Result, mon) -> np.ndarray:
+    """A field monitor's ``(6, nf, ni, nj, nk)`` phasors thinned by its ``interval_space``: the
+    box is accumulated at full resolution and the stride applies here (reference
+    ``openem/nb/backend.py:1958-1966``)."""
+    data = result.field_phasors[mon.name]
+    for ax, k in enumerate(getattr(mon, "interval_space", (1, 1, 1))):
+        data = np.take(data, downsample_indices(data.shape[2 + ax], k), axis=2 + ax)
+    return data

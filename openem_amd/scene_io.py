@@ -8,15 +8,14 @@ one in the same key layout, so scenes move between the two in both directions.
 
 Covered one for one (key names and meanings of ``openem/serialize.py:120-420``): the grid, eps /
 sigma / PEC arrays, CPML faces, plane-wave sources, mode sources, point dipoles, the pole-residue
-dispersion CSR, flux, field, field-time and flux-time monitors, ``dt``, ``num_time_steps``,
-``shutoff`` and
+dispersion CSR, flux, field (with its write-back stride), field-time, flux-time, permittivity,
+box-flux and projection monitors, ``dt``, ``num_time_steps``, ``shutoff`` and
 ``bloch_k``.
 
 Differences, each failing closed rather than silently:
 
 * entities this stepper does not model (mixed-interface dispersion, time modulation, tensor
-  media, absorber slabs, symmetry planes, permittivity / projection / box-flux monitors,
-  broadband mode-source terms, strided field monitors, 2D flat axes) raise
+  media, absorber slabs, symmetry planes, broadband mode-source terms, 2D flat axes) raise
   ``NotImplementedError`` on load;
 * TF/SF boxes: the reference stores its own 1D line (grid, eps, drive tables); this package
   builds its line from the scene (``openem_amd.tfsf``), so a loaded box keeps the geometry,
@@ -38,8 +37,9 @@ import numpy as np
 
 from . import cpml, waveform
 from .grid import Axis, Grid
-from .model import (Dispersion, FieldMonitor, FieldTimeMonitor, FluxMonitor, FluxTimeMonitor,
-                    ModeMonitor, ModeSource, PlaneWaveSource, PointDipole, Scene, TFSFSource)
+from .model import (BoxFluxMonitor, Dispersion, FieldMonitor, FieldTimeMonitor, FluxMonitor,
+                    FluxTimeMonitor, ModeMonitor, ModeSource, PermittivityMonitor, PlaneWaveSource,
+                    PointDipole, ProjectionMonitor, Scene, TFSFSource)
 
 FORMAT_VERSION = 10
 
@@ -47,8 +47,7 @@ _PML_FIELDS = ("a_E", "b_E", "inv_kappa_E", "a_H", "b_H", "inv_kappa_H")
 _MODE_FIELDS = ("ey_inc", "ez_inc", "hy_inc", "hz_inc", "amp_e", "amp_h")
 _UNSUPPORTED_PREFIXES = {"dmix_": "mixed-interface dispersion", "mod_": "time modulation",
                          "tensor_": "tensor media"}
-_UNSUPPORTED_LISTS = {"pmon_names": "permittivity monitors",
-                      "projmon_names": "projection monitors", "bfmon_names": "box flux monitors"}
+_UNSUPPORTED_LISTS: dict = {}
 
 
 def _opt(z, key):
@@ -176,7 +175,7 @@ def save(sc: Scene, path) -> pathlib.Path:
             d[f"mon{i}_tbounds"] = np.array(m.t_bounds, dtype=np.float64)
     d["fmon_names"] = np.array([m.name for m in sc.field_monitors])
     for i, m in enumerate(sc.field_monitors):
-        d[f"fmon{i}_meta"] = np.array([*m.origin, *m.box, 1, 1, 1])
+        d[f"fmon{i}_meta"] = np.array([*m.origin, *m.box, *m.interval_space])
         d[f"fmon{i}_freqs"] = np.asarray(m.freqs, dtype=np.float64)
         if m.apodization is not None:
             d[f"fmon{i}_apod"] = _apod_pack(m.apodization)
@@ -198,8 +197,17 @@ def save(sc: Scene, path) -> pathlib.Path:
                                         m.step_end, m.interval, m.num_slots])
         d[f"ftmon{i}_frac"] = np.array(float(m.frac))
         d[f"ftmon{i}_tbounds"] = np.array(m.t_bounds, dtype=np.float64)
-    for key in _UNSUPPORTED_LISTS:
-        d[key] = np.array([], dtype="<U1")
+    d["pmon_names"] = np.array([m.name for m in sc.permittivity_monitors])
+    for i, m in enumerate(sc.permittivity_monitors):
+        d[f"pmon{i}_meta"] = np.array([*m.origin, *m.box])
+        d[f"pmon{i}_freqs"] = np.asarray(m.freqs, dtype=np.float64)
+    d["projmon_names"] = np.array([m.name for m in sc.projection_monitors])
+    for i, m in enumerate(sc.projection_monitors):
+        d[f"projmon{i}_surfaces"] = np.array(m.surface_names)
+        d[f"projmon{i}_dirs"] = np.array(m.normal_dirs)
+    d["bfmon_names"] = np.array([m.name for m in sc.box_flux_monitors])
+    for i, m in enumerate(sc.box_flux_monitors):
+        d[f"bfmon{i}_faces"] = np.array(m.face_names)
     d["n_dipoles"] = np.array(len(sc.dipoles))
     for i, dp in enumerate(sc.dipoles):
         d[f"dip{i}_component"] = np.array(int(dp.component))
@@ -311,12 +319,12 @@ def load(path) -> Scene:
         field_mons = []
         for i, name in enumerate(_names(z, "fmon_names")):
             m = [int(v) for v in z[f"fmon{i}_meta"]]
-            if len(m) > 6 and tuple(m[6:9]) != (1, 1, 1):
-                raise NotImplementedError(f"field monitor {name}: strided sampling is not modelled")
             ap = _apod_unpack(z[f"fmon{i}_apod"]) if f"fmon{i}_apod" in z.files else None
             field_mons.append(FieldMonitor(name=name, freqs=np.asarray(z[f"fmon{i}_freqs"]),
                                            origin=tuple(m[0:3]), box=tuple(m[3:6]),
-                                           apodization=ap))
+                                           apodization=ap,
+                                           interval_space=tuple(m[6:9]) if len(m) > 6
+                                           else (1, 1, 1)))
         mode_mons = []
         planes = _names(z, "modemon_planes")
         for i, name in enumerate(_names(z, "modemon_names")):
@@ -345,6 +353,15 @@ def load(path) -> Scene:
                 origin=tuple(int(v) for v in z[f"tmon{i}_origin"]),
                 box=tuple(int(v) for v in z[f"tmon{i}_box"]), step_begin=beg, step_end=end,
                 interval=iv, num_slots=slots))
+        perm_mons = [PermittivityMonitor(name=n, freqs=np.asarray(z[f"pmon{i}_freqs"]),
+                                         origin=tuple(int(v) for v in z[f"pmon{i}_meta"][:3]),
+                                         box=tuple(int(v) for v in z[f"pmon{i}_meta"][3:6]))
+                     for i, n in enumerate(_names(z, "pmon_names"))]
+        proj_mons = [ProjectionMonitor(name=n, surface_names=tuple(_names(z, f"projmon{i}_surfaces")),
+                                       normal_dirs=tuple(_names(z, f"projmon{i}_dirs")))
+                     for i, n in enumerate(_names(z, "projmon_names"))]
+        box_mons = [BoxFluxMonitor(name=n, face_names=tuple(_names(z, f"bfmon{i}_faces")))
+                    for i, n in enumerate(_names(z, "bfmon_names"))]
         flux_time_mons = []
         for i, name in enumerate(_names(z, "ftmon_names")):
             axis, plane, ndir, beg, end, iv, slots = (int(v) for v in z[f"ftmon{i}_meta"])
@@ -373,4 +390,6 @@ def load(path) -> Scene:
                      else (0.0, 0.0, 0.0)),
             sources=sources, tfsf_sources=tfsf, dipoles=dipoles, mode_sources=mode_sources,
             flux_monitors=flux_mons, field_monitors=field_mons, mode_monitors=mode_mons,
-            field_time_monitors=time_mons, flux_time_monitors=flux_time_mons)
+            field_time_monitors=time_mons, flux_time_monitors=flux_time_mons,
+            permittivity_monitors=perm_mons, projection_monitors=proj_mons,
+            box_flux_monitors=box_mons)

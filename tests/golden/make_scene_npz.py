@@ -77,14 +77,19 @@ def main() -> None:
              model.FluxMonitor(name="R", axis=1, plane_index=2, normal_dir=-1, freqs=freqs[:2],
                                source_spectrum=np.ones(2, complex))]
     fldm = [model.FieldMonitor(name="box", freqs=freqs, origin=(1, 0, 2), box=(3, 4, 5),
-                               apodization=None),
+                               apodization=None, interval_space=(1, 2, 1)),
             model.FieldMonitor(name="mplane", freqs=freqs[1:], origin=(0, 0, 5), box=(6, 5, 2))]
     sc = model.Scene(grid=grid, dt=dt, num_time_steps=n, shutoff=1e-5, eps_ex=eps[0],
                      eps_ey=eps[1], eps_ez=eps[2], pml=pml, sigma_ex=sig[0], sigma_ey=sig[1],
                      sigma_ez=sig[2], pec_ex=pec[0], pec_ey=pec[1], pec_ez=pec[2],
                      dispersion=disp, bloch_k=(0.0, 0.0, 0.0), sources=[src], dipoles=dips,
                      mode_sources=[msrc], flux_monitors=fmons, field_monitors=fldm,
-                     mode_monitors=[model.ModeMonitorSpec(name="mm", plane_name="mplane")])
+                     mode_monitors=[model.ModeMonitorSpec(name="mm", plane_name="mplane")],
+                     permittivity_monitors=[model.PermittivityMonitor(
+                         name="perm", freqs=freqs[:1], origin=(2, 1, 3), box=(3, 2, 4))],
+                     projection_monitors=[model.ProjectionMonitor(
+                         name="far", surface_names=("box", "mplane"), normal_dirs=("+", "-"))],
+                     box_flux_monitors=[model.BoxFluxMonitor(name="net", face_names=("T", "R"))])
     serialize.save(sc, OUT)
     print("wrote", OUT, OUT.stat().st_size, "bytes")
 

@@ -120,3 +120,39 @@ def test_refused_colocation_falls_back_to_the_yee_sum(monkeypatch, capsys):
     ok = FluxMonitor(name="m", axis=0, plane_index=2, normal_dir=1, freqs=np.ones(2),
                      t_bounds=((2e-8, 10e-8), (2e-8, 10e-8)))
     assert np.array_equal(flux.monitor_flux(data, g, ok), yee)
+
+
+def test_box_flux_sums_faces_along_outward_normals():
+    """A closed-surface monitor is its faces' fluxes, each signed by its outward normal: two
+    faces holding the same phasors with opposite normals cancel, flipping every normal negates
+    the total, and the total is linear in each face's flux."""
+    from openem_amd import normalize
+    from openem_amd.model import BoxFluxMonitor, Scene
+    from openem_amd.results import Phasors, Result
+    rng = np.random.default_rng(11)
+    g = Grid(*(uniform_axis(n, 1e-8, 0.0, "Periodic") for n in (6, 5, 7)))
+    freqs = np.array([2e14, 2.2e14])
+
+    def mon(name, k, nd):
+        return FluxMonitor(name=name, axis=2, plane_index=k, normal_dir=nd, freqs=freqs)
+
+    def data():
+        return rng.normal(size=(4, 2, 6, 5)) + 1j * rng.normal(size=(4, 2, 6, 5))
+
+    a, b = data(), data()
+    faces = [mon("lo", 2, -1), mon("hi", 5, 1)]
+    one = np.ones(g.shape)
+    sc = Scene(grid=g, dt=1e-17, num_time_steps=1, eps_ex=one, eps_ey=one, eps_ez=one,
+               flux_monitors=faces, box_flux_monitors=[BoxFluxMonitor("net", ("lo", "hi"))])
+
+    def res(da, db):
+        return Result(phasors={"lo": Phasors(data=da, freqs=freqs),
+                               "hi": Phasors(data=db, freqs=freqs)})
+    fa, fb = (flux.monitor_flux(d, g, faces[0]) for d in (a, b))
+    assert np.allclose(normalize.box_flux(sc, res(a, a), "net"), 0.0, atol=1e-30)
+    assert np.allclose(normalize.box_flux(sc, res(a, b), "net"), fb - fa, rtol=1e-13)
+    flipped = Scene(grid=g, dt=1e-17, num_time_steps=1, eps_ex=one, eps_ey=one, eps_ez=one,
+                    flux_monitors=[mon("lo", 2, 1), mon("hi", 5, -1)],
+                    box_flux_monitors=sc.box_flux_monitors)
+    assert np.allclose(normalize.box_flux(flipped, res(a, b), "net"), fa - fb, rtol=1e-13)
+    assert np.allclose(normalize.box_flux(sc, res(a, 3.0 * b), "net"), 9.0 * fb - fa, rtol=1e-13)

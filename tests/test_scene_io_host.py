@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 from openem_amd import scene_io, scenes, tables
-from openem_amd.model import FieldTimeMonitor, FluxTimeMonitor
+from openem_amd.model import (BoxFluxMonitor, FieldTimeMonitor, FluxTimeMonitor,
+                              PermittivityMonitor, ProjectionMonitor)
 from oracle.yee64 import Oracle64
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "scene_v10_small.npz"
@@ -71,6 +72,27 @@ def test_reference_written_file_decodes_key_for_key():
     (mm,) = sc.mode_monitors
     assert (mm.name, mm.axis, mm.plane_index) == ("mm", 2, plane.origin[2] + plane.box[2] - 1)
     assert np.array_equal(mm.freqs, plane.freqs)
+    assert box.interval_space == tuple(int(v) for v in z["fmon0_meta"][6:9]) == (1, 2, 1)
+    (pm,) = sc.permittivity_monitors
+    assert (pm.name, pm.origin + pm.box) == ("perm", tuple(int(v) for v in z["pmon0_meta"]))
+    assert np.array_equal(pm.freqs, z["pmon0_freqs"])
+    (far,) = sc.projection_monitors
+    assert (far.name, far.surface_names, far.normal_dirs) == ("far", ("box", "mplane"), ("+", "-"))
+    (net,) = sc.box_flux_monitors
+    assert (net.name, net.face_names) == ("net", ("T", "R"))
+
+
+def test_eps_box_slices_the_yee_arrays():
+    """``Scene.eps_box`` is the three permittivity arrays cut to the monitor's index box."""
+    z = _raw()
+    sc = scene_io.load(GOLDEN)
+    (pm,) = sc.permittivity_monitors
+    exx, eyy, ezz = sc.eps_box(pm)
+    assert exx.shape == eyy.shape == ezz.shape == pm.box
+    (i0, j0, k0) = pm.origin
+    for got, key in zip((exx, eyy, ezz), ("eps_ex", "eps_ey", "eps_ez")):
+        for i, j, k in np.ndindex(*pm.box):
+            assert got[i, j, k] == z[key][i0 + i, j0 + j, k0 + k]
 
 
 def test_save_writes_the_reference_layout():
@@ -128,6 +150,17 @@ def _with_time_monitors(sc):
         FluxTimeMonitor("f", 2, 20, -1, 1, 28, 2, 12, t_bounds=span, frac=0.25)])
 
 
+def _with_passive_monitors(sc):
+    """Monitors the stepper never reads: they must survive the file and leave the step alone."""
+    fm = [replace(m, interval_space=(2, 1, 3)) for m in sc.field_monitors]
+    names = [m.name for m in sc.flux_monitors]
+    return replace(sc, field_monitors=fm, permittivity_monitors=[
+        PermittivityMonitor("p", np.array([2e14]), (1, 1, 2), (3, 2, 4))],
+        box_flux_monitors=[BoxFluxMonitor("b", tuple(names))],
+        projection_monitors=[ProjectionMonitor("far", tuple(m.name for m in fm),
+                                               tuple("+-"[i % 2] for i in range(len(fm))))])
+
+
 SCENES = {
     "mini_all": lambda: scenes.mini_scene(num_steps=30),
     "mini_pml_sigma_pec_hdip": lambda: scenes.mini_scene(num_steps=30, pml_xy=True, sigma_on=True,
@@ -140,6 +173,7 @@ SCENES = {
     "tfsf_box": lambda: scenes.tfsf_box_scene(num_steps=30, scatterer=True),
     "bloch": lambda: scenes.bloch_cell_scene(nxy=8, nz=40, n_pml=6, num_steps=30, nf=2),
     "time_monitors": lambda: _with_time_monitors(scenes.mini_scene(num_steps=30)),
+    "passive_monitors": lambda: _with_passive_monitors(scenes.mini_scene(num_steps=30)),
 }
 
 
@@ -159,7 +193,6 @@ def test_round_trip_keeps_every_field_and_the_step(tmp_path, name):
     ({"tensor_comp": np.array([0])}, "tensor"),
     ({"n_absorbers": np.array(1)}, "absorber"),
     ({"symmetry": np.array([1, 0, 0])}, "symmetry"),
-    ({"pmon_names": np.array(["t"])}, "permittivity"),
     ({"flat_y": np.array(True)}, "flat"),
     ({"msrc0_bb_ey": np.zeros(2)}, "broadband"),
 ])
@@ -178,3 +211,22 @@ def test_unknown_format_version_is_refused(tmp_path):
     np.savez(tmp_path / "v.npz", **z)
     with pytest.raises(ValueError, match="format version 11"):
         scene_io.load(tmp_path / "v.npz")
+
+
+def test_interval_space_keeps_every_kth_point_and_the_last():
+    """The stride rule's two worked cases in the reference (``openem/nb/backend.py:1946-1949``):
+    467 points at stride 3 keep 0, 3, ..., 465 and 466 (157 points); 241 points keep
+    0, 3, ..., 240 (81 points, the last already on the stride)."""
+    from openem_amd.results import Result, downsample_indices, strided_field_phasors
+    a = downsample_indices(467, 3)
+    assert a.size == 157 and a[:3].tolist() == [0, 3, 6] and a[-2:].tolist() == [465, 466]
+    b = downsample_indices(241, 3)
+    assert b.size == 81 and b[-2:].tolist() == [237, 240]
+    assert downsample_indices(5, 1).tolist() == [0, 1, 2, 3, 4]
+    assert downsample_indices(1, 4).tolist() == [0]
+    sc = scene_io.load(GOLDEN)
+    box = sc.field_monitors[0]
+    data = np.arange(6 * 3 * 3 * 4 * 5).reshape(6, 3, 3, 4, 5)
+    got = strided_field_phasors(Result(field_phasors={box.name: data}), box)
+    assert got.shape == (6, 3, 3, 3, 5)
+    assert np.array_equal(got, data[:, :, :, [0, 2, 3], :])
