@@ -492,182 +492,7 @@ k_update_e(F *__restrict__ E, const F *__restrict__ H, PsiDev ps, GridDev g, Mat
     update_e_tile<F>(E, H, ps, g, mat, cw, pw, stp, tc, s_lut, lds_lut);
 }
 
-// ------------------------------------------------------------------------------------------
-// Fused H + E step: one x march computes H^{n+1/2} and then E^{n+1} of every plane, so each
-// field is read once and written once per step (about 50 B per cell instead of 74 for the two
-// sweeps).  Old and new fields live in separate buffers (ping-pong between steps): every value
-// a wave reads is an old one, so waves never race and need no barrier.
-//
-// Each wave works alone on a strip of OEM_WR rows (j) x 63 cells (k) and marches along x.
-// Lane 0 is a halo lane at k0 - 1 (Z.prv[k0]); lanes 1..63 own k0 .. k0 + 62.  In each plane
-// the wave first computes the new H of the halo row Y.prv[j0] (not stored), then row by row
-// the new H (stored) and at once the new E of the same row, which needs the new H of
-//   j - 1: the previous row, carried in registers,
-//   k - 1: the lane to the left (a cross-lane shift),
-//   i - 1: the same row one plane back, carried in registers (a prologue computes it for the
-//          first plane of the chunk).
-// Halo values are recomputed from the same old values with the same arithmetic, so they are
-// bit-identical to what the owning wave stores.  Valid when no point correction acts on H
-// between the sweeps (the host checks); the plane-wave corrections are applied in the march as
-// in the separate sweeps.
-// ------------------------------------------------------------------------------------------
-#define OEM_WK 63           // owned k per wave (lane 0 is the k - 1 halo)
-#define OEM_WR 8            // owned rows per wave
-#define OEM_WW 4            // independent waves per work-group
-
-// One descriptor per field set (three components back to back, the component picked by the
-// scalar offset: `cb` bytes per component) keeps the march's SGPRs below the spill point; the
-// host limits the fused step to sets under 4 GiB.
-struct FusedIO {
-    rsrc_t eo, ho, en, hn;                  // old / new E and H
-    rsrc_t ph_o, ph_n, pe_o, pe_n;          // psi: H side old / new, E side old / new
-    uint32_t cb;                            // bytes per component
-};
-
-// Everything the new H at one cell reads: old E at the cell, at X.nxt[i] (ey, ez), at Y.nxt[j]
-// (ex, ez) and Z.nxt[k] (ex, ey), old H, and the H-side psi of the slabs holding the cell.
-template <typename F>
-struct HIn {
-    F ex, ey, ez, eyn, ezn, ex_j, ez_j, ex_k, ey_k, hx, hy, hz;
-    F px0, px1, py0, py1, pz0, pz1;
-};
-
-// Everything one row task reads: the H inputs and, for owned rows, the class index, the E-side
-// psi and the ADE history.
-template <typename F>
-struct RowIn {
-    HIn<F> v;
-    uint32_t cls;
-    F pe[6], ph[3];
-};
-
-// Offsets of the psi elements of cell (i, j, k) in the x / y / z slabs (ps.off order), and the
-// slab indices (-1 outside).
-struct PsiAt {
-    int sx, sy, sz;
-    uint32_t ox, oy, oz;        // per-lane byte offsets
-    uint32_t sx0, sx1, sy0, sy1, sz0, sz1;  // per-plane byte offsets
-};
-
-// Per-lane values of the H update at k, loaded once per kernel: inside the march the compiler
-// must assume the buffer stores may alias the axis tables and would reload them every row.
-struct KConst {
-    int k, kn, sz;              // cell, Z.nxt[k], z slab index (-1 outside)
-    float mk, mki, idz;         // forward mask (re, im) and 1/dl
-    float a, b, ik;             // H-side CPML of the z slab
-};
-
-__device__ __forceinline__ KConst k_const(const GridDev &g, int k) {
-    const AxisDev &Z = g.ax[2];
-    KConst c;
-    c.k = k;
-    c.kn = Z.nxt[k];
-    c.sz = slab_of(k, g.nz, Z.lo, Z.hi);
-    c.mk = axv(Z, k, AX_MNX);
-    c.mki = axv(Z, k, AX_MNXI);
-    c.idz = axv(Z, k, AX_IDLP);
-    c.a = 0.f; c.b = 1.f; c.ik = 1.f;
-    if (c.sz >= 0) { c.a = axv(Z, k, AX_AH); c.b = axv(Z, k, AX_BH); c.ik = axv(Z, k, AX_KH); }
-    return c;
-}
-
-
-template <typename F>
-__device__ __forceinline__ PsiAt psi_at(const PsiDev &ps, const GridDev &g, int i, int j,
-                                        const KConst &kc) {
-    typedef FT<F> T;
-    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
-    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
-    const int k = kc.k;
-    PsiAt q;
-    q.sx = slab_of(i, g.nx, X.lo, X.hi);
-    q.sy = slab_of(j, g.ny, Y.lo, Y.hi);
-    q.sz = kc.sz;
-    q.ox = (uint32_t)(j * g.nz + k) * T::B;
-    q.oy = (uint32_t)(max(q.sy, 0) * g.nz + k) * T::B;
-    q.oz = (uint32_t)(j * wz + max(q.sz, 0)) * T::B;
-    q.sx0 = (ps.off[0] + (uint32_t)max(q.sx, 0) * (uint32_t)g.plane) * T::B;
-    q.sx1 = (ps.off[1] + (uint32_t)max(q.sx, 0) * (uint32_t)g.plane) * T::B;
-    q.sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
-    q.sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
-    q.sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
-    q.sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
-    return q;
-}
-
-// Issue every load of the new H at (i, j, k); `in_` / `jn` are the x / y neighbours.
-template <typename F>
-__device__ __forceinline__ void fused_h_ld(const FusedIO &io, const GridDev &g, const PsiAt &q,
-                                           int i, int in_, int j, int jn, const KConst &kc,
-                                           HIn<F> &v) {
-    typedef FT<F> T;
-    const int nz = g.nz, k = kc.k;
-    const uint32_t planeB = (uint32_t)g.plane * T::B;
-    const uint32_t so = (uint32_t)i * planeB, sn = (uint32_t)in_ * planeB;
-    const uint32_t op = (uint32_t)(j * nz + k) * T::B;
-    const uint32_t oj = (uint32_t)(jn * nz + k) * T::B;
-    const uint32_t ok = (uint32_t)(j * nz + kc.kn) * T::B;
-    const uint32_t c1 = io.cb, c2 = 2u * io.cb;
-    v.ex = T::ld(io.eo, op + so, 0u); v.ey = T::ld(io.eo, op + so + c1, 0u); v.ez = T::ld(io.eo, op + so + c2, 0u);
-    v.eyn = T::ld(io.eo, op + sn + c1, 0u); v.ezn = T::ld(io.eo, op + sn + c2, 0u);
-    v.ex_j = T::ld(io.eo, oj + so, 0u); v.ez_j = T::ld(io.eo, oj + so + c2, 0u);
-    v.ex_k = T::ld(io.eo, ok + so, 0u); v.ey_k = T::ld(io.eo, ok + so + c1, 0u);
-    v.hx = T::ld(io.ho, op + so, 0u); v.hy = T::ld(io.ho, op + so + c1, 0u); v.hz = T::ld(io.ho, op + so + c2, 0u);
-    v.px0 = v.px1 = v.py0 = v.py1 = v.pz0 = v.pz1 = T::zero();
-    if (q.sx >= 0) { v.px0 = T::ld(io.ph_o, q.ox + q.sx0, 0u); v.px1 = T::ld(io.ph_o, q.ox + q.sx1, 0u); }
-    if (q.sy >= 0) { v.py0 = T::ld(io.ph_o, q.oy + q.sy0, 0u); v.py1 = T::ld(io.ph_o, q.oy + q.sy1, 0u); }
-    if (q.sz >= 0) { v.pz0 = T::ld(io.ph_o, q.oz + q.sz0, 0u); v.pz1 = T::ld(io.ph_o, q.oz + q.sz1, 0u); }
-}
-
-// H^{n+1/2} at (i, j, k) from loaded inputs (the arithmetic of update_h_tile); with `store` the
-// new H and H-side psi are written.
-template <typename F>
-__device__ __forceinline__ void fused_h_do(const FusedIO &io, const GridDev &g, const PsiAt &q,
-                                           int i, int j, const KConst &kc, HIn<F> &v, float ch,
-                                           float inj_hx, float inj_hy, bool store,
-                                           F &hx, F &hy, F &hz) {
-    typedef FT<F> T;
-    const AxisDev &X = g.ax[0], &Y = g.ax[1];
-    const uint32_t so = (uint32_t)i * ((uint32_t)g.plane * T::B);
-    const float mj = axv(Y, j, AX_MNX), mk = kc.mk;
-    const float mji = axv(Y, j, AX_MNXI), mki = kc.mki;
-    const float idy = axv(Y, j, AX_IDLP), idz = kc.idz;
-    const float mi = axv(X, i, AX_MNX), mii = axv(X, i, AX_MNXI), idx = axv(X, i, AX_IDLP);
-    const F dzy = (T::mmul(v.ez_j, mj, mji) - v.ez) * idy;
-    const F dyz = (T::mmul(v.ey_k, mk, mki) - v.ey) * idz;
-    const F dxz = (T::mmul(v.ex_k, mk, mki) - v.ex) * idz;
-    const F dzx = (T::mmul(v.ezn, mi, mii) - v.ez) * idx;
-    const F dyx = (T::mmul(v.eyn, mi, mii) - v.ey) * idx;
-    const F dxy = (T::mmul(v.ex_j, mj, mji) - v.ex) * idy;
-    F s_hx_y = dzy, s_hz_y = dxy, s_hx_z = dyz, s_hy_z = dxz, s_hy_x = dzx, s_hz_x = dyx;
-    if (q.sx >= 0) {
-        const float a_ = axv(X, i, AX_AH), b_ = axv(X, i, AX_BH), k_ = axv(X, i, AX_KH);
-        s_hy_x = stretch_v(v.px0, dzx, a_, b_, k_);
-        s_hz_x = stretch_v(v.px1, dyx, a_, b_, k_);
-        if (store) { T::st(io.ph_n, v.px0, q.ox + q.sx0, 0u); T::st(io.ph_n, v.px1, q.ox + q.sx1, 0u); }
-    }
-    if (q.sy >= 0) {
-        const float a_ = axv(Y, j, AX_AH), b_ = axv(Y, j, AX_BH), k_ = axv(Y, j, AX_KH);
-        s_hx_y = stretch_v(v.py0, dzy, a_, b_, k_);
-        s_hz_y = stretch_v(v.py1, dxy, a_, b_, k_);
-        if (store) { T::st(io.ph_n, v.py0, q.oy + q.sy0, 0u); T::st(io.ph_n, v.py1, q.oy + q.sy1, 0u); }
-    }
-    if (q.sz >= 0) {
-        s_hx_z = stretch_v(v.pz0, dyz, kc.a, kc.b, kc.ik);
-        s_hy_z = stretch_v(v.pz1, dxz, kc.a, kc.b, kc.ik);
-        if (store) { T::st(io.ph_n, v.pz0, q.oz + q.sz0, 0u); T::st(io.ph_n, v.pz1, q.oz + q.sz1, 0u); }
-    }
-    hx = T::addre(v.hx - ch * (s_hx_y - s_hx_z), inj_hx);
-    hy = T::addre(v.hy - ch * (s_hy_z - s_hy_x), inj_hy);
-    hz = v.hz - ch * (s_hz_x - s_hz_y);
-    if (store) {
-        T::st(io.hn, hx, q.ox + so, 0u);
-        T::st(io.hn, hy, q.ox + so + io.cb, 0u);
-        T::st(io.hn, hz, q.ox + so + 2u * io.cb, 0u);
-    }
-}
-
-// Plane-wave H-side correction at cell k (the fused update_h's injection term).
+// Plane-wave H-side correction at cell k (the injection term of update_h).
 __device__ __forceinline__ void pw_h_inj(const PlaneSrcDev &pw, int k, const int *stp,
                                          float &inj_hx, float &inj_hy) {
     inj_hx = 0.f;
@@ -681,95 +506,267 @@ __device__ __forceinline__ void pw_h_inj(const PlaneSrcDev &pw, int k, const int
         }
 }
 
-// Value of lane - 1 (lane 0 keeps its own); every lane of the wave must be active.
-__device__ __forceinline__ float lane_prev(float x) {
-    const int src = ((int)threadIdx.x - 1) & 63;
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
-}
-template <typename F> __device__ __forceinline__ F lane_prev_f(F x);
-template <> __device__ __forceinline__ float lane_prev_f<float>(float x) { return lane_prev(x); }
-template <> __device__ __forceinline__ cf2 lane_prev_f<cf2>(cf2 x) {
-    return {lane_prev(x.r), lane_prev(x.i)};
+// ------------------------------------------------------------------------------------------
+// Fused H + E step.  A work-group owns a tile of OEM_CR rows (j) x 64 cells (k)
+// and marches it along x; every thread of the OEM_CR row waves keeps one (j, k) column, as in
+// the separate sweeps, and computes H^{n+1/2} and then E^{n+1} of each plane:
+//   * the old Ey, Ez of the plane and the new Hy, Hz of the plane before are carried in
+//     registers (the x neighbours of H and of E);
+//   * the new H of the row above (j - 1) and of the lane to the left (k - 1) come from LDS, one
+//     barrier per plane, double-buffered by plane parity;
+//   * one more wave recomputes, from the same old values with the same arithmetic, the new H of
+//     the halo row Y.prv[j0] and of the halo column Z.prv[k0] (not stored): the values are
+//     bit-identical to what the owning work-groups store.
+// Old and new fields are separate buffers (ping-pong between steps), so no work-group reads a
+// value another one writes.  Each field is read once and written once per step.
+// ------------------------------------------------------------------------------------------
+#ifndef OEM_CR
+#define OEM_CR 7            // owned rows (waves) per work-group; one halo wave more
+#endif
+
+// Buffer descriptors of the fused step: a field set is E then H in one allocation, a psi set
+// the H side then the E side; one descriptor each keeps the SGPRs the march needs low.
+struct ColIO {
+    rsrc_t fo, fn;              // old / new field set
+    rsrc_t po, pn;              // old / new psi set
+    uint32_t cb;                // bytes per field component
+    uint32_t hof;               // bytes from E to H in a field set (3 * cb)
+    uint32_t pef;               // bytes from the H-side to the E-side psi in a psi set
+};
+
+// The H update at one (j, k) column: its per-column constants and the old Ey, Ez of the
+// current plane.
+template <typename F>
+struct HColumn {
+    uint32_t op, oj, ok;        // byte offsets in the plane: (j, k), (Y.nxt[j], k), (j, Z.nxt[k])
+    uint32_t oy, oz;            // psi offsets in the y / z slab planes
+    int sy, sz;                 // slab indices (-1 outside)
+    float mj, mji, idy, mk, mki, idz;
+    float ay, by, ky, az, bz, kz;
+    float ihx, ihy;             // plane-wave H corrections at k
+    F ey, ez;
+};
+
+template <typename F>
+__device__ __forceinline__ void hcolumn_init(HColumn<F> &c, const GridDev &g,
+                                             const PlaneSrcDev &pw, const int *stp, int j, int k,
+                                             int jn, int kn) {
+    typedef FT<F> T;
+    const AxisDev &Y = g.ax[1], &Z = g.ax[2];
+    const int nz = g.nz, wz = Z.lo + Z.hi;
+    c.op = (uint32_t)(j * nz + k) * T::B;
+    c.oj = (uint32_t)(jn * nz + k) * T::B;
+    c.ok = (uint32_t)(j * nz + kn) * T::B;
+    c.sy = slab_of(j, g.ny, Y.lo, Y.hi);
+    c.sz = slab_of(k, nz, Z.lo, Z.hi);
+    c.oy = (uint32_t)(max(c.sy, 0) * nz + k) * T::B;
+    c.oz = (uint32_t)(j * wz + max(c.sz, 0)) * T::B;
+    c.mj = axv(Y, j, AX_MNX); c.mji = axv(Y, j, AX_MNXI); c.idy = axv(Y, j, AX_IDLP);
+    c.mk = axv(Z, k, AX_MNX); c.mki = axv(Z, k, AX_MNXI); c.idz = axv(Z, k, AX_IDLP);
+    c.ay = 0.f; c.by = 1.f; c.ky = 1.f; c.az = 0.f; c.bz = 1.f; c.kz = 1.f;
+    if (c.sy >= 0) { c.ay = axv(Y, j, AX_AH); c.by = axv(Y, j, AX_BH); c.ky = axv(Y, j, AX_KH); }
+    if (c.sz >= 0) { c.az = axv(Z, k, AX_AH); c.bz = axv(Z, k, AX_BH); c.kz = axv(Z, k, AX_KH); }
+    pw_h_inj(pw, k, stp, c.ihx, c.ihy);
+    c.ey = c.ez = T::zero();
 }
 
-#ifdef OEM_HE_WAVES_PER_EU
-#define OEM_HE_ATTR __attribute__((amdgpu_waves_per_eu(OEM_HE_WAVES_PER_EU)))
-#else
-#define OEM_HE_ATTR
-#endif
+// Old Ey, Ez of plane i into the column.
 template <typename F>
-__global__ void __launch_bounds__(64 * OEM_WW) OEM_HE_ATTR
+__device__ __forceinline__ void hcolumn_load_e(HColumn<F> &c, const ColIO &io, const GridDev &g,
+                                               int i) {
+    typedef FT<F> T;
+    const uint32_t so = (uint32_t)i * ((uint32_t)g.plane * T::B);
+    c.ey = T::ld(io.fo, c.op, so + io.cb);
+    c.ez = T::ld(io.fo, c.op, so + 2u * io.cb);
+}
+
+// The old fields the H update of a column at plane i reads: E at the cell, at (j + 1), (k + 1)
+// and X.nxt[i] (Ey, Ez), and H at the cell.
+template <typename F>
+struct HLoads {
+    F ex, ex_j, ex_k, ey_k, ez_j, ey_n, ez_n, h0, h1, h2;
+};
+
+template <typename F>
+__device__ __forceinline__ void hcolumn_load(HLoads<F> &v, const HColumn<F> &c, const ColIO &io,
+                                             const GridDev &g, int i, int in_) {
+    typedef FT<F> T;
+    const uint32_t planeB = (uint32_t)g.plane * T::B;
+    const uint32_t so = (uint32_t)i * planeB, sn = (uint32_t)in_ * planeB, sh = so + io.hof;
+    const uint32_t c1 = io.cb, c2 = 2u * io.cb;
+    v.ex = T::ld(io.fo, c.op, so);
+    v.ex_j = T::ld(io.fo, c.oj, so);
+    v.ex_k = T::ld(io.fo, c.ok, so);
+    v.ey_k = T::ld(io.fo, c.ok, so + c1);
+    v.ez_j = T::ld(io.fo, c.oj, so + c2);
+    v.ey_n = T::ld(io.fo, c.op, sn + c1);
+    v.ez_n = T::ld(io.fo, c.op, sn + c2);
+    v.h0 = T::ld(io.fo, c.op, sh);
+    v.h1 = T::ld(io.fo, c.op, sh + c1);
+    v.h2 = T::ld(io.fo, c.op, sh + c2);
+}
+
+// H^{n+1/2} of the column at plane i from its loaded inputs (the arithmetic of update_h_tile);
+// `xr` is the x-axis table row of plane i.  The column's Ey, Ez move on to plane X.nxt[i].  With
+// `store` the new H and H-side psi are written.
+template <typename F>
+__device__ __forceinline__ void hcolumn_do(HColumn<F> &c, const HLoads<F> &v, const ColIO &io,
+                                           const GridDev &g, const PsiDev &ps, int i,
+                                           const float *xr, float ch, bool store, F &hx, F &hy,
+                                           F &hz) {
+    typedef FT<F> T;
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const uint32_t so = (uint32_t)i * ((uint32_t)g.plane * T::B), sh = so + io.hof;
+    const uint32_t c1 = io.cb, c2 = 2u * io.cb;
+    const int sx = slab_of(i, g.nx, X.lo, X.hi);
+    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+    const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
+    const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
+    const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
+    const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
+    const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
+    const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
+    F psx0 = T::zero(), psx1 = T::zero(), psy0 = T::zero(), psy1 = T::zero(),
+      psz0 = T::zero(), psz1 = T::zero();
+    if (sx >= 0) { psx0 = T::ld(io.po, c.op, sx0); psx1 = T::ld(io.po, c.op, sx1); }
+    if (c.sy >= 0) { psy0 = T::ld(io.po, c.oy, sy0); psy1 = T::ld(io.po, c.oy, sy1); }
+    if (c.sz >= 0) { psz0 = T::ld(io.po, c.oz, sz0); psz1 = T::ld(io.po, c.oz, sz1); }
+
+    const float mi = xr[AX_MNX], mii = xr[AX_MNXI], idx = xr[AX_IDLP];
+    const F dzy = (T::mmul(v.ez_j, c.mj, c.mji) - c.ez) * c.idy;
+    const F dyz = (T::mmul(v.ey_k, c.mk, c.mki) - c.ey) * c.idz;
+    const F dxz = (T::mmul(v.ex_k, c.mk, c.mki) - v.ex) * c.idz;
+    const F dzx = (T::mmul(v.ez_n, mi, mii) - c.ez) * idx;
+    const F dyx = (T::mmul(v.ey_n, mi, mii) - c.ey) * idx;
+    const F dxy = (T::mmul(v.ex_j, c.mj, c.mji) - v.ex) * c.idy;
+    F s_hx_y = dzy, s_hz_y = dxy, s_hx_z = dyz, s_hy_z = dxz, s_hy_x = dzx, s_hz_x = dyx;
+    if (sx >= 0) {
+        const float a_ = xr[AX_AH], b_ = xr[AX_BH], k_ = xr[AX_KH];
+        s_hy_x = stretch_v(psx0, dzx, a_, b_, k_);
+        s_hz_x = stretch_v(psx1, dyx, a_, b_, k_);
+        if (store) { T::st(io.pn, psx0, c.op, sx0); T::st(io.pn, psx1, c.op, sx1); }
+    }
+    if (c.sy >= 0) {
+        s_hx_y = stretch_v(psy0, dzy, c.ay, c.by, c.ky);
+        s_hz_y = stretch_v(psy1, dxy, c.ay, c.by, c.ky);
+        if (store) { T::st(io.pn, psy0, c.oy, sy0); T::st(io.pn, psy1, c.oy, sy1); }
+    }
+    if (c.sz >= 0) {
+        s_hx_z = stretch_v(psz0, dyz, c.az, c.bz, c.kz);
+        s_hy_z = stretch_v(psz1, dxz, c.az, c.bz, c.kz);
+        if (store) { T::st(io.pn, psz0, c.oz, sz0); T::st(io.pn, psz1, c.oz, sz1); }
+    }
+    hx = T::addre(v.h0 - ch * (s_hx_y - s_hx_z), c.ihx);
+    hy = T::addre(v.h1 - ch * (s_hy_z - s_hy_x), c.ihy);
+    hz = v.h2 - ch * (s_hz_x - s_hz_y);
+    if (store) {
+        T::st(io.fn, hx, c.op, sh);
+        T::st(io.fn, hy, c.op, sh + c1);
+        T::st(io.fn, hz, c.op, sh + c2);
+    }
+    c.ey = v.ey_n;
+    c.ez = v.ez_n;
+}
+
+template <typename F>
+__global__ void __launch_bounds__(64 * (OEM_CR + 1))
 k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ En,
-            F *__restrict__ Hn, PsiDev ps, PsiDev pn, GridDev g, MatDev mat, float ch, float cw,
-            PlaneSrcDev pw, const int *__restrict__ stp)
+                F *__restrict__ Hn, PsiDev ps, PsiDev pn, GridDev g, MatDev mat, float ch,
+                float cw, PlaneSrcDev pw, const int *__restrict__ stp)
 {
     typedef FT<F> T;
-    FusedIO io;
+    // Ho = Eo + 3 components and ps.e = ps.h + ps.total (the host allocates them so)
+    ColIO io;
     {
         const size_t nc = (size_t)g.plane * g.nx;
-        const uint32_t pb = ps.total * T::B;
         io.cb = (uint32_t)(nc * T::B);
-        const uint32_t sb = 3u * io.cb;
-        io.eo = mk_rsrc(Eo, sb); io.ho = mk_rsrc(Ho, sb);
-        io.en = mk_rsrc(En, sb); io.hn = mk_rsrc(Hn, sb);
-        io.ph_o = mk_rsrc(ps.h, pb); io.ph_n = mk_rsrc(pn.h, pb);
-        io.pe_o = mk_rsrc(ps.e, pb); io.pe_n = mk_rsrc(pn.e, pb);
+        io.hof = 3u * io.cb;
+        io.pef = ps.total * T::B;
+        io.fo = mk_rsrc(Eo, 2u * io.hof); io.fn = mk_rsrc(En, 2u * io.hof);
+        io.po = mk_rsrc(ps.h, 2u * io.pef); io.pn = mk_rsrc(pn.h, 2u * io.pef);
     }
     __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
+    // new H per row slot (0: the halo row, 1 + w: row wave w) and of the halo column, by
+    // plane parity
+    __shared__ F s_row[2][3][OEM_CR + 1][64];
+    __shared__ F s_col[2][2][OEM_CR];
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int nkt = (nz + 63) / 64;
+    // per-work-group quantities through readfirstlane: the compiler then keeps them in SGPRs
+    const int lb = xcd_remap(blockIdx.x, g.nblocks);
+    const int tile = __builtin_amdgcn_readfirstlane(lb % g.tiles);
+    const int chunk = __builtin_amdgcn_readfirstlane(lb / g.tiles);
+    const int j0 = __builtin_amdgcn_readfirstlane((tile / nkt) * OEM_CR);
+    const int k0 = __builtin_amdgcn_readfirstlane((tile % nkt) * 64);
+    const int i0 = chunk * g.chunk, i1 = min(i0 + g.chunk, nx);
     const bool lds_lut = lut_in_lds(mat);
     if (lds_lut) {
         const int tid = threadIdx.y * 64 + threadIdx.x;
         const float4 *src = reinterpret_cast<const float4 *>(mat.lut);
-        for (int q = tid; q < 2 * mat.n_classes; q += 64 * OEM_WW) s_lut[q] = src[q];
+        for (int q = tid; q < 2 * mat.n_classes; q += 64 * (OEM_CR + 1)) s_lut[q] = src[q];
         __syncthreads();
     }
-    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    // work item of this wave: (chunk, row group, k strip)
-    const int nks = (nz + OEM_WK - 1) / OEM_WK;
-    const int n_items = g.tiles * ((nx + g.chunk - 1) / g.chunk);
-    // the wave index through readfirstlane: the compiler then knows every per-wave quantity
-    // (item, plane i, rows j) is uniform, keeps it in SGPRs, gives the buffer ops scalar
-    // offsets and reads the axis tables with scalar loads
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    const int item = xcd_remap(blockIdx.x, g.nblocks) * OEM_WW + wave;
-    if (item >= n_items) return;
-    // the divisions run on the vector ALU: their wave-uniform results go back to SGPRs
-    const int tile = __builtin_amdgcn_readfirstlane(item % g.tiles);
-    const int chunk = __builtin_amdgcn_readfirstlane(item / g.tiles);
-    const int j0 = __builtin_amdgcn_readfirstlane((tile / nks) * OEM_WR);
-    const int k0 = __builtin_amdgcn_readfirstlane((tile % nks) * OEM_WK);
-    const int i0 = chunk * g.chunk, i1 = min(i0 + g.chunk, nx);
-    const int nr = min(OEM_WR, ny - j0);
-    // neighbour indices read from the tables are wave-uniform: say so, or every buffer access
-    // offset by them becomes a per-lane loop
-    const int jh = __builtin_amdgcn_readfirstlane(Y.prv[j0]);
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lane = threadIdx.x;
-    const bool act = lane == 0 || k0 + lane - 1 < nz;       // computes H
-    const bool own = lane > 0 && k0 + lane - 1 < nz;        // stores H and E
-    const int k = lane == 0 ? Z.prv[k0] : (act ? k0 + lane - 1 : 0);
-    const KConst kc = k_const(g, k);
     const int xn_last = __builtin_amdgcn_readfirstlane(X.nxt[nx - 1]);
     const int yn_last = __builtin_amdgcn_readfirstlane(Y.nxt[ny - 1]);
-    const int ip = __builtin_amdgcn_readfirstlane(X.prv[i0]);
+    const int zn_last = __builtin_amdgcn_readfirstlane(Z.nxt[nz - 1]);
     const uint32_t planeB = (uint32_t)g.plane * T::B, plane2 = (uint32_t)g.plane * 2u;
     const size_t ncell = (size_t)g.plane * nx;
 
-    // E-side per-k quantities (owners)
-    const float mkE = own ? axv(Z, k, AX_MPV) : 0.f, mkEi = own ? axv(Z, k, AX_MPVI) : 0.f;
-    const float idzE = own ? axv(Z, k, AX_IDLD) : 0.f, pecz = own ? axv(Z, k, AX_PEC) : 0.f;
-    float az = 0.f, bz = 1.f, kz = 1.f;
-    if (own && slab_of(k, nz, Z.lo, Z.hi) >= 0) {
-        az = axv(Z, k, AX_AE); bz = axv(Z, k, AX_BE); kz = axv(Z, k, AX_KE);
+    // Threads outside the grid work on a clamped in-range column and store nothing.
+    const int kl = min(k0 + lane, nz - 1);
+    const int xp0 = __builtin_amdgcn_readfirstlane(X.prv[i0]);
+    if (w == OEM_CR) {
+        // the halo wave: the new H of (Y.prv[j0], k0 + lane) and of (j0 + lane, Z.prv[k0]) for
+        // lane < OEM_CR, into LDS slot 0 and the column slots; one barrier per plane, as the
+        // row waves
+        const int jh = __builtin_amdgcn_readfirstlane(Y.prv[j0]);
+        const int kh = __builtin_amdgcn_readfirstlane(Z.prv[k0]);
+        const int jc = min(j0 + min(lane, OEM_CR - 1), ny - 1);
+        HColumn<F> cr, cc;
+        hcolumn_init<F>(cr, g, pw, stp, jh, kl, next_of(jh, ny, yn_last), next_of(kl, nz, zn_last));
+        hcolumn_init<F>(cc, g, pw, stp, jc, kh, next_of(jc, ny, yn_last), next_of(kh, nz, zn_last));
+        hcolumn_load_e<F>(cr, io, g, i0);
+        hcolumn_load_e<F>(cc, io, g, i0);
+#pragma unroll 1
+        for (int i = i0; i < i1; ++i) {
+            const int buf = (i - i0) & 1;
+            const int in_ = next_of(i, nx, xn_last);
+            F hx, hy, hz;
+            HLoads<F> vr, vc;
+            hcolumn_load<F>(vr, cr, io, g, i, in_);
+            hcolumn_load<F>(vc, cc, io, g, i, in_);
+            const float *xr = X.tab + (size_t)i * AX_W;
+            hcolumn_do<F>(cr, vr, io, g, ps, i, xr, ch, false, hx, hy, hz);
+            s_row[buf][0][0][lane] = hx;
+            s_row[buf][2][0][lane] = hz;
+            hcolumn_do<F>(cc, vc, io, g, ps, i, xr, ch, false, hx, hy, hz);
+            if (lane < OEM_CR) {
+                s_col[buf][0][lane] = hx;
+                s_col[buf][1][lane] = hy;
+            }
+            __syncthreads();
+        }
+        return;
     }
-    const rsrc_t rC = mk_rsrc(mat.cls, (uint32_t)(ncell * 2));
-    const int hbx = mat.hb[1] - mat.hb[0], hby = mat.hb[3] - mat.hb[2], hbz = mat.hb[5] - mat.hb[4];
-    const uint32_t hcomp = (uint32_t)(hbx * hby * hbz);
-    const rsrc_t rHist = mk_rsrc(mat.hist, 3u * hcomp * T::B);
-    const bool hcol = mat.hist && own && k >= mat.hb[4] && k < mat.hb[5];
+
+    // a row wave: column (j0 + w, k0 + lane)
+    const int j = min(j0 + w, ny - 1), k = kl;
+    const bool own = j0 + w < ny && k0 + lane < nz;
+    HColumn<F> ca;
+    hcolumn_init<F>(ca, g, pw, stp, j, k, next_of(j, ny, yn_last), next_of(k, nz, zn_last));
+    const float mjE = axv(Y, j, AX_MPV), mjEi = axv(Y, j, AX_MPVI);
+    const float idyE = axv(Y, j, AX_IDLD), pecy = axv(Y, j, AX_PEC);
+    const float mkE = axv(Z, k, AX_MPV), mkEi = axv(Z, k, AX_MPVI);
+    const float idzE = axv(Z, k, AX_IDLD), pecz = axv(Z, k, AX_PEC);
+    float ayE = 0.f, byE = 1.f, kyE = 1.f, azE = 0.f, bzE = 1.f, kzE = 1.f;
+    if (ca.sy >= 0) { ayE = axv(Y, j, AX_AE); byE = axv(Y, j, AX_BE); kyE = axv(Y, j, AX_KE); }
+    if (ca.sz >= 0) { azE = axv(Z, k, AX_AE); bzE = axv(Z, k, AX_BE); kzE = axv(Z, k, AX_KE); }
     float inj_ex = 0.f, inj_ey = 0.f;
     bool inj_x = false, inj_y = false;
-    if (pw.n && own) {
+    if (pw.n) {
         const int n = step_now(stp);
         for (int s = 0; s < pw.n; ++s)
             if (k == pw.ks[s]) {
@@ -777,164 +774,128 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
                 if (pw.pol[s] == 0) { inj_ex += v; inj_x = true; } else { inj_ey += v; inj_y = true; }
             }
     }
-    float ihx = 0.f, ihy = 0.f;
-    if (act) pw_h_inj(pw, k, stp, ihx, ihy);
+    const rsrc_t rC = mk_rsrc(mat.cls, (uint32_t)(ncell * 2));
+    const int hbx = mat.hb[1] - mat.hb[0], hby = mat.hb[3] - mat.hb[2], hbz = mat.hb[5] - mat.hb[4];
+    const uint32_t hcomp = (uint32_t)(hbx * hby * hbz);
+    const rsrc_t rHist = mk_rsrc(mat.hist, 3u * hcomp * T::B);
+    const bool hrow = mat.hist && own && j >= mat.hb[2] && j < mat.hb[3] && k >= mat.hb[4] &&
+                      k < mat.hb[5];
+    const uint32_t oh = hrow ? (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * T::B : 0u;
+    const uint32_t oc = (uint32_t)(j * nz + k) * 2u;
 
-    // new H of each row one plane back (the x neighbour of E), as a queue that rotates by one
-    // entry per row step: with the row loop rolled the compiler keeps one row's registers
-    F hy_q[OEM_WR], hz_q[OEM_WR];
-#pragma unroll
-    for (int r = 0; r < OEM_WR; ++r) {
-        hy_q[r] = T::zero();
-        hz_q[r] = T::zero();
+    // the new Hy, Hz of the plane before i0 (the x neighbours of E at i0), not stored
+    F hy_m, hz_m;
+    {
+        HColumn<F> cp = ca;
+        hcolumn_load_e<F>(cp, io, g, xp0);
+        HLoads<F> vp;
+        hcolumn_load<F>(vp, cp, io, g, xp0, next_of(xp0, nx, xn_last));
+        F hx_;
+        hcolumn_do<F>(cp, vp, io, g, ps, xp0, X.tab + (size_t)xp0 * AX_W, ch, false, hx_, hy_m,
+                      hz_m);
     }
-#pragma unroll 1
-    for (int r = 0; r < OEM_WR; ++r) {
-        F hy_ = T::zero(), hz_ = T::zero();
-        if (r < nr && own) {
-            const int j = j0 + r;
-            const PsiAt q = psi_at<F>(ps, g, ip, j, kc);
-            HIn<F> v;
-            F hx_;
-            fused_h_ld<F>(io, g, q, ip, next_of(ip, nx, xn_last), j, next_of(j, ny, yn_last), kc, v);
-            fused_h_do<F>(io, g, q, ip, j, kc, v, ch, ihx, ihy, false, hx_, hy_, hz_);
-        }
-#pragma unroll
-        for (int q = 0; q + 1 < OEM_WR; ++q) {
-            hy_q[q] = hy_q[q + 1];
-            hz_q[q] = hz_q[q + 1];
-        }
-        hy_q[OEM_WR - 1] = hy_;
-        hz_q[OEM_WR - 1] = hz_;
-    }
+    hcolumn_load_e<F>(ca, io, g, i0);
 
-    const int jhn = next_of(jh, ny, yn_last);
-    // The march as a flat list of tasks, OEM_WR + 1 per plane: the halo row, then the owned rows
-    // (rows past the last of a partial group are empty tasks that only rotate the queue).  The
-    // loads of task t + 1 are issued before the stores of task t: the memory counter is in
-    // order, so a load issued after a store could not be waited for without the store.
-    const int per_plane = OEM_WR + 1;
-    const int n_tasks = (i1 - i0) * per_plane;
-    RowIn<F> cur, nxt;
-    auto task_load = [&](int t, RowIn<F> &in) {
-        const int i = i0 + t / per_plane, r = t % per_plane - 1;
-        const bool halo = r < 0, valid = halo || r < nr;
-        const int j = halo ? jh : j0 + min(r, nr - 1);
-        const int jn = halo ? jhn : next_of(j, ny, yn_last);
-        const PsiAt q = psi_at<F>(ps, g, i, j, kc);
-        in.cls = 0u;
-        in.pe[0] = in.pe[1] = in.pe[2] = in.pe[3] = in.pe[4] = in.pe[5] = T::zero();
-        in.ph[0] = in.ph[1] = in.ph[2] = T::zero();
-        if (!(act && valid)) return;
-        fused_h_ld<F>(io, g, q, i, next_of(i, nx, xn_last), j, jn, kc, in.v);
-        if (!own || halo) return;
-        if (mat.mode == 0) in.cls = bld_u16(rC, (uint32_t)(j * nz + k) * 2u + (uint32_t)i * plane2, 0u);
-        if (q.sx >= 0) { in.pe[0] = T::ld(io.pe_o, q.ox + q.sx0, 0u); in.pe[1] = T::ld(io.pe_o, q.ox + q.sx1, 0u); }
-        if (q.sy >= 0) { in.pe[2] = T::ld(io.pe_o, q.oy + q.sy0, 0u); in.pe[3] = T::ld(io.pe_o, q.oy + q.sy1, 0u); }
-        if (q.sz >= 0) { in.pe[4] = T::ld(io.pe_o, q.oz + q.sz0, 0u); in.pe[5] = T::ld(io.pe_o, q.oz + q.sz1, 0u); }
-        const bool hin = hcol && i >= mat.hb[0] && i < mat.hb[1] && j >= mat.hb[2] && j < mat.hb[3];
-        if (hin) {
-            const uint32_t oh = (uint32_t)((j - mat.hb[2]) * hbz + (k - mat.hb[4])) * T::B;
-            const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * T::B;
-            in.ph[0] = T::ld(rHist, oh + sh, 0u);
-            in.ph[1] = T::ld(rHist, oh + sh + hcomp * T::B, 0u);
-            in.ph[2] = T::ld(rHist, oh + sh + 2u * hcomp * T::B, 0u);
-        }
-    };
-    F hxp = T::zero(), hzp = T::zero();
-    if (n_tasks > 0) task_load(0, cur);
 #pragma unroll 1
-    for (int t = 0; t < n_tasks; ++t) {
-        if (t + 1 < n_tasks) task_load(t + 1, nxt);
-        const int i = i0 + t / per_plane, r = t % per_plane - 1;
-        const bool halo = r < 0, valid = halo || r < nr;
-        const int j = halo ? jh : j0 + min(r, nr - 1);
-        const PsiAt q = psi_at<F>(ps, g, i, j, kc);
+    for (int i = i0; i < i1; ++i) {
+        const int buf = (i - i0) & 1;
+        const int in_ = next_of(i, nx, xn_last);
         const uint32_t so = (uint32_t)i * planeB;
-        F hx = T::zero(), hy = T::zero(), hz = T::zero();
-        if (act && valid)
-            fused_h_do<F>(io, g, q, i, j, kc, cur.v, ch, ihx, ihy, own && !halo, hx, hy, hz);
-        if (halo) {
-            // the halo row's Hx, Hz are the j - 1 neighbours of row 0
-            hxp = hx;
-            hzp = hz;
+        const int sx = slab_of(i, nx, X.lo, X.hi);
+        const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+        const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
+        const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
+        const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
+        const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * nz)) * T::B;
+        const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
+        const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
+        const bool hin = hrow && i >= mat.hb[0] && i < mat.hb[1];
+        // the E-side inputs first, so their loads overlap the H arithmetic
+        const uint32_t cls = mat.mode == 0 ? bld_u16(rC, oc, (uint32_t)i * plane2) : 0u;
+        F pe0 = T::zero(), pe1 = T::zero(), pe2 = T::zero(), pe3 = T::zero(), pe4 = T::zero(),
+          pe5 = T::zero();
+        if (sx >= 0) { pe0 = T::ld(io.po, ca.op, sx0 + io.pef); pe1 = T::ld(io.po, ca.op, sx1 + io.pef); }
+        if (ca.sy >= 0) { pe2 = T::ld(io.po, ca.oy, sy0 + io.pef); pe3 = T::ld(io.po, ca.oy, sy1 + io.pef); }
+        if (ca.sz >= 0) { pe4 = T::ld(io.po, ca.oz, sz0 + io.pef); pe5 = T::ld(io.po, ca.oz, sz1 + io.pef); }
+        HLoads<F> cv;
+        hcolumn_load<F>(cv, ca, io, g, i, in_);
+        // the update coefficients of the cell's class; the ADE history is read only where the
+        // class flags a dispersive component (the box around the dispersive cells is mostly
+        // other material)
+        float cax, cbx, cay, cby, caz, cbz;
+        uint32_t fl;
+        if (lds_lut) {
+            const float4 r0 = s_lut[2 * cls], r1 = s_lut[2 * cls + 1];
+            cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
+            fl = __float_as_uint(r1.z);
+        } else if (mat.mode == 0) {
+            const float4 *rw = reinterpret_cast<const float4 *>(mat.lut + 8u * cls);
+            const float4 r0 = rw[0], r1 = rw[1];
+            cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
+            fl = __float_as_uint(r1.z);
         } else {
-            // new H at k - 1 from the lane to the left (all lanes active here)
-            const F hx_k = lane_prev_f<F>(hx), hy_k = lane_prev_f<F>(hy);
-            if (own && valid) {
-                const float mi = axv(X, i, AX_MPV), mii = axv(X, i, AX_MPVI);
-                const float idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
-                const float mjE = axv(Y, j, AX_MPV), mjEi = axv(Y, j, AX_MPVI);
-                const float idyE = axv(Y, j, AX_IDLD), pecy = axv(Y, j, AX_PEC);
-                const bool hin = hcol && i >= mat.hb[0] && i < mat.hb[1] && j >= mat.hb[2] &&
-                                 j < mat.hb[3];
-                float cax, cbx, cay, cby, caz, cbz;
-                uint32_t fl;
-                if (lds_lut) {
-                    const float4 r0 = s_lut[2 * cur.cls], r1 = s_lut[2 * cur.cls + 1];
-                    cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
-                    fl = __float_as_uint(r1.z);
-                } else if (mat.mode == 0) {
-                    const float4 *rw = reinterpret_cast<const float4 *>(mat.lut + 8u * cur.cls);
-                    const float4 r0 = rw[0], r1 = rw[1];
-                    cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
-                    fl = __float_as_uint(r1.z);
-                } else {
-                    const size_t id = (size_t)i * g.plane + (size_t)j * nz + k;
-                    cax = mat.ca[id]; cay = mat.ca[ncell + id]; caz = mat.ca[2 * ncell + id];
-                    cbx = mat.cb[id]; cby = mat.cb[ncell + id]; cbz = mat.cb[2 * ncell + id];
-                    fl = hin ? 7u : 0u;
-                }
-                const F phx = (fl & 1u) ? cur.ph[0] : T::zero();
-                const F phy = (fl & 2u) ? cur.ph[1] : T::zero();
-                const F phz = (fl & 4u) ? cur.ph[2] : T::zero();
-                const F dzy = (hz - T::mmul(hzp, mjE, mjEi)) * idyE;
-                const F dyz = (hy - T::mmul(hy_k, mkE, mkEi)) * idzE;
-                const F dxz = (hx - T::mmul(hx_k, mkE, mkEi)) * idzE;
-                const F dzx = (hz - T::mmul(hz_q[0], mi, mii)) * idx;
-                const F dyx = (hy - T::mmul(hy_q[0], mi, mii)) * idx;
-                const F dxy = (hx - T::mmul(hxp, mjE, mjEi)) * idyE;
-                F s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
-                if (q.sx >= 0) {
-                    const float a_ = axv(X, i, AX_AE), b_ = axv(X, i, AX_BE), k_ = axv(X, i, AX_KE);
-                    s_ey_x = stretch_v(cur.pe[0], dzx, a_, b_, k_);
-                    s_ez_x = stretch_v(cur.pe[1], dyx, a_, b_, k_);
-                    T::st(io.pe_n, cur.pe[0], q.ox + q.sx0, 0u);
-                    T::st(io.pe_n, cur.pe[1], q.ox + q.sx1, 0u);
-                }
-                if (q.sy >= 0) {
-                    const float a_ = axv(Y, j, AX_AE), b_ = axv(Y, j, AX_BE), k_ = axv(Y, j, AX_KE);
-                    s_ex_y = stretch_v(cur.pe[2], dzy, a_, b_, k_);
-                    s_ez_y = stretch_v(cur.pe[3], dxy, a_, b_, k_);
-                    T::st(io.pe_n, cur.pe[2], q.oy + q.sy0, 0u);
-                    T::st(io.pe_n, cur.pe[3], q.oy + q.sy1, 0u);
-                }
-                if (q.sz >= 0) {
-                    s_ex_z = stretch_v(cur.pe[4], dyz, az, bz, kz);
-                    s_ey_z = stretch_v(cur.pe[5], dxz, az, bz, kz);
-                    T::st(io.pe_n, cur.pe[4], q.oz + q.sz0, 0u);
-                    T::st(io.pe_n, cur.pe[5], q.oz + q.sz1, 0u);
-                }
-                F nex = (cax * cur.v.ex + cbx * (s_ex_y - s_ex_z - cw * phx)) * pecy * pecz;
-                F ney = (cay * cur.v.ey + cby * (s_ey_z - s_ey_x - cw * phy)) * pecz * pecx;
-                F nez = (caz * cur.v.ez + cbz * (s_ez_x - s_ez_y - cw * phz)) * pecx * pecy;
-                if (inj_x) nex = T::addre(nex, cbx * inj_ex);
-                if (inj_y) ney = T::addre(ney, cby * inj_ey);
-                T::st(io.en, nex, q.ox + so, 0u);
-                T::st(io.en, ney, q.ox + so + io.cb, 0u);
-                T::st(io.en, nez, q.ox + so + 2u * io.cb, 0u);
-            }
-            hxp = hx;
-            hzp = hz;
-#pragma unroll
-            for (int qq = 0; qq + 1 < OEM_WR; ++qq) {
-                hy_q[qq] = hy_q[qq + 1];
-                hz_q[qq] = hz_q[qq + 1];
-            }
-            hy_q[OEM_WR - 1] = hy;
-            hz_q[OEM_WR - 1] = hz;
+            const size_t id = (size_t)i * g.plane + (size_t)j * nz + k;
+            cax = mat.ca[id]; cay = mat.ca[ncell + id]; caz = mat.ca[2 * ncell + id];
+            cbx = mat.cb[id]; cby = mat.cb[ncell + id]; cbz = mat.cb[2 * ncell + id];
+            fl = 7u;
         }
-        cur = nxt;
+        F phx = T::zero(), phy = T::zero(), phz = T::zero();
+        if (hin) {
+            const uint32_t sh = (uint32_t)(i - mat.hb[0]) * (uint32_t)(hby * hbz) * T::B;
+            if (fl & 1u) phx = T::ld(rHist, oh, sh);
+            if (fl & 2u) phy = T::ld(rHist, oh, sh + hcomp * T::B);
+            if (fl & 4u) phz = T::ld(rHist, oh, sh + 2u * hcomp * T::B);
+        }
+        const F ey0 = ca.ey, ez0 = ca.ez;
+        F hx, hy, hz;
+        const float *xr = X.tab + (size_t)i * AX_W;
+        const F ex0 = cv.ex;
+        hcolumn_do<F>(ca, cv, io, g, ps, i, xr, ch, own, hx, hy, hz);
+        s_row[buf][0][w + 1][lane] = hx;
+        s_row[buf][1][w + 1][lane] = hy;
+        s_row[buf][2][w + 1][lane] = hz;
+        __syncthreads();
+        const F hx_j = s_row[buf][0][w][lane], hz_j = s_row[buf][2][w][lane];
+        const F hx_k = lane ? s_row[buf][0][w + 1][lane - 1] : s_col[buf][0][w];
+        const F hy_k = lane ? s_row[buf][1][w + 1][lane - 1] : s_col[buf][1][w];
+        const F hpx = phx, hpy = phy, hpz = phz;
+        const float mi = xr[AX_MPV], mii = xr[AX_MPVI];
+        const float idx = xr[AX_IDLD], pecx = xr[AX_PEC];
+        const F dzy = (hz - T::mmul(hz_j, mjE, mjEi)) * idyE;
+        const F dyz = (hy - T::mmul(hy_k, mkE, mkEi)) * idzE;
+        const F dxz = (hx - T::mmul(hx_k, mkE, mkEi)) * idzE;
+        const F dzx = (hz - T::mmul(hz_m, mi, mii)) * idx;
+        const F dyx = (hy - T::mmul(hy_m, mi, mii)) * idx;
+        const F dxy = (hx - T::mmul(hx_j, mjE, mjEi)) * idyE;
+        F s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
+        if (sx >= 0) {
+            const float a_ = xr[AX_AE], b_ = xr[AX_BE], k_ = xr[AX_KE];
+            s_ey_x = stretch_v(pe0, dzx, a_, b_, k_);
+            s_ez_x = stretch_v(pe1, dyx, a_, b_, k_);
+            if (own) { T::st(io.pn, pe0, ca.op, sx0 + io.pef); T::st(io.pn, pe1, ca.op, sx1 + io.pef); }
+        }
+        if (ca.sy >= 0) {
+            s_ex_y = stretch_v(pe2, dzy, ayE, byE, kyE);
+            s_ez_y = stretch_v(pe3, dxy, ayE, byE, kyE);
+            if (own) { T::st(io.pn, pe2, ca.oy, sy0 + io.pef); T::st(io.pn, pe3, ca.oy, sy1 + io.pef); }
+        }
+        if (ca.sz >= 0) {
+            s_ex_z = stretch_v(pe4, dyz, azE, bzE, kzE);
+            s_ey_z = stretch_v(pe5, dxz, azE, bzE, kzE);
+            if (own) { T::st(io.pn, pe4, ca.oz, sz0 + io.pef); T::st(io.pn, pe5, ca.oz, sz1 + io.pef); }
+        }
+        F nex = (cax * ex0 + cbx * (s_ex_y - s_ex_z - cw * hpx)) * pecy * pecz;
+        F ney = (cay * ey0 + cby * (s_ey_z - s_ey_x - cw * hpy)) * pecz * pecx;
+        F nez = (caz * ez0 + cbz * (s_ez_x - s_ez_y - cw * hpz)) * pecx * pecy;
+        if (inj_x) nex = T::addre(nex, cbx * inj_ex);
+        if (inj_y) ney = T::addre(ney, cby * inj_ey);
+        if (own) {
+            T::st(io.fn, nex, ca.op, so);
+            T::st(io.fn, ney, ca.op, so + io.cb);
+            T::st(io.fn, nez, ca.op, so + 2u * io.cb);
+        }
+        hy_m = hy;
+        hz_m = hz;
     }
 }
 

@@ -8,7 +8,9 @@
 //   update_e (+ plane-wave E correction) -> TF/SF E faces -> electric dipoles -> ADE post
 //   (fused with the next pre) -> DFT snapshots -> step++
 //
-// with the DFT flushed every dft_batch steps.  Blocks of dft_batch steps aligned to the batch
+// with the DFT flushed every dft_batch steps.  When no point list corrects H between the sweeps,
+// update_h and update_e run as one fused kernel (update_he) that ping-pongs between two field
+// sets.  Blocks of dft_batch steps aligned to the batch
 // are captured once into a hipGraph and replayed; the device-side step counter lets one graph
 // serve every block (the reference does the same with a CUDA graph, `openem/solver.py:1165-1182`).
 
@@ -74,13 +76,14 @@ int ctx_init(oem_ctx *c, const oem_dims *d, int32_t device) {
         }
     if (total * 4 * c->esz > 0xffffffffull) return fail(OEM_ENOTSUP, "CPML slabs exceed 4 GiB per side");
     c->psi.total = (uint32_t)(total ? total : 1);
-    TRY(c->alloc(&c->E, 3 * nc * c->esz, true));
-    TRY(c->alloc(&c->H, 3 * nc * c->esz, true));
-    float *ph = nullptr, *pe = nullptr;
-    TRY(c->alloc(&ph, (size_t)c->psi.total * c->esz, true));
-    TRY(c->alloc(&pe, (size_t)c->psi.total * c->esz, true));
-    c->psi.h = ph;
-    c->psi.e = pe;
+    // E and H in one allocation (H right after E), and the H- and E-side psi in one: a kernel
+    // then reaches a whole field set, or a whole psi set, through one buffer descriptor
+    TRY(c->alloc(&c->E, 6 * nc * c->esz, true));
+    c->H = c->E + 3 * nc * c->esz;
+    float *pp = nullptr;
+    TRY(c->alloc(&pp, 2 * (size_t)c->psi.total * c->esz, true));
+    c->psi.h = pp;
+    c->psi.e = pp + (size_t)c->psi.total * c->esz;
     TRY(c->alloc(&c->d_step, 2, true));
     HIP_OK(hipStreamSynchronize(c->stream));
     return OEM_OK;
@@ -662,41 +665,35 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     const long long nb = (long long)g.tiles * ((g.nx + g.chunk - 1) / g.chunk);
     if (nb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
     g.nblocks = (int)nb;
-    // fused H+E step on request (OEM_FUSE=1), unless a point list corrects H between the sweeps
+    // the fused H+E step whenever it applies (OEM_FUSE=0 keeps the two sweeps): not in a serving
+    // pool, and not when a point list corrects H between the sweeps
     const char *fe = std::getenv("OEM_FUSE");
-    bool fuse = !ctx->pool && fe && fe[0] == '1';
-    // the fused march addresses a whole field set (three components) with 32-bit offsets
-    fuse &= 3ull * ctx->ncell * 4ull * (uint64_t)ctx->esz < (1ull << 32);
+    bool fuse = !ctx->pool && !(fe && fe[0] == '0');
+    // it addresses a whole field set (E and H, six components) and a whole psi set (both sides)
+    // with 32-bit offsets
+    fuse &= 6ull * ctx->ncell * 4ull * (uint64_t)ctx->esz < (1ull << 32);
+    fuse &= 2ull * ctx->psi.total * 4ull * (uint64_t)ctx->esz < (1ull << 32);
     for (auto &pt : ctx->points) fuse &= !pt.magnetic;
     if (fuse) {
         const size_t nc = ctx->ncell;
-        TRY(ctx->alloc(&ctx->E2, 3 * nc * ctx->esz, true));
-        TRY(ctx->alloc(&ctx->H2, 3 * nc * ctx->esz, true));
-        float *ph = nullptr, *pe = nullptr;
-        TRY(ctx->alloc(&ph, (size_t)ctx->psi.total * ctx->esz, true));
-        TRY(ctx->alloc(&pe, (size_t)ctx->psi.total * ctx->esz, true));
+        TRY(ctx->alloc(&ctx->E2, 6 * nc * ctx->esz, true));
+        ctx->H2 = ctx->E2 + 3 * nc * ctx->esz;
+        float *pp = nullptr;
+        TRY(ctx->alloc(&pp, 2 * (size_t)ctx->psi.total * ctx->esz, true));
         ctx->psi2 = ctx->psi;
-        ctx->psi2.h = ph;
-        ctx->psi2.e = pe;
+        ctx->psi2.h = pp;
+        ctx->psi2.e = pp + (size_t)ctx->psi.total * ctx->esz;
         HIP_OK(hipStreamSynchronize(ctx->stream));
+        // one work-group per (OEM_CR rows x 64 k) tile and x chunk; 16-plane chunks measured
+        // best at 512^3 (8: 2.20 ms, 16 and 32: 1.83 ms, 64: 2.15 ms per step of the kernel)
         ctx->fg = g;
-        // one wave per (OEM_WR rows x OEM_WK k) strip and x chunk, OEM_WW waves per work-group
-        ctx->fg.tiles = ((g.nz + OEM_WK - 1) / OEM_WK) * ((g.ny + OEM_WR - 1) / OEM_WR);
-        int fchunk;
-        if (ctx->chunk_req > 0) {
-            fchunk = ctx->chunk_req;
-        } else if (env && std::atoi(env) > 0) {
-            fchunk = std::atoi(env);
-        } else {
-            // short chunks (many waves per SIMD) measured best: 8 planes at 512^3
-            const int want = 32768;
-            const int need = std::max(1, (want + ctx->fg.tiles - 1) / ctx->fg.tiles);
-            fchunk = std::max(8, (g.nx + need - 1) / need);
-        }
+        ctx->fg.tiles = ((g.nz + 63) / 64) * ((g.ny + OEM_CR - 1) / OEM_CR);
+        int fchunk = 16;
+        if (ctx->chunk_req > 0) fchunk = ctx->chunk_req;
+        else if (env && std::atoi(env) > 0) fchunk = std::atoi(env);
         ctx->fg.chunk = std::min(fchunk, g.nx);
-        const long long items = (long long)ctx->fg.tiles * ((g.nx + ctx->fg.chunk - 1) / ctx->fg.chunk);
-        const long long fb = (items + OEM_WW - 1) / OEM_WW;
-        if (items > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
+        const long long fb = (long long)ctx->fg.tiles * ((g.nx + ctx->fg.chunk - 1) / ctx->fg.chunk);
+        if (fb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
         ctx->fg.nblocks = (int)fb;
         ctx->fused = true;
     }
@@ -766,7 +763,7 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
             TRY(mark(OEM_K_DISP_PRE, false));
         }
         TRY(mark(OEM_K_UPDATE_HE, true));
-        hipLaunchKernelGGL((k_update_he<F>), dim3(c->fg.nblocks), dim3(64, OEM_WW), 0,
+        hipLaunchKernelGGL((k_update_he<F>), dim3(c->fg.nblocks), dim3(64, OEM_CR + 1), 0,
                            c->stream, (const F *)Eo, (const F *)Ho, E, H, po, pn, c->fg, c->mat,
                            c->dims.ch, c->dims.cw, c->pw, c->d_step);
         TRY(mark(OEM_K_UPDATE_HE, false));
