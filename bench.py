@@ -182,7 +182,8 @@ def bloch_bench(steps: int, warmup: int = 10, profile_steps: int = 10) -> dict:
         st.close()
     cells = int(np.prod(sc.shape))
     ab = algorithmic_bytes(tabs)
-    k_dom = max(("update_h", "update_e"), key=lambda k: prof[k][0] / max(prof[k][1], 1))
+    k_dom = max(("update_h", "update_e", "update_he"),
+                key=lambda k: prof[k][0] / max(prof[k][1], 1))
     avg = prof[k_dom][0] / prof[k_dom][1]
     achieved = ab[k_dom] / (avg * 1e-3) / 1e9
     kern = {k: round(v[0] / v[1], 5) for k, v in prof.items() if v[1]}
@@ -196,7 +197,9 @@ def bloch_bench(steps: int, warmup: int = 10, profile_steps: int = 10) -> dict:
             "roofline": {"bound": "hbm", "kernel": k_dom, "achieved": round(achieved, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "fused": k_dom == "update_he",
                          "algorithmic_bytes_per_launch": ab[k_dom],
+                         "minimal_bytes_per_launch": ab["minimal_" + k_dom],
                          "avg_launch_ms": round(avg, 5)}}
 
 

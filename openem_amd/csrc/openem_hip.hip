@@ -8,9 +8,9 @@
 //   update_e (+ plane-wave E correction) -> TF/SF E faces -> electric dipoles -> ADE post
 //   (fused with the next pre) -> DFT snapshots -> step++
 //
-// with the DFT flushed every dft_batch steps.  When no point list corrects H between the sweeps,
-// update_h and update_e run as one fused kernel (update_he) that ping-pongs between two field
-// sets.  Blocks of dft_batch steps aligned to the batch
+// with the DFT flushed every dft_batch steps.  Outside a serving pool, update_h and update_e run
+// as one fused kernel (update_he) that ping-pongs between two field sets, the H-side point
+// corrections added to the old H just before it.  Blocks of dft_batch steps aligned to the batch
 // are captured once into a hipGraph and replayed; the device-side step counter lets one graph
 // serve every block (the reference does the same with a CUDA graph, `openem/solver.py:1165-1182`).
 
@@ -666,14 +666,13 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     if (nb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
     g.nblocks = (int)nb;
     // the fused H+E step whenever it applies (OEM_FUSE=0 keeps the two sweeps): not in a serving
-    // pool, and not when a point list corrects H between the sweeps
+    // pool
     const char *fe = std::getenv("OEM_FUSE");
     bool fuse = !ctx->pool && !(fe && fe[0] == '0');
     // it addresses a whole field set (E and H, six components) and a whole psi set (both sides)
     // with 32-bit offsets
     fuse &= 6ull * ctx->ncell * 4ull * (uint64_t)ctx->esz < (1ull << 32);
     fuse &= 2ull * ctx->psi.total * 4ull * (uint64_t)ctx->esz < (1ull << 32);
-    for (auto &pt : ctx->points) fuse &= !pt.magnetic;
     if (fuse) {
         const size_t nc = ctx->ncell;
         TRY(ctx->alloc(&ctx->E2, 6 * nc * ctx->esz, true));
@@ -762,6 +761,18 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
                                c->dims.ny, c->dims.nz, 0);
             TRY(mark(OEM_K_DISP_PRE, false));
         }
+        // H-side point corrections (magnetic dipoles, TF/SF faces, mode sources) are additive
+        // terms independent of the fields: added to the old H before the march, every H update
+        // and halo recompute sees them (the sweeps add them after update_h instead, which
+        // rounds (H - ch curl E) + c rather than (H + c) - ch curl E)
+        for (int slot = 0; slot < 3; ++slot)
+            for (auto &pt : c->points) {
+                if (points_slot(pt) != slot) continue;
+                TRY(mark(points_kid(pt), true));
+                hipLaunchKernelGGL((k_points<F>), dim3(blocks_for(pt.d.n_groups, 256)), dim3(256),
+                                   0, c->stream, Ho, pt.d, c->mat, c->dims.ch, nc, c->d_step);
+                TRY(mark(points_kid(pt), false));
+            }
         TRY(mark(OEM_K_UPDATE_HE, true));
         hipLaunchKernelGGL((k_update_he<F>), dim3(c->fg.nblocks), dim3(64, OEM_CR + 1), 0,
                            c->stream, (const F *)Eo, (const F *)Ho, E, H, po, pn, c->fg, c->mat,

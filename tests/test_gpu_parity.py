@@ -230,3 +230,34 @@ def test_fused_step_equals_the_two_sweeps(gpu_lib, monkeypatch, name, chunk):
     assert np.array_equal(f_fused, f_two), errs
     for a, b in zip(acc_fused, acc_two):
         assert np.array_equal(a, b)
+
+
+# scenes whose point lists correct H: the fused step adds those terms to the old H before the
+# march, the sweeps after update_h, so the two round differently at those cells
+H_POINT_CASES = {
+    "magnetic_dipole": lambda n: tables.build(scenes.mini_scene(num_steps=n,
+                                                                **CASES["pml_xy_sigma_pec"])),
+    "tfsf_box": lambda n: tables.build(scenes.tfsf_box_scene(num_steps=n, scatterer=True)),
+    "waveguide_mode": lambda n: tables.build(scenes.waveguide_scene(
+        shape=(30, 28, 26), n_pml=6, width_cells=8, thick_cells=4, num_steps=n, nf=1)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(H_POINT_CASES))
+def test_fused_step_with_h_point_lists_matches_the_two_sweeps(gpu_lib, monkeypatch, name):
+    """With H-side point corrections the fused step agrees with the two sweeps to float32
+    rounding (1e-5 of the peak after 150 steps) in fields and phasors."""
+    n = 150
+    tabs = H_POINT_CASES[name](n)
+    print(f"fused vs two sweeps with H points: {name} {tabs.shape}", flush=True)
+    monkeypatch.delenv("OEM_FUSE", raising=False)
+    f_fused, acc_fused = _gpu_state(tabs, n)
+    monkeypatch.setenv("OEM_FUSE", "0")
+    f_two, acc_two = _gpu_state(tabs, n)
+    peak = np.abs(f_two).max()
+    assert peak > 0
+    errs = [float(np.abs(f_fused[c] - f_two[c]).max() / peak) for c in range(6)]
+    print("per component:", errs, flush=True)
+    assert max(errs) < 1e-5, errs
+    for a, b in zip(acc_fused, acc_two):
+        assert _rel(a, b) < 1e-5
