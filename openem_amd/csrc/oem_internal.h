@@ -48,6 +48,9 @@ struct FluxTimeMonitor {
 // One grouped point list (dipoles or TF/SF face corrections) and which field it updates.
 enum { PT_DIPOLE = 0, PT_TFSF = 1, PT_MODE = 2 };
 
+// Smallest scene (cells) that takes the fused H+E step by default.
+#define OEM_FUSE_MIN_CELLS (12u << 20)
+
 struct Points {
     PointsDev d;
     int magnetic;       // 1: applied to H after update_h; 0: to E after update_e

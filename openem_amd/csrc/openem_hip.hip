@@ -665,10 +665,14 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     const long long nb = (long long)g.tiles * ((g.nx + g.chunk - 1) / g.chunk);
     if (nb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
     g.nblocks = (int)nb;
-    // the fused H+E step whenever it applies (OEM_FUSE=0 keeps the two sweeps): not in a serving
-    // pool
+    // the fused H+E step for scenes of OEM_FUSE_MIN_CELLS and more (OEM_FUSE=1: any size, 0:
+    // never), outside a serving pool.  Measured step times (box soup, two sweeps / fused):
+    // 64^3 30 / 29-55 us, 128^3 69 / 65-83 us, 160^3 98 / 132 us, 200^3 241 / 240 us, 256^3
+    // 393 / 355-361 us, 320^3 677 / 596 us, 512^3 2.54 / 1.99 ms: below ~8 M cells a fused
+    // work-group marches too few tiles to fill the GPU
     const char *fe = std::getenv("OEM_FUSE");
     bool fuse = !ctx->pool && !(fe && fe[0] == '0');
+    if (!(fe && fe[0] == '1')) fuse &= ctx->ncell >= OEM_FUSE_MIN_CELLS;
     // it addresses a whole field set (E and H, six components) and a whole psi set (both sides)
     // with 32-bit offsets
     fuse &= 6ull * ctx->ncell * 4ull * (uint64_t)ctx->esz < (1ull << 32);

@@ -1,0 +1,94 @@
+"""GPU tests of the features that read or restore the field sets, on the fused H+E step.
+
+The fused step ping-pongs between two field sets (after s steps the state is in set s % 2), so
+everything that reads the state (time monitors, shutoff reductions, returned fields) or restores
+it (adjoint checkpoints) must follow the parity.  Each test forces the fused step on a small
+scene (OEM_FUSE=1; by default it starts at 12 M cells) and compares with the two sweeps
+(OEM_FUSE=0): bit for bit where no point list corrects H, to float32 rounding where one does
+(the waveguide's mode source).
+"""
+
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from openem_amd import adjoint, scenes, shutoff, solver, tables
+from openem_amd.model import FieldTimeMonitor, FluxTimeMonitor
+from openem_amd.solver import Stepper
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(monkeypatch, fn):
+    out = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("OEM_FUSE", env)
+        out.append(fn())
+    monkeypatch.delenv("OEM_FUSE")
+    return out
+
+
+def test_time_and_flux_time_monitors_follow_the_parity(gpu_lib, monkeypatch):
+    sc = scenes.mini_scene(num_steps=121)
+    g = sc.grid
+    span = tuple((float(g.axes[t].edges[0]), float(g.axes[t].edges[-1])) for t in (0, 1))
+    sc = replace(sc, field_time_monitors=[
+        FieldTimeMonitor("all", tuple(range(6)), (1, 2, 6), (6, 7, 24), 5, 110, 3, 40)],
+        flux_time_monitors=[FluxTimeMonitor("f", 2, 30, 1, 0, 121, 2, 70, t_bounds=span)])
+    tabs = tables.build(sc)
+
+    def run():
+        st = Stepper(tabs, dft_batch=8)
+        try:
+            st.run(121)                 # odd: the fused state ends in the second set
+            return (st.read_time_monitor(0), st.read_flux_time_monitor(0),
+                    np.stack([st.read_field(c) for c in range(6)]))
+        finally:
+            st.close()
+    (t1, f1, e1), (t2, f2, e2) = _both(monkeypatch, run)
+    assert np.abs(t2).max() > 0 and np.abs(f2).max() > 0
+    assert np.array_equal(t1, t2) and np.array_equal(f1, f2) and np.array_equal(e1, e2)
+
+
+def test_shutoff_stops_at_the_same_step(gpu_lib, monkeypatch):
+    sc = scenes.mini_scene(shape=(14, 14, 40), pml_xy=True, plane_wave_on=False,
+                           lorentz_on=False, num_steps=3000)
+
+    def run():
+        return solver.run(sc, num_steps=3000, policy=shutoff.default_policy(1e-3, 20),
+                          verbose=False, return_fields=True)
+    a, b = _both(monkeypatch, run)
+    assert a.shutoff_triggered and b.shutoff_triggered
+    assert a.steps_run == b.steps_run and a.stop_reason == b.stop_reason
+    assert a.energy_trace == b.energy_trace
+    for k in b.fields:
+        assert np.array_equal(a.fields[k], b.fields[k]), k
+
+
+def test_adjoint_checkpoints_restore_either_set(gpu_lib, monkeypatch):
+    """Segments of 47 steps restore checkpoints at odd and even steps; the gradient equals the
+    one-segment gradient of the fused step and agrees with the two sweeps' to float32 rounding
+    (the mode source corrects H: 1e-4 of the largest |gradient|)."""
+    sc = scenes.waveguide_scene(shape=(30, 24, 22), n_pml=6, width_cells=6, thick_cells=4,
+                                fwidth_rel=0.3, num_steps=200, nf=1)
+    rng = np.random.default_rng(9)
+    ny, nz = sc.shape[1], sc.shape[2]
+    comp = rng.integers(0, 3, 6)
+    cell = np.array([(int(rng.integers(8, 20)) * ny + int(rng.integers(8, 16))) * nz
+                     + int(rng.integers(8, 14)) for _ in range(6)])
+
+    def run(every):
+        return adjoint.mode_power_gradient(sc, "fwd", comp, cell, checkpoint_every=every)
+    monkeypatch.setenv("OEM_FUSE", "1")
+    seg = run(47)
+    one = run(200)
+    monkeypatch.setenv("OEM_FUSE", "0")
+    two = run(200)
+    monkeypatch.delenv("OEM_FUSE")
+    assert seg.segments == 5 and one.segments == 1
+    assert np.allclose(seg.grad, one.grad, rtol=1e-12, atol=0) and seg.J == one.J
+    scale = np.abs(two.grad).max()
+    assert scale > 0
+    assert np.abs(one.grad - two.grad).max() <= 1e-4 * scale
+    assert abs(one.J - two.J) <= 1e-5 * two.J

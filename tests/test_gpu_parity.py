@@ -203,16 +203,17 @@ FUSED_CHUNKS = {"default": None, "chunk3": "3"}
 @pytest.mark.parametrize("chunk", sorted(FUSED_CHUNKS))
 @pytest.mark.parametrize("name", sorted(FUSED_CASES) + sorted(FUSED_COMPLEX))
 def test_fused_step_equals_the_two_sweeps(gpu_lib, monkeypatch, name, chunk):
-    """The fused H+E step (the default; ping-pong buffers, halo recompute) gives the fields and
-    phasors of the separate H and E sweeps (OEM_FUSE=0) bit for bit: every cell is updated from
-    the same inputs by the same expressions (no multiply-add contraction)."""
+    """The fused H+E step (OEM_FUSE=1, the default from 12 M cells; ping-pong buffers, halo
+    recompute) gives the fields and phasors of the separate H and E sweeps (OEM_FUSE=0) bit for
+    bit: every cell is updated from the same inputs by the same expressions (no multiply-add
+    contraction)."""
     n = 180
     if name in FUSED_COMPLEX:
         tabs = tables.build(FUSED_CASES[FUSED_COMPLEX[name]](n), force_complex=True)
     else:
         tabs = tables.build(FUSED_CASES[name](n))
     print(f"fused ({chunk}) vs two sweeps: {name} {tabs.shape}", flush=True)
-    monkeypatch.delenv("OEM_FUSE", raising=False)
+    monkeypatch.setenv("OEM_FUSE", "1")
     if FUSED_CHUNKS[chunk]:
         monkeypatch.setenv("OEM_CHUNK", FUSED_CHUNKS[chunk])
     f_fused, acc_fused = _gpu_state(tabs, n)
@@ -250,7 +251,7 @@ def test_fused_step_with_h_point_lists_matches_the_two_sweeps(gpu_lib, monkeypat
     n = 150
     tabs = H_POINT_CASES[name](n)
     print(f"fused vs two sweeps with H points: {name} {tabs.shape}", flush=True)
-    monkeypatch.delenv("OEM_FUSE", raising=False)
+    monkeypatch.setenv("OEM_FUSE", "1")
     f_fused, acc_fused = _gpu_state(tabs, n)
     monkeypatch.setenv("OEM_FUSE", "0")
     f_two, acc_two = _gpu_state(tabs, n)
