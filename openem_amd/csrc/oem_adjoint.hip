@@ -16,18 +16,13 @@
 
 #include "oem_internal.h"
 
-#define OEM_ADJ_THREADS 256
-#define OEM_ADJ_BLOCKS 4096
+// the transposed sweeps: a (k, j) column per thread from a 64 x 4 block; the transposes of the
+// curl march OEM_ADJ_CHUNK x planes per block (grid z), the cell-local parts one plane
+#define OEM_ADJ_TK 64
+#define OEM_ADJ_TJ 4
+#define OEM_ADJ_CHUNK 16
 
 namespace {
-
-struct Term { int c, b, a, sgn, slot; };
-// H_c and E_c share the term structure (curl): c = 0: +d_y(F_z) - d_z(F_y), ...
-__constant__ Term kTerms[6] = {
-    {0, 2, 1, +1, 2}, {0, 1, 2, -1, 4},     // x: +y(z) [y-slab, Hx/Ex], -z(y) [z-slab, Hx/Ex]
-    {1, 0, 2, +1, 5}, {1, 2, 0, -1, 0},     // y: +z(x) [z-slab, Hy/Ey], -x(z) [x-slab, Hy/Ey]
-    {2, 1, 0, +1, 1}, {2, 0, 1, -1, 3},     // z: +x(y) [x-slab, Hz/Ez], -y(x) [y-slab, Hz/Ez]
-};
 
 struct AdjDev {
     float *lE, *lH;              // 3 * ncell each
@@ -39,128 +34,273 @@ struct AdjDev {
     size_t ncell;
 };
 
-__device__ __forceinline__ void cell_ijk(size_t id, int ny, int nz, int &i, int &j, int &k) {
-    k = (int)(id % nz);
-    const size_t r = id / nz;
-    j = (int)(r % ny);
-    i = (int)(r / ny);
-}
-
-__device__ __forceinline__ size_t shift(size_t id, int a, int from, int to, int ny, int nz) {
-    const size_t st = a == 0 ? (size_t)ny * nz : (a == 1 ? (size_t)nz : 1);
-    return id + (size_t)((long long)to - from) * st;
-}
-
-// psi index of cell (i, j, k) in the slab of axis a (or -1 outside the slab)
-__device__ __forceinline__ long long psi_at(const AdjDev &d, int slot, int i, int j, int k) {
-    const GridDev &g = d.g;
-    const int a = slot >> 1;
-    const AxisDev &A = g.ax[a];
-    const int n = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
-    const int p = a == 0 ? i : (a == 1 ? j : k);
-    const int s = slab_of(p, n, A.lo, A.hi);
-    if (s < 0) return -1;
-    const int w = A.lo + A.hi;
-    size_t o;
-    if (a == 0) o = ((size_t)s * g.ny + j) * g.nz + k;
-    else if (a == 1) o = ((size_t)i * w + s) * g.nz + k;
-    else o = ((size_t)i * g.ny + j) * w + s;
-    return (long long)(d.psl.off[slot] + o);
-}
-
-__device__ __forceinline__ float cls_cb(const MatDev &m, int c, size_t id, size_t ncell) {
-    return m.mode == 0 ? m.lut[8 * (size_t)m.cls[id] + 2 * c + 1] : m.cb[(size_t)c * ncell + id];
-}
-__device__ __forceinline__ float cls_ca(const MatDev &m, int c, size_t id, size_t ncell) {
-    return m.mode == 0 ? m.lut[8 * (size_t)m.cls[id] + 2 * c] : m.ca[(size_t)c * ncell + id];
-}
-
-// E update's pec factor of component c at (i, j, k)
-__device__ __forceinline__ float pec_of(const GridDev &g, int c, int i, int j, int k) {
-    const float px = axv(g.ax[0], i, AX_PEC), py = axv(g.ax[1], j, AX_PEC), pz = axv(g.ax[2], k, AX_PEC);
-    return c == 0 ? py * pz : (c == 1 ? pz * px : px * py);
-}
-
-// q of an E-transpose term at cell (i, j, k): (1/kappa + a) sgn nu_c + a lpe, nu_c = cb pec lE_c
-__device__ __forceinline__ float q_e(const AdjDev &d, const Term &t, int i, int j, int k, size_t id) {
-    const AxisDev &A = d.g.ax[t.a];
-    const int p = t.a == 0 ? i : (t.a == 1 ? j : k);
-    const float nu = cls_cb(d.mat, t.c, id, d.ncell) * pec_of(d.g, t.c, i, j, k) *
-                     d.lE[(size_t)t.c * d.ncell + id];
-    const long long pi = psi_at(d, t.slot, i, j, k);
-    if (pi < 0) return (float)t.sgn * nu;
-    const float ae = axv(A, p, AX_AE), ke = axv(A, p, AX_KE);
-    return (ke + ae) * (float)t.sgn * nu + ae * d.lpe[pi];
-}
-
-// q of an H-transpose term: (1/kappa + a) f + a lph, f = -ch sgn lH_c
-__device__ __forceinline__ float q_h(const AdjDev &d, const Term &t, int i, int j, int k, size_t id) {
-    const AxisDev &A = d.g.ax[t.a];
-    const int p = t.a == 0 ? i : (t.a == 1 ? j : k);
-    const float f = -d.ch * (float)t.sgn * d.lH[(size_t)t.c * d.ncell + id];
-    const long long pi = psi_at(d, t.slot, i, j, k);
-    if (pi < 0) return f;
-    const float ah = axv(A, p, AX_AH), kh = axv(A, p, AX_KH);
-    return (kh + ah) * f + ah * d.lph[pi];
-}
-
 // E transpose, part 1: lH_b += D_bwd^T q_{c,a} for the two terms with source b.
 // D_bwd f[p] = (f[p] - f[prv p] mpv[p]) idd[p]  =>  (D_bwd^T q)[p] = q[p] idd[p]
 //   - sum over p' with prv[p'] == p of q[p'] mpv[p'] idd[p']  (p' = nxt p, or p itself at a
 //   clamped end)
-__global__ void __launch_bounds__(OEM_ADJ_THREADS) k_adj_e1(AdjDev d)
-{
-    const int ny = d.g.ny, nz = d.g.nz;
-    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < d.ncell;
-         id += (size_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        cell_ijk(id, ny, nz, i, j, k);
-        float acc[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const Term t = kTerms[q];
-            const AxisDev &A = d.g.ax[t.a];
-            const int p = t.a == 0 ? i : (t.a == 1 ? j : k);
-            const float qs = q_e(d, t, i, j, k, id);
-            float v = qs * axv(A, p, AX_IDLD);
-            if (A.prv[p] == p) v -= qs * axv(A, p, AX_MPV) * axv(A, p, AX_IDLD);   // clamped end
-            const int pn = A.nxt[p];
-            if (pn != p && A.prv[pn] == p) {
-                const int ii = t.a == 0 ? pn : i, jj = t.a == 1 ? pn : j, kk = t.a == 2 ? pn : k;
-                const size_t idn = shift(id, t.a, p, pn, ny, nz);
-                v -= q_e(d, t, ii, jj, kk, idn) * axv(A, pn, AX_MPV) * axv(A, pn, AX_IDLD);
-            }
-            acc[t.b] += v;
+// A thread keeps one (j, k) column and marches a chunk of x planes downwards, so the x term of
+// plane i + 1 is carried from the previous iteration; the y and z neighbours' terms are
+// recomputed from their cells.  Terms (component -> source, axis): t0 c0->b2 y, t1 c0->b1 z, t2 c1->b0 z,
+// t3 c1->b2 x, t4 c2->b1 x, t5 c2->b0 y.
+
+// cb of the three components at a cell (class LUT in LDS when it fits, else global; or dense)
+__device__ __forceinline__ void cb3(const MatDev &m, const float4 *s_lut, bool lds, uint32_t id,
+                                    uint32_t ncell, float &bx, float &by, float &bz) {
+    if (m.mode == 0) {
+        const uint32_t c = m.cls[id];
+        if (lds) {
+            const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
+            bx = r0.y; by = r0.w; bz = r1.y;
+        } else {
+            bx = m.lut[8 * (size_t)c + 1]; by = m.lut[8 * (size_t)c + 3]; bz = m.lut[8 * (size_t)c + 5];
         }
-#pragma unroll
-        for (int b = 0; b < 3; ++b) d.lH[(size_t)b * d.ncell + id] += acc[b];
+    } else {
+        bx = m.cb[id]; by = m.cb[ncell + id]; bz = m.cb[2 * ncell + id];
     }
 }
 
-// E transpose, part 2 (cell-local): lpe <- b (sgn nu + lpe), lE <- ca pec lE
-__global__ void __launch_bounds__(OEM_ADJ_THREADS) k_adj_e2(AdjDev d)
+// One axis' table values at coordinate p: the CPML (a, 1/kappa) of one side, the slab index,
+// 1/dl and the mask of the difference that the transposed term uses.
+struct AxAt {
+    float a, ik, idl, m, pec;
+    int s;
+};
+__device__ __forceinline__ AxAt ax_at(const AxisDev &A, int n, int p, int fa, int fk, int fidl,
+                                      int fm) {
+    AxAt r;
+    r.s = slab_of(p, n, A.lo, A.hi);
+    r.a = axv(A, p, fa); r.ik = axv(A, p, fk);
+    r.idl = axv(A, p, fidl); r.m = axv(A, p, fm);
+    r.pec = axv(A, p, AX_PEC);
+    return r;
+}
+
+// q of one transposed term: (1/kappa + a) sgn x + a psi in the slab, sgn x outside
+__device__ __forceinline__ float q_of(float sgn_x, const AxAt &t, float sgn, float x, float psi) {
+    return t.s >= 0 ? (t.ik + t.a) * sgn * x + t.a * psi : sgn_x;
+}
+
+__device__ __forceinline__ void stage_lut_adj(float4 *s_lut, const MatDev &mat) {
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const float4 *src = reinterpret_cast<const float4 *>(mat.lut);
+    for (int q = tid; q < 2 * mat.n_classes; q += blockDim.x * blockDim.y) s_lut[q] = src[q];
+}
+
+__global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
 {
-    const int ny = d.g.ny, nz = d.g.nz;
-    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < d.ncell;
-         id += (size_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        cell_ijk(id, ny, nz, i, j, k);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const Term t = kTerms[q];
-            const long long pi = psi_at(d, t.slot, i, j, k);
-            if (pi < 0) continue;
-            const AxisDev &A = d.g.ax[t.a];
-            const int p = t.a == 0 ? i : (t.a == 1 ? j : k);
-            const float nu = cls_cb(d.mat, t.c, id, d.ncell) * pec_of(d.g, t.c, i, j, k) *
-                             d.lE[(size_t)t.c * d.ncell + id];
-            d.lpe[pi] = axv(A, p, AX_BE) * ((float)t.sgn * nu + d.lpe[pi]);
+    __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
+    const MatDev &mat = d.mat;
+    const bool lds = lut_in_lds(mat);
+    if (lds) {
+        stage_lut_adj(s_lut, mat);
+        __syncthreads();
+    }
+    const GridDev &g = d.g;
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (k >= nz || j >= ny) return;
+    const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
+    const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
+    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+    const uint32_t *off = d.psl.off;
+    // y and z at the cell and at the forward neighbours Y.nxt[j], Z.nxt[k]
+    const AxAt yc = ax_at(Y, ny, j, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+    const AxAt zc = ax_at(Z, nz, k, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+    const int jn = Y.nxt[j], kn = Z.nxt[k];
+    const bool y_lo = Y.prv[j] == j, z_lo = Z.prv[k] == k;
+    const bool y_nb = jn != j && Y.prv[jn] == j, z_nb = kn != k && Z.prv[kn] == k;
+    const AxAt yn = ax_at(Y, ny, jn, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+    const AxAt zn = ax_at(Z, nz, kn, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+
+    // the x terms (t3: c1, sgn -1, slot 0; t4: c2, sgn +1, slot 1) of a cell on plane i
+    auto qx = [&](int i, float &q3, float &q4) {
+        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const AxAt xc = ax_at(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+        float bx, by, bz;
+        cb3(mat, s_lut, lds, id, nc, bx, by, bz);
+        const float nu1 = by * (zc.pec * xc.pec) * d.lE[nc + id];
+        const float nu2 = bz * (xc.pec * yc.pec) * d.lE[2 * nc + id];
+        float p0 = 0.f, p1 = 0.f;
+        if (xc.s >= 0) {
+            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
+            p0 = d.lpe[off[0] + o];
+            p1 = d.lpe[off[1] + o];
         }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float &l = d.lE[(size_t)c * d.ncell + id];
-            l = cls_ca(d.mat, c, id, d.ncell) * pec_of(d.g, c, i, j, k) * l;
+        q3 = q_of(-1.f * nu1, xc, -1.f, nu1, p0);
+        q4 = q_of(1.f * nu2, xc, 1.f, nu2, p1);
+        (void)bx;
+    };
+    // carried x terms of the plane above the current one
+    float q3n = 0.f, q4n = 0.f;
+    bool x_nb_top = false;
+    {
+        int inb = i1;
+        if (i1 < nx) {
+            x_nb_top = true;
+        } else {
+            inb = X.nxt[nx - 1];
+            x_nb_top = inb != nx - 1 && X.prv[inb] == nx - 1;
         }
+        if (x_nb_top) qx(inb, q3n, q4n);
+    }
+    for (int i = i1 - 1; i >= i0; --i) {
+        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const AxAt xc = ax_at(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+        float bx, by, bz;
+        cb3(mat, s_lut, lds, id, nc, bx, by, bz);
+        const float l0 = d.lE[id], l1 = d.lE[nc + id], l2 = d.lE[2 * nc + id];
+        // nu_c = cb_c pec_c lE_c
+        const float nu0 = bx * (yc.pec * zc.pec) * l0;
+        const float nu1 = by * (zc.pec * xc.pec) * l1;
+        const float nu2 = bz * (xc.pec * yc.pec) * l2;
+        float py0 = 0.f, py1 = 0.f, pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f;
+        if (yc.s >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yc.s) * (uint32_t)nz + (uint32_t)k;
+            py0 = d.lpe[off[2] + o]; py1 = d.lpe[off[3] + o];
+        }
+        if (zc.s >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zc.s;
+            pz0 = d.lpe[off[4] + o]; pz1 = d.lpe[off[5] + o];
+        }
+        if (xc.s >= 0) {
+            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
+            px0 = d.lpe[off[0] + o]; px1 = d.lpe[off[1] + o];
+        }
+        const float q0 = q_of(1.f * nu0, yc, 1.f, nu0, py0);      // t0: c0, y, slot 2
+        const float q5 = q_of(-1.f * nu2, yc, -1.f, nu2, py1);    // t5: c2, y, slot 3
+        const float q1 = q_of(-1.f * nu0, zc, -1.f, nu0, pz0);    // t1: c0, z, slot 4
+        const float q2 = q_of(1.f * nu1, zc, 1.f, nu1, pz1);      // t2: c1, z, slot 5
+        const float q3 = q_of(-1.f * nu1, xc, -1.f, nu1, px0);    // t3: c1, x, slot 0
+        const float q4 = q_of(1.f * nu2, xc, 1.f, nu2, px1);      // t4: c2, x, slot 1
+        // neighbours' terms: y (t0, t5) at Y.nxt[j], z (t1, t2) at Z.nxt[k], x carried
+        float q0n = 0.f, q5n = 0.f, q1n = 0.f, q2n = 0.f;
+        if (y_nb) {
+            const uint32_t idn = (uint32_t)i * plane + (uint32_t)(jn * nz + k);
+            float b0, b1, b2;
+            cb3(mat, s_lut, lds, idn, nc, b0, b1, b2);
+            const float n0 = b0 * (yn.pec * zc.pec) * d.lE[idn];
+            const float n2 = b2 * (xc.pec * yn.pec) * d.lE[2 * nc + idn];
+            float a0 = 0.f, a1 = 0.f;
+            if (yn.s >= 0) {
+                const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yn.s) * (uint32_t)nz + (uint32_t)k;
+                a0 = d.lpe[off[2] + o]; a1 = d.lpe[off[3] + o];
+            }
+            q0n = q_of(1.f * n0, yn, 1.f, n0, a0);
+            q5n = q_of(-1.f * n2, yn, -1.f, n2, a1);
+            (void)b1;
+        }
+        if (z_nb) {
+            const uint32_t idn = (uint32_t)i * plane + (uint32_t)(j * nz + kn);
+            float b0, b1, b2;
+            cb3(mat, s_lut, lds, idn, nc, b0, b1, b2);
+            const float n0 = b0 * (yc.pec * zn.pec) * d.lE[idn];
+            const float n1 = b1 * (zn.pec * xc.pec) * d.lE[nc + idn];
+            float a0 = 0.f, a1 = 0.f;
+            if (zn.s >= 0) {
+                const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zn.s;
+                a0 = d.lpe[off[4] + o]; a1 = d.lpe[off[5] + o];
+            }
+            q1n = q_of(-1.f * n0, zn, -1.f, n0, a0);
+            q2n = q_of(1.f * n1, zn, 1.f, n1, a1);
+            (void)b2;
+        }
+        const bool x_lo = X.prv[i] == i;
+        const bool x_nb = i + 1 < i1 || x_nb_top;
+        // the neighbour's mask and 1/dl along x: plane i + 1, or the wrap plane at the top
+        const int ixn = i + 1 < nx ? i + 1 : X.nxt[nx - 1];
+        const float mxn = axv(X, ixn, AX_MPV), ixdn = axv(X, ixn, AX_IDLD);
+        auto term = [&](float q, const AxAt &c, bool lo, bool nb, float qn, float mn, float idn) {
+            float v = q * c.idl;
+            if (lo) v -= q * c.m * c.idl;
+            if (nb) v -= qn * mn * idn;
+            return v;
+        };
+        const float v0 = term(q0, yc, y_lo, y_nb, q0n, yn.m, yn.idl);
+        const float v1 = term(q1, zc, z_lo, z_nb, q1n, zn.m, zn.idl);
+        const float v2 = term(q2, zc, z_lo, z_nb, q2n, zn.m, zn.idl);
+        const float v3 = term(q3, xc, x_lo, x_nb, q3n, mxn, ixdn);
+        const float v4 = term(q4, xc, x_lo, x_nb, q4n, mxn, ixdn);
+        const float v5 = term(q5, yc, y_lo, y_nb, q5n, yn.m, yn.idl);
+        float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+        acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
+        d.lH[id] += acc0;
+        d.lH[nc + id] += acc1;
+        d.lH[2 * nc + id] += acc2;
+        q3n = q3;
+        q4n = q4;
+    }
+}
+
+// E transpose, part 2 (cell-local): lpe <- b (sgn nu + lpe), lE <- ca pec lE; a column per
+// thread over a chunk of x planes, the column's y / z values loaded once
+__global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e2(AdjDev d)
+{
+    __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
+    const MatDev &mat = d.mat;
+    const bool lds = lut_in_lds(mat);
+    if (lds) {
+        stage_lut_adj(s_lut, mat);
+        __syncthreads();
+    }
+    const GridDev &g = d.g;
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (k >= nz || j >= ny) return;
+    const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
+    const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
+    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+    const uint32_t *off = d.psl.off;
+    const int sy = slab_of(j, ny, Y.lo, Y.hi), sz = slab_of(k, nz, Z.lo, Z.hi);
+    const float pecy = axv(Y, j, AX_PEC), pecz = axv(Z, k, AX_PEC);
+    const float by = sy >= 0 ? axv(Y, j, AX_BE) : 0.f, bz = sz >= 0 ? axv(Z, k, AX_BE) : 0.f;
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const int sx = slab_of(i, nx, X.lo, X.hi);
+        const float pecx = axv(X, i, AX_PEC);
+        float ca0, ca1, ca2, cb0, cb1, cb2;
+        if (mat.mode == 0) {
+            const uint32_t c = mat.cls[id];
+            if (lds) {
+                const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
+                ca0 = r0.x; cb0 = r0.y; ca1 = r0.z; cb1 = r0.w; ca2 = r1.x; cb2 = r1.y;
+            } else {
+                const float *r = mat.lut + 8 * (size_t)c;
+                ca0 = r[0]; cb0 = r[1]; ca1 = r[2]; cb1 = r[3]; ca2 = r[4]; cb2 = r[5];
+            }
+        } else {
+            ca0 = mat.ca[id]; ca1 = mat.ca[nc + id]; ca2 = mat.ca[2 * nc + id];
+            cb0 = mat.cb[id]; cb1 = mat.cb[nc + id]; cb2 = mat.cb[2 * nc + id];
+        }
+        // the pec factor of each component: x (py pz), y (pz px), z (px py)
+        const float p0 = pecy * pecz, p1 = pecz * pecx, p2 = pecx * pecy;
+        const float l0 = d.lE[id], l1 = d.lE[nc + id], l2 = d.lE[2 * nc + id];
+        const float nu0 = cb0 * p0 * l0, nu1 = cb1 * p1 * l1, nu2 = cb2 * p2 * l2;
+        // terms (component, sign, slot): x slab (Ey -, 0), (Ez +, 1); y slab (Ex +, 2), (Ez -, 3);
+        // z slab (Ex -, 4), (Ey +, 5)
+        if (sx >= 0) {
+            const float bx = axv(X, i, AX_BE);
+            const uint32_t o = ((uint32_t)sx * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
+            float &a0 = d.lpe[off[0] + o], &a1 = d.lpe[off[1] + o];
+            a0 = bx * (-1.f * nu1 + a0);
+            a1 = bx * (1.f * nu2 + a1);
+        }
+        if (sy >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)sy) * (uint32_t)nz + (uint32_t)k;
+            float &a0 = d.lpe[off[2] + o], &a1 = d.lpe[off[3] + o];
+            a0 = by * (1.f * nu0 + a0);
+            a1 = by * (-1.f * nu2 + a1);
+        }
+        if (sz >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)sz;
+            float &a0 = d.lpe[off[4] + o], &a1 = d.lpe[off[5] + o];
+            a0 = bz * (-1.f * nu0 + a0);
+            a1 = bz * (1.f * nu1 + a1);
+        }
+        d.lE[id] = ca0 * p0 * l0;
+        d.lE[nc + id] = ca1 * p1 * l1;
+        d.lE[2 * nc + id] = ca2 * p2 * l2;
     }
 }
 
@@ -168,52 +308,172 @@ __global__ void __launch_bounds__(OEM_ADJ_THREADS) k_adj_e2(AdjDev d)
 // D_fwd f[p] = (f[nxt p] mnx[p] - f[p]) idp[p]  =>  (D_fwd^T q)[p] = -q[p] idp[p]
 //   + sum over p' with nxt[p'] == p of q[p'] mnx[p'] idp[p']  (p' = prv p, or p itself at a
 //   clamped end)
-__global__ void __launch_bounds__(OEM_ADJ_THREADS) k_adj_h1(AdjDev d)
+// The march runs upwards in x, carrying the x terms of plane i - 1; the y and z neighbours
+// (Y.prv[j], Z.prv[k]) are recomputed.  q = (1/kappa + a) f + a psi_H, f = -ch sgn lH_c.
+__global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
 {
-    const int ny = d.g.ny, nz = d.g.nz;
-    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < d.ncell;
-         id += (size_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        cell_ijk(id, ny, nz, i, j, k);
-        float acc[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const Term t = kTerms[q];
-            const AxisDev &A = d.g.ax[t.a];
-            const int p = t.a == 0 ? i : (t.a == 1 ? j : k);
-            const float qs = q_h(d, t, i, j, k, id);
-            float v = -qs * axv(A, p, AX_IDLP);
-            if (A.nxt[p] == p) v += qs * axv(A, p, AX_MNX) * axv(A, p, AX_IDLP);   // clamped end
-            const int pp = A.prv[p];
-            if (pp != p && A.nxt[pp] == p) {
-                const int ii = t.a == 0 ? pp : i, jj = t.a == 1 ? pp : j, kk = t.a == 2 ? pp : k;
-                const size_t idp = shift(id, t.a, p, pp, ny, nz);
-                v += q_h(d, t, ii, jj, kk, idp) * axv(A, pp, AX_MNX) * axv(A, pp, AX_IDLP);
-            }
-            acc[t.b] += v;
+    const GridDev &g = d.g;
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (k >= nz || j >= ny) return;
+    const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
+    const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
+    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+    const uint32_t *off = d.psl.off;
+    const float mch = -d.ch;
+    const AxAt yc = ax_at(Y, ny, j, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+    const AxAt zc = ax_at(Z, nz, k, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+    const int jp = Y.prv[j], kp = Z.prv[k];
+    const bool y_hi = Y.nxt[j] == j, z_hi = Z.nxt[k] == k;
+    const bool y_nb = jp != j && Y.nxt[jp] == j, z_nb = kp != k && Z.nxt[kp] == k;
+    const AxAt yp = ax_at(Y, ny, jp, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+    const AxAt zp = ax_at(Z, nz, kp, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+
+    // the x terms (t3: c1, sgn -1, slot 0; t4: c2, sgn +1, slot 1) of the cell on plane i
+    auto qx = [&](int i, float &q3, float &q4) {
+        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const AxAt xc = ax_at(X, nx, i, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+        const float f1 = mch * -1.f * d.lH[nc + id], f2 = mch * 1.f * d.lH[2 * nc + id];
+        float p0 = 0.f, p1 = 0.f;
+        if (xc.s >= 0) {
+            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
+            p0 = d.lph[off[0] + o];
+            p1 = d.lph[off[1] + o];
         }
-#pragma unroll
-        for (int b = 0; b < 3; ++b) d.lE[(size_t)b * d.ncell + id] += acc[b];
+        q3 = xc.s >= 0 ? (xc.ik + xc.a) * f1 + xc.a * p0 : f1;
+        q4 = xc.s >= 0 ? (xc.ik + xc.a) * f2 + xc.a * p1 : f2;
+    };
+    float q3p = 0.f, q4p = 0.f;
+    bool x_nb_bot = false;
+    {
+        int ipb = i0 - 1;
+        if (i0 > 0) {
+            x_nb_bot = true;
+        } else {
+            ipb = X.prv[0];
+            x_nb_bot = ipb != 0 && X.nxt[ipb] == 0;
+        }
+        if (x_nb_bot) qx(ipb, q3p, q4p);
+    }
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const AxAt xc = ax_at(X, nx, i, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+        const float h0 = d.lH[id], h1 = d.lH[nc + id], h2 = d.lH[2 * nc + id];
+        // f of each term: -ch sgn lH_c
+        const float f0p = mch * 1.f * h0, f1m = mch * -1.f * h1, f2p = mch * 1.f * h2;
+        const float f0m = mch * -1.f * h0, f1p = mch * 1.f * h1, f2m = mch * -1.f * h2;
+        float py0 = 0.f, py1 = 0.f, pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f;
+        if (yc.s >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yc.s) * (uint32_t)nz + (uint32_t)k;
+            py0 = d.lph[off[2] + o]; py1 = d.lph[off[3] + o];
+        }
+        if (zc.s >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zc.s;
+            pz0 = d.lph[off[4] + o]; pz1 = d.lph[off[5] + o];
+        }
+        if (xc.s >= 0) {
+            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
+            px0 = d.lph[off[0] + o]; px1 = d.lph[off[1] + o];
+        }
+        auto qh = [](const AxAt &t, float f, float psi) {
+            return t.s >= 0 ? (t.ik + t.a) * f + t.a * psi : f;
+        };
+        const float q0 = qh(yc, f0p, py0);      // t0: c0 sgn +1, y, slot 2
+        const float q5 = qh(yc, f2m, py1);      // t5: c2 sgn -1, y, slot 3
+        const float q1 = qh(zc, f0m, pz0);      // t1: c0 sgn -1, z, slot 4
+        const float q2 = qh(zc, f1p, pz1);      // t2: c1 sgn +1, z, slot 5
+        const float q3 = qh(xc, f1m, px0);      // t3: c1 sgn -1, x, slot 0
+        const float q4 = qh(xc, f2p, px1);      // t4: c2 sgn +1, x, slot 1
+        float q0n = 0.f, q5n = 0.f, q1n = 0.f, q2n = 0.f;
+        if (y_nb) {
+            const uint32_t idn = (uint32_t)i * plane + (uint32_t)(jp * nz + k);
+            float a0 = 0.f, a1 = 0.f;
+            if (yp.s >= 0) {
+                const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yp.s) * (uint32_t)nz + (uint32_t)k;
+                a0 = d.lph[off[2] + o]; a1 = d.lph[off[3] + o];
+            }
+            q0n = qh(yp, mch * 1.f * d.lH[idn], a0);
+            q5n = qh(yp, mch * -1.f * d.lH[2 * nc + idn], a1);
+        }
+        if (z_nb) {
+            const uint32_t idn = (uint32_t)i * plane + (uint32_t)(j * nz + kp);
+            float a0 = 0.f, a1 = 0.f;
+            if (zp.s >= 0) {
+                const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zp.s;
+                a0 = d.lph[off[4] + o]; a1 = d.lph[off[5] + o];
+            }
+            q1n = qh(zp, mch * -1.f * d.lH[idn], a0);
+            q2n = qh(zp, mch * 1.f * d.lH[nc + idn], a1);
+        }
+        const bool x_hi = X.nxt[i] == i;
+        const bool x_nb = i > i0 || x_nb_bot;
+        const int ixp = i > 0 ? i - 1 : X.prv[0];
+        const float mxp = axv(X, ixp, AX_MNX), ixdp = axv(X, ixp, AX_IDLP);
+        auto term = [](float q, const AxAt &c, bool hi, bool nb, float qn, float mn, float idn) {
+            float v = -q * c.idl;
+            if (hi) v += q * c.m * c.idl;
+            if (nb) v += qn * mn * idn;
+            return v;
+        };
+        const float v0 = term(q0, yc, y_hi, y_nb, q0n, yp.m, yp.idl);
+        const float v1 = term(q1, zc, z_hi, z_nb, q1n, zp.m, zp.idl);
+        const float v2 = term(q2, zc, z_hi, z_nb, q2n, zp.m, zp.idl);
+        const float v3 = term(q3, xc, x_hi, x_nb, q3p, mxp, ixdp);
+        const float v4 = term(q4, xc, x_hi, x_nb, q4p, mxp, ixdp);
+        const float v5 = term(q5, yc, y_hi, y_nb, q5n, yp.m, yp.idl);
+        float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+        acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
+        d.lE[id] += acc0;
+        d.lE[nc + id] += acc1;
+        d.lE[2 * nc + id] += acc2;
+        q3p = q3;
+        q4p = q4;
     }
 }
 
-// H transpose, part 2 (cell-local): lph <- b (f + lph)
-__global__ void __launch_bounds__(OEM_ADJ_THREADS) k_adj_h2(AdjDev d)
+// H transpose, part 2 (cell-local): lph <- b (f + lph), f = -ch sgn lH_c; a column per thread
+// over a chunk of x planes
+__global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h2(AdjDev d)
 {
-    const int ny = d.g.ny, nz = d.g.nz;
-    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < d.ncell;
-         id += (size_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        cell_ijk(id, ny, nz, i, j, k);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const Term t = kTerms[q];
-            const long long pi = psi_at(d, t.slot, i, j, k);
-            if (pi < 0) continue;
-            const AxisDev &A = d.g.ax[t.a];
-            const int p = t.a == 0 ? i : (t.a == 1 ? j : k);
-            const float f = -d.ch * (float)t.sgn * d.lH[(size_t)t.c * d.ncell + id];
-            d.lph[pi] = axv(A, p, AX_BH) * (f + d.lph[pi]);
+    const GridDev &g = d.g;
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (k >= nz || j >= ny) return;
+    const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
+    const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
+    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+    const uint32_t *off = d.psl.off;
+    const int sy = slab_of(j, ny, Y.lo, Y.hi), sz = slab_of(k, nz, Z.lo, Z.hi);
+    if (sy < 0 && sz < 0 && X.lo == 0 && X.hi == 0) return;
+    const float by = sy >= 0 ? axv(Y, j, AX_BH) : 0.f, bz = sz >= 0 ? axv(Z, k, AX_BH) : 0.f;
+    const float mch = -d.ch;
+    for (int i = i0; i < i1; ++i) {
+        const int sx = slab_of(i, nx, X.lo, X.hi);
+        if (sx < 0 && sy < 0 && sz < 0) continue;
+        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const float h0 = d.lH[id], h1 = d.lH[nc + id], h2 = d.lH[2 * nc + id];
+        // terms as in the E part: x slab (Hy -, 0), (Hz +, 1); y (Hx +, 2), (Hz -, 3);
+        // z (Hx -, 4), (Hy +, 5)
+        if (sx >= 0) {
+            const float bx = axv(X, i, AX_BH);
+            const uint32_t o = ((uint32_t)sx * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
+            float &a0 = d.lph[off[0] + o], &a1 = d.lph[off[1] + o];
+            a0 = bx * (mch * -1.f * h1 + a0);
+            a1 = bx * (mch * 1.f * h2 + a1);
+        }
+        if (sy >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)sy) * (uint32_t)nz + (uint32_t)k;
+            float &a0 = d.lph[off[2] + o], &a1 = d.lph[off[3] + o];
+            a0 = by * (mch * 1.f * h0 + a0);
+            a1 = by * (mch * -1.f * h2 + a1);
+        }
+        if (sz >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)sz;
+            float &a0 = d.lph[off[4] + o], &a1 = d.lph[off[5] + o];
+            a0 = bz * (mch * -1.f * h0 + a0);
+            a1 = bz * (mch * 1.f * h1 + a1);
         }
     }
 }
@@ -277,6 +537,11 @@ static int adj_check(oem_ctx *ctx) {
     if (ctx->cplx || ctx->has_disp)
         return fail(OEM_ENOTSUP, "the adjoint covers real, non-dispersive scenes");
     if (ctx->pool) return fail(OEM_ENOTSUP, "pooled contexts have no adjoint");
+    if (ctx->dims.nx > 65535 || (ctx->dims.ny + OEM_ADJ_TJ - 1) / OEM_ADJ_TJ > 65535)
+        return fail(OEM_ENOTSUP, "the adjoint launch covers nx <= 65535 and ny <= 262140");
+    // the transposed sweeps index the three adjoint components with 32-bit offsets
+    if (3ull * ctx->ncell >= (1ull << 32) || ctx->psi.total >= (1u << 31))
+        return fail(OEM_ENOTSUP, "the adjoint covers up to 2^32 / 3 cells");
     if (!ctx->adj) ctx->adj = new oem_adjoint_state();
     return OEM_OK;
 }
@@ -431,8 +696,10 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
     d.mat = ctx->mat;
     d.ch = ctx->dims.ch;
     d.ncell = ctx->ncell;
-    const dim3 tb(OEM_ADJ_THREADS);
-    const dim3 tg((unsigned)std::min<size_t>(OEM_ADJ_BLOCKS, (ctx->ncell + OEM_ADJ_THREADS - 1) / OEM_ADJ_THREADS));
+    const dim3 tb(OEM_ADJ_TK, OEM_ADJ_TJ);
+    const dim3 tg((ctx->dims.nz + OEM_ADJ_TK - 1) / OEM_ADJ_TK,
+                  (ctx->dims.ny + OEM_ADJ_TJ - 1) / OEM_ADJ_TJ, ctx->dims.nx);
+    const dim3 tm(tg.x, tg.y, (ctx->dims.nx + OEM_ADJ_CHUNK - 1) / OEM_ADJ_CHUNK);
     const dim3 gb(256), gg((a->n_design + 255) / 256);
     for (int s = a->rec_n - 1; s >= 0; --s) {
         const int n = (int)(a->rec_s0 + s);          // forward step n: E^n -> E^{n+1}
@@ -444,10 +711,10 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
                            a->dca, a->dcb, a->ca, a->cb, a->rec + (size_t)s * a->n_design,
                            a->rec + (size_t)(s + 1) * a->n_design, a->n_design, ctx->ncell,
                            a->grad);
-        hipLaunchKernelGGL(k_adj_e1, tg, tb, 0, ctx->stream, d);
-        hipLaunchKernelGGL(k_adj_e2, tg, tb, 0, ctx->stream, d);
-        hipLaunchKernelGGL(k_adj_h1, tg, tb, 0, ctx->stream, d);
-        hipLaunchKernelGGL(k_adj_h2, tg, tb, 0, ctx->stream, d);
+        hipLaunchKernelGGL(k_adj_e1, tm, tb, 0, ctx->stream, d);
+        hipLaunchKernelGGL(k_adj_e2, tm, tb, 0, ctx->stream, d);
+        hipLaunchKernelGGL(k_adj_h1, tm, tb, 0, ctx->stream, d);
+        hipLaunchKernelGGL(k_adj_h2, tm, tb, 0, ctx->stream, d);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(ctx->stream));
