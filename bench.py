@@ -74,22 +74,54 @@ def measured_traffic(kernel: str):
         return None
 
 
-def cpu_baseline(n: int, steps: int = 2, nx_sample: int = 64) -> dict:
+def _oracle_sample_time(n: int, steps: int, nx_sample: int, barrier=None) -> float:
+    """Seconds for ``steps`` float64 oracle steps on the (nx_sample, n, n) box-soup sample (one
+    untimed step first); with ``barrier``, the timed steps start together with the other
+    workers'."""
     from openem_amd import scenes, tables
     from oracle.yee64 import Oracle64
     sc = scenes.box_soup_scene(shape=(nx_sample, n, n), num_steps=steps + 1)
     tabs = tables.build(sc, steps + 1)
     o = Oracle64(tabs)
     o.step()                       # first step touches every array once
+    if barrier is not None:
+        barrier.wait()
     t0 = time.perf_counter()
     o.run(steps)
-    dt = time.perf_counter() - t0
+    return time.perf_counter() - t0
+
+
+def _oracle_worker(n, steps, nx_sample, barrier, out):
+    out.put(_oracle_sample_time(n, steps, nx_sample, barrier))
+
+
+def cpu_baseline(n: int, steps: int = 2, nx_sample: int = 64, workers: int = 8) -> dict:
+    """The float64 oracle on the host: one process on one x-slab sample, and ``workers`` processes
+    stepping one x-slab each at the same time (an x-slab decomposition without its halo
+    exchange, which would add one plane per step and neighbour)."""
+    import multiprocessing as mp
+    dt = _oracle_sample_time(n, steps, nx_sample)
     cells = nx_sample * n * n
-    return {"value": round(cells * steps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"float64 NumPy oracle, {steps} timed steps on a {nx_sample}x{n}x{n} "
-                      f"x-slab of the same box-soup workload (same seed, PML, boxes, dipole, "
-                      f"monitor), single process"}
+    out = {"value": round(cells * steps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": 1,
+           "kind": "port",
+           "sample": f"float64 NumPy oracle, {steps} timed steps on a {nx_sample}x{n}x{n} "
+                     f"x-slab of the same box-soup workload (same seed, PML, boxes, dipole, "
+                     f"monitor), single process"}
+    if workers > 1:
+        ctx = mp.get_context("spawn")          # fresh interpreters: no fork of this process
+        bar, q = ctx.Barrier(workers), ctx.Queue()
+        procs = [ctx.Process(target=_oracle_worker, args=(n, steps, nx_sample, bar, q))
+                 for _ in range(workers)]
+        for pr in procs:
+            pr.start()
+        times = [q.get() for _ in procs]
+        for pr in procs:
+            pr.join()
+        out["parallel"] = {"value": round(workers * cells * steps / max(times) / 1e6, 3),
+                           "unit": "Mcell-steps/s", "cores": workers,
+                           "sample": f"{workers} processes, each the single-process sample "
+                                     f"(one {nx_sample}-plane x-slab), timed together"}
+    return out
 
 
 def serving_bench(n_req: int, arena_gib: int, workers: int) -> dict:
