@@ -241,13 +241,17 @@ H_POINT_CASES = {
     "tfsf_box": lambda n: tables.build(scenes.tfsf_box_scene(num_steps=n, scatterer=True)),
     "waveguide_mode": lambda n: tables.build(scenes.waveguide_scene(
         shape=(30, 28, 26), n_pml=6, width_cells=8, thick_cells=4, num_steps=n, nf=1)),
+    # complex fields: Bloch x/y wraps, the oblique TF/SF box's H faces, Drude + Lorentz metal
+    "bloch_tfsf_complex": lambda n: tables.build(scenes.bloch_cell_scene(nxy=10, nz=40, n_pml=6,
+                                                                         num_steps=n, nf=2)),
 }
 
 
 @pytest.mark.parametrize("name", sorted(H_POINT_CASES))
 def test_fused_step_with_h_point_lists_matches_the_two_sweeps(gpu_lib, monkeypatch, name):
-    """With H-side point corrections the fused step agrees with the two sweeps to float32
-    rounding (1e-5 of the peak after 150 steps) in fields and phasors."""
+    """With H-side point corrections the fused step rounds differently at the corrected cells;
+    both paths stay within the parity bound of the float64 oracle (1e-4 of the peak after 150
+    steps) and within 5e-5 of each other, in fields and phasors."""
     n = 150
     tabs = H_POINT_CASES[name](n)
     print(f"fused vs two sweeps with H points: {name} {tabs.shape}", flush=True)
@@ -255,10 +259,17 @@ def test_fused_step_with_h_point_lists_matches_the_two_sweeps(gpu_lib, monkeypat
     f_fused, acc_fused = _gpu_state(tabs, n)
     monkeypatch.setenv("OEM_FUSE", "0")
     f_two, acc_two = _gpu_state(tabs, n)
-    peak = np.abs(f_two).max()
+    o = Oracle64(tabs).run(n)
+    ref = np.concatenate([o.E, o.H])
+    peak = np.abs(ref).max()
     assert peak > 0
-    errs = [float(np.abs(f_fused[c] - f_two[c]).max() / peak) for c in range(6)]
-    print("per component:", errs, flush=True)
-    assert max(errs) < 1e-5, errs
-    for a, b in zip(acc_fused, acc_two):
-        assert _rel(a, b) < 1e-5
+    for label, f in (("fused", f_fused), ("two sweeps", f_two)):
+        errs = [float(np.abs(f[c] - ref[c]).max() / peak) for c in range(6)]
+        print(f"{label} vs oracle, per component:", errs, flush=True)
+        assert max(errs) < 1e-4, (label, errs)
+    diff = [float(np.abs(f_fused[c] - f_two[c]).max() / peak) for c in range(6)]
+    print("fused vs two sweeps, per component:", diff, flush=True)
+    assert max(diff) < 5e-5, diff
+    for a, b, r in zip(acc_fused, acc_two, o.acc):
+        assert _rel(a, r) < 1e-4 and _rel(b, r) < 1e-4
+        assert _rel(a, b) < 5e-5
