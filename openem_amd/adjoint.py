@@ -130,3 +130,13 @@ def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
     finally:
         st.close()
     return Gradient(J=abs(a) ** 2, a_plus=a, grad=out, steps=n, segments=n_seg)
+
+
+def mode_power_gradients(jobs, concurrency: int = 4, **kw) -> list:
+    """:func:`mode_power_gradient` for many designs, ``concurrency`` of them in flight at once:
+    each ``(scene, monitor, comp, cell)`` job has its own context (its own HIP stream, fields,
+    checkpoints and adjoint state), so one design's host work and launch gaps overlap another's
+    kernels.  Results are in job order."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, int(concurrency))) as ex:
+        return list(ex.map(lambda j: mode_power_gradient(*j, **kw), jobs))
