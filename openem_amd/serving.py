@@ -12,8 +12,8 @@ with the same launches (batched kernels over every resident scene in a hipGraph 
   of blocks in flight and hands finished scenes to the readout, which reads the phasors,
   computes the fluxes and releases the scene's memory;
 * scenes of at least ``solo_cells`` cells fill the GPU by themselves: they run in a second lane,
-  one at a time as standalone contexts (fused H+E step, hipGraph blocks) on their own stream,
-  concurrently with the batched small scenes.
+  one at a time as standalone contexts (hipGraph blocks; the fused H+E step from 12 M cells) on
+  their own stream, concurrently with the batched small scenes.
 
 :func:`serve_serial` is the plain loop the batched form is measured against: one
 :func:`openem_amd.solver.run` per request, in arrival order.
