@@ -285,6 +285,12 @@ int oem_adjoint_objective(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const i
                           const int32_t *cell, const float *z, const double *tab);
 /* Recompute n_steps forward from the loaded checkpoint (no DFT), recording E at the design. */
 int oem_adjoint_record(oem_ctx *ctx, int64_t n_steps);
+/* The forward run itself, n_steps with the monitors (as oem_run), recording E at the design
+ * after every step, so no checkpoint or recomputation is needed when the record ((n_steps + 1)
+ * x n_design floats, allocated by oem_adjoint_setup's max_seg) fits in memory.  The
+ * reference's counterpart is the same: tidy3d's forward run feeds its adjoint
+ * (`openem/nb/autograd_hook.py:1-28`). */
+int oem_adjoint_forward(oem_ctx *ctx, int64_t n_steps);
 /* The transposed steps over the recorded segment, last step first. */
 int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps);
 /* dJ/d eps of every design entry (float64). */

@@ -34,7 +34,7 @@ EXPORTS = ("oem_version", "oem_last_error", "oem_device_count", "oem_ctx_create"
            "oem_energy", "oem_set_flux_weights", "oem_monitor_flux", "oem_set_overlap_weights",
            "oem_monitor_overlap", "oem_checkpoint_alloc", "oem_checkpoint_save",
            "oem_checkpoint_load", "oem_adjoint_setup", "oem_adjoint_objective",
-           "oem_adjoint_record", "oem_adjoint_back", "oem_adjoint_gradient", "oem_pool_create",
+           "oem_adjoint_record", "oem_adjoint_forward", "oem_adjoint_back", "oem_adjoint_gradient", "oem_pool_create",
            "oem_pool_destroy", "oem_pool_ctx_create", "oem_pool_attach", "oem_pool_launch",
            "oem_pool_ctx_done", "oem_pool_wait", "oem_pool_stats", "oem_host_factorize")
 
@@ -105,6 +105,7 @@ _SIGS = {
     "oem_adjoint_setup": ([_P, C.c_int32] + [_P] * 6 + [C.c_int32], C.c_int),
     "oem_adjoint_objective": ([_P, C.c_int32, C.c_int32, _P, _P, _P, _P], C.c_int),
     "oem_adjoint_record": ([_P, C.c_int64], C.c_int),
+    "oem_adjoint_forward": ([_P, C.c_int64], C.c_int),
     "oem_adjoint_back": ([_P, C.c_int64], C.c_int),
     "oem_adjoint_gradient": ([_P, _P, C.c_size_t], C.c_int),
     "oem_pool_create": ([C.c_int32, C.c_uint64, C.c_int32, _P], C.c_int),

@@ -10,6 +10,10 @@ MI355X (``csrc/oem_adjoint.hip``) with segmented checkpoint recomputation:
    design cells, and step the adjoint back over it, accumulating
    ``dJ/d eps = sum_n lambda_E^{n+1} (ca' E^n + cb' S^n)``.
 
+When E at the design cells for every step fits in ``record_bytes`` of device memory, the forward
+run records it directly and the adjoint steps back over the whole run at once: no checkpoints
+and no recomputation (the adjoint recursion itself needs no forward fields).
+
 The gradient is of the discrete objective the stepper computes, so it agrees with central
 finite differences of the same stepper to their truncation error.
 """
@@ -68,9 +72,11 @@ def _plane_cells(shape, axis: int, k: int) -> np.ndarray:
 
 def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
                         checkpoint_every: int = 500, n_total: int | None = None,
-                        device: int = 0) -> Gradient:
+                        device: int = 0, record_bytes: int = 24 << 30) -> Gradient:
     """``J = |a+|^2`` of mode monitor ``monitor`` at frequency ``f_index`` and ``dJ/d eps`` at the
-    design entries (``comp`` 0..2, flat ``cell``)."""
+    design entries (``comp`` 0..2, flat ``cell``).  With ``record_bytes`` too small for the whole
+    record, the forward run is checkpointed every ``checkpoint_every`` steps and recomputed
+    segment by segment."""
     tabs = tables_mod.build(sc, n_total)
     names = [m.name for m in tabs.monitors]
     mi = names.index(monitor)
@@ -80,15 +86,23 @@ def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
     comp = np.ascontiguousarray(comp, dtype=np.int32)
     cell = np.ascontiguousarray(cell, dtype=np.int32)
     n = tabs.n_total
-    K = max(1, int(checkpoint_every))
+    one_pass = (n + 1) * comp.size * 4 <= record_bytes
+    K = n if one_pass else max(1, int(checkpoint_every))
     n_seg = (n + K - 1) // K
+    dca, dcb, ca, cb = design_coefficients(tabs, comp, cell)
     st = Stepper(tabs, device=device)
     lib, ctx = st.lib, st.ctx
     try:
-        _lib.check(lib, lib.oem_checkpoint_alloc(ctx, n_seg))
-        for s in range(n_seg):
-            _lib.check(lib, lib.oem_checkpoint_save(ctx, s))
-            st.run(min(K, n - s * K))
+        _lib.check(lib, lib.oem_adjoint_setup(ctx, comp.size, _lib.ptr(comp), _lib.ptr(cell),
+                                              _lib.ptr(dca), _lib.ptr(dcb), _lib.ptr(ca),
+                                              _lib.ptr(cb), K))
+        if one_pass:
+            _lib.check(lib, lib.oem_adjoint_forward(ctx, n))
+        else:
+            _lib.check(lib, lib.oem_checkpoint_alloc(ctx, n_seg))
+            for s in range(n_seg):
+                _lib.check(lib, lib.oem_checkpoint_save(ctx, s))
+                st.run(min(K, n - s * K))
         st.setup_modes()
         ap, _ = st.mode_amplitudes(mi)
         a = complex(ap[f_index])
@@ -102,10 +116,6 @@ def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
         z = 2.0 * np.conj(a) * Kw                         # (4, n1, n2)
         cells_k = _plane_cells(tabs.shape, m.axis, m.km)
         cells_km = _plane_cells(tabs.shape, m.axis, m.km - 1)
-        dca, dcb, ca, cb = design_coefficients(tabs, comp, cell)
-        _lib.check(lib, lib.oem_adjoint_setup(ctx, comp.size, _lib.ptr(comp), _lib.ptr(cell),
-                                              _lib.ptr(dca), _lib.ptr(dcb), _lib.ptr(ca),
-                                              _lib.ptr(cb), K))
         for magnetic, slot, fcomp, planes, tab, w in (
                 (0, 0, t1, (cells_k,), tab_e, 1.0), (0, 1, t2, (cells_k,), tab_e, 1.0),
                 (1, 2, t1, (cells_k, cells_km), tab_h, 0.5), (1, 3, t2, (cells_k, cells_km), tab_h, 0.5)):
@@ -120,11 +130,14 @@ def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
                 _lib.check(lib, lib.oem_adjoint_objective(ctx, magnetic, keep.size, _lib.ptr(pc),
                                                           _lib.ptr(pcell), _lib.ptr(zz),
                                                           _lib.ptr(tab)))
-        for s in range(n_seg - 1, -1, -1):
-            ln = min(K, n - s * K)
-            _lib.check(lib, lib.oem_checkpoint_load(ctx, s))
-            _lib.check(lib, lib.oem_adjoint_record(ctx, ln))
-            _lib.check(lib, lib.oem_adjoint_back(ctx, ln))
+        if one_pass:
+            _lib.check(lib, lib.oem_adjoint_back(ctx, n))
+        else:
+            for s in range(n_seg - 1, -1, -1):
+                ln = min(K, n - s * K)
+                _lib.check(lib, lib.oem_checkpoint_load(ctx, s))
+                _lib.check(lib, lib.oem_adjoint_record(ctx, ln))
+                _lib.check(lib, lib.oem_adjoint_back(ctx, ln))
         out = np.zeros(comp.size, np.float64)
         _lib.check(lib, lib.oem_adjoint_gradient(ctx, _lib.ptr(out), out.size))
     finally:

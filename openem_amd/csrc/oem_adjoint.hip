@@ -683,6 +683,39 @@ extern "C" int oem_adjoint_record(oem_ctx *ctx, int64_t n_steps) {
     return OEM_OK;
 }
 
+extern "C" int oem_adjoint_forward(oem_ctx *ctx, int64_t n_steps) {
+    CHECK_CTX(ctx);
+    TRY(adj_check(ctx));
+    oem_adjoint_state *a = ctx->adj;
+    if (!a->rec) return fail(OEM_ESTATE, "oem_adjoint_setup not called");
+    if (n_steps < 1 || n_steps > a->max_seg) return fail(OEM_EINVAL, "segment of %lld steps", (long long)n_steps);
+    if (ctx->step + n_steps > ctx->dims.n_steps) return fail(OEM_EINVAL, "beyond the step tables");
+    // the cached step graph is captured with or without the recording: drop it on both sides
+    if (ctx->graph) {
+        (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    const int64_t s0 = ctx->step;
+    hipLaunchKernelGGL(k_adj_gather, dim3((a->n_design + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->cur_E(), a->d_comp, a->d_cell, a->n_design, ctx->ncell, a->rec);
+    ctx->rec_hook.comp = a->d_comp;
+    ctx->rec_hook.cell = a->d_cell;
+    ctx->rec_hook.n = a->n_design;
+    ctx->rec_hook.rec = a->rec;
+    ctx->rec_hook.base = s0;
+    ctx->rec_hook.on = true;
+    const int rc = oem_run(ctx, n_steps);
+    ctx->rec_hook.on = false;
+    if (ctx->graph) {
+        (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    TRY(rc);
+    a->rec_s0 = s0;
+    a->rec_n = (int)n_steps;
+    return OEM_OK;
+}
+
 extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
     CHECK_CTX(ctx);
     TRY(adj_check(ctx));

@@ -142,6 +142,15 @@ struct oem_ctx {
     bool prepared = false;
     int chunk_req = 0;          // x-planes per work-group requested (0: heuristic)
     bool mons_off = false;      // adjoint recomputation: no DFT snapshots or flushes
+    // adjoint forward recording (oem_adjoint_forward): after each step, E at the design cells
+    // goes to row (step + 1 - base) of rec
+    struct RecHook {
+        bool on = false;
+        const int *comp = nullptr, *cell = nullptr;
+        int n = 0;
+        float *rec = nullptr;
+        int64_t base = 0;
+    } rec_hook;
     struct oem_adjoint_state *adj = nullptr;
     hipGraphExec_t graph = nullptr;
     std::vector<void *> allocs;

@@ -899,6 +899,20 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
     }
 }
 
+// E at the design cells of an adjoint after the step: row (step + 1 - base) of rec (the step read
+// from the device counter, so a captured step graph records every step it replays).
+template <typename F>
+__global__ void __launch_bounds__(256)
+k_rec_gather(const F *__restrict__ E, const int *__restrict__ comp,
+             const int *__restrict__ cell, int n, size_t ncell, F *__restrict__ rec,
+             const int *__restrict__ stp, int64_t base)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t row = (int64_t)step_now(stp) + 1 - base;
+    rec[(size_t)row * n + q] = E[(size_t)comp[q] * ncell + cell[q]];
+}
+
 // ------------------------------------------------------------------------------------------
 // Grouped point corrections: one thread per distinct (component, cell), its points applied in
 // input order (no atomics, deterministic).  Used for

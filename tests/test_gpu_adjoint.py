@@ -54,8 +54,23 @@ def test_checkpoint_segments_do_not_change_the_gradient(gpu_lib):
     t = tables.build(sc)
     mon = [m.name for m in t.monitors].index("fwd")
     comp, cell = _design(sc, t, mon, count=6, seed=9)
-    one = adjoint.mode_power_gradient(sc, "fwd", comp, cell, checkpoint_every=200)
-    seg = adjoint.mode_power_gradient(sc, "fwd", comp, cell, checkpoint_every=48)
+    one = adjoint.mode_power_gradient(sc, "fwd", comp, cell, checkpoint_every=200, record_bytes=0)
+    seg = adjoint.mode_power_gradient(sc, "fwd", comp, cell, checkpoint_every=48, record_bytes=0)
     assert seg.segments == 5 and one.segments == 1
     assert np.allclose(seg.grad, one.grad, rtol=1e-12, atol=0)
     assert seg.J == one.J
+
+
+def test_recording_forward_run_equals_checkpoint_recomputation(gpu_lib):
+    """Recording E at the design during the forward run (no checkpoints) gives the gradient of
+    checkpointed segments recomputed one by one, bit for bit: the same steps, recorded values and
+    transposed steps."""
+    sc = _guide(200)
+    t = tables.build(sc)
+    mon = [m.name for m in t.monitors].index("fwd")
+    comp, cell = _design(sc, t, mon, count=6, seed=9)
+    rec = adjoint.mode_power_gradient(sc, "fwd", comp, cell)
+    seg = adjoint.mode_power_gradient(sc, "fwd", comp, cell, checkpoint_every=48, record_bytes=0)
+    assert rec.segments == 1 and seg.segments == 5
+    assert rec.J == seg.J
+    assert np.array_equal(rec.grad, seg.grad)

@@ -79,7 +79,8 @@ def test_adjoint_checkpoints_restore_either_set(gpu_lib, monkeypatch):
                      + int(rng.integers(8, 14)) for _ in range(6)])
 
     def run(every):
-        return adjoint.mode_power_gradient(sc, "fwd", comp, cell, checkpoint_every=every)
+        return adjoint.mode_power_gradient(sc, "fwd", comp, cell, checkpoint_every=every,
+                                           record_bytes=0)
     monkeypatch.setenv("OEM_FUSE", "1")
     seg = run(47)
     one = run(200)
