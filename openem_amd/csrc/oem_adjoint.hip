@@ -8,7 +8,9 @@
 // E^{n+1} and H^{n+1/2} (monitor samples), the eps gradient lambda_E . (ca' E^n + cb' S^n), the
 // transpose of the E update (gather from the forward neighbour along each axis, then the
 // cell-local scaling), and the transpose of the H update (gather from the backward neighbour).
-// Each transpose is two kernels so that no kernel reads a value another thread overwrites.
+// Each transpose is one kernel that reads the old adjoint fields and writes the new cell-local
+// ones (lambda_E, the psi adjoints) to a second buffer, so no thread reads a value another
+// overwrites; the buffers swap after each kernel.
 //
 // Each term of an update: (target component c, source component b, derivative axis a, sign,
 // psi slot).  psi slot = off[2 a + m] of the slab layout (x: Hy/Ey 0, Hz/Ez 1; y: Hx/Ex 0,
@@ -16,8 +18,8 @@
 
 #include "oem_internal.h"
 
-// the transposed sweeps: a (k, j) column per thread from a 64 x 4 block; the transposes of the
-// curl march OEM_ADJ_CHUNK x planes per block (grid z), the cell-local parts one plane
+// the transposed sweeps: a (k, j) column per thread from a 64 x 4 block, marching OEM_ADJ_CHUNK
+// x planes per block (grid z)
 #define OEM_ADJ_TK 64
 #define OEM_ADJ_TJ 4
 #define OEM_ADJ_CHUNK 16
@@ -27,6 +29,7 @@ namespace {
 struct AdjDev {
     float *lE, *lH;              // 3 * ncell each
     float *lpe, *lph;            // psi adjoints, slab layout of PsiDev
+    float *lE_n, *lpe_n, *lph_n; // where the transposes write the new lE, lpe, lph (ping-pong)
     PsiDev psl;                  // offsets (pointers unused)
     GridDev g;
     MatDev mat;
@@ -58,11 +61,26 @@ __device__ __forceinline__ void cb3(const MatDev &m, const float4 *s_lut, bool l
         bx = m.cb[id]; by = m.cb[ncell + id]; bz = m.cb[2 * ncell + id];
     }
 }
+// ca of the three components at a cell
+__device__ __forceinline__ void ca3(const MatDev &m, const float4 *s_lut, bool lds, uint32_t id,
+                                    uint32_t ncell, float &ax, float &ay, float &az) {
+    if (m.mode == 0) {
+        const uint32_t c = m.cls[id];
+        if (lds) {
+            const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
+            ax = r0.x; ay = r0.z; az = r1.x;
+        } else {
+            ax = m.lut[8 * (size_t)c]; ay = m.lut[8 * (size_t)c + 2]; az = m.lut[8 * (size_t)c + 4];
+        }
+    } else {
+        ax = m.ca[id]; ay = m.ca[ncell + id]; az = m.ca[2 * ncell + id];
+    }
+}
 
 // One axis' table values at coordinate p: the CPML (a, 1/kappa) of one side, the slab index,
 // 1/dl and the mask of the difference that the transposed term uses.
 struct AxAt {
-    float a, ik, idl, m, pec;
+    float a, b, ik, idl, m, pec;
     int s;
 };
 __device__ __forceinline__ AxAt ax_at(const AxisDev &A, int n, int p, int fa, int fk, int fidl,
@@ -70,6 +88,7 @@ __device__ __forceinline__ AxAt ax_at(const AxisDev &A, int n, int p, int fa, in
     AxAt r;
     r.s = slab_of(p, n, A.lo, A.hi);
     r.a = axv(A, p, fa); r.ik = axv(A, p, fk);
+    r.b = axv(A, p, fa == AX_AE ? AX_BE : AX_BH);
     r.idl = axv(A, p, fidl); r.m = axv(A, p, fm);
     r.pec = axv(A, p, AX_PEC);
     return r;
@@ -227,80 +246,30 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         d.lH[id] += acc0;
         d.lH[nc + id] += acc1;
         d.lH[2 * nc + id] += acc2;
+        // the cell-local part, into the other buffers (neighbours read the old values):
+        // lpe <- b (sgn nu + lpe) per slab term, lE <- ca pec lE
+        if (xc.s >= 0) {
+            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
+            d.lpe_n[off[0] + o] = xc.b * (-1.f * nu1 + px0);
+            d.lpe_n[off[1] + o] = xc.b * (1.f * nu2 + px1);
+        }
+        if (yc.s >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yc.s) * (uint32_t)nz + (uint32_t)k;
+            d.lpe_n[off[2] + o] = yc.b * (1.f * nu0 + py0);
+            d.lpe_n[off[3] + o] = yc.b * (-1.f * nu2 + py1);
+        }
+        if (zc.s >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zc.s;
+            d.lpe_n[off[4] + o] = zc.b * (-1.f * nu0 + pz0);
+            d.lpe_n[off[5] + o] = zc.b * (1.f * nu1 + pz1);
+        }
+        float ax_, ay_, az_;
+        ca3(mat, s_lut, lds, id, nc, ax_, ay_, az_);
+        d.lE_n[id] = ax_ * (yc.pec * zc.pec) * l0;
+        d.lE_n[nc + id] = ay_ * (zc.pec * xc.pec) * l1;
+        d.lE_n[2 * nc + id] = az_ * (xc.pec * yc.pec) * l2;
         q3n = q3;
         q4n = q4;
-    }
-}
-
-// E transpose, part 2 (cell-local): lpe <- b (sgn nu + lpe), lE <- ca pec lE; a column per
-// thread over a chunk of x planes, the column's y / z values loaded once
-__global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e2(AdjDev d)
-{
-    __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
-    const MatDev &mat = d.mat;
-    const bool lds = lut_in_lds(mat);
-    if (lds) {
-        stage_lut_adj(s_lut, mat);
-        __syncthreads();
-    }
-    const GridDev &g = d.g;
-    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (k >= nz || j >= ny) return;
-    const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
-    const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
-    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
-    const uint32_t *off = d.psl.off;
-    const int sy = slab_of(j, ny, Y.lo, Y.hi), sz = slab_of(k, nz, Z.lo, Z.hi);
-    const float pecy = axv(Y, j, AX_PEC), pecz = axv(Z, k, AX_PEC);
-    const float by = sy >= 0 ? axv(Y, j, AX_BE) : 0.f, bz = sz >= 0 ? axv(Z, k, AX_BE) : 0.f;
-    for (int i = i0; i < i1; ++i) {
-        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
-        const int sx = slab_of(i, nx, X.lo, X.hi);
-        const float pecx = axv(X, i, AX_PEC);
-        float ca0, ca1, ca2, cb0, cb1, cb2;
-        if (mat.mode == 0) {
-            const uint32_t c = mat.cls[id];
-            if (lds) {
-                const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
-                ca0 = r0.x; cb0 = r0.y; ca1 = r0.z; cb1 = r0.w; ca2 = r1.x; cb2 = r1.y;
-            } else {
-                const float *r = mat.lut + 8 * (size_t)c;
-                ca0 = r[0]; cb0 = r[1]; ca1 = r[2]; cb1 = r[3]; ca2 = r[4]; cb2 = r[5];
-            }
-        } else {
-            ca0 = mat.ca[id]; ca1 = mat.ca[nc + id]; ca2 = mat.ca[2 * nc + id];
-            cb0 = mat.cb[id]; cb1 = mat.cb[nc + id]; cb2 = mat.cb[2 * nc + id];
-        }
-        // the pec factor of each component: x (py pz), y (pz px), z (px py)
-        const float p0 = pecy * pecz, p1 = pecz * pecx, p2 = pecx * pecy;
-        const float l0 = d.lE[id], l1 = d.lE[nc + id], l2 = d.lE[2 * nc + id];
-        const float nu0 = cb0 * p0 * l0, nu1 = cb1 * p1 * l1, nu2 = cb2 * p2 * l2;
-        // terms (component, sign, slot): x slab (Ey -, 0), (Ez +, 1); y slab (Ex +, 2), (Ez -, 3);
-        // z slab (Ex -, 4), (Ey +, 5)
-        if (sx >= 0) {
-            const float bx = axv(X, i, AX_BE);
-            const uint32_t o = ((uint32_t)sx * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
-            float &a0 = d.lpe[off[0] + o], &a1 = d.lpe[off[1] + o];
-            a0 = bx * (-1.f * nu1 + a0);
-            a1 = bx * (1.f * nu2 + a1);
-        }
-        if (sy >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)sy) * (uint32_t)nz + (uint32_t)k;
-            float &a0 = d.lpe[off[2] + o], &a1 = d.lpe[off[3] + o];
-            a0 = by * (1.f * nu0 + a0);
-            a1 = by * (-1.f * nu2 + a1);
-        }
-        if (sz >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)sz;
-            float &a0 = d.lpe[off[4] + o], &a1 = d.lpe[off[5] + o];
-            a0 = bz * (-1.f * nu0 + a0);
-            a1 = bz * (1.f * nu1 + a1);
-        }
-        d.lE[id] = ca0 * p0 * l0;
-        d.lE[nc + id] = ca1 * p1 * l1;
-        d.lE[2 * nc + id] = ca2 * p2 * l2;
     }
 }
 
@@ -427,54 +396,24 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         d.lE[id] += acc0;
         d.lE[nc + id] += acc1;
         d.lE[2 * nc + id] += acc2;
+        // the cell-local part, into the other buffer: lph <- b (f + lph) per slab term
+        if (xc.s >= 0) {
+            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
+            d.lph_n[off[0] + o] = xc.b * (f1m + px0);
+            d.lph_n[off[1] + o] = xc.b * (f2p + px1);
+        }
+        if (yc.s >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yc.s) * (uint32_t)nz + (uint32_t)k;
+            d.lph_n[off[2] + o] = yc.b * (f0p + py0);
+            d.lph_n[off[3] + o] = yc.b * (f2m + py1);
+        }
+        if (zc.s >= 0) {
+            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zc.s;
+            d.lph_n[off[4] + o] = zc.b * (f0m + pz0);
+            d.lph_n[off[5] + o] = zc.b * (f1p + pz1);
+        }
         q3p = q3;
         q4p = q4;
-    }
-}
-
-// H transpose, part 2 (cell-local): lph <- b (f + lph), f = -ch sgn lH_c; a column per thread
-// over a chunk of x planes
-__global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h2(AdjDev d)
-{
-    const GridDev &g = d.g;
-    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (k >= nz || j >= ny) return;
-    const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
-    const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
-    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
-    const uint32_t *off = d.psl.off;
-    const int sy = slab_of(j, ny, Y.lo, Y.hi), sz = slab_of(k, nz, Z.lo, Z.hi);
-    if (sy < 0 && sz < 0 && X.lo == 0 && X.hi == 0) return;
-    const float by = sy >= 0 ? axv(Y, j, AX_BH) : 0.f, bz = sz >= 0 ? axv(Z, k, AX_BH) : 0.f;
-    const float mch = -d.ch;
-    for (int i = i0; i < i1; ++i) {
-        const int sx = slab_of(i, nx, X.lo, X.hi);
-        if (sx < 0 && sy < 0 && sz < 0) continue;
-        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
-        const float h0 = d.lH[id], h1 = d.lH[nc + id], h2 = d.lH[2 * nc + id];
-        // terms as in the E part: x slab (Hy -, 0), (Hz +, 1); y (Hx +, 2), (Hz -, 3);
-        // z (Hx -, 4), (Hy +, 5)
-        if (sx >= 0) {
-            const float bx = axv(X, i, AX_BH);
-            const uint32_t o = ((uint32_t)sx * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
-            float &a0 = d.lph[off[0] + o], &a1 = d.lph[off[1] + o];
-            a0 = bx * (mch * -1.f * h1 + a0);
-            a1 = bx * (mch * 1.f * h2 + a1);
-        }
-        if (sy >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)sy) * (uint32_t)nz + (uint32_t)k;
-            float &a0 = d.lph[off[2] + o], &a1 = d.lph[off[3] + o];
-            a0 = by * (mch * 1.f * h0 + a0);
-            a1 = by * (mch * -1.f * h2 + a1);
-        }
-        if (sz >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)sz;
-            float &a0 = d.lph[off[4] + o], &a1 = d.lph[off[5] + o];
-            a0 = bz * (mch * -1.f * h0 + a0);
-            a1 = bz * (mch * 1.f * h1 + a1);
-        }
     }
 }
 
@@ -516,6 +455,7 @@ struct oem_adjoint_state {
     std::vector<int64_t> ckpt_step;
     // adjoint fields
     float *lE = nullptr, *lH = nullptr, *lpe = nullptr, *lph = nullptr;
+    float *lE2 = nullptr, *lpe2 = nullptr, *lph2 = nullptr;   // the other halves of the ping-pong
     // design
     int n_design = 0, max_seg = 0;
     int *d_comp = nullptr, *d_cell = nullptr;
@@ -622,6 +562,9 @@ extern "C" int oem_adjoint_setup(oem_ctx *ctx, int32_t n_design, const int32_t *
     TRY(ctx->alloc(&a->lH, 3 * nc, true));
     TRY(ctx->alloc(&a->lpe, (size_t)ctx->psi.total, true));
     TRY(ctx->alloc(&a->lph, (size_t)ctx->psi.total, true));
+    TRY(ctx->alloc(&a->lE2, 3 * nc, true));
+    TRY(ctx->alloc(&a->lpe2, (size_t)ctx->psi.total, true));
+    TRY(ctx->alloc(&a->lph2, (size_t)ctx->psi.total, true));
     TRY(ctx->upload(&a->d_comp, (const int *)comp, (size_t)n_design));
     TRY(ctx->upload(&a->d_cell, (const int *)cell, (size_t)n_design));
     TRY(ctx->upload(&a->dca, dca, (size_t)n_design));
@@ -723,16 +666,16 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
     if (a->rec_s0 < 0 || n_steps != a->rec_n)
         return fail(OEM_ESTATE, "back over exactly the recorded segment (%d steps)", a->rec_n);
     AdjDev d{};
-    d.lE = a->lE; d.lH = a->lH; d.lpe = a->lpe; d.lph = a->lph;
+    d.lH = a->lH;
     d.psl = ctx->psi;
     d.g = ctx->g;
     d.mat = ctx->mat;
     d.ch = ctx->dims.ch;
     d.ncell = ctx->ncell;
     const dim3 tb(OEM_ADJ_TK, OEM_ADJ_TJ);
-    const dim3 tg((ctx->dims.nz + OEM_ADJ_TK - 1) / OEM_ADJ_TK,
-                  (ctx->dims.ny + OEM_ADJ_TJ - 1) / OEM_ADJ_TJ, ctx->dims.nx);
-    const dim3 tm(tg.x, tg.y, (ctx->dims.nx + OEM_ADJ_CHUNK - 1) / OEM_ADJ_CHUNK);
+    const dim3 tm((ctx->dims.nz + OEM_ADJ_TK - 1) / OEM_ADJ_TK,
+                  (ctx->dims.ny + OEM_ADJ_TJ - 1) / OEM_ADJ_TJ,
+                  (ctx->dims.nx + OEM_ADJ_CHUNK - 1) / OEM_ADJ_CHUNK);
     const dim3 gb(256), gg((a->n_design + 255) / 256);
     for (int s = a->rec_n - 1; s >= 0; --s) {
         const int n = (int)(a->rec_s0 + s);          // forward step n: E^n -> E^{n+1}
@@ -744,10 +687,15 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
                            a->dca, a->dcb, a->ca, a->cb, a->rec + (size_t)s * a->n_design,
                            a->rec + (size_t)(s + 1) * a->n_design, a->n_design, ctx->ncell,
                            a->grad);
+        // E transpose: reads lE and lpe, adds to lH, writes the new lE and lpe to the other
+        // buffers; then the H transpose: reads lH and lph, adds to the new lE, writes the new lph
+        d.lE = a->lE; d.lpe = a->lpe; d.lE_n = a->lE2; d.lpe_n = a->lpe2;
         hipLaunchKernelGGL(k_adj_e1, tm, tb, 0, ctx->stream, d);
-        hipLaunchKernelGGL(k_adj_e2, tm, tb, 0, ctx->stream, d);
+        std::swap(a->lE, a->lE2);
+        std::swap(a->lpe, a->lpe2);
+        d.lE = a->lE; d.lph = a->lph; d.lph_n = a->lph2;
         hipLaunchKernelGGL(k_adj_h1, tm, tb, 0, ctx->stream, d);
-        hipLaunchKernelGGL(k_adj_h2, tm, tb, 0, ctx->stream, d);
+        std::swap(a->lph, a->lph2);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(ctx->stream));
