@@ -273,3 +273,19 @@ def test_fused_step_with_h_point_lists_matches_the_two_sweeps(gpu_lib, monkeypat
     for a, b, r in zip(acc_fused, acc_two, o.acc):
         assert _rel(a, r) < 1e-4 and _rel(b, r) < 1e-4
         assert _rel(a, b) < 5e-5
+
+
+@pytest.mark.parametrize("fuse", ["0", "1"])
+def test_dense_materials_on_either_step_equal_the_lut(gpu_lib, monkeypatch, fuse):
+    """Dense per-cell (ca, cb) arrays, the fallback beyond 65535 classes, give the class LUT's
+    fields and phasors bit for bit on the fused step as on the two sweeps (a box soup with CPML,
+    ADE history and a dipole, ragged tiles)."""
+    n = 80
+    tabs = tables.build(FUSED_CASES["soup_24x22x150"](n))
+    monkeypatch.setenv("OEM_FUSE", fuse)
+    f1, a1 = _gpu_state(tabs, n)
+    f2, a2 = _gpu_state(tabs, n, dense_materials=True)
+    assert np.abs(f1).max() > 0
+    assert np.array_equal(f1, f2)
+    for x, y in zip(a1, a2):
+        assert np.array_equal(x, y)
