@@ -33,7 +33,10 @@ inline int fail(int code, const char *fmt, ...) {
 
 struct Monitor {
     MonDev d;
-    size_t count;     // ncomp*nf*npts
+    size_t count;         // ncomp*nf*npts
+    bool fused = false;   // the fused step writes its snapshots; `patch` lists the points an
+    int *patch = nullptr; // E-side point list corrects afterwards, snapped again
+    int n_patch = 0;
 };
 
 struct TimeMonitor {
@@ -55,6 +58,7 @@ struct Points {
     PointsDev d;
     int magnetic;       // 1: applied to H after update_h; 0: to E after update_e
     int kind;           // PT_*: launch order H: dipole, TF/SF, mode; E: TF/SF, mode, dipole
+    std::vector<int> cells;   // the group cells (host copy)
 };
 
 // Kernel family of a point list (profiling ids of include/openem_hip.h).
@@ -124,6 +128,7 @@ struct oem_ctx {
     float *cur_H() const { return (fused && (step & 1)) ? H2 : H; }
     GridDev g{};
     GridDev fg{};               // launch geometry of the fused H+E step
+    FuseSnap fsnap{};           // the DFT monitors the fused step snaps
     bool axis_set[3] = {false, false, false};
     MatDev mat{};
     bool mat_set = false;

@@ -668,11 +668,141 @@ __device__ __forceinline__ void hcolumn_do(HColumn<F> &c, const HLoads<F> &v, co
     c.ez = v.ez_n;
 }
 
+// DFT monitors whose snapshot the fused step writes as it stores the new fields, in the layout
+// of k_dft_snap (a flux plane: E_t1, E_t2 and the H averages over the plane and the one below;
+// a field box: E and H); no work-group meets two of them.  Cells that an E-side point list
+// corrects after the step are snapped again afterwards (k_dft_snap_list).
+#define OEM_FUSE_MONS 4
+struct FuseMon {
+    int kind, axis, km;         // kind 0: a plane (normal axis, index); 1: a box
+    int o[3], b[3];             // box
+    int npts, T;
+    void *snap;
+};
+struct FuseSnap {
+    int n;
+    FuseMon m[OEM_FUSE_MONS];
+};
+
+// The monitors of `fs` (bit t: monitor t) whose cells meet the box [i0, i1) x [j0, j1) x [k0, k1).
+__device__ __forceinline__ uint32_t fuse_snap_mask(const FuseSnap &fs, int nx, int ny, int nz,
+                                                   int i0, int i1, int j0, int j1, int k0, int k1)
+{
+    uint32_t mask = 0;
+    for (int t = 0; t < fs.n; ++t) {
+        const FuseMon m = fs.m[t];
+        int lo[3] = {0, 0, 0}, hi[3] = {nx, ny, nz};
+        if (m.kind == 1) {
+            for (int a = 0; a < 3; ++a) { lo[a] = m.o[a]; hi[a] = m.o[a] + m.b[a]; }
+        } else {
+            lo[m.axis] = m.km;
+            hi[m.axis] = m.km + 1;
+        }
+        if (lo[0] < i1 && hi[0] > i0 && lo[1] < j1 && hi[1] > j0 && lo[2] < k1 && hi[2] > k0)
+            mask |= 1u << t;
+    }
+    return mask;
+}
+
+// One monitor's snapshot for the columns of a work-group, set up before the march: point
+// q = (i - ilo) * si + qjk of the plane i in [ilo, ihi) (qjk < 0: the column is not on the
+// monitor); `sel` 0: a field box, 1 + a: a flux plane normal to axis a.
+template <typename F>
+struct SnapPlan {
+    F *row;          // the ring row of this step
+    int np, sel, ilo, ihi, si, qjk;
+};
+
+template <typename F>
+__device__ __forceinline__ SnapPlan<F> snap_plan(const FuseMon &m, int step, int nx, int ny,
+                                                 int nz, int j, int k)
+{
+    SnapPlan<F> p;
+    p.np = m.npts;
+    if (m.kind == 1) {
+        p.sel = 0;
+        p.ilo = m.o[0]; p.ihi = m.o[0] + m.b[0]; p.si = m.b[1] * m.b[2];
+        p.qjk = ((unsigned)(j - m.o[1]) < (unsigned)m.b[1] && (unsigned)(k - m.o[2]) < (unsigned)m.b[2])
+                    ? (j - m.o[1]) * m.b[2] + (k - m.o[2]) : -1;
+    } else {
+        p.sel = 1 + m.axis;
+        p.ilo = 0; p.ihi = nx;
+        if (m.axis == 0) { p.ilo = m.km; p.ihi = m.km + 1; p.si = 0; p.qjk = j * nz + k; }
+        else if (m.axis == 1) { p.si = nz; p.qjk = j == m.km ? k : -1; }
+        else { p.si = ny; p.qjk = k == m.km ? j : -1; }
+    }
+    p.row = static_cast<F *>(m.snap) + (size_t)(step % m.T) * (m.kind == 1 ? 6 : 4) * (size_t)m.npts;
+    return p;
+}
+
+// The snapshot values of one cell under plan p, to the ring (stage == nullptr) or to slot `slot`
+// of an LDS stage laid out [component][64].
+template <typename F>
+__device__ __forceinline__ void snap_planned(const SnapPlan<F> &p, int i, F *stage, int slot, F ex,
+                                             F ey, F ez, F hx, F hy, F hz, F hym_i, F hzm_i,
+                                             F hxm_j, F hzm_j, F hxm_k, F hym_k)
+{
+    size_t np = 64, q = (size_t)slot;
+    F *row = stage;
+    if (!stage) {
+        np = (size_t)p.np;
+        q = (size_t)((i - p.ilo) * p.si + p.qjk);
+        row = p.row;
+    }
+    if (p.sel == 0) {
+        row[q] = ex; row[np + q] = ey; row[2 * np + q] = ez;
+        row[3 * np + q] = hx; row[4 * np + q] = hy; row[5 * np + q] = hz;
+    } else if (p.sel == 1) {
+        row[q] = ey; row[np + q] = ez;
+        row[2 * np + q] = 0.5f * (hy + hym_i); row[3 * np + q] = 0.5f * (hz + hzm_i);
+    } else if (p.sel == 2) {
+        row[q] = ex; row[np + q] = ez;
+        row[2 * np + q] = 0.5f * (hx + hxm_j); row[3 * np + q] = 0.5f * (hz + hzm_j);
+    } else {
+        row[q] = ex; row[np + q] = ey;
+        row[2 * np + q] = 0.5f * (hx + hxm_k); row[3 * np + q] = 0.5f * (hy + hym_k);
+    }
+}
+
+// Rank the row waves' snapshot points (`part`) across the work-group, one barrier that every
+// wave takes: the number of points if at most 64 (each point's slot and ring offset qjk go to
+// `slot` and fq[slot]), else -1 (the row waves store their points themselves).
+__device__ __forceinline__ int snap_rank(bool part, int w, int lane, int qjk, int *s_fcnt,
+                                         int *s_fq, int &slot)
+{
+    const uint64_t bal = __ballot(part);
+    if (w < OEM_CR && lane == 0) s_fcnt[w] = __popcll(bal);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int r = 0; r < OEM_CR; ++r) {
+        const int c = s_fcnt[r];
+        off += r < w ? c : 0;
+        tot += c;
+    }
+    if (tot > 64) return -1;
+    if (part) {
+        slot = off + __popcll(bal & ((1ull << lane) - 1ull));
+        s_fq[slot] = qjk;
+    }
+    return __builtin_amdgcn_readfirstlane(tot);
+}
+
+// The staged snapshot of plane i (cnt points, ring offsets fq) to the ring, by one wave.
+template <typename F>
+__device__ __forceinline__ void snap_flush(const SnapPlan<F> &p, const F *stage,
+                                           const int *fq, int cnt, int i, int lane)
+{
+    if (lane >= cnt) return;
+    const int nc = p.sel == 0 ? 6 : 4;
+    const size_t np = (size_t)p.np, q = (size_t)((i - p.ilo) * p.si + fq[lane]);
+    for (int c = 0; c < nc; ++c) p.row[c * np + q] = stage[c * 64 + lane];
+}
+
 template <typename F>
 __global__ void __launch_bounds__(64 * (OEM_CR + 1))
 k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ En,
                 F *__restrict__ Hn, PsiDev ps, PsiDev pn, GridDev g, MatDev mat, float ch,
-                float cw, PlaneSrcDev pw, const int *__restrict__ stp)
+                float cw, PlaneSrcDev pw, const int *__restrict__ stp, FuseSnap fs)
 {
     typedef FT<F> T;
     // Ho = Eo + 3 components and ps.e = ps.h + ps.total (the host allocates them so)
@@ -690,6 +820,11 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
     // plane parity
     __shared__ F s_row[2][3][OEM_CR + 1][64];
     __shared__ F s_col[2][2][OEM_CR];
+    // snapshot stage by plane parity ([component][point]), the points' ring offsets, and the
+    // row waves' point counts
+    __shared__ F s_fsn[2][6 * 64];
+    __shared__ int s_fq[64];
+    __shared__ int s_fcnt[OEM_CR];
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int nkt = (nz + 63) / 64;
@@ -718,6 +853,37 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
     // Threads outside the grid work on a clamped in-range column and store nothing.
     const int kl = min(k0 + lane, nz - 1);
     const int xp0 = __builtin_amdgcn_readfirstlane(X.prv[i0]);
+    // the fused DFT monitor that meets this work-group's tile and chunk (the host fuses monitors
+    // whose work-groups are disjoint; most work-groups meet none), set up here so that the march
+    // reads no descriptor.  Up to 64 points go through an LDS stage that the halo wave stores
+    // with full-width stores after the next barrier (a row wave holds only a few of them: 2
+    // lanes of a 2-layer box, 1 of a z plane)
+    int fcnt = -1, fslot = -1;
+    bool fplan = false;
+    SnapPlan<F> fp{};
+    fp.ilo = fp.ihi = 0;
+    if (fs.n) {
+        const uint32_t fmask = __builtin_amdgcn_readfirstlane(
+            fuse_snap_mask(fs, nx, ny, nz, i0, i1, j0, j0 + OEM_CR, k0, k0 + 64));
+        if (fmask) {
+            const int fstep = __builtin_amdgcn_readfirstlane(step_now(stp));
+            const int jr = min(j0 + min(w, OEM_CR - 1), ny - 1);
+            fp = snap_plan<F>(fs.m[__builtin_ctz(fmask)], fstep, nx, ny, nz, jr, kl);
+            {
+                // all but qjk are the same across the work-group
+                const uint64_t rp = reinterpret_cast<uint64_t>(fp.row);
+                const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)rp);
+                const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(rp >> 32));
+                fp.row = reinterpret_cast<F *>(((uint64_t)hi << 32) | lo);
+                fp.np = __builtin_amdgcn_readfirstlane(fp.np);
+                fp.sel = __builtin_amdgcn_readfirstlane(fp.sel);
+                fp.ilo = __builtin_amdgcn_readfirstlane(fp.ilo);
+                fp.ihi = __builtin_amdgcn_readfirstlane(fp.ihi);
+                fp.si = __builtin_amdgcn_readfirstlane(fp.si);
+            }
+            fplan = true;
+        }
+    }
     if (w == OEM_CR) {
         // the halo wave: the new H of (Y.prv[j0], k0 + lane) and of (j0 + lane, Z.prv[k0]) for
         // lane < OEM_CR, into LDS slot 0 and the column slots; one barrier per plane, as the
@@ -730,6 +896,7 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
         hcolumn_init<F>(cc, g, pw, stp, jc, kh, next_of(jc, ny, yn_last), next_of(kh, nz, zn_last));
         hcolumn_load_e<F>(cr, io, g, i0);
         hcolumn_load_e<F>(cc, io, g, i0);
+        if (fplan) fcnt = snap_rank(false, w, lane, 0, s_fcnt, s_fq, fslot);
 #pragma unroll 1
         for (int i = i0; i < i1; ++i) {
             const int buf = (i - i0) & 1;
@@ -748,6 +915,14 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
                 s_col[buf][1][lane] = hy;
             }
             __syncthreads();
+            // the row waves staged plane i - 1 before this barrier
+            if (fcnt > 0 && i > i0 && i - 1 >= fp.ilo && i - 1 < fp.ihi)
+                snap_flush<F>(fp, s_fsn[buf ^ 1], s_fq, fcnt, i - 1, lane);
+        }
+        if (fcnt > 0) {
+            __syncthreads();
+            if (i1 - 1 >= fp.ilo && i1 - 1 < fp.ihi)
+                snap_flush<F>(fp, s_fsn[(i1 - 1 - i0) & 1], s_fq, fcnt, i1 - 1, lane);
         }
         return;
     }
@@ -795,6 +970,7 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
                       hz_m);
     }
     hcolumn_load_e<F>(ca, io, g, i0);
+    if (fplan) fcnt = snap_rank(own && fp.qjk >= 0, w, lane, fp.qjk, s_fcnt, s_fq, fslot);
 
 #pragma unroll 1
     for (int i = i0; i < i1; ++i) {
@@ -893,10 +1069,15 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
             T::st(io.fn, nex, ca.op, so);
             T::st(io.fn, ney, ca.op, so + io.cb);
             T::st(io.fn, nez, ca.op, so + 2u * io.cb);
+            if (i >= fp.ilo && i < fp.ihi && fp.qjk >= 0)
+                snap_planned<F>(fp, i, fcnt >= 0 ? s_fsn[buf] : nullptr, fslot, nex, ney, nez, hx,
+                                hy, hz, hy_m, hz_m, hx_j, hz_j, hx_k, hy_k);
         }
         hy_m = hy;
         hz_m = hz;
     }
+    // the halo wave stores the last staged plane after this barrier
+    if (fcnt > 0) __syncthreads();
 }
 
 // E at the design cells of an adjoint after the step: row (step + 1 - base) of rec (the step read
@@ -1297,6 +1478,17 @@ k_dft_snap(MonDev m, const F *__restrict__ E, const F *__restrict__ H, size_t nc
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.npts) return;
     snap_point<F>(m, E, H, ncell, ny, nz, step_now(stp) % m.T, q);
+}
+
+// The points `qs[0..nq)` of a monitor whose other points the fused step snapped.
+template <typename F>
+__global__ void __launch_bounds__(256)
+k_dft_snap_list(MonDev m, const F *__restrict__ E, const F *__restrict__ H, size_t ncell, int ny,
+                int nz, const int *__restrict__ stp, const int *__restrict__ qs, int nq)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    snap_point<F>(m, E, H, ncell, ny, nz, step_now(stp) % m.T, qs[t]);
 }
 
 __device__ __forceinline__ void dft_acc(double &sr, double &si, float v, double c, double s) {

@@ -16,6 +16,8 @@
 
 #include "oem_internal.h"
 
+#include <array>
+
 __global__ void k_step_advance(int *stp) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *stp += 1;
 }
@@ -346,6 +348,7 @@ int oemi::build_points(oem_ctx *ctx, int magnetic, int kind, int tfsf, int cplx_
     if (phase) TRY(ctx->upload(&pp, ph.data(), ph.size()));
     TRY(ctx->upload(&pt, tab, tab_doubles));
     d.g_comp = pa; d.g_cell = pb; d.g_ofs = po; d.coef = pf; d.col = pc; d.phase = pp; d.tab = pt;
+    P.cells = gl;
     *out = P;
     return OEM_OK;
 }
@@ -609,6 +612,69 @@ extern "C" int oem_read_flux_time_monitor(oem_ctx *ctx, int32_t mon_id, double *
     return OEM_OK;
 }
 
+// The DFT monitors whose snapshots the fused step writes: up to OEM_FUSE_MONS, in order, whose
+// footprints in work-groups (x chunks x row tiles x k tiles) are disjoint, so that a work-group
+// meets at most one of them; the others keep k_dft_snap.  Each comes with the list of its
+// points at cells that an E-side point list corrects after the step (those are snapped again
+// after the corrections, so every snapshot holds the final fields).
+static int plan_fused_snaps(oem_ctx *ctx) {
+    const int nx = ctx->dims.nx, ny = ctx->dims.ny, nz = ctx->dims.nz;
+    const int tile[3] = {ctx->fg.chunk, OEM_CR, 64};
+    std::vector<std::array<int, 6>> taken;   // fused footprints: first, last tile per axis
+    auto footprint = [&](const MonDev &m) {
+        int lo[3] = {0, 0, 0}, hi[3] = {nx, ny, nz};
+        if (m.kind == 1) {
+            for (int a = 0; a < 3; ++a) { lo[a] = m.o[a]; hi[a] = m.o[a] + m.b[a]; }
+        } else {
+            lo[m.axis] = m.km;
+            hi[m.axis] = m.km + 1;
+        }
+        std::array<int, 6> f{};
+        for (int a = 0; a < 3; ++a) { f[2 * a] = lo[a] / tile[a]; f[2 * a + 1] = (hi[a] - 1) / tile[a]; }
+        return f;
+    };
+    std::vector<int> ecells;
+    for (auto &pt : ctx->points)
+        if (!pt.magnetic) ecells.insert(ecells.end(), pt.cells.begin(), pt.cells.end());
+    std::sort(ecells.begin(), ecells.end());
+    ecells.erase(std::unique(ecells.begin(), ecells.end()), ecells.end());
+    ctx->fsnap.n = 0;
+    for (auto &mon : ctx->mons) {
+        if (ctx->fsnap.n == OEM_FUSE_MONS) break;
+        const MonDev &m = mon.d;
+        const std::array<int, 6> fp = footprint(m);
+        bool meets = false;
+        for (const auto &t : taken)
+            meets |= fp[0] <= t[1] && t[0] <= fp[1] && fp[2] <= t[3] && t[2] <= fp[3] &&
+                     fp[4] <= t[5] && t[4] <= fp[5];
+        if (meets) continue;
+        taken.push_back(fp);
+        std::vector<int> qs;
+        for (const int id : ecells) {
+            const int i = id / (ny * nz), r = id - i * ny * nz, j = r / nz, k = r - j * nz;
+            if (m.kind == 1) {
+                if (i < m.o[0] || i >= m.o[0] + m.b[0] || j < m.o[1] || j >= m.o[1] + m.b[1] ||
+                    k < m.o[2] || k >= m.o[2] + m.b[2])
+                    continue;
+                qs.push_back(((i - m.o[0]) * m.b[1] + (j - m.o[1])) * m.b[2] + (k - m.o[2]));
+            } else {
+                if ((m.axis == 0 ? i : m.axis == 1 ? j : k) != m.km) continue;
+                qs.push_back(m.axis == 0 ? j * nz + k : m.axis == 1 ? i * nz + k : i * ny + j);
+            }
+        }
+        FuseMon &f = ctx->fsnap.m[ctx->fsnap.n++];
+        f.kind = m.kind; f.axis = m.axis; f.km = m.km;
+        for (int a = 0; a < 3; ++a) { f.o[a] = m.o[a]; f.b[a] = m.b[a]; }
+        f.npts = m.npts; f.T = m.T; f.snap = m.snap;
+        mon.fused = true;
+        if (!qs.empty()) {
+            TRY(ctx->upload(&mon.patch, qs.data(), qs.size()));
+            mon.n_patch = (int)qs.size();
+        }
+    }
+    return OEM_OK;
+}
+
 extern "C" int oem_prepare(oem_ctx *ctx) {
     CHECK_CTX(ctx);
     if (ctx->prepared) return OEM_OK;
@@ -699,6 +765,8 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
         if (fb > 0x7fffffffLL) return fail(OEM_EINVAL, "launch too large");
         ctx->fg.nblocks = (int)fb;
         ctx->fused = true;
+        const char *fsn = std::getenv("OEM_FUSE_SNAP");
+        if (!(fsn && fsn[0] == '0')) TRY(plan_fused_snaps(ctx));
     }
     ctx->prepared = true;
     return OEM_OK;
@@ -780,7 +848,8 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
         TRY(mark(OEM_K_UPDATE_HE, true));
         hipLaunchKernelGGL((k_update_he<F>), dim3(c->fg.nblocks), dim3(64, OEM_CR + 1), 0,
                            c->stream, (const F *)Eo, (const F *)Ho, E, H, po, pn, c->fg, c->mat,
-                           c->dims.ch, c->dims.cw, c->pw, c->d_step);
+                           c->dims.ch, c->dims.cw, c->pw, c->d_step,
+                           c->mons_off ? FuseSnap{} : c->fsnap);
         TRY(mark(OEM_K_UPDATE_HE, false));
     } else {
     TRY(mark(OEM_K_UPDATE_H, true));
@@ -853,10 +922,16 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
     }
     for (auto &m : c->mons) {
         if (c->mons_off) break;
+        if (c->fused && m.fused && !m.n_patch) continue;
         TRY(mark(OEM_K_DFT_SNAP, true));
-        hipLaunchKernelGGL((k_dft_snap<F>), dim3(blocks_for(m.d.npts, 256)), dim3(256), 0,
-                           c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
-                           c->dims.nz, c->d_step);
+        if (c->fused && m.fused)
+            hipLaunchKernelGGL((k_dft_snap_list<F>), dim3(blocks_for(m.n_patch, 256)), dim3(256), 0,
+                               c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
+                               c->dims.nz, c->d_step, (const int *)m.patch, m.n_patch);
+        else
+            hipLaunchKernelGGL((k_dft_snap<F>), dim3(blocks_for(m.d.npts, 256)), dim3(256), 0,
+                               c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
+                               c->dims.nz, c->d_step);
         TRY(mark(OEM_K_DFT_SNAP, false));
     }
     TRY(mark(OEM_K_STEP, true));

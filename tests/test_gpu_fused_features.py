@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from openem_amd import adjoint, scenes, shutoff, solver, tables
-from openem_amd.model import FieldTimeMonitor, FluxTimeMonitor
+from openem_amd.model import FieldMonitor, FieldTimeMonitor, FluxMonitor, FluxTimeMonitor
 from openem_amd.solver import Stepper
 
 pytestmark = pytest.mark.gpu
@@ -93,3 +93,63 @@ def test_adjoint_checkpoints_restore_either_set(gpu_lib, monkeypatch):
     assert scale > 0
     assert np.abs(one.grad - two.grad).max() <= 1e-4 * scale
     assert abs(one.J - two.J) <= 1e-5 * two.J
+
+
+# Monitor sets on the 24 x 22 x 150 box soup (dipole at the centre cell (12, 11, 75)).  The
+# fused step snaps monitors whose work-groups (x chunks x 7-row tiles x 64-wide k tiles) are
+# disjoint, the others keep the snapshot kernel; each set names what it covers.  Flux planes at
+# the first plane of an x chunk (3-plane chunks: 6, 15; 16-plane chunks: 16), of a row tile (7,
+# 14) and of a k tile (64, 128) take the H below from the prologue, the halo row and the halo
+# column; boxes and planes through the dipole are snapped again after its correction; a
+# work-group with more than 64 points of a monitor stores them itself, fewer go through LDS.
+SNAP_SETS = {
+    # x planes at chunk starts: every work-group of the chunk has 448 points (direct stores)
+    "x6_x15_chunk3": ("3", lambda f: [FluxMonitor("x6", 0, 6, +1, f[:2]),
+                                      FluxMonitor("x15", 0, 15, -1, f[1:3])], lambda f: []),
+    # y planes at row-tile starts: one row of 64 points per work-group (staged)
+    "y7_y14": (None, lambda f: [FluxMonitor("y7", 1, 7, -1, f[1:3]),
+                                FluxMonitor("y14", 1, 14, +1, f[:2])], lambda f: []),
+    # z planes at k-tile starts (7 points per work-group, staged) and z75 sharing z64's tiles
+    # (snapshot kernel)
+    "z64_z128_z75": (None, lambda f: [FluxMonitor("z64", 2, 64, +1, f[:3]),
+                                      FluxMonitor("z128", 2, 128, +1, f[:2]),
+                                      FluxMonitor("z75", 2, 75, -1, f[:2])], lambda f: []),
+    # a z plane through the dipole (corrected cells snapped again)
+    "z75_dipole": (None, lambda f: [FluxMonitor("z75", 2, 75, -1, f[:2])], lambda f: []),
+    # a box around the dipole (staged, corrected cells snapped again) and an x plane at the
+    # second 16-plane chunk
+    "box_dipole_x16": (None, lambda f: [FluxMonitor("x16", 0, 16, +1, f[:2])],
+                       lambda f: [FieldMonitor("dip", f[:2], (10, 9, 70), (5, 6, 10))]),
+    # a box with up to 196 points per work-group (direct stores) and a z plane in other k tiles
+    "wide_box_z40": (None, lambda f: [FluxMonitor("z40", 2, 40, +1, f[:2])],
+                     lambda f: [FieldMonitor("wide", f[1:3], (0, 0, 100), (24, 22, 40))]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SNAP_SETS))
+def test_fused_snapshots_equal_the_sweeps(gpu_lib, monkeypatch, name):
+    """The fused step writes the DFT snapshots as it stores the new fields; the phasors equal
+    those of the two sweeps and their snapshot kernel bit for bit."""
+    chunk, flux, boxes = SNAP_SETS[name]
+    n = 120
+    sc = scenes.box_soup_scene(shape=(24, 22, 150), n_pml=6, n_boxes=8, num_steps=n, seed=3,
+                               moment=1e-18)
+    f = sc.field_monitors[0].freqs
+    sc = replace(sc, flux_monitors=flux(f), field_monitors=boxes(f))
+    tabs = tables.build(sc)
+
+    def run():
+        st = Stepper(tabs, dft_batch=8)
+        try:
+            st.run(n)
+            return ([st.read_monitor(i) for i in range(len(tabs.monitors))],
+                    np.stack([st.read_field(c) for c in range(6)]))
+        finally:
+            st.close()
+    if chunk:
+        monkeypatch.setenv("OEM_CHUNK", chunk)
+    (a1, e1), (a2, e2) = _both(monkeypatch, run)
+    assert np.array_equal(e1, e2)
+    for x, y in zip(a1, a2):
+        assert np.abs(y).max() > 0
+        assert np.array_equal(x, y)
