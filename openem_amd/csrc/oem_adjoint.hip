@@ -18,7 +18,7 @@
 
 #include "oem_internal.h"
 
-// the transposed sweeps: a (k, j) column per thread from a 64 x 4 block, marching OEM_ADJ_CHUNK
+// the transposed sweeps: a (j, k) column per thread from a block of 256, marching OEM_ADJ_CHUNK
 // x planes per block (grid z)
 #define OEM_ADJ_TK 64
 #define OEM_ADJ_TJ 4
@@ -117,8 +117,11 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     const GridDev &g = d.g;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (k >= nz || j >= ny) return;
+    // one (j, k) column per thread, numbered along the plane: no idle lanes when nz is not a
+    // multiple of 64 (a 48-cell-thick slab leaves a quarter of (k, j) tiles empty)
+    const int q = blockIdx.x * (OEM_ADJ_TK * OEM_ADJ_TJ) + threadIdx.y * OEM_ADJ_TK + threadIdx.x;
+    if (q >= ny * nz) return;
+    const int j = q / nz, k = q - j * nz;
     const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
     const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
@@ -284,8 +287,11 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
     const GridDev &g = d.g;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (k >= nz || j >= ny) return;
+    // one (j, k) column per thread, numbered along the plane: no idle lanes when nz is not a
+    // multiple of 64 (a 48-cell-thick slab leaves a quarter of (k, j) tiles empty)
+    const int q = blockIdx.x * (OEM_ADJ_TK * OEM_ADJ_TJ) + threadIdx.y * OEM_ADJ_TK + threadIdx.x;
+    if (q >= ny * nz) return;
+    const int j = q / nz, k = q - j * nz;
     const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
     const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
@@ -477,8 +483,8 @@ static int adj_check(oem_ctx *ctx) {
     if (ctx->cplx || ctx->has_disp)
         return fail(OEM_ENOTSUP, "the adjoint covers real, non-dispersive scenes");
     if (ctx->pool) return fail(OEM_ENOTSUP, "pooled contexts have no adjoint");
-    if (ctx->dims.nx > 65535 || (ctx->dims.ny + OEM_ADJ_TJ - 1) / OEM_ADJ_TJ > 65535)
-        return fail(OEM_ENOTSUP, "the adjoint launch covers nx <= 65535 and ny <= 262140");
+    if ((ctx->dims.nx + OEM_ADJ_CHUNK - 1) / OEM_ADJ_CHUNK > 65535)
+        return fail(OEM_ENOTSUP, "the adjoint launch covers nx <= %d", 65535 * OEM_ADJ_CHUNK);
     // the transposed sweeps index the three adjoint components with 32-bit offsets
     if (3ull * ctx->ncell >= (1ull << 32) || ctx->psi.total >= (1u << 31))
         return fail(OEM_ENOTSUP, "the adjoint covers up to 2^32 / 3 cells");
@@ -673,8 +679,8 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
     d.ch = ctx->dims.ch;
     d.ncell = ctx->ncell;
     const dim3 tb(OEM_ADJ_TK, OEM_ADJ_TJ);
-    const dim3 tm((ctx->dims.nz + OEM_ADJ_TK - 1) / OEM_ADJ_TK,
-                  (ctx->dims.ny + OEM_ADJ_TJ - 1) / OEM_ADJ_TJ,
+    const int cols = ctx->dims.ny * ctx->dims.nz;
+    const dim3 tm((cols + OEM_ADJ_TK * OEM_ADJ_TJ - 1) / (OEM_ADJ_TK * OEM_ADJ_TJ), 1,
                   (ctx->dims.nx + OEM_ADJ_CHUNK - 1) / OEM_ADJ_CHUNK);
     const dim3 gb(256), gg((a->n_design + 255) / 256);
     for (int s = a->rec_n - 1; s >= 0; --s) {
