@@ -112,9 +112,10 @@ struct GridDev {
     AxisDev ax[3];
     int nx, ny, nz;
     int plane;          // ny*nz
-    int tiles;          // k-tiles * j-tiles of the (ny, nz) plane
+    int tiles;          // k-tiles * j-tiles of the (ny, nz) plane (flat: blocks of the plane)
     int chunk;          // i-planes per block
     int nblocks;        // tiles * ceil(nx / chunk)
+    int flat;           // the sweeps number the (j, k) columns along the plane (nz % 64 != 0)
 };
 
 // psi slabs: one allocation per side (H, E) holding six sub-arrays at element offsets
@@ -168,17 +169,27 @@ struct TileCoord {
     bool ok;
 };
 
-// Logical block lb of a scene -> (j, k) of this thread and the i-range of the x march.
+// Logical block lb of a scene -> (j, k) of this thread and the i-range of the x march: an
+// OEM_TJ x OEM_TK tile of the (j, k) plane, or, when nz is not a multiple of OEM_TK (a 48-cell
+// slab would leave a quarter of every wave idle), the block's OEM_TJ * OEM_TK consecutive
+// columns of the plane.
 __device__ __forceinline__ TileCoord tile_coord(const GridDev &g, int lb) {
-    const int nkt = (g.nz + OEM_TK - 1) / OEM_TK;
     const int tile = lb % g.tiles, chunk = lb / g.tiles;
-    const int jt = tile / nkt, kt = tile - jt * nkt;
     TileCoord t;
-    t.k = kt * OEM_TK + (int)threadIdx.x;
-    t.j = jt * OEM_TJ + (int)threadIdx.y;
+    if (g.flat) {
+        const int q = tile * (OEM_TK * OEM_TJ) + (int)threadIdx.y * OEM_TK + (int)threadIdx.x;
+        t.ok = q < g.plane;
+        t.j = t.ok ? q / g.nz : 0;
+        t.k = t.ok ? q - t.j * g.nz : 0;
+    } else {
+        const int nkt = (g.nz + OEM_TK - 1) / OEM_TK;
+        const int jt = tile / nkt, kt = tile - jt * nkt;
+        t.k = kt * OEM_TK + (int)threadIdx.x;
+        t.j = jt * OEM_TJ + (int)threadIdx.y;
+        t.ok = t.k < g.nz && t.j < g.ny;
+    }
     t.i0 = chunk * g.chunk;
     t.i1 = min(t.i0 + g.chunk, g.nx);
-    t.ok = t.k < g.nz && t.j < g.ny;
     return t;
 }
 

@@ -708,7 +708,9 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     g.plane = g.ny * g.nz;
     if ((size_t)g.plane * 4 >= ((size_t)1 << 32))
         return fail(OEM_EINVAL, "plane ny*nz=%d too large for 32-bit in-plane offsets", g.plane);
-    g.tiles = ((g.nz + OEM_TK - 1) / OEM_TK) * ((g.ny + OEM_TJ - 1) / OEM_TJ);
+    g.flat = g.nz % OEM_TK != 0;
+    g.tiles = g.flat ? (g.plane + OEM_TK * OEM_TJ - 1) / (OEM_TK * OEM_TJ)
+                     : ((g.nz + OEM_TK - 1) / OEM_TK) * ((g.ny + OEM_TJ - 1) / OEM_TJ);
     int chunk;
     const char *env = std::getenv("OEM_CHUNK");
     if (ctx->chunk_req > 0) {
