@@ -18,11 +18,13 @@
 
 #include "oem_internal.h"
 
-// the transposed sweeps: a (j, k) column per thread from a block of 256, marching OEM_ADJ_CHUNK
-// x planes per block (grid z)
+// the transposed sweeps: a (j, k) column per thread from a block of 256, marching `chunk` x
+// planes per block (grid z; OEM_ADJ_CHUNK by default, the environment's OEM_ADJ_CHUNK if set)
 #define OEM_ADJ_TK 64
 #define OEM_ADJ_TJ 4
-#define OEM_ADJ_CHUNK 16
+// 8 planes measured best on the 400 x 400 x 48 bend (s per design: 4 3.04-3.06, 6 3.01, 8
+// 3.02-3.09, 12 3.08, 16 3.11-3.18, 24 3.10, 32 3.17)
+#define OEM_ADJ_CHUNK 8
 
 namespace {
 
@@ -35,6 +37,7 @@ struct AdjDev {
     MatDev mat;
     float ch;
     size_t ncell;
+    int chunk;                   // x planes per block of the transposes
 };
 
 // E transpose, part 1: lH_b += D_bwd^T q_{c,a} for the two terms with source b.
@@ -122,7 +125,7 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     const int q = blockIdx.x * (OEM_ADJ_TK * OEM_ADJ_TJ) + threadIdx.y * OEM_ADJ_TK + threadIdx.x;
     if (q >= ny * nz) return;
     const int j = q / nz, k = q - j * nz;
-    const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
+    const int i0 = blockIdx.z * d.chunk, i1 = min(i0 + d.chunk, nx);
     const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
     const uint32_t *off = d.psl.off;
@@ -292,7 +295,7 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
     const int q = blockIdx.x * (OEM_ADJ_TK * OEM_ADJ_TJ) + threadIdx.y * OEM_ADJ_TK + threadIdx.x;
     if (q >= ny * nz) return;
     const int j = q / nz, k = q - j * nz;
-    const int i0 = blockIdx.z * OEM_ADJ_CHUNK, i1 = min(i0 + OEM_ADJ_CHUNK, nx);
+    const int i0 = blockIdx.z * d.chunk, i1 = min(i0 + d.chunk, nx);
     const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
     const uint32_t *off = d.psl.off;
@@ -478,13 +481,20 @@ namespace oemi {
 void adjoint_state_free(oem_adjoint_state *a) { delete a; }
 }
 
+// x planes per block of the transposed sweeps
+static int adj_chunk() {
+    const char *e = std::getenv("OEM_ADJ_CHUNK");
+    const int v = e ? std::atoi(e) : 0;
+    return v > 0 ? v : OEM_ADJ_CHUNK;
+}
+
 static int adj_check(oem_ctx *ctx) {
     if (!ctx->prepared) return fail(OEM_ESTATE, "oem_prepare not called");
     if (ctx->cplx || ctx->has_disp)
         return fail(OEM_ENOTSUP, "the adjoint covers real, non-dispersive scenes");
     if (ctx->pool) return fail(OEM_ENOTSUP, "pooled contexts have no adjoint");
-    if ((ctx->dims.nx + OEM_ADJ_CHUNK - 1) / OEM_ADJ_CHUNK > 65535)
-        return fail(OEM_ENOTSUP, "the adjoint launch covers nx <= %d", 65535 * OEM_ADJ_CHUNK);
+    if ((ctx->dims.nx + adj_chunk() - 1) / adj_chunk() > 65535)
+        return fail(OEM_ENOTSUP, "the adjoint launch covers nx <= %d", 65535 * adj_chunk());
     // the transposed sweeps index the three adjoint components with 32-bit offsets
     if (3ull * ctx->ncell >= (1ull << 32) || ctx->psi.total >= (1u << 31))
         return fail(OEM_ENOTSUP, "the adjoint covers up to 2^32 / 3 cells");
@@ -678,10 +688,11 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
     d.mat = ctx->mat;
     d.ch = ctx->dims.ch;
     d.ncell = ctx->ncell;
+    d.chunk = adj_chunk();
     const dim3 tb(OEM_ADJ_TK, OEM_ADJ_TJ);
     const int cols = ctx->dims.ny * ctx->dims.nz;
     const dim3 tm((cols + OEM_ADJ_TK * OEM_ADJ_TJ - 1) / (OEM_ADJ_TK * OEM_ADJ_TJ), 1,
-                  (ctx->dims.nx + OEM_ADJ_CHUNK - 1) / OEM_ADJ_CHUNK);
+                  (ctx->dims.nx + d.chunk - 1) / d.chunk);
     const dim3 gb(256), gg((a->n_design + 255) / 256);
     for (int s = a->rec_n - 1; s >= 0; --s) {
         const int n = (int)(a->rec_s0 + s);          // forward step n: E^n -> E^{n+1}
