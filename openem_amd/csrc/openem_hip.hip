@@ -755,11 +755,13 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
         ctx->psi2.h = pp;
         ctx->psi2.e = pp + (size_t)ctx->psi.total * ctx->esz;
         HIP_OK(hipStreamSynchronize(ctx->stream));
-        // one work-group per (OEM_CR rows x 64 k) tile and x chunk; 16-plane chunks measured
-        // best at 512^3 (8: 2.20 ms, 16 and 32: 1.83 ms, 64: 2.15 ms per step of the kernel)
+        // one work-group per (OEM_CR rows x 64 k) tile and x chunk.  Measured step times:
+        // real fields at 512^3 (3 work-groups per CU), chunks 8 / 10 / 12 / 16 / 24 / 32: 2.00 /
+        // 1.96 / 1.93 / 1.94 / 1.93 / 1.94 ms; the complex Bloch cell (2 per CU), chunks 12 / 16
+        // / 24 / 32: 1.24 / 1.20 / 1.22 / 1.20 ms
         ctx->fg = g;
         ctx->fg.tiles = ((g.nz + 63) / 64) * ((g.ny + OEM_CR - 1) / OEM_CR);
-        int fchunk = 16;
+        int fchunk = ctx->cplx ? 16 : 12;
         if (ctx->chunk_req > 0) fchunk = ctx->chunk_req;
         else if (env && std::atoi(env) > 0) fchunk = std::atoi(env);
         ctx->fg.chunk = std::min(fchunk, g.nx);
