@@ -299,8 +299,9 @@ int oem_adjoint_gradient(oem_ctx *ctx, double *out, size_t n);
 /* ---- Serving pool: many resident scenes advanced by the same launches ------------------
  * No counterpart in the reference, which runs one scene per openem.solver.run call
  * (`openem/solver.py:403-511`) with its own allocations.  A pool owns one device arena of
- * arena_bytes (the admission budget) and a hipGraph of dft_batch steps whose persistent
- * kernels walk the tiles, point lists, ADE entries and monitors of every attached scene; each
+ * arena_bytes (the admission budget) and a hipGraph of dft_batch steps whose kernels, on
+ * fixed over-sized grids (blocks past the work exit), walk the tiles, point lists, ADE entries
+ * and monitors of every attached scene; each
  * scene keeps its own device step counter, so scenes at different steps share every launch and
  * one that reaches its n_steps simply stops.  Contexts made by oem_pool_ctx_create are built
  * and uploaded with the single-scene calls above (from any host thread), then attached. */

@@ -125,7 +125,7 @@ def _flux_of(st: solver.Stepper) -> list:
     out = []
     for i, m in enumerate(st.tabs.monitors):
         data = st.read_monitor(i)
-        mon = st.tabs.scene.flux_monitors[i] if m.kind == "plane" else None
+        mon = m.spec if m.kind == "plane" and m.modes is None else None
         out.append(flux.monitor_flux(data, st.tabs.scene.grid, mon) if mon is not None else data)
     return out
 

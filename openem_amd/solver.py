@@ -31,6 +31,24 @@ def _f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def plane_weights(tabs: tables_mod.StepTables, i: int, transverse: bool = True):
+    """Face weights ``(w1, w2)`` (float64, the plane's ``(n1, n2)`` layout) of plane monitor
+    ``i`` for the device flux reduction.  A flux monitor's transverse index window is applied
+    when ``transverse``; a mode monitor always spans its whole plane."""
+    m = tabs.monitors[i]
+    if m.kind != "plane":
+        raise ValueError(f"monitor {m.name} is not a plane monitor")
+    w1, w2 = flux_mod.area_weights(tabs.scene.grid, m.axis)
+    win = getattr(m.spec, "transverse", None) if m.modes is None else None
+    if transverse and win is not None:
+        mask = np.zeros(w1.shape)
+        (j0, j1), (k0, k1) = win
+        mask[j0:j1 + 1, k0:k1 + 1] = 1.0
+        w1, w2 = w1 * mask, w2 * mask
+    return (np.ascontiguousarray(w1, dtype=np.float64),
+            np.ascontiguousarray(w2, dtype=np.float64))
+
+
 class Stepper:
     """One scene resident on the GPU.  With ``pool`` (a :class:`openem_amd.serving.Pool`) its
     device memory comes from the pool's arena and the pool steps it once attached."""
@@ -268,19 +286,10 @@ class Stepper:
     def setup_flux(self, transverse: bool = True) -> None:
         """Upload the face weights of every plane monitor (``transverse``: apply the flux
         monitor's transverse window, as :func:`openem_amd.flux.monitor_flux` does)."""
-        sc = self.tabs.scene
         for i, m in enumerate(self.tabs.monitors):
             if m.kind != "plane":
                 continue
-            w1, w2 = flux_mod.area_weights(sc.grid, m.axis)
-            fm = sc.flux_monitors[i]
-            if transverse and fm.transverse is not None:
-                mask = np.zeros(w1.shape)
-                (j0, j1), (k0, k1) = fm.transverse
-                mask[j0:j1 + 1, k0:k1 + 1] = 1.0
-                w1, w2 = w1 * mask, w2 * mask
-            w1 = np.ascontiguousarray(w1, dtype=np.float64)
-            w2 = np.ascontiguousarray(w2, dtype=np.float64)
+            w1, w2 = plane_weights(self.tabs, i, transverse)
             _lib.check(self.lib, self.lib.oem_set_flux_weights(self.ctx, self.mon_ids[i],
                                                                _lib.ptr(w1), _lib.ptr(w2)))
 

@@ -94,6 +94,8 @@ class MonitorTables:
     origin: tuple = (0, 0, 0)
     box: tuple = (0, 0, 0)
     modes: list | None = None   # mode monitors: one solved Mode per frequency
+    # the scene monitor these tables were built from (FluxMonitor, FieldMonitor or ModeMonitor)
+    spec: object = None
 
 
 @dataclass
@@ -480,7 +482,7 @@ def monitor_tables(sc: Scene, n_total: int) -> list[MonitorTables]:
         out.append(MonitorTables(
             name=m.name, kind="plane", nf=len(m.freqs), freqs=np.asarray(m.freqs, np.float64),
             phase=phase_table(m.freqs, sc.dt, n_total, m.apodization),
-            axis=int(m.axis), km=int(m.plane_index), n1=shape[t1], n2=shape[t2]))
+            axis=int(m.axis), km=int(m.plane_index), n1=shape[t1], n2=shape[t2], spec=m))
     for m in sc.field_monitors:
         o, b = tuple(int(v) for v in m.origin), tuple(int(v) for v in m.box)
         for a in range(3):
@@ -488,7 +490,8 @@ def monitor_tables(sc: Scene, n_total: int) -> list[MonitorTables]:
                 raise ValueError(f"field monitor {m.name}: box outside the grid")
         out.append(MonitorTables(
             name=m.name, kind="box", nf=len(m.freqs), freqs=np.asarray(m.freqs, np.float64),
-            phase=phase_table(m.freqs, sc.dt, n_total, m.apodization), origin=o, box=b))
+            phase=phase_table(m.freqs, sc.dt, n_total, m.apodization), origin=o, box=b,
+            spec=m))
     if sc.mode_monitors:
         from . import modes as modes_mod
     for m in sc.mode_monitors:
@@ -500,7 +503,7 @@ def monitor_tables(sc: Scene, n_total: int) -> list[MonitorTables]:
         out.append(MonitorTables(
             name=m.name, kind="plane", nf=len(m.freqs), freqs=np.asarray(m.freqs, np.float64),
             phase=phase_table(m.freqs, sc.dt, n_total, None), axis=int(m.axis),
-            km=int(m.plane_index), n1=shape[t1], n2=shape[t2], modes=sol))
+            km=int(m.plane_index), n1=shape[t1], n2=shape[t2], modes=sol, spec=m))
     return out
 
 

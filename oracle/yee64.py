@@ -34,7 +34,10 @@ CPML reflection bound (``tests/test_pml_reflection.py:57-60`` of the reference),
 one-sidedness, T+R=1 and the slab transmission against the reference's published value, the
 dispersive == equivalent-lossy identity (``tests/test_dispersion.py:224-274``), zero-residue
 poles as no-ops, the windowed DFT == tensordot of the time series
-(``tests/test_apodization.py:171-205``), and brute-force loops on tiny grids.
+(``tests/test_apodization.py:171-205``), brute-force loops on tiny grids, and, for the point
+dipoles, the 1D Yee Green's function and the discrete Poynting balance of an electric and of a
+magnetic current sheet (tests/test_oracle_dipoles.py, which also plants sign, scale and sample
+time mistakes in a copy of :meth:`Oracle64._dipoles` and sees them fail).
 """
 
 from __future__ import annotations
@@ -140,12 +143,7 @@ class Oracle64:
         H[2] -= ch * (sx - sy)
         H[0] = hx_new
         H[1] = hy_new
-        for dp in t.dipoles:
-            if dp.magnetic:
-                v = ch * dp.coef * dp.amp[dp.src_of, n]
-                for c in range(3):
-                    sel = dp.comp == c
-                    np.subtract.at(H[c].reshape(-1), dp.cell[sel].astype(np.int64), v[sel])
+        self._dipoles(n, magnetic=True)
         for ps in t.plane_srcs:
             hc = 1 if ps.pol == 0 else 0
             H[hc][:, :, ps.kh] += ps.coef_h * ps.ex_inc[n]
@@ -186,13 +184,7 @@ class Oracle64:
             self._tfsf(tf, n, magnetic=False)
         for ms in self.sc.mode_sources:
             self._mode(ms, n, magnetic=False)
-        for dp in t.dipoles:
-            if not dp.magnetic:
-                for c in range(3):
-                    sel = dp.comp == c
-                    cells = dp.cell[sel].astype(np.int64)
-                    v = cb[c].reshape(-1)[cells] * dp.coef[sel] * dp.amp[dp.src_of[sel], n]
-                    np.subtract.at(E[c].reshape(-1), cells, v)
+        self._dipoles(n, magnetic=False)
         if d is not None:
             en = E[self._idx4]
             self.P += d.b * en.real[self._pole_entry]
@@ -202,6 +194,23 @@ class Oracle64:
         self._sample_time(n)
         self._sample_flux_time(n)
         self.n += 1
+
+    def _dipoles(self, n, magnetic):
+        """Point dipoles of one kind (``source_dft.cu:147-206``): a magnetic current sample
+        ``M^n = coef amp_int[n]`` (``t = n dt``) enters the H update as ``H -= ch M``, an electric
+        current sample ``J^{n+1/2} = coef amp_half[n]`` (``t = (n+1/2) dt``) the E update as
+        ``E -= cb J`` with the cell's cb; points on one (component, cell) add up."""
+        t = self.t
+        F = self.H if magnetic else self.E
+        for dp in t.dipoles:
+            if bool(dp.magnetic) != magnetic:
+                continue
+            for c in range(3):
+                sel = dp.comp == c
+                cells = dp.cell[sel].astype(np.int64)
+                scale = t.ch if magnetic else t.cb[c].reshape(-1)[cells]
+                v = scale * dp.coef[sel] * dp.amp[dp.src_of[sel], n]
+                np.subtract.at(F[c].reshape(-1), cells, v)
 
     def _mode(self, ms, n, magnetic):
         """Mode-source plane corrections: H side ``H_u += s_hu ch/dl Re(E_v amp_e[n])``,

@@ -63,3 +63,22 @@ def test_run_reports_mode_amplitudes(gpu_lib):
     am = abs(res.mode_amps["back"]["a_minus"][0]) / S
     print(f"|a+|/|S| = {ap:.5f}, backward {am:.2e}", flush=True)
     assert abs(ap - 0.5) < 5e-3 and am < 1e-3
+
+
+def test_run_with_default_arguments_on_a_mode_scene(gpu_lib):
+    """``solver.run`` with its defaults (shutoff on) on a scene holding mode monitors: the flux
+    weights of every plane monitor upload, and the mode amplitudes equal those of the run
+    without shutoff (the scene stops no earlier than its last step here, so both runs step the
+    same number of steps)."""
+    n = 600
+    sc = scenes.waveguide_scene(shape=(44, 36, 32), n_pml=8, width_cells=10, thick_cells=6,
+                                fwidth_rel=0.2, num_steps=n, nf=2)
+    print(f"run() defaults on waveguide {sc.shape}, {n} steps", flush=True)
+    a = solver.run(sc, verbose=False)
+    b = solver.run(sc, use_shutoff=False, verbose=False)
+    assert a.steps_run == b.steps_run == n, (a.steps_run, a.stop_reason)
+    for name in ("fwd", "back"):
+        for key in ("a_plus", "a_minus"):
+            assert np.array_equal(a.mode_amps[name][key], b.mode_amps[name][key]), (name, key)
+    for name in ("fwd_flux", "back_flux"):
+        assert np.array_equal(a.phasors[name].data, b.phasors[name].data), name

@@ -133,3 +133,32 @@ def test_mode_point_tables_equal_the_oracle_plane_corrections():
                           scale * coef[sel] * np.real(ph[sel] * amp[n]))
             assert np.abs(want).max() > 0
             assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+def test_plane_weights_map_every_plane_monitor_to_its_own_spec():
+    """Mode monitors are planes too and follow the flux monitors in the table list: each plane
+    monitor's face weights come from its own scene monitor (a flux monitor's transverse window;
+    a mode monitor's whole plane), never from ``sc.flux_monitors`` by table index."""
+    import dataclasses
+    from openem_amd.solver import plane_weights
+    sc = scenes.waveguide_scene(shape=(30, 20, 18), n_pml=5, width_cells=6, thick_cells=4,
+                                num_steps=20, nf=1)
+    win = ((2, 15), (3, 12))
+    fl = [dataclasses.replace(sc.flux_monitors[0], transverse=win), sc.flux_monitors[1]]
+    sc = dataclasses.replace(sc, flux_monitors=fl)
+    t = tables.build(sc)
+    kinds = [(m.name, m.kind, m.modes is not None) for m in t.monitors]
+    assert kinds == [("fwd_flux", "plane", False), ("back_flux", "plane", False),
+                     ("fwd", "plane", True), ("back", "plane", True)]
+    full = flux.area_weights(sc.grid, 0)
+    for i, m in enumerate(t.monitors):
+        assert m.spec.name == m.name
+        w1, w2 = plane_weights(t, i)
+        if i == 0:
+            mask = np.zeros(full[0].shape)
+            mask[2:16, 3:13] = 1.0
+            assert np.array_equal(w1, full[0] * mask) and np.array_equal(w2, full[1] * mask)
+            u1, _ = plane_weights(t, i, transverse=False)
+            assert np.array_equal(u1, full[0])
+        else:
+            assert np.array_equal(w1, full[0]) and np.array_equal(w2, full[1])
