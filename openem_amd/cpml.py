@@ -1,19 +1,25 @@
-"""CPML profiles and recursion coefficients, plus SI constants.
+"""CPML coefficients of one PML face, and the SI constants.
 
-The grading follows tidy3d's PML definition as the reference restates it
-(``openem/cpml.py:1-16``, ``:55-128``)::
+Grading: tidy3d's published PML definition (``PML`` / ``PMLParams`` in tidy3d's documentation;
+the reference states it in ``openem/cpml.py:1-16``).  At normalised depth ``d`` in [0, 1]
+(0 at the interface with the domain, 1 at the outer wall)::
 
-    sigma(x) = sigma_max * v / (eta0 * dl) * step**sigma_order
-    kappa(x) = kappa_min + (kappa_max - kappa_min) * step**kappa_order
-    alpha(x) = alpha_min + (alpha_max - alpha_min) * step**alpha_order
+    sigma(d) = sigma_max * v / (eta0 * dl) * d**sigma_order      dl: the grid step at that face
+    kappa(d) = kappa_min + (kappa_max - kappa_min) * d**kappa_order
+    alpha(d) = alpha_min + (alpha_max - alpha_min) * d**alpha_order
 
-``step`` is the normalised depth at the E or H position of each layer (E positions are the
-backward-difference points, H positions are half a cell further out). The psi recursion is the
-Roden-Gedney form ``psi <- b*psi + a*d`` with
+Depth of a sample: a face of ``n`` layers spans ``n`` cells; E samples sit on primal edges and H
+samples at cell centres, so a sample ``p`` cells from the interface has depth ``p / n``.  On a
+low face the interface is the stack's high edge (the E sample on the outer wall has depth 1,
+the last H sample depth 1/(2n)); on a high face it is the stack's low edge (the first E sample
+lies on the interface itself, depth 0).
 
-    b = exp(-dt/eps0 * (sigma/kappa + alpha)),   a = (b - 1) * sigma / (sigma + alpha*kappa) / kappa
+Recursion: the convolutional PML of Roden & Gedney (2000), ``psi <- b psi + a d`` with::
 
-and ``a = 0`` where ``sigma = 0``.
+    b = exp(-(sigma / kappa + alpha) dt / eps0)
+    a = sigma (b - 1) / (kappa (sigma + kappa alpha))      (a = 0 where sigma = 0)
+
+and the stretched derivative ``d / kappa + psi``.
 """
 
 from __future__ import annotations
@@ -30,7 +36,8 @@ C_0 = 299792458.0                  # m/s
 
 @dataclass(frozen=True)
 class PMLParams:
-    """``td.PML()`` defaults (reference ``openem/cpml.py:37-52``)."""
+    """Grading parameters of one face; the defaults are ``td.PML()``'s (field names as tidy3d's
+    ``PMLParams``)."""
 
     num_layers: int
     sigma_max: float = 1.5
@@ -42,40 +49,39 @@ class PMLParams:
     alpha_max: float = 0.0
     alpha_order: int = 1
 
+    def grade(self, depth: np.ndarray, dl: float, avg_speed: float = 1.0):
+        """``(sigma, kappa, alpha)`` at the given normalised depths."""
+        sigma_peak = self.sigma_max * avg_speed / (ETA_0 * dl)
+        sigma = sigma_peak * depth ** self.sigma_order
+        kappa = self.kappa_min + (self.kappa_max - self.kappa_min) * depth ** self.kappa_order
+        alpha = self.alpha_min + (self.alpha_max - self.alpha_min) * depth ** self.alpha_order
+        return sigma, kappa, alpha
 
-def steps(n_pml: int, side: str, at: str) -> np.ndarray:
-    """Normalised depth of each layer, ordered by increasing global index
-    (reference ``openem/cpml.py:55-76``). The E side is deliberately asymmetric: at the low end
-    the outermost E layer has depth 1, at the high end the innermost E layer has depth 0."""
-    if n_pml <= 0:
+
+def sample_depths(n_layers: int, side: str, at: str) -> np.ndarray:
+    """Normalised depth of each layer's E (``at="E"``) or H sample, by increasing global
+    index (module docstring)."""
+    if n_layers <= 0:
         return np.zeros(0)
-    i = np.arange(n_pml, dtype=np.float64)
-    if side == "lo":
-        return (n_pml - i) / n_pml if at == "E" else (n_pml - i - 0.5) / n_pml
-    return i / n_pml if at == "E" else (i + 0.5) / n_pml
+    pos = np.arange(n_layers, dtype=np.float64) + (0.5 if at == "H" else 0.0)
+    from_interface = n_layers - pos if side == "lo" else pos
+    return from_interface / n_layers
 
 
-def profile(p: PMLParams, dl: float, side: str, at: str, avg_speed: float = 1.0):
-    """``(sigma, kappa, alpha)`` per layer (reference ``openem/cpml.py:79-112``)."""
-    s = steps(p.num_layers, side, at)
-    sigma = p.sigma_max * avg_speed / (ETA_0 * dl) * s ** p.sigma_order
-    kappa = p.kappa_min + (p.kappa_max - p.kappa_min) * s ** p.kappa_order
-    alpha = p.alpha_min + (p.alpha_max - p.alpha_min) * s ** p.alpha_order
-    return sigma, kappa, alpha
-
-
-def recursion_coeffs(sigma, kappa, alpha, dt: float):
-    """``(a, b)`` of ``psi <- b*psi + a*d`` (reference ``openem/cpml.py:115-128``)."""
-    b = np.exp(-dt / EPSILON_0 * (sigma / kappa + alpha))
-    denom = sigma + alpha * kappa
-    with np.errstate(divide="ignore", invalid="ignore"):
-        a = (b - 1.0) * sigma / denom / kappa
-    return np.nan_to_num(a, nan=0.0, posinf=0.0, neginf=0.0), b
+def recursion(sigma: np.ndarray, kappa: np.ndarray, alpha: np.ndarray, dt: float):
+    """``(a, b)`` of the psi recursion at each sample."""
+    b = np.exp(-(dt / EPSILON_0) * (sigma / kappa + alpha))
+    a = np.zeros_like(b)
+    live = sigma > 0
+    s, k, al = sigma[live], kappa[live], alpha[live]
+    a[live] = s * (b[live] - 1.0) / (k * (s + k * al))
+    return a, b
 
 
 @dataclass(frozen=True)
 class PMLCoeffs:
-    """Coefficients of one PML face, each ``(num_layers,)``."""
+    """Coefficients of one PML face, each ``(num_layers,)`` (the Scene contract of the
+    reference's ``cpml.PMLCoeffs``)."""
 
     a_E: np.ndarray
     b_E: np.ndarray
@@ -89,16 +95,16 @@ class PMLCoeffs:
         return int(self.a_E.size)
 
 
-def build(p: PMLParams, dl: float, side: str, dt: float, avg_speed: float = 1.0) -> PMLCoeffs:
-    """Parameters -> profile -> recursion coefficients (reference ``openem/cpml.py:149-164``)."""
-    out = {}
-    for at in ("E", "H"):
-        sigma, kappa, alpha = profile(p, dl, side, at, avg_speed)
-        a, b = recursion_coeffs(sigma, kappa, alpha, dt)
-        out[f"a_{at}"] = a
-        out[f"b_{at}"] = b
-        out[f"inv_kappa_{at}"] = 1.0 / kappa
-    return PMLCoeffs(**out)
+def face_coeffs(p: PMLParams, dl: float, side: str, dt: float,
+                avg_speed: float = 1.0) -> PMLCoeffs:
+    """The face's coefficients at its E and H samples."""
+    def at(where):
+        sigma, kappa, alpha = p.grade(sample_depths(p.num_layers, side, where), dl, avg_speed)
+        a, b = recursion(sigma, kappa, alpha, dt)
+        return a, b, 1.0 / kappa
+
+    (ae, be, ke), (ah, bh, kh) = at("E"), at("H")
+    return PMLCoeffs(a_E=ae, b_E=be, inv_kappa_E=ke, a_H=ah, b_H=bh, inv_kappa_H=kh)
 
 
 def courant_dt(dls: tuple[float, float, float], courant: float = 0.99) -> float:

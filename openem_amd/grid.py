@@ -86,56 +86,59 @@ class Axis:
 
     @property
     def dl_dual(self) -> np.ndarray:
-        """Dual spacing ``(n,)`` (E update): ``(dl[i] + dl[i-1])/2``; at i=0 a periodic axis
-        wraps to ``dl[n-1]``, an absorbing end extends with ``dl[0]``
-        (reference ``openem/grid.py:198-215``)."""
-        dl = self.dl
-        prev = np.empty_like(dl)
-        prev[1:] = dl[:-1]
-        prev[0] = dl[-1] if self.is_periodic else dl[0]
-        return 0.5 * (dl + prev)
+        """Dual spacing ``(n,)`` (E update): the mean of the two primal cells that meet at each
+        primal edge.  Before cell 0 a periodic axis sees its last cell, an absorbing end a copy
+        of cell 0 (the grid continued one cell outwards)."""
+        half = 0.5 * self.dl
+        out = half + np.roll(half, 1)
+        if not self.is_periodic:
+            out[0] = self.dl[0]
+        return out
 
     @property
     def centers(self) -> np.ndarray:
         return 0.5 * (self.edges[:-1] + self.edges[1:])
 
-    def index_tables(self, bloch_k: float = 0.0) -> dict[str, np.ndarray]:
-        """Neighbour indices and masks that encode the boundary topology
-        (reference ``openem/grid.py:219-304``).
+    def _end_image(self, side: str, bloch_k: float):
+        """What a difference reads past one end of the axis: ``(index, weight, pec)``.
 
-        - ``nxt``/``prv``: forward/backward neighbour (wrap on a periodic axis, clamp otherwise)
-        - ``mnx``/``mpv``: 1 where that neighbour exists, 0 outside an absorbing end; on a Bloch
-          axis the wrap entries carry ``exp(+-i 2 pi k)`` and the arrays are complex128
-        - ``pec``: 0 at i=0 of an absorbing axis (the PML is backed by PEC)
+        The forward difference at the last cell reads E one edge past the high end, the backward
+        difference at cell 0 reads H one centre before the low end.  ``weight`` multiplies that
+        read (0 where the field outside is zero), ``index`` is the in-range cell it is read
+        from, and ``pec`` (low end only) is what tangential E on edge 0 is multiplied by.
         """
         n = self.n
-        i = np.arange(n)
         if self.is_periodic:
-            mnx = np.ones(n)
-            mpv = np.ones(n)
-            if bloch_k != 0.0:
-                mnx = mnx.astype(np.complex128)
-                mpv = mpv.astype(np.complex128)
-                mnx[n - 1] = np.exp(+2j * np.pi * bloch_k)
-                mpv[0] = np.exp(-2j * np.pi * bloch_k)
-            return {"nxt": ((i + 1) % n).astype(np.int32),
-                    "prv": ((i - 1) % n).astype(np.int32),
-                    "mnx": mnx, "mpv": mpv, "pec": np.ones(n)}
+            # Bloch condition u(x + L) = u(x) exp(+i 2 pi k): the wrap picks up the phase
+            ph = np.exp((2j if side == "hi" else -2j) * np.pi * bloch_k) if bloch_k else 1.0
+            return (0 if side == "hi" else n - 1), ph, 1.0
         if bloch_k != 0.0:
             raise ValueError(f"bloch_k={bloch_k} needs a periodic axis, this one is "
                              f"{self.boundary_lo}/{self.boundary_hi}")
-        pec = np.ones(n)
-        mpv = (i > 0).astype(np.float64)
-        nxt = np.minimum(i + 1, n - 1).astype(np.int32)
-        mnx = (i + 1 < n).astype(np.float64)
-        if self.boundary_lo == "PMCBoundary":
-            mpv[0] = -1.0
-        else:
-            pec[0] = 0.0
-        if self.boundary_hi == "PMCBoundary":
-            mnx[n - 1] = 1.0
-        return {"nxt": nxt, "prv": np.maximum(i - 1, 0).astype(np.int32),
-                "mnx": mnx, "mpv": mpv, "pec": pec}
+        kind = self.boundary_lo if side == "lo" else self.boundary_hi
+        if kind.startswith("Symmetry"):
+            raise NotImplementedError(f"{kind}: symmetry folding is not supported")
+        if kind == "PMCBoundary":
+            # magnetic wall: tangential H is odd about it, tangential E even
+            return (0, -1.0, 1.0) if side == "lo" else (n - 1, 1.0, 1.0)
+        # PEC wall, or the PEC that backs a PML: nothing is read from outside, and tangential
+        # E on the low wall is held at zero
+        return (0 if side == "lo" else n - 1), 0.0, 0.0
+
+    def index_tables(self, bloch_k: float = 0.0) -> dict[str, np.ndarray]:
+        """Neighbour indices and weights encoding the boundary topology (the reference's
+        contract, ``openem/grid.py:219-304``): ``nxt``/``prv`` the forward/backward neighbour of
+        every cell (int32), ``mnx``/``mpv`` the weight of that neighbour (1 inside; at the ends
+        the image of :meth:`_end_image`, complex128 on a Bloch axis), ``pec`` 1 except on a PEC
+        low wall."""
+        n = self.n
+        cell = np.arange(n, dtype=np.int32)
+        nxt, prv = np.roll(cell, -1), np.roll(cell, 1)
+        dtype = np.complex128 if (self.is_periodic and bloch_k != 0.0) else np.float64
+        mnx, mpv, pec = np.ones(n, dtype), np.ones(n, dtype), np.ones(n)
+        nxt[n - 1], mnx[n - 1], _ = self._end_image("hi", bloch_k)
+        prv[0], mpv[0], pec[0] = self._end_image("lo", bloch_k)
+        return {"nxt": nxt, "prv": prv, "mnx": mnx, "mpv": mpv, "pec": pec}
 
 
 @dataclass(frozen=True)
@@ -157,26 +160,26 @@ class Grid:
         nx, ny, nz = self.shape
         return nx * ny * nz
 
+    def _yee_lengths(self, name: str) -> list[np.ndarray]:
+        """Per-axis lengths of the dual cell around component ``name``'s Yee points: the dual
+        spacing along an axis where the point sits on a primal edge, the primal one where it
+        sits at a cell centre."""
+        return [a.dl_dual if on else a.dl for a, on in zip(self.axes, YEE_ON_EDGE[name])]
+
     def yee_dual_volumes(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
-        """Dual volume of Ex/Ey/Ez at their own Yee positions
-        (reference ``openem/grid.py:350-368``)."""
-        xp, yp, zp = (a.dl for a in self.axes)
-        xd, yd, zd = (a.dl_dual for a in self.axes)
-        return outer3(xp, yd, zd), outer3(xd, yp, zd), outer3(xd, yd, zp)
+        """Volume ``(nx, ny, nz)`` of the dual cell of Ex, Ey, Ez at their Yee points
+        (reference ``openem/grid.py:350-368``: Ex -> dx dy_dual dz_dual, ...)."""
+        return tuple(outer3(*self._yee_lengths(c)) for c in ("Ex", "Ey", "Ez"))
+
+    def yee_h_dual_volumes(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """The same for Hx, Hy, Hz (reference ``openem/grid.py:370-386``)."""
+        return tuple(outer3(*self._yee_lengths(c)) for c in ("Hx", "Hy", "Hz"))
 
     def dual_volume_at(self, comp: int, magnetic: bool, i: int, j: int, k: int) -> float:
         """One entry of :meth:`yee_dual_volumes` (``magnetic``: :meth:`yee_h_dual_volumes`),
         with the same multiplication order."""
-        primal = (comp != 0, comp != 1, comp != 2) if magnetic else \
-            (comp == 0, comp == 1, comp == 2)
-        f = [a.dl if p else a.dl_dual for a, p in zip(self.axes, primal)]
-        return float((f[0][i] * f[1][j]) * f[2][k])
-
-    def yee_h_dual_volumes(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
-        """Dual volume of Hx/Hy/Hz (reference ``openem/grid.py:370-386``)."""
-        xp, yp, zp = (a.dl for a in self.axes)
-        xd, yd, zd = (a.dl_dual for a in self.axes)
-        return outer3(xd, yp, zp), outer3(xp, yd, zp), outer3(xp, yp, zd)
+        lx, ly, lz = self._yee_lengths(("H" if magnetic else "E") + "xyz"[comp])
+        return float((lx[i] * ly[j]) * lz[k])
 
     def yee_coords(self, comp: int, magnetic: bool) -> tuple[np.ndarray, ...]:
         """Per-axis coordinates of a component's Yee points."""
@@ -193,12 +196,20 @@ def uniform_axis(n: int, dl: float, lo: float, boundary_lo: str,
 
 
 def nearest_edge_index(edges: np.ndarray, coord: float) -> int:
-    """Snap a plane coordinate to the nearest primal boundary index (within half a local cell),
-    as the reference does for source and flux planes (``openem/grid.py:397-425``)."""
-    i = int(np.argmin(np.abs(edges - coord)))
-    resid = abs(float(edges[i]) - coord)
-    dl = np.diff(edges)
-    local = max(float(dl[i]) if i < dl.size else 0.0, float(dl[i - 1]) if i > 0 else 0.0)
-    if resid > 0.5 * local:
-        raise ValueError(f"coordinate {coord!r} is {resid!r} off the nearest edge {edges[i]!r}")
+    """Index of the primal edge nearest to a plane coordinate (sources and flux planes sit on
+    edges, where tangential E lives; the reference snaps the same way, ``openem/grid.py:397``).
+    A coordinate inside the grid always has an edge within half its cell; one outside must lie
+    within half of the end cell, else it is refused (ties go to the lower edge)."""
+    e = np.asarray(edges, dtype=np.float64)
+    k = int(np.clip(np.searchsorted(e, coord), 1, e.size - 1))
+    i = k - 1 if coord - e[k - 1] <= e[k] - coord else k
+    if coord < e[0]:
+        reach = 0.5 * (e[1] - e[0])
+    elif coord > e[-1]:
+        reach = 0.5 * (e[-1] - e[-2])
+    else:
+        reach = np.inf
+    if abs(coord - e[i]) > reach:
+        raise ValueError(f"coordinate {coord!r} is {abs(coord - e[i])!r} off the grid's end "
+                         f"edge {e[i]!r}")
     return i

@@ -92,7 +92,7 @@ def layer_eps_z(grid: Grid, z0: float, z1: float, eps_in: float, eps_out: float 
 
 def pml_faces(dt: float, dls, axes_sides, n_layers: int = 12) -> dict:
     p = cpml.PMLParams(num_layers=n_layers)
-    return {(a, s): cpml.build(p, dls[a], s, dt) for a, s in axes_sides}
+    return {(a, s): cpml.face_coeffs(p, dls[a], s, dt) for a, s in axes_sides}
 
 
 def slab_scene(dl: float = 0.02 * UM, size=(0.6 * UM, 0.6 * UM, 3.0 * UM), eps_slab: float = 4.0,

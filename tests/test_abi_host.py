@@ -85,7 +85,7 @@ def test_cpml_profile_closed_form():
     (reference openem/cpml.py:55-112)."""
     p = cpml.PMLParams(num_layers=12)
     dl, dt = 20e-9, 3.8e-17
-    c = cpml.build(p, dl, "lo", dt)
+    c = cpml.face_coeffs(p, dl, "lo", dt)
     s_e = (12 - np.arange(12)) / 12
     sig = 1.5 / (cpml.ETA_0 * dl) * s_e ** 3
     kap = 1 + 2 * s_e ** 3
@@ -93,7 +93,7 @@ def test_cpml_profile_closed_form():
     assert np.allclose(c.b_E, b, rtol=1e-14)
     assert np.allclose(c.a_E, (b - 1) * sig / sig / kap, rtol=1e-14)
     assert np.allclose(c.inv_kappa_E, 1 / kap)
-    hi = cpml.build(p, dl, "hi", dt)
+    hi = cpml.face_coeffs(p, dl, "hi", dt)
     assert hi.a_E[0] == 0.0 and hi.b_E[0] == 1.0          # innermost E layer at depth 0
 
 
