@@ -40,7 +40,7 @@ enum oem_kernel_id {
     OEM_K_UPDATE_H = 0, OEM_K_UPDATE_E = 1, OEM_K_DIPOLE_E = 2, OEM_K_DIPOLE_H = 3,
     OEM_K_DISP_PRE = 4, OEM_K_DISP_POST = 5, OEM_K_DFT_SNAP = 6, OEM_K_DFT_FLUSH = 7,
     OEM_K_STEP = 8, OEM_K_TFSF_H = 9, OEM_K_TFSF_E = 10, OEM_K_MODE_H = 11, OEM_K_MODE_E = 12,
-    OEM_K_UPDATE_HE = 13, OEM_K_COUNT = 14
+    OEM_K_UPDATE_HE = 13, OEM_K_STEP_TAIL = 14, OEM_K_COUNT = 15
 };
 
 typedef struct oem_ctx oem_ctx;
@@ -197,8 +197,19 @@ int oem_read_flux_time_monitor(oem_ctx *ctx, int32_t mon_id, double *out, size_t
 
 /* Launch shape: x-planes each work-group marches (0 = the library's choice).  Before
  * oem_prepare.  No counterpart in the reference (its launch shapes are fixed in the kernels'
- * callers, `openem/solver.py:876-1162`); it is the knob the launch autotuner sets. */
+ * callers, `openem/solver.py:876-1162`, and its block size is the hand-set BLOCK3 knob,
+ * `openem/knobs.py:64`); set by the launch autotuner (openem_amd/autotune.py). */
 int oem_set_chunk(oem_ctx *ctx, int32_t chunk);
+
+/* Step form: 0 = the library's choice (the fused H+E march from 12 M cells, else the two
+ * sweeps; OEM_FUSE=0/1 forces one), 1 = the two sweeps, 2 = the fused march (OEM_ENOTSUP when a
+ * field set exceeds its 32-bit offsets or the context is pooled).  Before oem_prepare.  No
+ * counterpart in the reference (its H+E fusion is a global knob, `openem/knobs.py:61`); set by
+ * the launch autotuner (openem_amd/autotune.py) together with oem_set_chunk. */
+int oem_set_step_form(oem_ctx *ctx, int32_t form);
+
+/* The launch plan oem_prepare chose: form 1 (two sweeps) or 2 (fused) and its x chunk. */
+int oem_launch_plan(oem_ctx *ctx, int32_t *form, int32_t *chunk);
 
 /* Freeze the tables, build the launch plan.  Must precede oem_run. */
 int oem_prepare(oem_ctx *ctx);

@@ -18,7 +18,7 @@ LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 OEM_OK, OEM_EINVAL, OEM_ENOMEM, OEM_EHIP, OEM_ESTATE, OEM_ENOTSUP = range(6)
 KERNELS = ("update_h", "update_e", "dipole_e", "dipole_h", "disp_pre", "disp_post",
            "dft_snap", "dft_flush", "step_advance", "tfsf_h", "tfsf_e", "mode_h", "mode_e",
-           "update_he")
+           "update_he", "step_tail")
 K_COUNT = len(KERNELS)
 
 #: every symbol include/openem_hip.h and include/openem_host.h declare (checked by the CPU
@@ -30,8 +30,10 @@ EXPORTS = ("oem_version", "oem_last_error", "oem_device_count", "oem_ctx_create"
            "oem_add_field_box", "oem_add_time_monitor", "oem_read_time_monitor",
            "oem_add_flux_time_monitor", "oem_read_flux_time_monitor", "oem_prepare",
            "oem_run", "oem_profile_run", "oem_timed_run", "oem_step_index", "oem_read_monitor",
-           "oem_read_field", "oem_write_field", "oem_set_chunk", "oem_set_energy_tables",
-           "oem_energy", "oem_set_flux_weights", "oem_monitor_flux", "oem_set_overlap_weights",
+           "oem_read_field", "oem_write_field", "oem_set_chunk", "oem_set_step_form",
+           "oem_launch_plan", "oem_set_energy_tables",
+           "oem_energy", "oem_set_flux_weights", "oem_monitor_flux",
+           "oem_set_overlap_weights",
            "oem_monitor_overlap", "oem_checkpoint_alloc", "oem_checkpoint_save",
            "oem_checkpoint_load", "oem_adjoint_setup", "oem_adjoint_objective",
            "oem_adjoint_record", "oem_adjoint_forward", "oem_adjoint_back", "oem_adjoint_gradient", "oem_pool_create",
@@ -93,6 +95,8 @@ _SIGS = {
     "oem_read_field": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "oem_write_field": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "oem_set_chunk": ([_P, C.c_int32], C.c_int),
+    "oem_set_step_form": ([_P, C.c_int32], C.c_int),
+    "oem_launch_plan": ([_P, _P, _P], C.c_int),
     "oem_set_energy_tables": ([_P] + [_P] * 7 + [C.c_int32], C.c_int),
     "oem_energy": ([_P, _P], C.c_int),
     "oem_set_flux_weights": ([_P, C.c_int32, _P, _P], C.c_int),

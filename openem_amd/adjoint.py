@@ -145,11 +145,21 @@ def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
     return Gradient(J=abs(a) ** 2, a_plus=a, grad=out, steps=n, segments=n_seg)
 
 
-def mode_power_gradients(jobs, concurrency: int = 4, **kw) -> list:
+def mode_power_gradients(jobs, concurrency: int = 4, on_done=None, **kw) -> list:
     """:func:`mode_power_gradient` for many designs, ``concurrency`` of them in flight at once:
     each ``(scene, monitor, comp, cell)`` job has its own context (its own HIP stream, fields,
     checkpoints and adjoint state), so one design's host work and launch gaps overlap another's
-    kernels.  Results are in job order."""
+    kernels (``concurrency`` 1: one after another on the calling thread).  ``on_done(index,
+    gradient)`` is called as each design finishes.  Results are in job order."""
     from concurrent.futures import ThreadPoolExecutor
-    with ThreadPoolExecutor(max(1, int(concurrency))) as ex:
-        return list(ex.map(lambda j: mode_power_gradient(*j, **kw), jobs))
+
+    def one(ij):
+        g = mode_power_gradient(*ij[1], **kw)
+        if on_done is not None:
+            on_done(ij[0], g)
+        return g
+
+    if int(concurrency) <= 1:               # one at a time, on the calling thread
+        return [one(ij) for ij in enumerate(jobs)]
+    with ThreadPoolExecutor(int(concurrency)) as ex:
+        return list(ex.map(one, enumerate(jobs)))

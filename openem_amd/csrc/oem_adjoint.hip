@@ -121,14 +121,23 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     // one (j, k) column per thread, numbered along the plane: no idle lanes when nz is not a
-    // multiple of 64 (a 48-cell-thick slab leaves a quarter of (k, j) tiles empty)
-    const int q = blockIdx.x * (OEM_ADJ_TK * OEM_ADJ_TJ) + threadIdx.y * OEM_ADJ_TK + threadIdx.x;
-    if (q >= ny * nz) return;
+    // multiple of 64 (a 48-cell-thick slab leaves a quarter of (k, j) tiles empty).  Threads
+    // past the last column take the last one and store nothing: every thread reaches the
+    // per-plane barrier.
+    const int tid = threadIdx.y * OEM_ADJ_TK + threadIdx.x;
+    const int base = blockIdx.x * (OEM_ADJ_TK * OEM_ADJ_TJ);
+    const bool act = base + tid < ny * nz;
+    const int q = act ? base + tid : ny * nz - 1;
     const int j = q / nz, k = q - j * nz;
     const int i0 = blockIdx.z * d.chunk, i1 = min(i0 + d.chunk, nx);
     const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
     const uint32_t *off = d.psl.off;
+    // the y and z terms every thread computes for its own cell, published per plane so the
+    // threads of the backward neighbours (j - 1, k - 1 in the interior) read them instead of
+    // recomputing them from the neighbour's cell (double-buffered by plane parity: one barrier
+    // per plane)
+    __shared__ float s_q[2][4][OEM_ADJ_TK * OEM_ADJ_TJ];
     // y and z at the cell and at the forward neighbours Y.nxt[j], Z.nxt[k]
     const AxAt yc = ax_at(Y, ny, j, AX_AE, AX_KE, AX_IDLD, AX_MPV);
     const AxAt zc = ax_at(Z, nz, k, AX_AE, AX_KE, AX_IDLD, AX_MPV);
@@ -137,6 +146,11 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     const bool y_nb = jn != j && Y.prv[jn] == j, z_nb = kn != k && Z.prv[kn] == k;
     const AxAt yn = ax_at(Y, ny, jn, AX_AE, AX_KE, AX_IDLD, AX_MPV);
     const AxAt zn = ax_at(Z, nz, kn, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+    // the neighbours' threads in this block, if they are in it
+    const int ly = jn * nz + k - base, lz = j * nz + kn - base;
+    const bool y_lds = y_nb && ly >= 0 && ly < OEM_ADJ_TK * OEM_ADJ_TJ;
+    const bool z_lds = z_nb && lz >= 0 && lz < OEM_ADJ_TK * OEM_ADJ_TJ;
+    int buf = 0;
 
     // the x terms (t3: c1, sgn -1, slot 0; t4: c2, sgn +1, slot 1) of a cell on plane i
     auto qx = [&](int i, float &q3, float &q4) {
@@ -198,9 +212,19 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         const float q2 = q_of(1.f * nu1, zc, 1.f, nu1, pz1);      // t2: c1, z, slot 5
         const float q3 = q_of(-1.f * nu1, xc, -1.f, nu1, px0);    // t3: c1, x, slot 0
         const float q4 = q_of(1.f * nu2, xc, 1.f, nu2, px1);      // t4: c2, x, slot 1
-        // neighbours' terms: y (t0, t5) at Y.nxt[j], z (t1, t2) at Z.nxt[k], x carried
+        s_q[buf][0][tid] = q0;
+        s_q[buf][1][tid] = q5;
+        s_q[buf][2][tid] = q1;
+        s_q[buf][3][tid] = q2;
+        __syncthreads();
+        // neighbours' terms: y (t0, t5) at Y.nxt[j], z (t1, t2) at Z.nxt[k], x carried; from
+        // the neighbour's thread when it is in the block, else recomputed with the same
+        // expressions
         float q0n = 0.f, q5n = 0.f, q1n = 0.f, q2n = 0.f;
-        if (y_nb) {
+        if (y_lds) {
+            q0n = s_q[buf][0][ly];
+            q5n = s_q[buf][1][ly];
+        } else if (y_nb) {
             const uint32_t idn = (uint32_t)i * plane + (uint32_t)(jn * nz + k);
             float b0, b1, b2;
             cb3(mat, s_lut, lds, idn, nc, b0, b1, b2);
@@ -215,7 +239,10 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
             q5n = q_of(-1.f * n2, yn, -1.f, n2, a1);
             (void)b1;
         }
-        if (z_nb) {
+        if (z_lds) {
+            q1n = s_q[buf][2][lz];
+            q2n = s_q[buf][3][lz];
+        } else if (z_nb) {
             const uint32_t idn = (uint32_t)i * plane + (uint32_t)(j * nz + kn);
             float b0, b1, b2;
             cb3(mat, s_lut, lds, idn, nc, b0, b1, b2);
@@ -247,6 +274,10 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         const float v3 = term(q3, xc, x_lo, x_nb, q3n, mxn, ixdn);
         const float v4 = term(q4, xc, x_lo, x_nb, q4n, mxn, ixdn);
         const float v5 = term(q5, yc, y_lo, y_nb, q5n, yn.m, yn.idl);
+        buf ^= 1;
+        q3n = q3;
+        q4n = q4;
+        if (!act) continue;
         float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
         acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
         d.lH[id] += acc0;
@@ -274,8 +305,6 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         d.lE_n[id] = ax_ * (yc.pec * zc.pec) * l0;
         d.lE_n[nc + id] = ay_ * (zc.pec * xc.pec) * l1;
         d.lE_n[2 * nc + id] = az_ * (xc.pec * yc.pec) * l2;
-        q3n = q3;
-        q4n = q4;
     }
 }
 
@@ -290,16 +319,20 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
     const GridDev &g = d.g;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    // one (j, k) column per thread, numbered along the plane: no idle lanes when nz is not a
-    // multiple of 64 (a 48-cell-thick slab leaves a quarter of (k, j) tiles empty)
-    const int q = blockIdx.x * (OEM_ADJ_TK * OEM_ADJ_TJ) + threadIdx.y * OEM_ADJ_TK + threadIdx.x;
-    if (q >= ny * nz) return;
+    // one (j, k) column per thread, numbered along the plane (threads past the last column take
+    // the last one and store nothing, so every thread reaches the per-plane barrier)
+    const int tid = threadIdx.y * OEM_ADJ_TK + threadIdx.x;
+    const int base = blockIdx.x * (OEM_ADJ_TK * OEM_ADJ_TJ);
+    const bool act = base + tid < ny * nz;
+    const int q = act ? base + tid : ny * nz - 1;
     const int j = q / nz, k = q - j * nz;
     const int i0 = blockIdx.z * d.chunk, i1 = min(i0 + d.chunk, nx);
     const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
     const uint32_t *off = d.psl.off;
     const float mch = -d.ch;
+    // each thread's y and z terms, published per plane for the forward neighbours' threads
+    __shared__ float s_q[2][4][OEM_ADJ_TK * OEM_ADJ_TJ];
     const AxAt yc = ax_at(Y, ny, j, AX_AH, AX_KH, AX_IDLP, AX_MNX);
     const AxAt zc = ax_at(Z, nz, k, AX_AH, AX_KH, AX_IDLP, AX_MNX);
     const int jp = Y.prv[j], kp = Z.prv[k];
@@ -307,6 +340,10 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
     const bool y_nb = jp != j && Y.nxt[jp] == j, z_nb = kp != k && Z.nxt[kp] == k;
     const AxAt yp = ax_at(Y, ny, jp, AX_AH, AX_KH, AX_IDLP, AX_MNX);
     const AxAt zp = ax_at(Z, nz, kp, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+    const int ly = jp * nz + k - base, lz = j * nz + kp - base;
+    const bool y_lds = y_nb && ly >= 0 && ly < OEM_ADJ_TK * OEM_ADJ_TJ;
+    const bool z_lds = z_nb && lz >= 0 && lz < OEM_ADJ_TK * OEM_ADJ_TJ;
+    int buf = 0;
 
     // the x terms (t3: c1, sgn -1, slot 0; t4: c2, sgn +1, slot 1) of the cell on plane i
     auto qx = [&](int i, float &q3, float &q4) {
@@ -363,8 +400,16 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         const float q2 = qh(zc, f1p, pz1);      // t2: c1 sgn +1, z, slot 5
         const float q3 = qh(xc, f1m, px0);      // t3: c1 sgn -1, x, slot 0
         const float q4 = qh(xc, f2p, px1);      // t4: c2 sgn +1, x, slot 1
+        s_q[buf][0][tid] = q0;
+        s_q[buf][1][tid] = q5;
+        s_q[buf][2][tid] = q1;
+        s_q[buf][3][tid] = q2;
+        __syncthreads();
         float q0n = 0.f, q5n = 0.f, q1n = 0.f, q2n = 0.f;
-        if (y_nb) {
+        if (y_lds) {
+            q0n = s_q[buf][0][ly];
+            q5n = s_q[buf][1][ly];
+        } else if (y_nb) {
             const uint32_t idn = (uint32_t)i * plane + (uint32_t)(jp * nz + k);
             float a0 = 0.f, a1 = 0.f;
             if (yp.s >= 0) {
@@ -374,7 +419,10 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
             q0n = qh(yp, mch * 1.f * d.lH[idn], a0);
             q5n = qh(yp, mch * -1.f * d.lH[2 * nc + idn], a1);
         }
-        if (z_nb) {
+        if (z_lds) {
+            q1n = s_q[buf][2][lz];
+            q2n = s_q[buf][3][lz];
+        } else if (z_nb) {
             const uint32_t idn = (uint32_t)i * plane + (uint32_t)(j * nz + kp);
             float a0 = 0.f, a1 = 0.f;
             if (zp.s >= 0) {
@@ -400,6 +448,10 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         const float v3 = term(q3, xc, x_hi, x_nb, q3p, mxp, ixdp);
         const float v4 = term(q4, xc, x_hi, x_nb, q4p, mxp, ixdp);
         const float v5 = term(q5, yc, y_hi, y_nb, q5n, yp.m, yp.idl);
+        buf ^= 1;
+        q3p = q3;
+        q4p = q4;
+        if (!act) continue;
         float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
         acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
         d.lE[id] += acc0;
@@ -421,8 +473,6 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
             d.lph_n[off[4] + o] = zc.b * (f0m + pz0);
             d.lph_n[off[5] + o] = zc.b * (f1p + pz1);
         }
-        q3p = q3;
-        q4p = q4;
     }
 }
 

@@ -146,6 +146,7 @@ struct oem_ctx {
     bool pre_ready = false;     // ADE: hist/P already advanced by the previous fused step
     bool prepared = false;
     int chunk_req = 0;          // x-planes per work-group requested (0: heuristic)
+    int form_req = 0;           // step form requested: 0 the library's choice, 1 two sweeps, 2 fused
     bool mons_off = false;      // adjoint recomputation: no DFT snapshots or flushes
     // adjoint forward recording (oem_adjoint_forward): after each step, E at the design cells
     // goes to row (step + 1 - base) of rec

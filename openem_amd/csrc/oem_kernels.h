@@ -1491,6 +1491,37 @@ k_dft_snap(MonDev m, const F *__restrict__ E, const F *__restrict__ H, size_t nc
     snap_point<F>(m, E, H, ncell, ny, nz, step_now(stp) % m.T, q);
 }
 
+// The end of a step as one single-block launch, where three small launches would otherwise
+// close it: the E-side point lists in step order (each list after the one before it), then the
+// monitor points those lists corrected, snapped again, then the step counter.  Used when every
+// list fits the block and nothing else reads the step counter after the E-side points.
+#define OEM_TAIL_LISTS 4
+#define OEM_TAIL_THREADS 1024
+struct TailDev {
+    int n_pts, n_snap;
+    PointsDev pts[OEM_TAIL_LISTS];
+    MonDev mon[OEM_TAIL_LISTS];
+    const int *qs[OEM_TAIL_LISTS];
+    int nq[OEM_TAIL_LISTS];
+};
+
+template <typename F>
+__global__ void __launch_bounds__(OEM_TAIL_THREADS)
+k_step_tail(F *__restrict__ E, const F *__restrict__ H, TailDev t, MatDev mat, float ch,
+            size_t ncell, int ny, int nz, int *__restrict__ stp)
+{
+    const int n = step_now(stp);
+    const int tid = threadIdx.x;
+    for (int p = 0; p < t.n_pts; ++p) {
+        if (tid < t.pts[p].n_groups) point_group<F>(E, t.pts[p], mat, ch, ncell, n, tid);
+        __syncthreads();
+    }
+    for (int m = 0; m < t.n_snap; ++m)
+        if (tid < t.nq[m]) snap_point<F>(t.mon[m], E, H, ncell, ny, nz, n % t.mon[m].T, t.qs[m][tid]);
+    __syncthreads();
+    if (tid == 0) *stp = n + 1;
+}
+
 // The points `qs[0..nq)` of a monitor whose other points the fused step snapped.
 template <typename F>
 __global__ void __launch_bounds__(256)
@@ -1537,15 +1568,40 @@ __device__ __forceinline__ void flush_point(const MonDev &m, int first, int last
     }
 }
 
-// grid: x over points, y over frequencies.  Launched after the step counter advanced past the
-// last snapshot: the cnt snapshots ending at step *stp - 1 are replayed.
-template <typename F, int NC>
+// Replay of a single-scene monitor: a thread per (point, component) and group of up to
+// OEM_FLUSH_FG frequencies, so each snapshot value is read once per group, not once per
+// frequency (the phase rows are the same for the whole block and come through scalar loads).
+// Each (point, component, frequency) sum is formed in the order flush_point forms it, so both
+// give the same bits.  grid: x over points, y over (component, frequency group).  Launched after
+// the step counter advanced past the last snapshot: the cnt snapshots ending at step *stp - 1
+// are replayed.
+#define OEM_FLUSH_FG 8
+template <typename F>
 __global__ void __launch_bounds__(256)
 k_dft_flush(MonDev m, int cnt, const int *__restrict__ stp)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int f = blockIdx.y;
-    if (q >= m.npts || f >= m.nf) return;
-    const int last = step_now(stp) - 1;
-    flush_point<F, NC>(m, last - cnt + 1, last, f, q);
+    const int ngrp = (m.nf + OEM_FLUSH_FG - 1) / OEM_FLUSH_FG;
+    const int c = blockIdx.y / ngrp, f0 = (blockIdx.y - c * ngrp) * OEM_FLUSH_FG;
+    if (q >= m.npts || c >= m.ncomp) return;
+    const int nfl = min(OEM_FLUSH_FG, m.nf - f0);
+    const int last = step_now(stp) - 1, first = last - cnt + 1;
+    const int po = c < m.ncomp / 2 ? 0 : 2;        // E: whole-step phase, H: half-step phase
+    double sre[OEM_FLUSH_FG], sim[OEM_FLUSH_FG];
+#pragma unroll
+    for (int u = 0; u < OEM_FLUSH_FG; ++u) { sre[u] = 0.0; sim[u] = 0.0; }
+    for (int s = first; s <= last; ++s) {
+        const F v = static_cast<const F *>(m.snap)[((size_t)(s % m.T) * m.ncomp + c) * m.npts + q];
+        const double *ph = m.phase + ((size_t)s * m.nf + f0) * 4 + po;
+#pragma unroll
+        for (int u = 0; u < OEM_FLUSH_FG; ++u)
+            if (u < nfl) dft_acc(sre[u], sim[u], v, ph[4 * u], ph[4 * u + 1]);
+    }
+#pragma unroll
+    for (int u = 0; u < OEM_FLUSH_FG; ++u)
+        if (u < nfl) {
+            const size_t o = ((size_t)c * m.nf + f0 + u) * m.npts + q;
+            m.re[o] += sre[u];
+            m.im[o] += sim[u];
+        }
 }

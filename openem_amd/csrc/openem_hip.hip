@@ -121,6 +121,24 @@ extern "C" int oem_set_chunk(oem_ctx *ctx, int32_t chunk) {
     return OEM_OK;
 }
 
+extern "C" int oem_set_step_form(oem_ctx *ctx, int32_t form) {
+    CHECK_CTX(ctx);
+    if (ctx->prepared) return fail(OEM_ESTATE, "context already prepared");
+    if (form < 0 || form > 2) return fail(OEM_EINVAL, "step form %d", form);
+    if (form == 2 && ctx->pool) return fail(OEM_ENOTSUP, "a pooled context steps with the two sweeps");
+    ctx->form_req = form;
+    return OEM_OK;
+}
+
+extern "C" int oem_launch_plan(oem_ctx *ctx, int32_t *form, int32_t *chunk) {
+    CHECK_CTX(ctx);
+    if (!ctx->prepared) return fail(OEM_ESTATE, "oem_prepare not called");
+    if (!form || !chunk) return fail(OEM_EINVAL, "null output");
+    *form = ctx->fused ? 2 : 1;
+    *chunk = ctx->fused ? ctx->fg.chunk : ctx->g.chunk;
+    return OEM_OK;
+}
+
 extern "C" int oem_set_axis(oem_ctx *ctx, int32_t axis, const float *idl_p, const float *idl_d,
                             const int32_t *nxt, const int32_t *prv, const float *mnx,
                             const float *mpv, const float *pec, const float *a_e,
@@ -738,13 +756,18 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
     // 64^3 30 / 29-55 us, 128^3 69 / 65-83 us, 160^3 98 / 132 us, 200^3 241 / 240 us, 256^3
     // 393 / 355-361 us, 320^3 677 / 596 us, 512^3 2.54 / 1.99 ms: below ~8 M cells a fused
     // work-group marches too few tiles to fill the GPU
+    // (oem_set_step_form overrides the rule and OEM_FUSE)
     const char *fe = std::getenv("OEM_FUSE");
     bool fuse = !ctx->pool && !(fe && fe[0] == '0');
     if (!(fe && fe[0] == '1')) fuse &= ctx->ncell >= OEM_FUSE_MIN_CELLS;
+    if (ctx->form_req) fuse = ctx->form_req == 2;
     // it addresses a whole field set (E and H, six components) and a whole psi set (both sides)
     // with 32-bit offsets
-    fuse &= 6ull * ctx->ncell * 4ull * (uint64_t)ctx->esz < (1ull << 32);
-    fuse &= 2ull * ctx->psi.total * 4ull * (uint64_t)ctx->esz < (1ull << 32);
+    const bool fits = 6ull * ctx->ncell * 4ull * (uint64_t)ctx->esz < (1ull << 32) &&
+                      2ull * ctx->psi.total * 4ull * (uint64_t)ctx->esz < (1ull << 32);
+    if (ctx->form_req == 2 && !fits)
+        return fail(OEM_ENOTSUP, "the fused step addresses a field set with 32-bit offsets");
+    fuse &= fits;
     if (fuse) {
         const size_t nc = ctx->ncell;
         TRY(ctx->alloc(&ctx->E2, 6 * nc * ctx->esz, true));
@@ -789,11 +812,9 @@ inline int blocks_for(int n, int bs) { return (n + bs - 1) / bs; }
 template <typename F>
 int launch_flush_t(oem_ctx *c, int cnt) {
     for (auto &m : c->mons) {
-        dim3 grid(blocks_for(m.d.npts, 256), m.d.nf);
-        if (m.d.ncomp == 4)
-            hipLaunchKernelGGL((k_dft_flush<F, 4>), grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
-        else
-            hipLaunchKernelGGL((k_dft_flush<F, 6>), grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
+        const int ngrp = (m.d.nf + OEM_FLUSH_FG - 1) / OEM_FLUSH_FG;
+        const dim3 grid(blocks_for(m.d.npts, 256), m.d.ncomp * ngrp);
+        hipLaunchKernelGGL((k_dft_flush<F>), grid, dim3(256), 0, c->stream, m.d, cnt, c->d_step);
     }
     HIP_OK(hipGetLastError());
     return OEM_OK;
@@ -801,6 +822,34 @@ int launch_flush_t(oem_ctx *c, int cnt) {
 
 int launch_flush(oem_ctx *c, int cnt) {
     return c->cplx ? launch_flush_t<cf2>(c, cnt) : launch_flush_t<float>(c, cnt);
+}
+
+// Whether a step can end with k_step_tail, and its lists: the E-side point lists in step order
+// and the monitor points they correct that the fused step snapped before the correction.  Not
+// when anything else follows the E-side points and reads the step counter (an adjoint record,
+// time monitors, monitors the snapshot kernel takes) or a list exceeds the block.
+// OEM_TAIL=0 keeps the separate launches.
+static bool tail_plan(const oem_ctx *c, TailDev *t) {
+    const char *te = std::getenv("OEM_TAIL");
+    if ((te && te[0] == '0') || c->rec_hook.on) return false;
+    if (!c->mons_off && (!c->tmons.empty() || !c->ftmons.empty())) return false;
+    t->n_pts = t->n_snap = 0;
+    for (int slot = 3; slot < 6; ++slot)
+        for (const auto &pt : c->points) {
+            if (points_slot(pt) != slot) continue;
+            if (t->n_pts == OEM_TAIL_LISTS || pt.d.n_groups > OEM_TAIL_THREADS) return false;
+            t->pts[t->n_pts++] = pt.d;
+        }
+    if (!c->mons_off)
+        for (const auto &m : c->mons) {
+            if (!(c->fused && m.fused)) return false;
+            if (!m.n_patch) continue;
+            if (t->n_snap == OEM_TAIL_LISTS || m.n_patch > OEM_TAIL_THREADS) return false;
+            t->mon[t->n_snap] = m.d;
+            t->qs[t->n_snap] = m.patch;
+            t->nq[t->n_snap++] = m.n_patch;
+        }
+    return true;
 }
 
 // One time step.  `flush_after`: number of snapshots to flush after the step counter advanced
@@ -879,68 +928,85 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
                        c->mat, c->dims.cw, c->pw, c->d_step);
     TRY(mark(OEM_K_UPDATE_E, false));
     }
-    // E-side corrections: TF/SF faces, mode sources, then electric dipoles (reference order,
-    // `openem/solver.py:945-968`)
-    for (int slot = 3; slot < 6; ++slot)
-        for (auto &pt : c->points) {
-            if (points_slot(pt) != slot) continue;
-            TRY(mark(points_kid(pt), true));
-            hipLaunchKernelGGL((k_points<F>), dim3(blocks_for(pt.d.n_groups, 256)), dim3(256), 0,
-                               c->stream, E, pt.d, c->mat, c->dims.ch, nc, c->d_step);
-            TRY(mark(points_kid(pt), false));
-        }
-    if (c->has_disp) {
+    // ADE post on the new E (after every E-side correction); the post of a run's last step is
+    // plain, any other is fused with the next step's pre
+    auto disp_post = [&]() -> int {
+        if (!c->has_disp) return OEM_OK;
         TRY(mark(OEM_K_DISP_POST, true));
         hipLaunchKernelGGL((k_disp<F>), dg, db, 0, c->stream, c->disp, (const F *)E, nc,
                            c->dims.ny, c->dims.nz, last ? 1 : 2);
         TRY(mark(OEM_K_DISP_POST, false));
         c->pre_ready = !last;
-    }
-    if constexpr (sizeof(F) == 4) {
-        if (c->rec_hook.on)
-            hipLaunchKernelGGL((k_rec_gather<float>), dim3(blocks_for(c->rec_hook.n, 256)), dim3(256), 0,
-                               c->stream, (const float *)E, c->rec_hook.comp, c->rec_hook.cell,
-                               c->rec_hook.n, nc, c->rec_hook.rec, c->d_step, c->rec_hook.base);
-    }
-    for (auto &tm : c->tmons) {
-        if (c->mons_off) break;
-        for (int shot = 0; shot < 2; ++shot) {
-            TRY(mark(OEM_K_DFT_SNAP, true));
-            hipLaunchKernelGGL((k_time_sample<F>), dim3(blocks_for(tm.d.cells, 256)), dim3(256), 0,
-                               c->stream, tm.d, (const F *)E, (const F *)H, nc, c->dims.ny,
-                               c->dims.nz, c->d_step, shot);
-            TRY(mark(OEM_K_DFT_SNAP, false));
+        return OEM_OK;
+    };
+    TailDev tail{};
+    if (tail_plan(c, &tail)) {
+        // one launch: the E-side points, the monitor points they corrected snapped again, the
+        // step counter; then the ADE post, which reads no step counter
+        TRY(mark(OEM_K_STEP_TAIL, true));
+        hipLaunchKernelGGL((k_step_tail<F>), dim3(1), dim3(OEM_TAIL_THREADS), 0, c->stream, E,
+                           (const F *)H, tail, c->mat, c->dims.ch, nc, c->dims.ny, c->dims.nz,
+                           c->d_step);
+        TRY(mark(OEM_K_STEP_TAIL, false));
+        TRY(disp_post());
+    } else {
+        // E-side corrections: TF/SF faces, mode sources, then electric dipoles (reference order,
+        // `openem/solver.py:945-968`)
+        for (int slot = 3; slot < 6; ++slot)
+            for (auto &pt : c->points) {
+                if (points_slot(pt) != slot) continue;
+                TRY(mark(points_kid(pt), true));
+                hipLaunchKernelGGL((k_points<F>), dim3(blocks_for(pt.d.n_groups, 256)), dim3(256), 0,
+                                   c->stream, E, pt.d, c->mat, c->dims.ch, nc, c->d_step);
+                TRY(mark(points_kid(pt), false));
+            }
+        TRY(disp_post());
+        if constexpr (sizeof(F) == 4) {
+            if (c->rec_hook.on)
+                hipLaunchKernelGGL((k_rec_gather<float>), dim3(blocks_for(c->rec_hook.n, 256)), dim3(256), 0,
+                                   c->stream, (const float *)E, c->rec_hook.comp, c->rec_hook.cell,
+                                   c->rec_hook.n, nc, c->rec_hook.rec, c->d_step, c->rec_hook.base);
         }
-    }
-    if constexpr (sizeof(F) == 4) {
-        for (auto &fm : c->ftmons) {
+        for (auto &tm : c->tmons) {
             if (c->mons_off) break;
+            for (int shot = 0; shot < 2; ++shot) {
+                TRY(mark(OEM_K_DFT_SNAP, true));
+                hipLaunchKernelGGL((k_time_sample<F>), dim3(blocks_for(tm.d.cells, 256)), dim3(256), 0,
+                                   c->stream, tm.d, (const F *)E, (const F *)H, nc, c->dims.ny,
+                                   c->dims.nz, c->d_step, shot);
+                TRY(mark(OEM_K_DFT_SNAP, false));
+            }
+        }
+        if constexpr (sizeof(F) == 4) {
+            for (auto &fm : c->ftmons) {
+                if (c->mons_off) break;
+                TRY(mark(OEM_K_DFT_SNAP, true));
+                hipLaunchKernelGGL((k_flux_time_partial<float>), dim3(fm.d.n_blocks), dim3(OEM_FT_BLK), 0,
+                                   c->stream, fm.d, (const float *)E, (const float *)H, nc,
+                                   c->dims.ny, c->dims.nz, c->d_step);
+                hipLaunchKernelGGL((k_flux_time_sum<float>), dim3(1), dim3(OEM_FT_BLK), 0, c->stream, fm.d,
+                                   c->d_step);
+                TRY(mark(OEM_K_DFT_SNAP, false));
+            }
+        }
+        for (auto &m : c->mons) {
+            if (c->mons_off) break;
+            if (c->fused && m.fused && !m.n_patch) continue;
             TRY(mark(OEM_K_DFT_SNAP, true));
-            hipLaunchKernelGGL((k_flux_time_partial<float>), dim3(fm.d.n_blocks), dim3(OEM_FT_BLK), 0,
-                               c->stream, fm.d, (const float *)E, (const float *)H, nc,
-                               c->dims.ny, c->dims.nz, c->d_step);
-            hipLaunchKernelGGL((k_flux_time_sum<float>), dim3(1), dim3(OEM_FT_BLK), 0, c->stream, fm.d,
-                               c->d_step);
+            if (c->fused && m.fused)
+                hipLaunchKernelGGL((k_dft_snap_list<F>), dim3(blocks_for(m.n_patch, 256)), dim3(256), 0,
+                                   c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
+                                   c->dims.nz, c->d_step, (const int *)m.patch, m.n_patch);
+            else
+                hipLaunchKernelGGL((k_dft_snap<F>), dim3(blocks_for(m.d.npts, 256)), dim3(256), 0,
+                                   c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
+                                   c->dims.nz, c->d_step);
             TRY(mark(OEM_K_DFT_SNAP, false));
         }
+        TRY(mark(OEM_K_STEP, true));
+        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, c->stream, c->d_step);
+        TRY(mark(OEM_K_STEP, false));
     }
-    for (auto &m : c->mons) {
-        if (c->mons_off) break;
-        if (c->fused && m.fused && !m.n_patch) continue;
-        TRY(mark(OEM_K_DFT_SNAP, true));
-        if (c->fused && m.fused)
-            hipLaunchKernelGGL((k_dft_snap_list<F>), dim3(blocks_for(m.n_patch, 256)), dim3(256), 0,
-                               c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
-                               c->dims.nz, c->d_step, (const int *)m.patch, m.n_patch);
-        else
-            hipLaunchKernelGGL((k_dft_snap<F>), dim3(blocks_for(m.d.npts, 256)), dim3(256), 0,
-                               c->stream, m.d, (const F *)E, (const F *)H, nc, c->dims.ny,
-                               c->dims.nz, c->d_step);
-        TRY(mark(OEM_K_DFT_SNAP, false));
-    }
-    TRY(mark(OEM_K_STEP, true));
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, c->stream, c->d_step);
-    TRY(mark(OEM_K_STEP, false));
     if (flush_after > 0 && !c->mons.empty() && !c->mons_off) {
         TRY(mark(OEM_K_DFT_FLUSH, true));
         TRY(launch_flush(c, flush_after));

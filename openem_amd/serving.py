@@ -11,9 +11,11 @@ with the same launches (batched kernels over every resident scene in a hipGraph 
 * the stepping thread attaches ready scenes at the next block boundary, keeps a bounded number
   of blocks in flight and hands finished scenes to the readout, which reads the phasors,
   computes the fluxes and releases the scene's memory;
-* scenes of at least ``solo_cells`` cells fill the GPU by themselves: they run in a second lane,
-  one at a time as standalone contexts (hipGraph blocks; the fused H+E step from 12 M cells) on
-  their own stream, concurrently with the batched small scenes.
+* scenes of at least ``solo_cells`` cells nearly fill the GPU by themselves: they run as
+  standalone contexts (hipGraph blocks; the fused H+E step from 12 M cells) in ``solo_lanes``
+  lanes beside the batch, each lane one scene at a time on its own stream, so one large scene's
+  setup, readout and release overlap another's stepping, and two large scenes together fill the
+  GPU better than one.  A free lane takes the waiting large scene with the most cell-steps.
 
 :func:`serve_serial` is the plain loop the batched form is measured against: one
 :func:`openem_amd.solver.run` per request, in arrival order.
@@ -23,6 +25,7 @@ from __future__ import annotations
 
 import ctypes as C
 import queue
+import sys
 import threading
 import time
 from dataclasses import dataclass, field
@@ -117,6 +120,7 @@ class Request:
     t_arrival: float = 0.0          # when the request entered the system (seconds, run clock)
     t_admit: float = 0.0            # attached to the batch
     t_done: float = 0.0             # results read back
+    run_s: float = 0.0              # wall seconds of its own stepping call (standalone runs)
     flux: list = field(default_factory=list)   # per monitor, (nf,) float64
     error: str | None = None
 
@@ -128,6 +132,13 @@ def _flux_of(st: solver.Stepper) -> list:
         mon = m.spec if m.kind == "plane" and m.modes is None else None
         out.append(flux.monitor_flux(data, st.tabs.scene.grid, mon) if mon is not None else data)
     return out
+
+
+def _report(results: list, total: int, t0: float, label: str, every: int) -> None:
+    """One line on stderr each time another ``every`` requests have finished."""
+    if every and (len(results) % every == 0 or len(results) == total):
+        print(f"[{label}] {len(results)}/{total} requests done at "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
 
 def _client(specs, t0, outq, build_ahead: int, stop):
@@ -168,16 +179,21 @@ SOLO_CELLS = 8_000_000
 
 def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers: int = 6,
           max_in_flight: int = 2, build_ahead: int = 8, device: int = 0,
-          batch_cells: int = BATCH_CELLS, solo_cells: int = SOLO_CELLS, on_admit=None) -> dict:
+          batch_cells: int = BATCH_CELLS, solo_cells: int = SOLO_CELLS, solo_lanes: int = 2,
+          on_admit=None, report_every: int = 0) -> dict:
     """Continuous batching of ``specs`` (from :func:`openem_amd.scenes.serving_requests`).
     Ready scenes join the running batch in arrival order while the batch's cells stay within
-    ``batch_cells`` (a scene that alone exceeds it runs when the batch is empty).
+    ``batch_cells`` (a scene that alone exceeds it runs when the batch is empty).  With
+    ``report_every`` a progress line goes to stderr each time that many more requests finish.
     Returns ``dict(requests=[Request...], wall=seconds, peak_running=int)``."""
     pool = Pool(arena_bytes, dft_batch, device)
     t0 = time.perf_counter()
     arrivals: queue.Queue = queue.Queue()
     ready: queue.Queue = queue.Queue()
-    solo: queue.Queue = queue.Queue()
+    # the solo lanes take the waiting large scene with the most cell-steps first: those scenes
+    # set the end of the stream, and the longest-first order shortens its tail
+    solo: queue.PriorityQueue = queue.PriorityQueue()
+    seq = iter(range(1 << 62))
     done_q: queue.Queue = queue.Queue()
     stop = threading.Event()
     mem = threading.Condition()
@@ -195,7 +211,8 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
             try:
                 tabs = tables_mod.build(sc)
                 if int(np.prod(tabs.shape)) >= solo_cells:
-                    solo.put((req, tabs))
+                    work = int(np.prod(tabs.shape)) * int(tabs.n_total)
+                    solo.put((-work, next(seq), (req, tabs)))
                     continue
                 need = device_bytes(tabs, dft_batch)
                 with mem:
@@ -233,11 +250,13 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
                     budget["free"] += need
                     mem.notify_all()
                 req.t_done = time.perf_counter() - t0
-            results.append(req)
+            with mem:
+                results.append(req)
+                _report(results, len(specs), t0, "serve", report_every)
 
     def solo_worker():
         while True:
-            item = solo.get()
+            item = solo.get()[2]
             if item is None:
                 return
             req, tabs = item
@@ -245,20 +264,26 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
             try:
                 st = solver.Stepper(tabs, device=device, dft_batch=dft_batch)
                 try:
+                    tr = time.perf_counter()
                     st.run(tabs.n_total)
+                    req.run_s = time.perf_counter() - tr
                     req.flux = _flux_of(st)
                 finally:
                     st.close()
             except Exception as e:  # noqa: BLE001
                 req.error = repr(e)
             req.t_done = time.perf_counter() - t0
-            results.append(req)
+            with mem:
+                results.append(req)
+                _report(results, len(specs), t0, "serve", report_every)
 
     client = threading.Thread(target=_client, args=(specs, t0, arrivals, build_ahead, stop),
                               daemon=True)
     setups = [threading.Thread(target=setup_worker, daemon=True) for _ in range(workers)]
-    solo_th = threading.Thread(target=solo_worker, daemon=True)
-    solo_th.start()
+    solo_ths = [threading.Thread(target=solo_worker, daemon=True)
+                for _ in range(max(1, int(solo_lanes)))]
+    for th in solo_ths:
+        th.start()
     reader = threading.Thread(target=readout_worker, daemon=True)
     client.start()
     for th in setups:
@@ -313,8 +338,10 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
             running = still
     finally:
         stop.set()
-        solo.put(None)
-        solo_th.join()
+        for _ in solo_ths:
+            solo.put((float("inf"), next(seq), None))
+        for th in solo_ths:
+            th.join()
         done_q.put(None)
         reader.join()
         client.join(timeout=5)
@@ -325,7 +352,8 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
                 peak_running=peak)
 
 
-def serve_serial(specs: list, dft_batch: int = 16, device: int = 0, build_ahead: int = 8) -> dict:
+def serve_serial(specs: list, dft_batch: int = 16, device: int = 0, build_ahead: int = 8,
+                 report_every: int = 0) -> dict:
     """The plain loop: requests in arrival order, one :func:`openem_amd.solver.run`-equivalent
     each (tables, context, stepping, readout, release)."""
     t0 = time.perf_counter()
@@ -344,7 +372,9 @@ def serve_serial(specs: list, dft_batch: int = 16, device: int = 0, build_ahead:
             tabs = tables_mod.build(sc)
             st = solver.Stepper(tabs, device=device, dft_batch=dft_batch)
             try:
+                tr = time.perf_counter()
                 st.run(tabs.n_total)
+                req.run_s = time.perf_counter() - tr
                 req.flux = _flux_of(st)
             finally:
                 st.close()
@@ -352,6 +382,7 @@ def serve_serial(specs: list, dft_batch: int = 16, device: int = 0, build_ahead:
             req.error = repr(e)
         req.t_done = time.perf_counter() - t0
         results.append(req)
+        _report(results, len(specs), t0, "serial", report_every)
     client.join(timeout=5)
     wall = max((r.t_done for r in results), default=0.0)
     return dict(requests=results, wall=wall)

@@ -53,8 +53,12 @@ class Stepper:
     """One scene resident on the GPU.  With ``pool`` (a :class:`openem_amd.serving.Pool`) its
     device memory comes from the pool's arena and the pool steps it once attached."""
 
+    #: step forms of oem_set_step_form
+    FORMS = {None: 0, "sweeps": 1, "fused": 2}
+
     def __init__(self, tabs: tables_mod.StepTables, device: int = 0, dft_batch: int = 16,
-                 dense_materials: bool = False, pool=None, chunk: int = 0):
+                 dense_materials: bool = False, pool=None, chunk: int = 0,
+                 form: str | None = None):
         self.lib = lib = _lib.load()
         self.pool = pool
         self.cplx = bool(tabs.bloch)
@@ -78,6 +82,8 @@ class Stepper:
         try:
             if chunk:
                 _lib.check(lib, lib.oem_set_chunk(self.ctx, int(chunk)))
+            if form is not None:
+                _lib.check(lib, lib.oem_set_step_form(self.ctx, self.FORMS[form]))
             self._upload(dense_materials)
             _lib.check(lib, lib.oem_prepare(self.ctx))
         except Exception:
@@ -191,6 +197,13 @@ class Stepper:
                 _lib.ptr(wlo), _lib.ptr(whi), float(g["sign"]), m.step_begin, m.step_end,
                 m.interval, m.num_slots, C.byref(mid)))
             self.ftmon_ids.append(int(mid.value))
+
+    @property
+    def launch(self) -> dict:
+        """The launch plan in use: ``{"form": "sweeps" | "fused", "chunk": x planes}``."""
+        f, c = C.c_int32(0), C.c_int32(0)
+        _lib.check(self.lib, self.lib.oem_launch_plan(self.ctx, C.byref(f), C.byref(c)))
+        return {"form": "fused" if f.value == 2 else "sweeps", "chunk": int(c.value)}
 
     # -- stepping ---------------------------------------------------------------------------
     def run(self, n_steps: int) -> None:

@@ -49,6 +49,16 @@ class Adjoint64:
         self.lpe = {k: np.zeros(shp) for k in _PSI}
         self.lph = {k: np.zeros(shp) for k in _PSI}
 
+    @staticmethod
+    def _scatter_add(acc, idx, src):
+        """``acc[idx[i]] += src[i]`` along the first axis, repeated targets summed (the end
+        cells of a non-periodic axis name one neighbour twice): one fancy-indexed add over the
+        first occurrence of every target, the repeats one by one."""
+        _, first = np.unique(idx, return_index=True)
+        acc[idx[first]] += src[first]
+        for i in np.setdiff1d(np.arange(idx.size), first):
+            acc[idx[i]] += src[i]
+
     # transposes of the oracle's one-sided differences along axis a
     def _dfwd_T(self, g, a):
         m = self.o._m[a]
@@ -57,7 +67,7 @@ class Adjoint64:
         out = -w
         src = np.moveaxis(w * _bc(m["mnx"], a), a, 0)
         acc = np.moveaxis(out, a, 0)
-        np.add.at(acc, m["nxt"], src)
+        self._scatter_add(acc, m["nxt"], src)
         return np.moveaxis(acc, 0, a)
 
     def _dbwd_T(self, g, a):
@@ -67,7 +77,7 @@ class Adjoint64:
         out = w.copy()
         src = np.moveaxis(-w * _bc(m["mpv"], a), a, 0)
         acc = np.moveaxis(out, a, 0)
-        np.add.at(acc, m["prv"], src)
+        self._scatter_add(acc, m["prv"], src)
         return np.moveaxis(acc, 0, a)
 
     def _coef(self, a, side):
@@ -140,14 +150,19 @@ def mode_objective_weights(tabs, mon_index: int, f_index: int = 0):
 
 def gradient_mode_power(sc, tabs, mon_index: int, design: np.ndarray, f_index: int = 0):
     """J = |a+|^2 of a mode monitor after ``tabs.n_total`` steps and dJ/d eps at ``design``
-    ((n, 2) int: component, flat cell), by a stored-trajectory float64 adjoint."""
+    ((n, 2) int: component, flat cell), by a float64 adjoint over the stored trajectory of E at
+    the design entries (E^n for n = 0 .. n_total, the only forward values the gradient needs)."""
     from openem_amd.grid import transverse_axes
     o = Oracle64(tabs)
     n_total = tabs.n_total
-    traj = [o.E.copy()]
-    for _ in range(n_total):
+    dc = np.asarray(design[:, 0], dtype=np.int64)
+    dcell = np.asarray(design[:, 1], dtype=np.int64)
+    flat = dc * int(np.prod(tabs.shape)) + dcell          # into E.reshape(-1)
+    traj = np.empty((n_total + 1, len(design)))
+    traj[0] = o.E.reshape(-1)[flat]
+    for n in range(n_total):
         o.step()
-        traj.append(o.E.copy())
+        traj[n + 1] = o.E.reshape(-1)[flat]
     m = tabs.monitors[mon_index]
     K = mode_objective_weights(tabs, mon_index, f_index)
     ph = m.phase[:, f_index]                              # (n_total, 4): cos/sin at t_e, t_h
@@ -157,37 +172,27 @@ def gradient_mode_power(sc, tabs, mon_index: int, design: np.ndarray, f_index: i
     t1, t2 = transverse_axes(m.axis)
     adj = Adjoint64(tabs)
     dca_dcb = eps_derivatives(tabs, sc)
+    ca = np.stack([tabs.ca[c] for c in range(3)]).reshape(-1)[flat]
+    cb = np.stack([tabs.cb[c] for c in range(3)]).reshape(-1)[flat]
+    dca = np.stack([dca_dcb[c][0] for c in range(3)]).reshape(-1)[flat]
+    dcb = np.stack([dca_dcb[c][1] for c in range(3)]).reshape(-1)[flat]
     grad = np.zeros(len(design))
-    dc, dcell = design[:, 0], design[:, 1]
+    plane = [slice(None)] * 3
+    below = [slice(None)] * 3
+    plane[m.axis], below[m.axis] = m.km, m.km - 1
+    plane, below = tuple(plane), tuple(below)
     for n in range(n_total - 1, -1, -1):
         we = ph[n, 0] + 1j * ph[n, 1]
         wh = ph[n, 2] + 1j * ph[n, 3]
-        gE = np.zeros((3,) + tabs.shape)
-        gH = np.zeros((3,) + tabs.shape)
-        for slot, comp, w in ((0, t1, we), (1, t2, we)):
-            gE[comp] += _on_plane(tabs.shape, m.axis, m.km, 2.0 * np.real(np.conj(a) * K[slot] * w))
+        # dJ/dE^{n+1} on the monitor plane, dJ/dH^{n+1/2} split over the two planes the
+        # colocated H sample averages
+        for slot, comp in ((0, t1), (1, t2)):
+            adj.lE[comp][plane] += 2.0 * np.real(np.conj(a) * K[slot] * we)
         for slot, comp in ((2, t1), (3, t2)):
             v = 2.0 * np.real(np.conj(a) * K[slot] * wh) * 0.5
-            gH[comp] += _on_plane(tabs.shape, m.axis, m.km, v)
-            gH[comp] += _on_plane(tabs.shape, m.axis, m.km - 1, v)
-        adj.lE += gE
-        adj.lH += gH
-        E0, E1 = traj[n], traj[n + 1]
-        for q in range(len(design)):
-            c, cell = int(dc[q]), int(dcell[q])
-            ca = tabs.ca[c].reshape(-1)[cell]
-            cb = tabs.cb[c].reshape(-1)[cell]
-            e0, e1 = E0[c].reshape(-1)[cell], E1[c].reshape(-1)[cell]
-            dca = dca_dcb[c][0].reshape(-1)[cell]
-            dcb = dca_dcb[c][1].reshape(-1)[cell]
-            grad[q] += adj.lE[c].reshape(-1)[cell] * (dca * e0 + dcb * (e1 - ca * e0) / cb)
+            adj.lH[comp][plane] += v
+            adj.lH[comp][below] += v
+        e0, e1 = traj[n], traj[n + 1]
+        grad += adj.lE.reshape(-1)[flat] * (dca * e0 + dcb * (e1 - ca * e0) / cb)
         adj.step_back()
     return J, grad
-
-
-def _on_plane(shape, axis, k, val):
-    out = np.zeros(shape)
-    idx = [slice(None)] * 3
-    idx[axis] = k
-    out[tuple(idx)] = val
-    return out

@@ -153,3 +153,37 @@ def test_fused_snapshots_equal_the_sweeps(gpu_lib, monkeypatch, name):
     for x, y in zip(a1, a2):
         assert np.abs(y).max() > 0
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_step_tail_equals_separate_launches(gpu_lib, monkeypatch, fuse):
+    """A step that ends with one launch (E-side dipole, the monitor points it corrects snapped
+    again, the step counter) gives the fields and phasors of the same step ending with separate
+    launches (OEM_TAIL=0), bit for bit; on the two sweeps with no monitor the tail applies too."""
+    n = 120
+    sc = scenes.box_soup_scene(shape=(24, 22, 150), n_pml=6, n_boxes=8, num_steps=n, seed=3,
+                               moment=1e-18)
+    f = sc.field_monitors[0].freqs
+    sc = replace(sc, flux_monitors=[],
+                 field_monitors=[FieldMonitor("dip", f[:2], (10, 9, 70), (5, 6, 10))]
+                 if fuse == "1" else [])
+    tabs = tables.build(sc)
+    monkeypatch.setenv("OEM_FUSE", fuse)
+    out = []
+    for tail in ("1", "0"):
+        monkeypatch.setenv("OEM_TAIL", tail)
+        st = Stepper(tabs, dft_batch=8)
+        try:
+            st.run(n - 1)
+            prof = st.profile_run(1)
+            assert prof["step_tail"][1] == (1 if tail == "1" else 0)
+            out.append(([st.read_monitor(i) for i in range(len(tabs.monitors))],
+                        np.stack([st.read_field(c) for c in range(6)])))
+        finally:
+            st.close()
+    (a1, e1), (a2, e2) = out
+    assert np.abs(e2).max() > 0
+    assert np.array_equal(e1, e2)
+    for x, y in zip(a1, a2):
+        assert np.abs(y).max() > 0
+        assert np.array_equal(x, y)

@@ -1,8 +1,9 @@
 """GPU tests of mode sources and mode monitors.
 
 Tolerances: fields and monitor phasors of the fp32 stepper against the float64 oracle on the
-same tables, 1e-4 of their peak; the device overlap reduction against the host overlap of the
-same float64 phasors, 1e-12 relative.
+same tables, 1e-4 of their peak; mode amplitudes against those of the oracle's phasors, 1e-5 of
+the larger of |a+| and |a-| (measured at most 2e-6); the device overlap reduction against the
+host overlap of the same float64 phasors, 1e-12 relative.
 """
 
 import numpy as np
@@ -48,6 +49,14 @@ def test_waveguide_mode_source_matches_oracle(gpu_lib, direction):
                                       modes.orientation(m.axis))
             assert abs(ap[fi] - hp[0]) <= 1e-12 * max(abs(hp[0]), abs(hm[0]))
             assert abs(am[fi] - hm[0]) <= 1e-12 * max(abs(hp[0]), abs(hm[0]))
+            # against the amplitudes of the oracle's float64 phasors
+            op, om = modes.amplitudes(o.acc[i][:, fi:fi + 1], m.modes[fi], sc.grid,
+                                      modes.orientation(m.axis))
+            big = max(abs(op[0]), abs(om[0]))
+            ep, em = abs(ap[fi] - op[0]) / big, abs(am[fi] - om[0]) / big
+            print(f"{m.name} f{fi}: |a+| {abs(op[0]):.4e} |a-| {abs(om[0]):.4e}, device error "
+                  f"{ep:.2e} / {em:.2e} of the larger", flush=True)
+            assert ep < 1e-5 and em < 1e-5
 
 
 def test_run_reports_mode_amplitudes(gpu_lib):

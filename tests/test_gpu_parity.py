@@ -116,6 +116,33 @@ def test_slab_flux_within_1e5_of_oracle(gpu_lib):
         assert rel < 1e-5, (m.name, rel)
 
 
+def test_full_slab_cross_section_equals_the_narrow_one(gpu_lib):
+    """Config 0 as specified (30 x 30 x 174 cells, 10,491 steps) on the device: normal incidence
+    on a laterally uniform slab with periodic x, y keeps every (x, y) column of the 30 x 30
+    cross-section bitwise equal to every other and to the 2 x 2 run's columns, so each
+    monitor's flux per unit area equals the 2 x 2 run's (which the test above holds within 1e-5
+    of the float64 oracle) to float64 summation rounding."""
+    wide = scenes.slab_scene()
+    narrow = scenes.slab_scene(size=(0.04 * UM, 0.04 * UM, 3.0 * UM))
+    assert wide.shape == (30, 30, 174)
+    tw, tn = tables.build(wide), tables.build(narrow)
+    n = wide.num_time_steps
+    assert n == narrow.num_time_steps
+    fw, aw = _gpu_state(tw, n)
+    fn, an = _gpu_state(tn, n)
+    for c in range(6):
+        assert np.array_equal(fw[c], np.broadcast_to(fw[c][:1, :1, :], fw[c].shape)), c
+        assert np.array_equal(fw[c][0, 0], fn[c][0, 0]), c
+    area_w = 30 * 30 * (0.02 * UM) ** 2
+    area_n = 2 * 2 * (0.02 * UM) ** 2
+    for i, m in enumerate(tw.monitors):
+        pw = flux.plane_flux(aw[i], wide.grid) / area_w
+        pn = flux.plane_flux(an[i], narrow.grid) / area_n
+        rel = np.abs(pw - pn).max() / np.abs(pn).max()
+        print(f"{m.name}: 30x30 against 2x2 flux per area {rel:.2e}", flush=True)
+        assert rel < 1e-12, (m.name, rel)
+
+
 def test_box_soup_small_matches_oracle(gpu_lib):
     """The roofline workload's generator at a small size: CPML on all six faces, seeded boxes
     (plain and Lorentz), centred Ez dipole, central-plane field monitor."""
@@ -165,6 +192,18 @@ def test_bloch_cell_matches_oracle(gpu_lib, metal):
     for i, m in enumerate(tabs.monitors):
         err = np.abs(acc[i] - o.acc[i]).max() / scale
         assert err < 1e-4, (m.name, err)
+    # flux of each plane from the complex phasors, against the oracle's: relative to the
+    # largest flux through the cell's planes (the reflection plane of the empty cell carries
+    # only the TF/SF residual)
+    fl = [flux.plane_flux(acc[i], sc.grid) for i in range(len(tabs.monitors))]
+    fo = [flux.plane_flux(o.acc[i], sc.grid) for i in range(len(tabs.monitors))]
+    big = max(np.abs(x).max() for x in fo)
+    for i, m in enumerate(tabs.monitors):
+        rel = np.abs(fl[i] - fo[i]).max() / big
+        own = np.abs(fl[i] - fo[i]).max() / max(np.abs(fo[i]).max(), 1e-300)
+        print(f"bloch metal={metal} {m.name}: flux error {rel:.2e} of the largest flux, "
+              f"{own:.2e} of its own", flush=True)
+        assert rel < 1e-5, (m.name, rel)
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(scatterer=True, jxy=(0.6, 0.8)),
