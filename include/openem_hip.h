@@ -262,6 +262,14 @@ int oem_set_flux_weights(oem_ctx *ctx, int32_t mon_id, const double *w1, const d
  * (`openem/flux.py:75-110`) on the shutoff and readout paths. */
 int oem_monitor_flux(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
 
+/* oem_monitor_flux without any device allocation, for callers that cannot allocate (serving
+ * loops that must not stall the device with a first-call hipMalloc, harnesses that forbid
+ * allocation).  No counterpart in the reference.  Same arguments, checks and bits as
+ * oem_monitor_flux; the partial sums live in one file-scope device array per GPU (1.04 MB,
+ * held until the process ends), so monitors of at most 1024 frequencies are taken (more:
+ * OEM_EINVAL).  Calls on one GPU take turns on that array (a host mutex per GPU). */
+int oem_monitor_flux_noalloc(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
+
 /* Overlap weights of a plane monitor: n_sets (1..16) sets of 4 * nf * npts complex values
  * (re, im doubles), layout (set, component, f, point) over the monitor's (E_t1, E_t2, H_t1,
  * H_t2) accumulators. */

@@ -32,7 +32,7 @@ EXPORTS = ("oem_version", "oem_last_error", "oem_device_count", "oem_ctx_create"
            "oem_run", "oem_profile_run", "oem_timed_run", "oem_step_index", "oem_read_monitor",
            "oem_read_field", "oem_write_field", "oem_set_chunk", "oem_set_step_form",
            "oem_launch_plan", "oem_set_energy_tables",
-           "oem_energy", "oem_set_flux_weights", "oem_monitor_flux",
+           "oem_energy", "oem_set_flux_weights", "oem_monitor_flux", "oem_monitor_flux_noalloc",
            "oem_set_overlap_weights",
            "oem_monitor_overlap", "oem_checkpoint_alloc", "oem_checkpoint_save",
            "oem_checkpoint_load", "oem_adjoint_setup", "oem_adjoint_objective",
@@ -101,6 +101,7 @@ _SIGS = {
     "oem_energy": ([_P, _P], C.c_int),
     "oem_set_flux_weights": ([_P, C.c_int32, _P, _P], C.c_int),
     "oem_monitor_flux": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "oem_monitor_flux_noalloc": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "oem_set_overlap_weights": ([_P, C.c_int32, C.c_int32, _P], C.c_int),
     "oem_monitor_overlap": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "oem_checkpoint_alloc": ([_P, C.c_int32], C.c_int),

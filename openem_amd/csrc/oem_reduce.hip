@@ -6,6 +6,8 @@
 
 #include "oem_internal.h"
 
+#include <mutex>
+
 #define OEM_RED_BLOCKS 2048
 #define OEM_RED_THREADS 256
 
@@ -330,6 +332,73 @@ extern "C" int oem_monitor_overlap(oem_ctx *ctx, int32_t mon_id, double *out, si
                        r->partial, nblk, ny, r->out);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(out, r->out, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return OEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Plane flux without allocation.  The partial sums (64 blocks x nf x 2 doubles at most) and the
+// 2 nf results of oem_monitor_flux_noalloc live in `scratch`, one file-scope device array per
+// GPU, instead of the context's lazily allocated reduction buffers, so the call never reaches
+// hipMalloc.  Its device address is looked up once per GPU, with that GPU current, on the first
+// call there; host code only ever passes that address, never the array's name.  Calls on one
+// GPU share the array: a host mutex per GPU holds it for the whole call (launches, copy, stream
+// synchronisation), so calls from several threads or contexts on one GPU run one after another;
+// calls issued outside this entry point that overlap it on the same GPU would race.
+#define OEM_NOALLOC_MAX_NF 1024
+#define OEM_NOALLOC_BLOCKS 64
+#define OEM_MAX_DEVICES 64
+
+static __device__ __align__(256) double
+    scratch[2 * OEM_NOALLOC_BLOCKS * OEM_NOALLOC_MAX_NF + 2 * OEM_NOALLOC_MAX_NF];
+
+static std::mutex g_scratch_mu[OEM_MAX_DEVICES];
+static double *g_scratch_ptr[OEM_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device` (the current device), looked up on first use.
+static int scratch_ptr(int device, double **out) {
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OEM_EHIP, "device %d: address of the flux scratch array: %s", device,
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return OEM_OK;
+}
+
+extern "C" int oem_monitor_flux_noalloc(oem_ctx *ctx, int32_t mon_id, double *out, size_t count) {
+    CHECK_CTX(ctx);
+    if (ctx->device < 0 || ctx->device >= OEM_MAX_DEVICES)
+        return fail(OEM_EINVAL, "device %d beyond the %d the scratch table covers", ctx->device,
+                    OEM_MAX_DEVICES);
+    if (mon_id < 0 || mon_id >= (int)ctx->mons.size()) return fail(OEM_EINVAL, "monitor %d", mon_id);
+    const MonDev &m = ctx->mons[mon_id].d;
+    if (!out || count != 2 * (size_t)m.nf)
+        return fail(OEM_EINVAL, "monitor %d: output holds 2*nf = %d doubles", mon_id, 2 * m.nf);
+    if (m.nf > OEM_NOALLOC_MAX_NF)
+        return fail(OEM_EINVAL, "monitor %d: %d frequencies, the allocation-free flux takes at most %d",
+                    mon_id, m.nf, OEM_NOALLOC_MAX_NF);
+    oem_reduce_state *r = ctx->red;
+    if (!r || (int)r->flux_w.size() <= mon_id || !r->flux_w[mon_id].first)
+        return fail(OEM_ESTATE, "flux weights of monitor %d not set", mon_id);
+    if (ctx->pool) TRY(oemi::pool_wait_done(ctx));
+    std::lock_guard<std::mutex> hold(g_scratch_mu[ctx->device]);
+    double *base;
+    TRY(scratch_ptr(ctx->device, &base));
+    double *partial = base;
+    double *res = base + 2 * OEM_NOALLOC_BLOCKS * OEM_NOALLOC_MAX_NF;
+    // the same launch shape as oem_monitor_flux, so both give the same bits
+    const int nblk = std::max(1, std::min(OEM_NOALLOC_BLOCKS, (m.npts + OEM_RED_THREADS - 1) / OEM_RED_THREADS));
+    hipLaunchKernelGGL(k_flux_partial, dim3(nblk, m.nf), dim3(OEM_RED_THREADS), 0, ctx->stream,
+                       m.re, m.im, r->flux_w[mon_id].first, r->flux_w[mon_id].second, m.nf,
+                       m.npts, partial);
+    hipLaunchKernelGGL(k_sum_flux, dim3((2 * m.nf + 255) / 256), dim3(256), 0, ctx->stream,
+                       partial, nblk, m.nf, res);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, res, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return OEM_OK;
 }
