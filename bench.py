@@ -95,7 +95,7 @@ def _oracle_worker(n, steps, nx_sample, barrier, out):
     out.put(_oracle_sample_time(n, steps, nx_sample, barrier))
 
 
-def cpu_baseline(n: int, steps: int = 6, nx_sample: int = 64, workers: int = 8) -> dict:
+def cpu_baseline(n: int, steps: int = 4, nx_sample: int = 64, workers: int = 8) -> dict:
     """The float64 oracle on the host: one process on one x-slab sample, and ``workers`` processes
     stepping one x-slab each at the same time (an x-slab decomposition without its halo
     exchange, which would add one plane per step and neighbour).  ``steps`` timed steps after
@@ -149,6 +149,12 @@ def serving_bench(n_req: int, arena_gib: int, workers: int, solo_lanes: int = 2,
     s = serving.serve_serial(specs, report_every=16)
     ss = serving.summary(s)
     print(f"[bench] serial {ss}", file=sys.stderr, flush=True)
+    errors = [f"batched #{r.spec['id']} ({r.spec['kind']} {r.spec['side']}^3): {r.error}"
+              for r in b["requests"] if r.error] + \
+             [f"serial #{r.spec['id']} ({r.spec['kind']} {r.spec['side']}^3): {r.error}"
+              for r in s["requests"] if r.error]
+    for e in errors:
+        print(f"[bench] failed request {e}", file=sys.stderr, flush=True)
     worst = 0.0
     for rb, rs in zip(b["requests"], s["requests"]):
         for x, y in zip(rb.flux, rs.flux):
@@ -164,7 +170,7 @@ def serving_bench(n_req: int, arena_gib: int, workers: int, solo_lanes: int = 2,
            "serial_stepping_s": round(sum(r.run_s for r in s["requests"]), 2),
            "speedup": round(sb["sims_per_s"] / ss["sims_per_s"], 3),
            "peak_batch": b["peak_running"], "solo_lanes": solo_lanes,
-           "failed": sb["failed"] + ss["failed"],
+           "failed": sb["failed"] + ss["failed"], "errors": errors[:8],
            "max_rel_flux_diff_vs_serial": worst, "arena_gib": arena_gib}
     if best_rate:
         bound = cs / (best_rate * 1e6)
@@ -292,10 +298,10 @@ def _mode_power(sc) -> float:
 
 def _serving_estimate_s(specs, best_rate: float) -> float:
     """Wall seconds the serving leg (batched, then serial) is expected to take: the stream's
-    cell-steps at 0.6 and 0.52 of the single-scene rate, host work included (the whole stream
-    measured 0.62 and 0.55)."""
+    cell-steps at 0.63 and 0.53 of the single-scene rate, host work included (the whole stream
+    measured 0.64-0.66 and 0.54-0.55)."""
     cs = sum(sp["side"] ** 3 * sp["n_steps"] for sp in specs)
-    return cs / (best_rate * 1e6) * (1 / 0.6 + 1 / 0.52)
+    return cs / (best_rate * 1e6) * (1 / 0.63 + 1 / 0.53)
 
 
 def main() -> int:
@@ -360,7 +366,7 @@ def main() -> int:
             flush=True)
         return 0
 
-    n_total = max(args.warmup + args.steps + args.profile_steps + 8, 32)
+    n_total = max(args.warmup + args.dft_batch + args.steps + args.profile_steps + 8, 32)
     t0 = time.time()
     sc = scenes.box_soup_scene(n=args.n, num_steps=max(n_total, 4000))
     tabs = tables.build(sc, n_total)
@@ -380,7 +386,10 @@ def main() -> int:
     print(f"[bench] setup {setup_s:.1f}s, {sc.shape}, classes {st.n_classes}, "
           f"ADE entries {tabs.disp.comp.size if tabs.disp else 0}, launch {launch}",
           file=sys.stderr, flush=True)
-    st.run(args.warmup)
+    # the warmup runs on to a whole graph block, so the timed steps replay captured blocks
+    # (they start at a block boundary) rather than capture one
+    warm = args.warmup + (-args.warmup) % args.dft_batch
+    st.run(warm)
     t1 = time.perf_counter()
     ms = st.timed_run(args.steps)
     wall = time.perf_counter() - t1
@@ -402,15 +411,16 @@ def main() -> int:
     print(f"[bench] timed {args.steps} steps: {ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall)",
           file=sys.stderr, flush=True)
     del sc, tabs
-    cpu = None if args.no_cpu else cpu_baseline(args.n)
     blo = bloch_bench(args.bloch_steps) if args.bloch_steps > 0 else None
+    cpu = None if args.no_cpu else cpu_baseline(args.n)
 
     # serving, then gradients, each shortened when it would overrun the budget
     srv = None
     if args.serve_requests > 0:
         specs = scenes.serving_requests(256, rate=20.0, seed=0)[:args.serve_requests]
         # keep room for the smallest gradient leg (4 designs and the central differences)
-        left = args.budget_s - (time.perf_counter() - t_start) - (4 * 3.2 + 2.2 * args.adjoint_check + 10)
+        left = (args.budget_s - (time.perf_counter() - t_start)
+                - (4 * 2.5 + 2.0 * args.adjoint_check + 10 if args.adjoint_designs > 0 else 0))
         n_req = len(specs)
         while n_req > 16 and _serving_estimate_s(specs[:n_req], value) > left:
             n_req -= 16
@@ -425,7 +435,7 @@ def main() -> int:
     if args.adjoint_designs > 0:
         left = args.budget_s - (time.perf_counter() - t_start)
         designs = args.adjoint_designs
-        per, fd = 3.2, 2.2 * args.adjoint_check        # seconds (measured 2.84 / 1.96)
+        per, fd = 2.5, 2.0 * args.adjoint_check        # seconds per design / per pixel pair
         while designs > 4 and designs * per + fd + 10 > left:
             designs -= 4
         if designs < args.adjoint_designs:
@@ -437,7 +447,8 @@ def main() -> int:
             adj["shortened_from"] = args.adjoint_designs
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "steps": args.steps, "warmup": args.warmup, "warmup_steps_run": warm,
+        "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"box-soup {args.n}^3 (CPML x6, 32 boxes eps/2-pole Lorentz, "

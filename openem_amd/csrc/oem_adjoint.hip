@@ -18,6 +18,8 @@
 
 #include "oem_internal.h"
 
+#include <unordered_set>
+
 // the transposed sweeps: a (j, k) column per thread from a block of 256, marching `chunk` x
 // planes per block (grid z; OEM_ADJ_CHUNK by default, the environment's OEM_ADJ_CHUNK if set)
 #define OEM_ADJ_TK 64
@@ -48,37 +50,6 @@ struct AdjDev {
 // plane i + 1 is carried from the previous iteration; the y and z neighbours' terms are
 // recomputed from their cells.  Terms (component -> source, axis): t0 c0->b2 y, t1 c0->b1 z, t2 c1->b0 z,
 // t3 c1->b2 x, t4 c2->b1 x, t5 c2->b0 y.
-
-// cb of the three components at a cell (class LUT in LDS when it fits, else global; or dense)
-__device__ __forceinline__ void cb3(const MatDev &m, const float4 *s_lut, bool lds, uint32_t id,
-                                    uint32_t ncell, float &bx, float &by, float &bz) {
-    if (m.mode == 0) {
-        const uint32_t c = m.cls[id];
-        if (lds) {
-            const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
-            bx = r0.y; by = r0.w; bz = r1.y;
-        } else {
-            bx = m.lut[8 * (size_t)c + 1]; by = m.lut[8 * (size_t)c + 3]; bz = m.lut[8 * (size_t)c + 5];
-        }
-    } else {
-        bx = m.cb[id]; by = m.cb[ncell + id]; bz = m.cb[2 * ncell + id];
-    }
-}
-// ca of the three components at a cell
-__device__ __forceinline__ void ca3(const MatDev &m, const float4 *s_lut, bool lds, uint32_t id,
-                                    uint32_t ncell, float &ax, float &ay, float &az) {
-    if (m.mode == 0) {
-        const uint32_t c = m.cls[id];
-        if (lds) {
-            const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
-            ax = r0.x; ay = r0.z; az = r1.x;
-        } else {
-            ax = m.lut[8 * (size_t)c]; ay = m.lut[8 * (size_t)c + 2]; az = m.lut[8 * (size_t)c + 4];
-        }
-    } else {
-        ax = m.ca[id]; ay = m.ca[ncell + id]; az = m.ca[2 * ncell + id];
-    }
-}
 
 // One axis' table values at coordinate p: the CPML (a, 1/kappa) of one side, the slab index,
 // 1/dl and the mask of the difference that the transposed term uses.
@@ -133,6 +104,13 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     const uint32_t nc = (uint32_t)d.ncell, plane = (uint32_t)(ny * nz);
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
     const uint32_t *off = d.psl.off;
+    // buffer resources (the x march's per-plane offsets go in SGPRs): the three components of
+    // lambda_E (old, new) and lambda_H at c * cB, the psi adjoints (old, new), the classes
+    const uint32_t cB = nc * 4u;
+    const rsrc_t rE = mk_rsrc(d.lE, 3u * cB), rEn = mk_rsrc(d.lE_n, 3u * cB);
+    const rsrc_t rH = mk_rsrc(d.lH, 3u * cB);
+    const rsrc_t rP = mk_rsrc(d.lpe, d.psl.total * 4u), rPn = mk_rsrc(d.lpe_n, d.psl.total * 4u);
+    const rsrc_t rC = mk_rsrc(mat.cls, nc * 2u);
     // the y and z terms every thread computes for its own cell, published per plane so the
     // threads of the backward neighbours (j - 1, k - 1 in the interior) read them instead of
     // recomputing them from the neighbour's cell (double-buffered by plane parity: one barrier
@@ -150,21 +128,57 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     const int ly = jn * nz + k - base, lz = j * nz + kn - base;
     const bool y_lds = y_nb && ly >= 0 && ly < OEM_ADJ_TK * OEM_ADJ_TJ;
     const bool z_lds = z_nb && lz >= 0 && lz < OEM_ADJ_TK * OEM_ADJ_TJ;
+    // per-thread byte offsets within a plane: the cell, its y and z neighbours, their psi rows
+    const uint32_t vo = (uint32_t)(j * nz + k) * 4u, vyn = (uint32_t)(jn * nz + k) * 4u,
+                   vzn = (uint32_t)(j * nz + kn) * 4u;
+    const uint32_t vpy = (uint32_t)(max(yc.s, 0) * nz + k) * 4u;
+    const uint32_t vpyn = (uint32_t)(max(yn.s, 0) * nz + k) * 4u;
+    const uint32_t vpz = (uint32_t)(j * wz + max(zc.s, 0)) * 4u;
+    const uint32_t vpzn = (uint32_t)(j * wz + max(zn.s, 0)) * 4u;
     int buf = 0;
+    // the class of the cell at byte offset v of plane offset sp (class LUT materials), and the
+    // cell's cb / ca from its class or from the dense arrays (flat cell id)
+    auto cls_at = [&](uint32_t v, uint32_t sp) -> uint32_t {
+        return mat.mode == 0 ? bld_u16(rC, v >> 1, sp >> 1) : 0u;
+    };
+    auto cbs = [&](uint32_t c, uint32_t id, float &bx, float &by, float &bz) {
+        if (mat.mode == 0) {
+            if (lds) {
+                const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
+                bx = r0.y; by = r0.w; bz = r1.y;
+            } else {
+                bx = mat.lut[8 * (size_t)c + 1]; by = mat.lut[8 * (size_t)c + 3]; bz = mat.lut[8 * (size_t)c + 5];
+            }
+        } else {
+            bx = mat.cb[id]; by = mat.cb[nc + id]; bz = mat.cb[2 * nc + id];
+        }
+    };
+    auto cas = [&](uint32_t c, uint32_t id, float &ax, float &ay, float &az) {
+        if (mat.mode == 0) {
+            if (lds) {
+                const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
+                ax = r0.x; ay = r0.z; az = r1.x;
+            } else {
+                ax = mat.lut[8 * (size_t)c]; ay = mat.lut[8 * (size_t)c + 2]; az = mat.lut[8 * (size_t)c + 4];
+            }
+        } else {
+            ax = mat.ca[id]; ay = mat.ca[nc + id]; az = mat.ca[2 * nc + id];
+        }
+    };
 
     // the x terms (t3: c1, sgn -1, slot 0; t4: c2, sgn +1, slot 1) of a cell on plane i
     auto qx = [&](int i, float &q3, float &q4) {
-        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const uint32_t sp = (uint32_t)i * plane * 4u;
         const AxAt xc = ax_at(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
         float bx, by, bz;
-        cb3(mat, s_lut, lds, id, nc, bx, by, bz);
-        const float nu1 = by * (zc.pec * xc.pec) * d.lE[nc + id];
-        const float nu2 = bz * (xc.pec * yc.pec) * d.lE[2 * nc + id];
+        cbs(cls_at(vo, sp), (uint32_t)i * plane + (uint32_t)(j * nz + k), bx, by, bz);
+        const float nu1 = by * (zc.pec * xc.pec) * FT<float>::ld(rE, vo, sp + cB);
+        const float nu2 = bz * (xc.pec * yc.pec) * FT<float>::ld(rE, vo, sp + 2u * cB);
         float p0 = 0.f, p1 = 0.f;
         if (xc.s >= 0) {
-            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
-            p0 = d.lpe[off[0] + o];
-            p1 = d.lpe[off[1] + o];
+            const uint32_t sx = (uint32_t)xc.s * plane * 4u;
+            p0 = FT<float>::ld(rP, vo, off[0] * 4u + sx);
+            p1 = FT<float>::ld(rP, vo, off[1] * 4u + sx);
         }
         q3 = q_of(-1.f * nu1, xc, -1.f, nu1, p0);
         q4 = q_of(1.f * nu2, xc, 1.f, nu2, p1);
@@ -185,27 +199,24 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     }
     for (int i = i1 - 1; i >= i0; --i) {
         const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const uint32_t sp = (uint32_t)i * plane * 4u;
+        const uint32_t sy = (uint32_t)i * (uint32_t)(wy * nz) * 4u;
+        const uint32_t sz = (uint32_t)i * (uint32_t)(ny * wz) * 4u;
         const AxAt xc = ax_at(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+        const uint32_t sx = (uint32_t)max(xc.s, 0) * plane * 4u;
+        const uint32_t cls = cls_at(vo, sp);
         float bx, by, bz;
-        cb3(mat, s_lut, lds, id, nc, bx, by, bz);
-        const float l0 = d.lE[id], l1 = d.lE[nc + id], l2 = d.lE[2 * nc + id];
+        cbs(cls, id, bx, by, bz);
+        const float l0 = FT<float>::ld(rE, vo, sp), l1 = FT<float>::ld(rE, vo, sp + cB),
+                    l2 = FT<float>::ld(rE, vo, sp + 2u * cB);
         // nu_c = cb_c pec_c lE_c
         const float nu0 = bx * (yc.pec * zc.pec) * l0;
         const float nu1 = by * (zc.pec * xc.pec) * l1;
         const float nu2 = bz * (xc.pec * yc.pec) * l2;
         float py0 = 0.f, py1 = 0.f, pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f;
-        if (yc.s >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yc.s) * (uint32_t)nz + (uint32_t)k;
-            py0 = d.lpe[off[2] + o]; py1 = d.lpe[off[3] + o];
-        }
-        if (zc.s >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zc.s;
-            pz0 = d.lpe[off[4] + o]; pz1 = d.lpe[off[5] + o];
-        }
-        if (xc.s >= 0) {
-            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
-            px0 = d.lpe[off[0] + o]; px1 = d.lpe[off[1] + o];
-        }
+        if (yc.s >= 0) { py0 = FT<float>::ld(rP, vpy, off[2] * 4u + sy); py1 = FT<float>::ld(rP, vpy, off[3] * 4u + sy); }
+        if (zc.s >= 0) { pz0 = FT<float>::ld(rP, vpz, off[4] * 4u + sz); pz1 = FT<float>::ld(rP, vpz, off[5] * 4u + sz); }
+        if (xc.s >= 0) { px0 = FT<float>::ld(rP, vo, off[0] * 4u + sx); px1 = FT<float>::ld(rP, vo, off[1] * 4u + sx); }
         const float q0 = q_of(1.f * nu0, yc, 1.f, nu0, py0);      // t0: c0, y, slot 2
         const float q5 = q_of(-1.f * nu2, yc, -1.f, nu2, py1);    // t5: c2, y, slot 3
         const float q1 = q_of(-1.f * nu0, zc, -1.f, nu0, pz0);    // t1: c0, z, slot 4
@@ -225,16 +236,12 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
             q0n = s_q[buf][0][ly];
             q5n = s_q[buf][1][ly];
         } else if (y_nb) {
-            const uint32_t idn = (uint32_t)i * plane + (uint32_t)(jn * nz + k);
             float b0, b1, b2;
-            cb3(mat, s_lut, lds, idn, nc, b0, b1, b2);
-            const float n0 = b0 * (yn.pec * zc.pec) * d.lE[idn];
-            const float n2 = b2 * (xc.pec * yn.pec) * d.lE[2 * nc + idn];
+            cbs(cls_at(vyn, sp), (uint32_t)i * plane + (uint32_t)(jn * nz + k), b0, b1, b2);
+            const float n0 = b0 * (yn.pec * zc.pec) * FT<float>::ld(rE, vyn, sp);
+            const float n2 = b2 * (xc.pec * yn.pec) * FT<float>::ld(rE, vyn, sp + 2u * cB);
             float a0 = 0.f, a1 = 0.f;
-            if (yn.s >= 0) {
-                const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yn.s) * (uint32_t)nz + (uint32_t)k;
-                a0 = d.lpe[off[2] + o]; a1 = d.lpe[off[3] + o];
-            }
+            if (yn.s >= 0) { a0 = FT<float>::ld(rP, vpyn, off[2] * 4u + sy); a1 = FT<float>::ld(rP, vpyn, off[3] * 4u + sy); }
             q0n = q_of(1.f * n0, yn, 1.f, n0, a0);
             q5n = q_of(-1.f * n2, yn, -1.f, n2, a1);
             (void)b1;
@@ -243,16 +250,12 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
             q1n = s_q[buf][2][lz];
             q2n = s_q[buf][3][lz];
         } else if (z_nb) {
-            const uint32_t idn = (uint32_t)i * plane + (uint32_t)(j * nz + kn);
             float b0, b1, b2;
-            cb3(mat, s_lut, lds, idn, nc, b0, b1, b2);
-            const float n0 = b0 * (yc.pec * zn.pec) * d.lE[idn];
-            const float n1 = b1 * (zn.pec * xc.pec) * d.lE[nc + idn];
+            cbs(cls_at(vzn, sp), (uint32_t)i * plane + (uint32_t)(j * nz + kn), b0, b1, b2);
+            const float n0 = b0 * (yc.pec * zn.pec) * FT<float>::ld(rE, vzn, sp);
+            const float n1 = b1 * (zn.pec * xc.pec) * FT<float>::ld(rE, vzn, sp + cB);
             float a0 = 0.f, a1 = 0.f;
-            if (zn.s >= 0) {
-                const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zn.s;
-                a0 = d.lpe[off[4] + o]; a1 = d.lpe[off[5] + o];
-            }
+            if (zn.s >= 0) { a0 = FT<float>::ld(rP, vpzn, off[4] * 4u + sz); a1 = FT<float>::ld(rP, vpzn, off[5] * 4u + sz); }
             q1n = q_of(-1.f * n0, zn, -1.f, n0, a0);
             q2n = q_of(1.f * n1, zn, 1.f, n1, a1);
             (void)b2;
@@ -280,31 +283,28 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         if (!act) continue;
         float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
         acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
-        d.lH[id] += acc0;
-        d.lH[nc + id] += acc1;
-        d.lH[2 * nc + id] += acc2;
+        FT<float>::st(rH, FT<float>::ld(rH, vo, sp) + acc0, vo, sp);
+        FT<float>::st(rH, FT<float>::ld(rH, vo, sp + cB) + acc1, vo, sp + cB);
+        FT<float>::st(rH, FT<float>::ld(rH, vo, sp + 2u * cB) + acc2, vo, sp + 2u * cB);
         // the cell-local part, into the other buffers (neighbours read the old values):
         // lpe <- b (sgn nu + lpe) per slab term, lE <- ca pec lE
         if (xc.s >= 0) {
-            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
-            d.lpe_n[off[0] + o] = xc.b * (-1.f * nu1 + px0);
-            d.lpe_n[off[1] + o] = xc.b * (1.f * nu2 + px1);
+            FT<float>::st(rPn, xc.b * (-1.f * nu1 + px0), vo, off[0] * 4u + sx);
+            FT<float>::st(rPn, xc.b * (1.f * nu2 + px1), vo, off[1] * 4u + sx);
         }
         if (yc.s >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yc.s) * (uint32_t)nz + (uint32_t)k;
-            d.lpe_n[off[2] + o] = yc.b * (1.f * nu0 + py0);
-            d.lpe_n[off[3] + o] = yc.b * (-1.f * nu2 + py1);
+            FT<float>::st(rPn, yc.b * (1.f * nu0 + py0), vpy, off[2] * 4u + sy);
+            FT<float>::st(rPn, yc.b * (-1.f * nu2 + py1), vpy, off[3] * 4u + sy);
         }
         if (zc.s >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zc.s;
-            d.lpe_n[off[4] + o] = zc.b * (-1.f * nu0 + pz0);
-            d.lpe_n[off[5] + o] = zc.b * (1.f * nu1 + pz1);
+            FT<float>::st(rPn, zc.b * (-1.f * nu0 + pz0), vpz, off[4] * 4u + sz);
+            FT<float>::st(rPn, zc.b * (1.f * nu1 + pz1), vpz, off[5] * 4u + sz);
         }
         float ax_, ay_, az_;
-        ca3(mat, s_lut, lds, id, nc, ax_, ay_, az_);
-        d.lE_n[id] = ax_ * (yc.pec * zc.pec) * l0;
-        d.lE_n[nc + id] = ay_ * (zc.pec * xc.pec) * l1;
-        d.lE_n[2 * nc + id] = az_ * (xc.pec * yc.pec) * l2;
+        cas(cls, id, ax_, ay_, az_);
+        FT<float>::st(rEn, ax_ * (yc.pec * zc.pec) * l0, vo, sp);
+        FT<float>::st(rEn, ay_ * (zc.pec * xc.pec) * l1, vo, sp + cB);
+        FT<float>::st(rEn, az_ * (xc.pec * yc.pec) * l2, vo, sp + 2u * cB);
     }
 }
 
@@ -331,6 +331,10 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
     const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
     const uint32_t *off = d.psl.off;
     const float mch = -d.ch;
+    // buffer resources: lambda_H, lambda_E (three components at c * cB), the psi adjoints
+    const uint32_t cB = nc * 4u;
+    const rsrc_t rH = mk_rsrc(d.lH, 3u * cB), rE = mk_rsrc(d.lE, 3u * cB);
+    const rsrc_t rP = mk_rsrc(d.lph, d.psl.total * 4u), rPn = mk_rsrc(d.lph_n, d.psl.total * 4u);
     // each thread's y and z terms, published per plane for the forward neighbours' threads
     __shared__ float s_q[2][4][OEM_ADJ_TK * OEM_ADJ_TJ];
     const AxAt yc = ax_at(Y, ny, j, AX_AH, AX_KH, AX_IDLP, AX_MNX);
@@ -343,18 +347,26 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
     const int ly = jp * nz + k - base, lz = j * nz + kp - base;
     const bool y_lds = y_nb && ly >= 0 && ly < OEM_ADJ_TK * OEM_ADJ_TJ;
     const bool z_lds = z_nb && lz >= 0 && lz < OEM_ADJ_TK * OEM_ADJ_TJ;
+    // per-thread byte offsets within a plane: the cell, its y and z neighbours, their psi rows
+    const uint32_t vo = (uint32_t)(j * nz + k) * 4u, vyp = (uint32_t)(jp * nz + k) * 4u,
+                   vzp = (uint32_t)(j * nz + kp) * 4u;
+    const uint32_t vpy = (uint32_t)(max(yc.s, 0) * nz + k) * 4u;
+    const uint32_t vpyp = (uint32_t)(max(yp.s, 0) * nz + k) * 4u;
+    const uint32_t vpz = (uint32_t)(j * wz + max(zc.s, 0)) * 4u;
+    const uint32_t vpzp = (uint32_t)(j * wz + max(zp.s, 0)) * 4u;
     int buf = 0;
 
     // the x terms (t3: c1, sgn -1, slot 0; t4: c2, sgn +1, slot 1) of the cell on plane i
     auto qx = [&](int i, float &q3, float &q4) {
-        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const uint32_t sp = (uint32_t)i * plane * 4u;
         const AxAt xc = ax_at(X, nx, i, AX_AH, AX_KH, AX_IDLP, AX_MNX);
-        const float f1 = mch * -1.f * d.lH[nc + id], f2 = mch * 1.f * d.lH[2 * nc + id];
+        const float f1 = mch * -1.f * FT<float>::ld(rH, vo, sp + cB);
+        const float f2 = mch * 1.f * FT<float>::ld(rH, vo, sp + 2u * cB);
         float p0 = 0.f, p1 = 0.f;
         if (xc.s >= 0) {
-            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
-            p0 = d.lph[off[0] + o];
-            p1 = d.lph[off[1] + o];
+            const uint32_t sx = (uint32_t)xc.s * plane * 4u;
+            p0 = FT<float>::ld(rP, vo, off[0] * 4u + sx);
+            p1 = FT<float>::ld(rP, vo, off[1] * 4u + sx);
         }
         q3 = xc.s >= 0 ? (xc.ik + xc.a) * f1 + xc.a * p0 : f1;
         q4 = xc.s >= 0 ? (xc.ik + xc.a) * f2 + xc.a * p1 : f2;
@@ -372,25 +384,20 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         if (x_nb_bot) qx(ipb, q3p, q4p);
     }
     for (int i = i0; i < i1; ++i) {
-        const uint32_t id = (uint32_t)i * plane + (uint32_t)(j * nz + k);
+        const uint32_t sp = (uint32_t)i * plane * 4u;
+        const uint32_t sy = (uint32_t)i * (uint32_t)(wy * nz) * 4u;
+        const uint32_t sz = (uint32_t)i * (uint32_t)(ny * wz) * 4u;
         const AxAt xc = ax_at(X, nx, i, AX_AH, AX_KH, AX_IDLP, AX_MNX);
-        const float h0 = d.lH[id], h1 = d.lH[nc + id], h2 = d.lH[2 * nc + id];
+        const uint32_t sx = (uint32_t)max(xc.s, 0) * plane * 4u;
+        const float h0 = FT<float>::ld(rH, vo, sp), h1 = FT<float>::ld(rH, vo, sp + cB),
+                    h2 = FT<float>::ld(rH, vo, sp + 2u * cB);
         // f of each term: -ch sgn lH_c
         const float f0p = mch * 1.f * h0, f1m = mch * -1.f * h1, f2p = mch * 1.f * h2;
         const float f0m = mch * -1.f * h0, f1p = mch * 1.f * h1, f2m = mch * -1.f * h2;
         float py0 = 0.f, py1 = 0.f, pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f;
-        if (yc.s >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yc.s) * (uint32_t)nz + (uint32_t)k;
-            py0 = d.lph[off[2] + o]; py1 = d.lph[off[3] + o];
-        }
-        if (zc.s >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zc.s;
-            pz0 = d.lph[off[4] + o]; pz1 = d.lph[off[5] + o];
-        }
-        if (xc.s >= 0) {
-            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
-            px0 = d.lph[off[0] + o]; px1 = d.lph[off[1] + o];
-        }
+        if (yc.s >= 0) { py0 = FT<float>::ld(rP, vpy, off[2] * 4u + sy); py1 = FT<float>::ld(rP, vpy, off[3] * 4u + sy); }
+        if (zc.s >= 0) { pz0 = FT<float>::ld(rP, vpz, off[4] * 4u + sz); pz1 = FT<float>::ld(rP, vpz, off[5] * 4u + sz); }
+        if (xc.s >= 0) { px0 = FT<float>::ld(rP, vo, off[0] * 4u + sx); px1 = FT<float>::ld(rP, vo, off[1] * 4u + sx); }
         auto qh = [](const AxAt &t, float f, float psi) {
             return t.s >= 0 ? (t.ik + t.a) * f + t.a * psi : f;
         };
@@ -410,27 +417,19 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
             q0n = s_q[buf][0][ly];
             q5n = s_q[buf][1][ly];
         } else if (y_nb) {
-            const uint32_t idn = (uint32_t)i * plane + (uint32_t)(jp * nz + k);
             float a0 = 0.f, a1 = 0.f;
-            if (yp.s >= 0) {
-                const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yp.s) * (uint32_t)nz + (uint32_t)k;
-                a0 = d.lph[off[2] + o]; a1 = d.lph[off[3] + o];
-            }
-            q0n = qh(yp, mch * 1.f * d.lH[idn], a0);
-            q5n = qh(yp, mch * -1.f * d.lH[2 * nc + idn], a1);
+            if (yp.s >= 0) { a0 = FT<float>::ld(rP, vpyp, off[2] * 4u + sy); a1 = FT<float>::ld(rP, vpyp, off[3] * 4u + sy); }
+            q0n = qh(yp, mch * 1.f * FT<float>::ld(rH, vyp, sp), a0);
+            q5n = qh(yp, mch * -1.f * FT<float>::ld(rH, vyp, sp + 2u * cB), a1);
         }
         if (z_lds) {
             q1n = s_q[buf][2][lz];
             q2n = s_q[buf][3][lz];
         } else if (z_nb) {
-            const uint32_t idn = (uint32_t)i * plane + (uint32_t)(j * nz + kp);
             float a0 = 0.f, a1 = 0.f;
-            if (zp.s >= 0) {
-                const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zp.s;
-                a0 = d.lph[off[4] + o]; a1 = d.lph[off[5] + o];
-            }
-            q1n = qh(zp, mch * -1.f * d.lH[idn], a0);
-            q2n = qh(zp, mch * 1.f * d.lH[nc + idn], a1);
+            if (zp.s >= 0) { a0 = FT<float>::ld(rP, vpzp, off[4] * 4u + sz); a1 = FT<float>::ld(rP, vpzp, off[5] * 4u + sz); }
+            q1n = qh(zp, mch * -1.f * FT<float>::ld(rH, vzp, sp), a0);
+            q2n = qh(zp, mch * 1.f * FT<float>::ld(rH, vzp, sp + cB), a1);
         }
         const bool x_hi = X.nxt[i] == i;
         const bool x_nb = i > i0 || x_nb_bot;
@@ -454,24 +453,21 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         if (!act) continue;
         float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
         acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
-        d.lE[id] += acc0;
-        d.lE[nc + id] += acc1;
-        d.lE[2 * nc + id] += acc2;
+        FT<float>::st(rE, FT<float>::ld(rE, vo, sp) + acc0, vo, sp);
+        FT<float>::st(rE, FT<float>::ld(rE, vo, sp + cB) + acc1, vo, sp + cB);
+        FT<float>::st(rE, FT<float>::ld(rE, vo, sp + 2u * cB) + acc2, vo, sp + 2u * cB);
         // the cell-local part, into the other buffer: lph <- b (f + lph) per slab term
         if (xc.s >= 0) {
-            const uint32_t o = ((uint32_t)xc.s * (uint32_t)ny + (uint32_t)j) * (uint32_t)nz + (uint32_t)k;
-            d.lph_n[off[0] + o] = xc.b * (f1m + px0);
-            d.lph_n[off[1] + o] = xc.b * (f2p + px1);
+            FT<float>::st(rPn, xc.b * (f1m + px0), vo, off[0] * 4u + sx);
+            FT<float>::st(rPn, xc.b * (f2p + px1), vo, off[1] * 4u + sx);
         }
         if (yc.s >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)wy + (uint32_t)yc.s) * (uint32_t)nz + (uint32_t)k;
-            d.lph_n[off[2] + o] = yc.b * (f0p + py0);
-            d.lph_n[off[3] + o] = yc.b * (f2m + py1);
+            FT<float>::st(rPn, yc.b * (f0p + py0), vpy, off[2] * 4u + sy);
+            FT<float>::st(rPn, yc.b * (f2m + py1), vpy, off[3] * 4u + sy);
         }
         if (zc.s >= 0) {
-            const uint32_t o = ((uint32_t)i * (uint32_t)ny + (uint32_t)j) * (uint32_t)wz + (uint32_t)zc.s;
-            d.lph_n[off[4] + o] = zc.b * (f0m + pz0);
-            d.lph_n[off[5] + o] = zc.b * (f1p + pz1);
+            FT<float>::st(rPn, zc.b * (f0m + pz0), vpz, off[4] * 4u + sz);
+            FT<float>::st(rPn, zc.b * (f1p + pz1), vpz, off[5] * 4u + sz);
         }
     }
 }
@@ -482,6 +478,25 @@ __global__ void __launch_bounds__(256) k_adj_obj(float *F, PointsDev pd, MatDev 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= pd.n_groups) return;
     point_group<float>(F, pd, mat, 1.0f, ncell, n, t);
+}
+
+// Every objective point list of a step in one launch, when no two lists touch the same
+// (field, component, cell): thread t applies group t - ofs[l] of list l, as k_adj_obj would.
+#define OEM_ADJ_OBJ_MAX 8
+struct ObjDev {
+    int n;
+    int ofs[OEM_ADJ_OBJ_MAX + 1];
+    int mag[OEM_ADJ_OBJ_MAX];
+    PointsDev p[OEM_ADJ_OBJ_MAX];
+};
+__global__ void __launch_bounds__(256) k_adj_obj_all(float *lE, float *lH, ObjDev o, MatDev mat,
+                                                     size_t ncell, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= o.ofs[o.n]) return;
+    int l = 0;
+    while (t >= o.ofs[l + 1]) ++l;
+    point_group<float>(o.mag[l] ? lH : lE, o.p[l], mat, 1.0f, ncell, n, t - o.ofs[l]);
 }
 
 // eps gradient: g[q] += lE . (dca e0 + dcb (e1 - ca e0) / cb)
@@ -523,8 +538,11 @@ struct oem_adjoint_state {
     float *rec = nullptr;            // (max_seg + 1) * n_design
     int64_t rec_s0 = -1;
     int rec_n = 0;
-    // objective point lists: E side, H side
+    // objective point lists: E side, H side; the (field, component, cell) keys they touch, and
+    // whether every key is touched by one list only (then one launch applies them all)
     std::vector<Points> obj;
+    std::unordered_set<uint64_t> obj_keys;
+    bool obj_disjoint = true;
 };
 
 namespace oemi {
@@ -545,9 +563,10 @@ static int adj_check(oem_ctx *ctx) {
     if (ctx->pool) return fail(OEM_ENOTSUP, "pooled contexts have no adjoint");
     if ((ctx->dims.nx + adj_chunk() - 1) / adj_chunk() > 65535)
         return fail(OEM_ENOTSUP, "the adjoint launch covers nx <= %d", 65535 * adj_chunk());
-    // the transposed sweeps index the three adjoint components with 32-bit offsets
-    if (3ull * ctx->ncell >= (1ull << 32) || ctx->psi.total >= (1u << 31))
-        return fail(OEM_ENOTSUP, "the adjoint covers up to 2^32 / 3 cells");
+    // the transposed sweeps address the three components of an adjoint field, and the psi
+    // adjoints of one side, through buffer resources with 32-bit byte offsets
+    if (12ull * ctx->ncell >= (1ull << 32) || 4ull * ctx->psi.total >= (1ull << 32))
+        return fail(OEM_ENOTSUP, "the adjoint covers up to 2^32 / 12 cells");
     if (!ctx->adj) ctx->adj = new oem_adjoint_state();
     return OEM_OK;
 }
@@ -660,7 +679,16 @@ extern "C" int oem_adjoint_objective(oem_ctx *ctx, int32_t magnetic, int32_t n_p
     TRY(oemi::build_points(ctx, magnetic, PT_TFSF, 1, 1, n_pts, comp, cell, one.data(), col.data(),
                            z, 1, tab, (size_t)ctx->dims.n_steps * 2, &P));
     P.d.e_side = 0;                      // scale given by the kernel argument (1)
-    ctx->adj->obj.push_back(P);
+    oem_adjoint_state *a = ctx->adj;
+    std::unordered_set<uint64_t> mine;
+    for (int q = 0; q < n_pts; ++q) {
+        const uint64_t key = ((uint64_t)(magnetic ? 1 : 0) << 63) | ((uint64_t)comp[q] << 40) |
+                             (uint64_t)cell[q];
+        mine.insert(key);
+    }
+    for (uint64_t key : mine)
+        if (!a->obj_keys.insert(key).second) a->obj_disjoint = false;
+    a->obj.push_back(P);
     return OEM_OK;
 }
 
@@ -744,12 +772,30 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
     const dim3 tm((cols + OEM_ADJ_TK * OEM_ADJ_TJ - 1) / (OEM_ADJ_TK * OEM_ADJ_TJ), 1,
                   (ctx->dims.nx + d.chunk - 1) / d.chunk);
     const dim3 gb(256), gg((a->n_design + 255) / 256);
+    // the objective lists in one launch per step when they touch disjoint cells
+    ObjDev od{};
+    const bool all = a->obj_disjoint && a->obj.size() <= OEM_ADJ_OBJ_MAX &&
+                     !(std::getenv("OEM_ADJ_OBJ_ALL") && std::getenv("OEM_ADJ_OBJ_ALL")[0] == '0');
+    if (all) {
+        od.n = (int)a->obj.size();
+        for (int l = 0; l < od.n; ++l) {
+            od.p[l] = a->obj[l].d;
+            od.mag[l] = a->obj[l].magnetic ? 1 : 0;
+            od.ofs[l + 1] = od.ofs[l] + a->obj[l].d.n_groups;
+        }
+    }
     for (int s = a->rec_n - 1; s >= 0; --s) {
         const int n = (int)(a->rec_s0 + s);          // forward step n: E^n -> E^{n+1}
-        for (auto &P : a->obj)
-            hipLaunchKernelGGL(k_adj_obj, dim3((P.d.n_groups + 255) / 256), dim3(256), 0,
-                               ctx->stream, P.magnetic ? a->lH : a->lE, P.d, ctx->mat,
-                               ctx->ncell, n);
+        if (all) {
+            if (od.ofs[od.n] > 0)
+                hipLaunchKernelGGL(k_adj_obj_all, dim3((od.ofs[od.n] + 255) / 256), dim3(256), 0,
+                                   ctx->stream, a->lE, a->lH, od, ctx->mat, ctx->ncell, n);
+        } else {
+            for (auto &P : a->obj)
+                hipLaunchKernelGGL(k_adj_obj, dim3((P.d.n_groups + 255) / 256), dim3(256), 0,
+                                   ctx->stream, P.magnetic ? a->lH : a->lE, P.d, ctx->mat,
+                                   ctx->ncell, n);
+        }
         hipLaunchKernelGGL(k_adj_grad, gg, gb, 0, ctx->stream, a->lE, a->d_comp, a->d_cell,
                            a->dca, a->dcb, a->ca, a->cb, a->rec + (size_t)s * a->n_design,
                            a->rec + (size_t)(s + 1) * a->n_design, a->n_design, ctx->ncell,

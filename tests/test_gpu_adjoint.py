@@ -92,3 +92,20 @@ def test_transpose_chunks_do_not_change_the_gradient(gpu_lib, monkeypatch, chunk
     assert np.abs(ref.grad).max() > 0
     assert alt.J == ref.J
     assert np.array_equal(alt.grad, ref.grad)
+
+
+def test_objective_lists_in_one_launch_equal_one_launch_each(gpu_lib, monkeypatch):
+    """The objective's point lists touch disjoint cells, so one launch applies them all per
+    backward step; the gradient equals that of one launch per list (OEM_ADJ_OBJ_ALL=0) bit for
+    bit."""
+    sc = _guide(200)
+    t = tables.build(sc)
+    mon = [m.name for m in t.monitors].index("fwd")
+    comp, cell = _design(sc, t, mon, count=6, seed=9)
+    monkeypatch.delenv("OEM_ADJ_OBJ_ALL", raising=False)
+    ref = adjoint.mode_power_gradient(sc, "fwd", comp, cell)
+    monkeypatch.setenv("OEM_ADJ_OBJ_ALL", "0")
+    alt = adjoint.mode_power_gradient(sc, "fwd", comp, cell)
+    assert np.abs(ref.grad).max() > 0
+    assert alt.J == ref.J
+    assert np.array_equal(alt.grad, ref.grad)
