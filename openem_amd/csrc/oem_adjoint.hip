@@ -823,7 +823,7 @@ extern "C" int oem_adjoint_gradient(oem_ctx *ctx, double *out, size_t n) {
     oem_adjoint_state *a = ctx->adj;
     if (!a->grad || !out || n != (size_t)a->n_design)
         return fail(OEM_EINVAL, "gradient holds %d values", a->n_design);
+    HIP_OK(hipMemcpyAsync(out, a->grad, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    HIP_OK(hipMemcpy(out, a->grad, n * sizeof(double), hipMemcpyDeviceToHost));
     return OEM_OK;
 }

@@ -7,6 +7,8 @@
 #include "../../include/openem_hip.h"
 #include "oem_kernels.h"
 
+#include <mutex>
+
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -80,6 +82,17 @@ int pool_arena_alloc(oem_pool *pool, size_t bytes, void **out);
 void pool_arena_free(oem_pool *pool, void *p);
 
 }  // namespace oemi
+
+namespace oemi {
+// Held while a stream is captured into a graph (begin to instantiate) and around every call
+// that may synchronise the device (hipFree, hipHostFree, hipStreamDestroy, hipMalloc):
+// contexts step on several host threads at once (serving lanes, designs in flight), and a
+// device-wide synchronisation issued by one thread while another thread's stream is capturing
+// invalidates that capture.  For the same reason no copy here goes through the legacy stream:
+// every host copy is hipMemcpyAsync on the context's (or pool's) own stream, then a
+// synchronisation of that stream.
+std::mutex &device_sync_mutex();
+}
 
 struct oem_reduce_state;            // energy / flux reduction tables (oem_reduce.hip)
 struct oem_adjoint_state;           // checkpoints, adjoint fields, design gradient (oem_adjoint.hip)
@@ -172,6 +185,7 @@ struct oem_ctx {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (red) oemi::reduce_state_free(red);
         if (adj) oemi::adjoint_state_free(adj);
+        std::lock_guard<std::mutex> hold(oemi::device_sync_mutex());
         for (void *p : allocs) {
             if (pool) oemi::pool_arena_free(pool, p);
             else (void)hipFree(p);
@@ -186,6 +200,7 @@ struct oem_ctx {
         if (pool) {
             TRY(oemi::pool_arena_alloc(pool, n * sizeof(T), &p));
         } else {
+            std::lock_guard<std::mutex> hold(oemi::device_sync_mutex());
             HIP_OK(hipMalloc(&p, n * sizeof(T)));
         }
         allocs.push_back(p);

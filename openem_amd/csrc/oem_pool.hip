@@ -422,6 +422,7 @@ int build_desc(oem_pool *pool, PoolDev *h) {
 }
 
 int capture_graph(oem_pool *pool) {
+    std::lock_guard<std::mutex> hold(oemi::device_sync_mutex());
     hipGraph_t gr = nullptr;
     HIP_OK(hipStreamBeginCapture(pool->stream, hipStreamCaptureModeThreadLocal));
     const PoolDev *P = pool->d_desc;
@@ -459,6 +460,7 @@ void destroy_pool(oem_pool *p) {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (auto &f : p->inflight) (void)hipEventDestroy(f.second);
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
+    std::lock_guard<std::mutex> hold(oemi::device_sync_mutex());
     for (int i = 0; i < OEM_POOL_STAGES; ++i) {
         if (p->h_stage[i]) (void)hipHostFree(p->h_stage[i]);
         if (p->stage_ev[i]) (void)hipEventDestroy(p->stage_ev[i]);
@@ -490,19 +492,24 @@ extern "C" int oem_pool_create(int32_t device, uint64_t arena_bytes, int32_t dft
         if (e != hipSuccess && !rc)
             rc = fail(e == hipErrorOutOfMemory ? OEM_ENOMEM : OEM_EHIP, "%s: %s", what, hipGetErrorString(e));
     };
-    hip(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
-    if (!rc) hip(hipMalloc((void **)&p->base, cap), "arena hipMalloc");
-    if (!rc) hip(hipMalloc((void **)&p->d_desc, sizeof(PoolDev)), "descriptor hipMalloc");
-    for (int i = 0; i < OEM_POOL_STAGES && !rc; ++i) {
-        hip(hipHostMalloc((void **)&p->h_stage[i], sizeof(PoolDev), hipHostMallocDefault), "hipHostMalloc");
-        if (!rc) hip(hipEventCreateWithFlags(&p->stage_ev[i], hipEventDisableTiming), "hipEventCreate");
+    {
+        std::lock_guard<std::mutex> hold(oemi::device_sync_mutex());
+        hip(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+        if (!rc) hip(hipMalloc((void **)&p->base, cap), "arena hipMalloc");
+        if (!rc) hip(hipMalloc((void **)&p->d_desc, sizeof(PoolDev)), "descriptor hipMalloc");
+        for (int i = 0; i < OEM_POOL_STAGES && !rc; ++i) {
+            hip(hipHostMalloc((void **)&p->h_stage[i], sizeof(PoolDev), hipHostMallocDefault), "hipHostMalloc");
+            if (!rc) hip(hipEventCreateWithFlags(&p->stage_ev[i], hipEventDisableTiming), "hipEventCreate");
+        }
     }
     if (!rc) {
         p->capacity = cap;
         p->free_at[0] = cap;
         // an empty table, so a graph replayed before the first upload does nothing
         std::memset(p->h_stage[0], 0, sizeof(PoolDev));
-        hip(hipMemcpy(p->d_desc, p->h_stage[0], sizeof(PoolDev), hipMemcpyHostToDevice), "hipMemcpy");
+        hip(hipMemcpyAsync(p->d_desc, p->h_stage[0], sizeof(PoolDev), hipMemcpyHostToDevice,
+                           p->stream), "hipMemcpyAsync");
+        if (!rc) hip(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
     }
     if (!rc) {
         // Over-sized grids (blocks past the end of a list exit at once): with few long items

@@ -22,6 +22,11 @@ __global__ void k_step_advance(int *stp) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *stp += 1;
 }
 
+std::mutex &oemi::device_sync_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 extern "C" int oem_version(void) { return OEM_ABI_VERSION; }
 
 extern "C" int oem_last_error(char *buf, size_t n) {
@@ -61,8 +66,11 @@ int ctx_init(oem_ctx *c, const oem_dims *d, int32_t device) {
     c->ncell = (size_t)d->nx * d->ny * d->nz;
     c->cplx = d->complex_fields != 0;
     c->esz = c->cplx ? 2 : 1;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail(OEM_EHIP, "hipStreamCreate failed");
+    {
+        std::lock_guard<std::mutex> hold(oemi::device_sync_mutex());
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(OEM_EHIP, "hipStreamCreate failed");
+    }
     const size_t nc = c->ncell;
     if (nc * 4 * c->esz > 0xffffffffull)
         return fail(OEM_ENOTSUP, "%zu cells: one field component exceeds 4 GiB (limit 2^%d cells)",
@@ -547,8 +555,8 @@ extern "C" int oem_read_time_monitor(oem_ctx *ctx, int32_t mon_id, float *out, s
     if (!out || count != m.count * ctx->esz)
         return fail(OEM_EINVAL, "time monitor %d holds %zu floats, asked %zu", mon_id,
                     m.count * ctx->esz, count);
+    HIP_OK(hipMemcpyAsync(out, m.d.buf, count * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    HIP_OK(hipMemcpy(out, m.d.buf, count * sizeof(float), hipMemcpyDeviceToHost));
     return OEM_OK;
 }
 
@@ -625,8 +633,8 @@ extern "C" int oem_read_flux_time_monitor(oem_ctx *ctx, int32_t mon_id, double *
     const FluxTimeMonitor &m = ctx->ftmons[mon_id];
     if (!out || count != (size_t)m.d.slots)
         return fail(OEM_EINVAL, "flux-time monitor %d holds %d values, asked %zu", mon_id, m.d.slots, count);
+    HIP_OK(hipMemcpyAsync(out, m.d.buf, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    HIP_OK(hipMemcpy(out, m.d.buf, count * sizeof(double), hipMemcpyDeviceToHost));
     return OEM_OK;
 }
 
@@ -706,8 +714,9 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
         if (any) {
             if (!ctx->has_disp) return fail(OEM_EINVAL, "classes flag dispersion but no ADE tables set");
             std::vector<uint16_t> cls(ctx->ncell);
-            HIP_OK(hipMemcpy(cls.data(), ctx->mat.cls, ctx->ncell * sizeof(uint16_t),
-                             hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpyAsync(cls.data(), ctx->mat.cls, ctx->ncell * sizeof(uint16_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIP_OK(hipStreamSynchronize(ctx->stream));
             const int ny = ctx->dims.ny, nz = ctx->dims.nz;
             const int *hb = ctx->mat.hb;
             for (size_t id = 0; id < ctx->ncell; ++id) {
@@ -1050,6 +1059,8 @@ static int run_steps(oem_ctx *ctx, int64_t n) {
         if (use_graph && ctx->pending == 0 && ctx->step % T == 0 && left > T && disp_ok &&
             !(ctx->fused && (T & 1))) {
             if (!ctx->graph) {
+                // no other thread synchronises the device while this stream captures
+                std::lock_guard<std::mutex> hold(oemi::device_sync_mutex());
                 hipGraph_t gr = nullptr;
                 HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
                 // each captured step sees its own step index: on the fused path its parity picks
@@ -1177,9 +1188,9 @@ extern "C" int oem_read_monitor(oem_ctx *ctx, int32_t mon_id, double *re, double
     if (count != m.count || !re || !im)
         return fail(OEM_EINVAL, "monitor %d holds %zu values, asked %zu", mon_id, m.count, count);
     if (ctx->pool) TRY(oemi::pool_wait_done(ctx));
+    HIP_OK(hipMemcpyAsync(re, m.d.re, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipMemcpyAsync(im, m.d.im, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    HIP_OK(hipMemcpy(re, m.d.re, count * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(im, m.d.im, count * sizeof(double), hipMemcpyDeviceToHost));
     return OEM_OK;
 }
 
@@ -1197,8 +1208,8 @@ extern "C" int oem_read_field(oem_ctx *ctx, int32_t comp, float *out, size_t cou
     TRY(field_ptr(ctx, comp, count, &p));
     if (!out) return fail(OEM_EINVAL, "null output");
     if (ctx->pool) TRY(oemi::pool_wait_done(ctx));
+    HIP_OK(hipMemcpyAsync(out, p, count * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    HIP_OK(hipMemcpy(out, p, count * sizeof(float), hipMemcpyDeviceToHost));
     return OEM_OK;
 }
 
@@ -1208,7 +1219,7 @@ extern "C" int oem_write_field(oem_ctx *ctx, int32_t comp, const float *in, size
     TRY(field_ptr(ctx, comp, count, &p));
     if (!in) return fail(OEM_EINVAL, "null input");
     if (ctx->attached) return fail(OEM_ESTATE, "context is stepped by its pool");
+    HIP_OK(hipMemcpyAsync(p, in, count * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    HIP_OK(hipMemcpy(p, in, count * sizeof(float), hipMemcpyHostToDevice));
     return OEM_OK;
 }
