@@ -130,7 +130,7 @@ def cpu_baseline(n: int, steps: int = 4, nx_sample: int = 64, workers: int = 8) 
 
 
 def serving_bench(n_req: int, arena_gib: int, workers: int, solo_lanes: int = 2,
-                  best_rate: float | None = None) -> dict:
+                  best_rate: float | None = None, solo_running: int = 2) -> dict:
     """The seeded serving stream, served by continuous batching and by the serial plain loop.
     ``best_rate`` (Mcell-steps/s, the single-scene headline rate) gives the stream's lower
     bound: its cell-steps at that rate, the time no schedule of these scenes can beat on this
@@ -142,7 +142,7 @@ def serving_bench(n_req: int, arena_gib: int, workers: int, solo_lanes: int = 2,
     print(f"[bench] serving {n_req} requests ({cs / 1e12:.2f} Tcell-steps): batched",
           file=sys.stderr, flush=True)
     b = serving.serve(specs, arena_bytes=arena_gib << 30, workers=workers,
-                      solo_lanes=solo_lanes, report_every=16)
+                      solo_lanes=solo_lanes, solo_running=solo_running, report_every=16)
     sb = serving.summary(b)
     print(f"[bench] batched {sb}, peak running {b['peak_running']}; serial", file=sys.stderr,
           flush=True)
@@ -170,6 +170,7 @@ def serving_bench(n_req: int, arena_gib: int, workers: int, solo_lanes: int = 2,
            "serial_stepping_s": round(sum(r.run_s for r in s["requests"]), 2),
            "speedup": round(sb["sims_per_s"] / ss["sims_per_s"], 3),
            "peak_batch": b["peak_running"], "solo_lanes": solo_lanes,
+           "solo_running": solo_running,
            "failed": sb["failed"] + ss["failed"], "errors": errors[:8],
            "max_rel_flux_diff_vs_serial": worst, "arena_gib": arena_gib}
     if best_rate:

@@ -180,10 +180,12 @@ SOLO_CELLS = 8_000_000
 def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers: int = 6,
           max_in_flight: int = 2, build_ahead: int = 8, device: int = 0,
           batch_cells: int = BATCH_CELLS, solo_cells: int = SOLO_CELLS, solo_lanes: int = 2,
-          on_admit=None, report_every: int = 0) -> dict:
+          solo_running: int = 2, on_admit=None, report_every: int = 0) -> dict:
     """Continuous batching of ``specs`` (from :func:`openem_amd.scenes.serving_requests`).
     Ready scenes join the running batch in arrival order while the batch's cells stay within
-    ``batch_cells`` (a scene that alone exceeds it runs when the batch is empty).  With
+    ``batch_cells`` (a scene that alone exceeds it runs when the batch is empty).  Large scenes
+    go to ``solo_lanes`` lanes, of which at most ``solo_running`` step at once (the others set
+    up, read out or release meanwhile).  With
     ``report_every`` a progress line goes to stderr each time that many more requests finish.
     Returns ``dict(requests=[Request...], wall=seconds, peak_running=int)``."""
     pool = Pool(arena_bytes, dft_batch, device)
@@ -194,6 +196,7 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
     # set the end of the stream, and the longest-first order shortens its tail
     solo: queue.PriorityQueue = queue.PriorityQueue()
     seq = iter(range(1 << 62))
+    stepping = threading.Semaphore(max(1, int(solo_running)))
     done_q: queue.Queue = queue.Queue()
     stop = threading.Event()
     mem = threading.Condition()
@@ -264,9 +267,10 @@ def serve(specs: list, arena_bytes: int = 64 << 30, dft_batch: int = 16, workers
             try:
                 st = solver.Stepper(tabs, device=device, dft_batch=dft_batch)
                 try:
-                    tr = time.perf_counter()
-                    st.run(tabs.n_total)
-                    req.run_s = time.perf_counter() - tr
+                    with stepping:
+                        tr = time.perf_counter()
+                        st.run(tabs.n_total)
+                        req.run_s = time.perf_counter() - tr
                     req.flux = _flux_of(st)
                 finally:
                     st.close()
