@@ -340,6 +340,8 @@ def main() -> int:
                          "checkpoints every 500 steps and recomputation)")
     ap.add_argument("--bloch-steps", type=int, default=60,
                     help="steps of the full-size Bloch unit cell to time (0: skip)")
+    ap.add_argument("--bloch-only", action="store_true",
+                    help="only the Bloch unit cell leg")
     ap.add_argument("--budget-s", type=float, default=540.0,
                     help="wall-clock budget of the whole run: a leg that would overrun it is "
                          "shortened (and says so in its record)")
@@ -360,6 +362,9 @@ def main() -> int:
                                                    concurrency=args.adjoint_concurrency,
                                                    record_gib=args.adjoint_record_gib)}),
               flush=True)
+        return 0
+    if args.bloch_only:
+        print(json.dumps({"metric": METRIC, "bloch": bloch_bench(args.bloch_steps)}), flush=True)
         return 0
     if args.serve_only:
         print(json.dumps({"metric": METRIC, "serving": serving_bench(
