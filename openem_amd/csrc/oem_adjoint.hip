@@ -33,7 +33,7 @@ namespace {
 struct AdjDev {
     float *lE, *lH;              // 3 * ncell each
     float *lpe, *lph;            // psi adjoints, slab layout of PsiDev
-    float *lE_n, *lpe_n, *lph_n; // where the transposes write the new lE, lpe, lph (ping-pong)
+    float *lpe_n, *lph_n;        // where the transposes write the new lpe, lph (ping-pong)
     PsiDev psl;                  // offsets (pointers unused)
     GridDev g;
     MatDev mat;
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     // buffer resources (the x march's per-plane offsets go in SGPRs): the three components of
     // lambda_E (old, new) and lambda_H at c * cB, the psi adjoints (old, new), the classes
     const uint32_t cB = nc * 4u;
-    const rsrc_t rE = mk_rsrc(d.lE, 3u * cB), rEn = mk_rsrc(d.lE_n, 3u * cB);
+    const rsrc_t rE = mk_rsrc(d.lE, 3u * cB);
     const rsrc_t rH = mk_rsrc(d.lH, 3u * cB);
     const rsrc_t rP = mk_rsrc(d.lpe, d.psl.total * 4u), rPn = mk_rsrc(d.lpe_n, d.psl.total * 4u);
     const rsrc_t rC = mk_rsrc(mat.cls, nc * 2u);
@@ -151,18 +151,6 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
             }
         } else {
             bx = mat.cb[id]; by = mat.cb[nc + id]; bz = mat.cb[2 * nc + id];
-        }
-    };
-    auto cas = [&](uint32_t c, uint32_t id, float &ax, float &ay, float &az) {
-        if (mat.mode == 0) {
-            if (lds) {
-                const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
-                ax = r0.x; ay = r0.z; az = r1.x;
-            } else {
-                ax = mat.lut[8 * (size_t)c]; ay = mat.lut[8 * (size_t)c + 2]; az = mat.lut[8 * (size_t)c + 4];
-            }
-        } else {
-            ax = mat.ca[id]; ay = mat.ca[nc + id]; az = mat.ca[2 * nc + id];
         }
     };
 
@@ -286,8 +274,9 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         FT<float>::st(rH, FT<float>::ld(rH, vo, sp) + acc0, vo, sp);
         FT<float>::st(rH, FT<float>::ld(rH, vo, sp + cB) + acc1, vo, sp + cB);
         FT<float>::st(rH, FT<float>::ld(rH, vo, sp + 2u * cB) + acc2, vo, sp + 2u * cB);
-        // the cell-local part, into the other buffers (neighbours read the old values):
-        // lpe <- b (sgn nu + lpe) per slab term, lE <- ca pec lE
+        // the cell-local part of the psi adjoints, into the other buffer (neighbours read the
+        // old values): lpe <- b (sgn nu + lpe) per slab term.  The cell-local part of lambda_E
+        // (ca pec lambda_E) is left to the H transpose, which rewrites lambda_E in place
         if (xc.s >= 0) {
             FT<float>::st(rPn, xc.b * (-1.f * nu1 + px0), vo, off[0] * 4u + sx);
             FT<float>::st(rPn, xc.b * (1.f * nu2 + px1), vo, off[1] * 4u + sx);
@@ -300,11 +289,6 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
             FT<float>::st(rPn, zc.b * (-1.f * nu0 + pz0), vpz, off[4] * 4u + sz);
             FT<float>::st(rPn, zc.b * (1.f * nu1 + pz1), vpz, off[5] * 4u + sz);
         }
-        float ax_, ay_, az_;
-        cas(cls, id, ax_, ay_, az_);
-        FT<float>::st(rEn, ax_ * (yc.pec * zc.pec) * l0, vo, sp);
-        FT<float>::st(rEn, ay_ * (zc.pec * xc.pec) * l1, vo, sp + cB);
-        FT<float>::st(rEn, az_ * (xc.pec * yc.pec) * l2, vo, sp + 2u * cB);
     }
 }
 
@@ -316,6 +300,13 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
 // (Y.prv[j], Z.prv[k]) are recomputed.  q = (1/kappa + a) f + a psi_H, f = -ch sgn lH_c.
 __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
 {
+    __shared__ float4 s_lut[2 * OEM_LDS_CLASSES];
+    const MatDev &mat = d.mat;
+    const bool lds = lut_in_lds(mat);
+    if (lds) {
+        stage_lut_adj(s_lut, mat);
+        __syncthreads();
+    }
     const GridDev &g = d.g;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -335,6 +326,21 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
     const uint32_t cB = nc * 4u;
     const rsrc_t rH = mk_rsrc(d.lH, 3u * cB), rE = mk_rsrc(d.lE, 3u * cB);
     const rsrc_t rP = mk_rsrc(d.lph, d.psl.total * 4u), rPn = mk_rsrc(d.lph_n, d.psl.total * 4u);
+    const rsrc_t rC = mk_rsrc(mat.cls, nc * 2u);
+    // ca of the three components of a cell, from its class or the dense arrays (flat id)
+    auto cas = [&](uint32_t v, uint32_t sp, uint32_t id, float &ax, float &ay, float &az) {
+        if (mat.mode == 0) {
+            const uint32_t c = bld_u16(rC, v >> 1, sp >> 1);
+            if (lds) {
+                const float4 r0 = s_lut[2 * c], r1 = s_lut[2 * c + 1];
+                ax = r0.x; ay = r0.z; az = r1.x;
+            } else {
+                ax = mat.lut[8 * (size_t)c]; ay = mat.lut[8 * (size_t)c + 2]; az = mat.lut[8 * (size_t)c + 4];
+            }
+        } else {
+            ax = mat.ca[id]; ay = mat.ca[nc + id]; az = mat.ca[2 * nc + id];
+        }
+    };
     // each thread's y and z terms, published per plane for the forward neighbours' threads
     __shared__ float s_q[2][4][OEM_ADJ_TK * OEM_ADJ_TJ];
     const AxAt yc = ax_at(Y, ny, j, AX_AH, AX_KH, AX_IDLP, AX_MNX);
@@ -453,9 +459,16 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         if (!act) continue;
         float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
         acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
-        FT<float>::st(rE, FT<float>::ld(rE, vo, sp) + acc0, vo, sp);
-        FT<float>::st(rE, FT<float>::ld(rE, vo, sp + cB) + acc1, vo, sp + cB);
-        FT<float>::st(rE, FT<float>::ld(rE, vo, sp + 2u * cB) + acc2, vo, sp + 2u * cB);
+        // lambda_E <- ca pec lambda_E (the E update's cell-local part, transposed) + the sum,
+        // in place: no thread reads another cell's lambda_E here
+        float ax_, ay_, az_;
+        cas(vo, sp, (uint32_t)i * plane + (uint32_t)(j * nz + k), ax_, ay_, az_);
+        const float e0 = ax_ * (yc.pec * zc.pec) * FT<float>::ld(rE, vo, sp);
+        const float e1 = ay_ * (zc.pec * xc.pec) * FT<float>::ld(rE, vo, sp + cB);
+        const float e2 = az_ * (xc.pec * yc.pec) * FT<float>::ld(rE, vo, sp + 2u * cB);
+        FT<float>::st(rE, e0 + acc0, vo, sp);
+        FT<float>::st(rE, e1 + acc1, vo, sp + cB);
+        FT<float>::st(rE, e2 + acc2, vo, sp + 2u * cB);
         // the cell-local part, into the other buffer: lph <- b (f + lph) per slab term
         if (xc.s >= 0) {
             FT<float>::st(rPn, xc.b * (f1m + px0), vo, off[0] * 4u + sx);
@@ -529,7 +542,7 @@ struct oem_adjoint_state {
     std::vector<int64_t> ckpt_step;
     // adjoint fields
     float *lE = nullptr, *lH = nullptr, *lpe = nullptr, *lph = nullptr;
-    float *lE2 = nullptr, *lpe2 = nullptr, *lph2 = nullptr;   // the other halves of the ping-pong
+    float *lpe2 = nullptr, *lph2 = nullptr;   // the other halves of the ping-pong
     // design
     int n_design = 0, max_seg = 0;
     int *d_comp = nullptr, *d_cell = nullptr;
@@ -647,7 +660,6 @@ extern "C" int oem_adjoint_setup(oem_ctx *ctx, int32_t n_design, const int32_t *
     TRY(ctx->alloc(&a->lH, 3 * nc, true));
     TRY(ctx->alloc(&a->lpe, (size_t)ctx->psi.total, true));
     TRY(ctx->alloc(&a->lph, (size_t)ctx->psi.total, true));
-    TRY(ctx->alloc(&a->lE2, 3 * nc, true));
     TRY(ctx->alloc(&a->lpe2, (size_t)ctx->psi.total, true));
     TRY(ctx->alloc(&a->lph2, (size_t)ctx->psi.total, true));
     TRY(ctx->upload(&a->d_comp, (const int *)comp, (size_t)n_design));
@@ -800,13 +812,13 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
                            a->dca, a->dcb, a->ca, a->cb, a->rec + (size_t)s * a->n_design,
                            a->rec + (size_t)(s + 1) * a->n_design, a->n_design, ctx->ncell,
                            a->grad);
-        // E transpose: reads lE and lpe, adds to lH, writes the new lE and lpe to the other
-        // buffers; then the H transpose: reads lH and lph, adds to the new lE, writes the new lph
-        d.lE = a->lE; d.lpe = a->lpe; d.lE_n = a->lE2; d.lpe_n = a->lpe2;
+        // E transpose: reads lE and lpe, adds to lH, writes the new lpe to the other buffer;
+        // then the H transpose: reads lH and lph, rewrites lE in place as ca pec lE + its
+        // sum, writes the new lph to the other buffer
+        d.lE = a->lE; d.lpe = a->lpe; d.lpe_n = a->lpe2;
         hipLaunchKernelGGL(k_adj_e1, tm, tb, 0, ctx->stream, d);
-        std::swap(a->lE, a->lE2);
         std::swap(a->lpe, a->lpe2);
-        d.lE = a->lE; d.lph = a->lph; d.lph_n = a->lph2;
+        d.lph = a->lph; d.lph_n = a->lph2;
         hipLaunchKernelGGL(k_adj_h1, tm, tb, 0, ctx->stream, d);
         std::swap(a->lph, a->lph2);
     }
