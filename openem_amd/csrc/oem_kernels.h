@@ -1191,6 +1191,16 @@ struct DispDev {
     float *q_re, *q_im;     // pole state driven by Im E (complex fields only)
     void *hist;             // F elements, (3, bx, by, bz)
     int hb[6];
+    // Compact form, used when every entry has the same pole count np (1..4): per entry
+    // {comp << 30 | cell, history index}; per pole a 16-bit coefficient index and the state as
+    // (re, im) pairs, entry-major (pole q of entry e at e * np + q); coefficients as
+    // (A-1 re, A-1 im, B re, B im).  Every input of an entry is then one load away from its
+    // index (no CSR offsets, no index division), and the arithmetic is pole_update's.
+    int np;                 // 0: the CSR arrays above
+    const uint2 *meta;
+    const uint16_t *c16;
+    const float4 *coef;
+    float2 *sp, *sq;        // pole state driven by Re E / by Im E (complex fields only)
 };
 
 __device__ __forceinline__ size_t hist_index(const DispDev &d, int c, int cell, int ny, int nz) {
@@ -1226,11 +1236,77 @@ __device__ __forceinline__ void hist_store(void *h, size_t q, float w, float) {
     static_cast<float *>(h)[q] = w;
 }
 
+// pole_update on a (re, im) state pair with its coefficients (A-1 re, A-1 im, B re, B im).
+__device__ __forceinline__ float pole_pair(float2 &s, const float4 cf, float e, int mode) {
+    const float br = cf.z, bi = cf.w;
+    float pr = s.x, pi = s.y;
+    if (mode >= 1) {
+        pr = pr + br * e;
+        pi = pi + bi * e;
+        if (mode == 1) {
+            s = make_float2(pr, pi);
+            return 0.f;
+        }
+    }
+    const float ar = cf.x, ai = cf.y;
+    const float dr = ar * pr - ai * pi + br * e;
+    const float di = ar * pi + ai * pr + bi * e;
+    s = make_float2(pr + dr, pi + di);
+    return 2.0f * dr;
+}
+
+// Entry e of the compact form with NP poles: every load first, then the updates in pole order
+// (the sums round as the CSR form's loop does).
+template <typename F, int NP>
+__device__ __forceinline__ void disp_entry_np(const DispDev &d, const F *__restrict__ E,
+                                              size_t ncell, int mode, int e)
+{
+    const uint2 mt = d.meta[e];
+    const uint32_t c = mt.x >> 30, id = mt.x & 0x3fffffffu;
+    const size_t t0 = (size_t)e * NP;
+    float2 sp[NP];
+    float4 cf[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        sp[q] = d.sp[t0 + q];
+        cf[q] = d.coef[d.c16[t0 + q]];
+    }
+    const F en = E[(size_t)c * ncell + id];
+    if constexpr (sizeof(F) == 4) {
+        float w = 0.f;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) w += pole_pair(sp[q], cf[q], en, mode);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) d.sp[t0 + q] = sp[q];
+        if (mode != 1) static_cast<float *>(d.hist)[mt.y] = w;
+    } else {
+        float2 sq[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) sq[q] = d.sq[t0 + q];
+        float wr = 0.f, wi = 0.f;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            wr += pole_pair(sp[q], cf[q], en.r, mode);
+            wi += pole_pair(sq[q], cf[q], en.i, mode);
+        }
+#pragma unroll
+        for (int q = 0; q < NP; ++q) { d.sp[t0 + q] = sp[q]; d.sq[t0 + q] = sq[q]; }
+        if (mode != 1) static_cast<cf2 *>(d.hist)[mt.y] = cf2{wr, wi};
+    }
+}
+
 // Entry e: mode 0 = pre, 1 = post, 2 = fused post + pre.
 template <typename F>
 __device__ __forceinline__ void disp_entry(const DispDev &d, const F *__restrict__ E,
                                            size_t ncell, int ny, int nz, int mode, int e)
 {
+    switch (d.np) {
+    case 1: disp_entry_np<F, 1>(d, E, ncell, mode, e); return;
+    case 2: disp_entry_np<F, 2>(d, E, ncell, mode, e); return;
+    case 3: disp_entry_np<F, 3>(d, E, ncell, mode, e); return;
+    case 4: disp_entry_np<F, 4>(d, E, ncell, mode, e); return;
+    default: break;
+    }
     const int c = d.comp[e], id = d.cell[e];
     const F en = E[(size_t)c * ncell + id];
     if constexpr (sizeof(F) == 4) {

@@ -266,8 +266,46 @@ extern "C" int oem_set_dispersion(oem_ctx *ctx, int32_t n_entry, const int32_t *
     }
     for (int t = 0; t < n_pole; ++t)
         if (cidx[t] < 0 || cidx[t] >= n_coef) return fail(OEM_EINVAL, "cidx out of range at %d", t);
+    const size_t hsz = 3ull * (hb[1] - hb[0]) * (hb[3] - hb[2]) * (hb[5] - hb[4]);
+    if (hsz * 4 * ctx->esz > 0xffffffffull) return fail(OEM_ENOTSUP, "ADE history box exceeds 4 GiB");
     DispDev &D = ctx->disp;
+    D = DispDev{};
     D.n_entry = n_entry;
+    // the compact form when every entry has the same pole count (1..4)
+    const int np0 = ofs[1] - ofs[0];
+    const char *csr = std::getenv("OEM_ADE_CSR");   // "1": keep the CSR form
+    bool uniform = np0 >= 1 && np0 <= 4 && ctx->ncell <= (1ull << 30) && n_coef <= 65536 &&
+                   !(csr && csr[0] == '1');
+    for (int e = 1; uniform && e < n_entry; ++e) uniform = ofs[e + 1] - ofs[e] == np0;
+    if (uniform) {
+        const int bx = hb[1] - hb[0], by = hb[3] - hb[2], bz = hb[5] - hb[4];
+        std::vector<uint32_t> meta(2 * (size_t)n_entry);
+        for (int e = 0; e < n_entry; ++e) {
+            const int i = cell[e] / pl, r = cell[e] % pl, j = r / d.nz, k = r % d.nz;
+            meta[2 * e] = (uint32_t)comp[e] << 30 | (uint32_t)cell[e];
+            meta[2 * e + 1] = (uint32_t)((((size_t)comp[e] * bx + (i - hb[0])) * by + (j - hb[2])) * bz +
+                                         (k - hb[4]));
+        }
+        std::vector<uint16_t> c16((size_t)n_pole);
+        for (int t = 0; t < n_pole; ++t) c16[t] = (uint16_t)cidx[t];
+        std::vector<float> coef(4 * (size_t)n_coef);
+        for (int q = 0; q < n_coef; ++q) {
+            coef[4 * q] = am1_re[q]; coef[4 * q + 1] = am1_im[q];
+            coef[4 * q + 2] = b_re[q]; coef[4 * q + 3] = b_im[q];
+        }
+        uint32_t *p_meta;
+        uint16_t *p_c16;
+        float *p_coef;
+        TRY(ctx->upload(&p_meta, meta.data(), meta.size()));
+        TRY(ctx->upload(&p_c16, c16.data(), c16.size()));
+        TRY(ctx->upload(&p_coef, coef.data(), coef.size()));
+        D.np = np0;
+        D.meta = reinterpret_cast<const uint2 *>(p_meta);
+        D.c16 = p_c16;
+        D.coef = reinterpret_cast<const float4 *>(p_coef);
+        TRY(ctx->alloc(&D.sp, (size_t)n_pole, true));
+        if (ctx->cplx) TRY(ctx->alloc(&D.sq, (size_t)n_pole, true));
+    } else {
     int *p_comp, *p_cell, *p_ofs, *p_cidx;
     float *ar, *ai, *br, *bi;
     TRY(ctx->upload(&p_comp, (const int *)comp, (size_t)n_entry));
@@ -290,8 +328,7 @@ extern "C" int oem_set_dispersion(oem_ctx *ctx, int32_t n_entry, const int32_t *
         TRY(ctx->alloc(&D.q_re, (size_t)n_pole, true));
         TRY(ctx->alloc(&D.q_im, (size_t)n_pole, true));
     }
-    const size_t hsz = 3ull * (hb[1] - hb[0]) * (hb[3] - hb[2]) * (hb[5] - hb[4]);
-    if (hsz * 4 * ctx->esz > 0xffffffffull) return fail(OEM_ENOTSUP, "ADE history box exceeds 4 GiB");
+    }
     float *hist = nullptr;
     TRY(ctx->alloc(&hist, hsz * ctx->esz, true));
     D.hist = hist;

@@ -328,3 +328,32 @@ def test_dense_materials_on_either_step_equal_the_lut(gpu_lib, monkeypatch, fuse
     assert np.array_equal(f1, f2)
     for x, y in zip(a1, a2):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("kind", ["box_soup", "bloch_metal", "mini"])
+def test_compact_ade_tables_equal_the_csr_form(gpu_lib, monkeypatch, kind):
+    """The ADE pass keeps its entries in a compact form when every entry has the same pole
+    count (2 Lorentz poles in the box soup, Drude + 2 Lorentz in the Bloch metal) and in the
+    CSR form otherwise (the mini scene mixes 1- and 2-pole media; ``OEM_ADE_CSR=1`` forces
+    it): both run the same arithmetic, so fields and phasors agree bit for bit."""
+    if kind == "box_soup":
+        sc = scenes.box_soup_scene(shape=(40, 36, 44), n_boxes=10, num_steps=120, seed=5,
+                                   moment=1e-18)
+    elif kind == "bloch_metal":
+        sc = scenes.bloch_cell_scene(nxy=16, nz=80, n_pml=8, num_steps=120, with_metal=True,
+                                     nf=5)
+    else:
+        sc = scenes.mini_scene(shape=(6, 8, 32), num_steps=120, n_pml=5)
+    tabs = tables.build(sc)
+    d = tabs.disp
+    assert d is not None and d.comp.size > 0
+    counts = np.unique(np.diff(d.ofs))
+    print(f"{kind}: {d.comp.size} entries, pole counts {counts.tolist()}", flush=True)
+    monkeypatch.delenv("OEM_ADE_CSR", raising=False)
+    fa, aa = _gpu_state(tabs, 120)
+    monkeypatch.setenv("OEM_ADE_CSR", "1")
+    fb, ab = _gpu_state(tabs, 120)
+    assert np.abs(fa).max() > 0
+    assert np.array_equal(fa, fb)
+    for x, y in zip(aa, ab):
+        assert np.array_equal(x, y)
