@@ -67,6 +67,18 @@ __device__ __forceinline__ AxAt ax_at(const AxisDev &A, int n, int p, int fa, in
     r.pec = axv(A, p, AX_PEC);
     return r;
 }
+// ax_at of a wave-uniform plane p, through scalar loads
+__device__ __forceinline__ AxAt ax_at_row(const AxisDev &A, int n, int p, int fa, int fk, int fidl,
+                                          int fm) {
+    const ctab_t t = ctab_row(A, p);
+    AxAt r;
+    r.s = slab_of(p, n, A.lo, A.hi);
+    r.a = t[fa]; r.ik = t[fk];
+    r.b = t[fa == AX_AE ? AX_BE : AX_BH];
+    r.idl = t[fidl]; r.m = t[fm];
+    r.pec = t[AX_PEC];
+    return r;
+}
 
 // q of one transposed term: (1/kappa + a) sgn x + a psi in the slab, sgn x outside
 __device__ __forceinline__ float q_of(float sgn_x, const AxAt &t, float sgn, float x, float psi) {
@@ -157,7 +169,7 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
     // the x terms (t3: c1, sgn -1, slot 0; t4: c2, sgn +1, slot 1) of a cell on plane i
     auto qx = [&](int i, float &q3, float &q4) {
         const uint32_t sp = (uint32_t)i * plane * 4u;
-        const AxAt xc = ax_at(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+        const AxAt xc = ax_at_row(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
         float bx, by, bz;
         cbs(cls_at(vo, sp), (uint32_t)i * plane + (uint32_t)(j * nz + k), bx, by, bz);
         const float nu1 = by * (zc.pec * xc.pec) * FT<float>::ld(rE, vo, sp + cB);
@@ -190,7 +202,7 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         const uint32_t sp = (uint32_t)i * plane * 4u;
         const uint32_t sy = (uint32_t)i * (uint32_t)(wy * nz) * 4u;
         const uint32_t sz = (uint32_t)i * (uint32_t)(ny * wz) * 4u;
-        const AxAt xc = ax_at(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
+        const AxAt xc = ax_at_row(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
         const uint32_t sx = (uint32_t)max(xc.s, 0) * plane * 4u;
         const uint32_t cls = cls_at(vo, sp);
         float bx, by, bz;
@@ -248,11 +260,11 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
             q2n = q_of(1.f * n1, zn, 1.f, n1, a1);
             (void)b2;
         }
-        const bool x_lo = X.prv[i] == i;
+        const bool x_lo = cidx_at(X.prv, i) == i;
         const bool x_nb = i + 1 < i1 || x_nb_top;
         // the neighbour's mask and 1/dl along x: plane i + 1, or the wrap plane at the top
-        const int ixn = i + 1 < nx ? i + 1 : X.nxt[nx - 1];
-        const float mxn = axv(X, ixn, AX_MPV), ixdn = axv(X, ixn, AX_IDLD);
+        const int ixn = i + 1 < nx ? i + 1 : cidx_at(X.nxt, nx - 1);
+        const float mxn = ctab_row(X, ixn)[AX_MPV], ixdn = ctab_row(X, ixn)[AX_IDLD];
         auto term = [&](float q, const AxAt &c, bool lo, bool nb, float qn, float mn, float idn) {
             float v = q * c.idl;
             if (lo) v -= q * c.m * c.idl;
@@ -365,7 +377,7 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
     // the x terms (t3: c1, sgn -1, slot 0; t4: c2, sgn +1, slot 1) of the cell on plane i
     auto qx = [&](int i, float &q3, float &q4) {
         const uint32_t sp = (uint32_t)i * plane * 4u;
-        const AxAt xc = ax_at(X, nx, i, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+        const AxAt xc = ax_at_row(X, nx, i, AX_AH, AX_KH, AX_IDLP, AX_MNX);
         const float f1 = mch * -1.f * FT<float>::ld(rH, vo, sp + cB);
         const float f2 = mch * 1.f * FT<float>::ld(rH, vo, sp + 2u * cB);
         float p0 = 0.f, p1 = 0.f;
@@ -393,7 +405,7 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         const uint32_t sp = (uint32_t)i * plane * 4u;
         const uint32_t sy = (uint32_t)i * (uint32_t)(wy * nz) * 4u;
         const uint32_t sz = (uint32_t)i * (uint32_t)(ny * wz) * 4u;
-        const AxAt xc = ax_at(X, nx, i, AX_AH, AX_KH, AX_IDLP, AX_MNX);
+        const AxAt xc = ax_at_row(X, nx, i, AX_AH, AX_KH, AX_IDLP, AX_MNX);
         const uint32_t sx = (uint32_t)max(xc.s, 0) * plane * 4u;
         const float h0 = FT<float>::ld(rH, vo, sp), h1 = FT<float>::ld(rH, vo, sp + cB),
                     h2 = FT<float>::ld(rH, vo, sp + 2u * cB);
@@ -437,10 +449,10 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
             q1n = qh(zp, mch * -1.f * FT<float>::ld(rH, vzp, sp), a0);
             q2n = qh(zp, mch * 1.f * FT<float>::ld(rH, vzp, sp + cB), a1);
         }
-        const bool x_hi = X.nxt[i] == i;
+        const bool x_hi = cidx_at(X.nxt, i) == i;
         const bool x_nb = i > i0 || x_nb_bot;
-        const int ixp = i > 0 ? i - 1 : X.prv[0];
-        const float mxp = axv(X, ixp, AX_MNX), ixdp = axv(X, ixp, AX_IDLP);
+        const int ixp = i > 0 ? i - 1 : cidx_at(X.prv, 0);
+        const float mxp = ctab_row(X, ixp)[AX_MNX], ixdp = ctab_row(X, ixp)[AX_IDLP];
         auto term = [](float q, const AxAt &c, bool hi, bool nb, float qn, float mn, float idn) {
             float v = -q * c.idl;
             if (hi) v += q * c.m * c.idl;

@@ -108,6 +108,18 @@ struct AxisDev {
 
 __device__ __forceinline__ float axv(const AxisDev &A, int i, int f) { return A.tab[i * AX_W + f]; }
 
+// A table row read through the constant address space: the tables never change during a
+// launch, so a wave-uniform row comes in through scalar loads, whose wait does not hold the
+// vector loads and stores in flight (those share one in-order counter).
+typedef const __attribute__((address_space(4))) float *ctab_t;
+__device__ __forceinline__ ctab_t ctab_row(const AxisDev &A, int i) {
+    return (ctab_t)(A.tab + (size_t)i * AX_W);
+}
+// A wave-uniform entry of a read-only index table, through a scalar load.
+__device__ __forceinline__ int cidx_at(const int *p, int i) {
+    return ((const __attribute__((address_space(4))) int *)p)[i];
+}
+
 struct GridDev {
     AxisDev ax[3];
     int nx, ny, nz;
@@ -271,7 +283,8 @@ __device__ __forceinline__ void update_h_tile(F *__restrict__ H, const F *__rest
         if (sy >= 0) { psy0 = T::ld(rP, oy, sy0); psy1 = T::ld(rP, oy, sy1); }
         if (sz >= 0) { psz0 = T::ld(rP, oz, sz0); psz1 = T::ld(rP, oz, sz1); }
 
-        const float mi = axv(X, i, AX_MNX), mii = axv(X, i, AX_MNXI), idx = axv(X, i, AX_IDLP);
+        const ctab_t xr = ctab_row(X, i);
+        const float mi = xr[AX_MNX], mii = xr[AX_MNXI], idx = xr[AX_IDLP];
         const F dzy = (T::mmul(ez_j, mj, mji) - ez) * idy;
         const F dyz = (T::mmul(ey_k, mk, mki) - ey) * idz;
         const F dxz = (T::mmul(ex_k, mk, mki) - ex) * idz;
@@ -281,7 +294,7 @@ __device__ __forceinline__ void update_h_tile(F *__restrict__ H, const F *__rest
 
         F s_hx_y = dzy, s_hz_y = dxy, s_hx_z = dyz, s_hy_z = dxz, s_hy_x = dzx, s_hz_x = dyx;
         if (sx >= 0) {
-            const float ax_ = axv(X, i, AX_AH), bx_ = axv(X, i, AX_BH), kx_ = axv(X, i, AX_KH);
+            const float ax_ = xr[AX_AH], bx_ = xr[AX_BH], kx_ = xr[AX_KH];
             s_hy_x = stretch_v(psx0, dzx, ax_, bx_, kx_);
             s_hz_x = stretch_v(psx1, dyx, ax_, bx_, kx_);
             T::st(rP, psx0, op, sx0);
@@ -445,8 +458,9 @@ __device__ __forceinline__ void update_e_tile(F *__restrict__ E, const F *__rest
             if (fl & 4u) phz = T::ld(rHist, oh, sh + 2u * hcomp * T::B);
         }
 
-        const float mi = axv(X, i, AX_MPV), mii = axv(X, i, AX_MPVI);
-        const float idx = axv(X, i, AX_IDLD), pecx = axv(X, i, AX_PEC);
+        const ctab_t xr = ctab_row(X, i);
+        const float mi = xr[AX_MPV], mii = xr[AX_MPVI];
+        const float idx = xr[AX_IDLD], pecx = xr[AX_PEC];
         const F dzy = (hz - T::mmul(hz_j, mj, mji)) * idy;
         const F dyz = (hy - T::mmul(hy_k, mk, mki)) * idz;
         const F dxz = (hx - T::mmul(hx_k, mk, mki)) * idz;
@@ -456,7 +470,7 @@ __device__ __forceinline__ void update_e_tile(F *__restrict__ E, const F *__rest
 
         F s_ex_y = dzy, s_ez_y = dxy, s_ex_z = dyz, s_ey_z = dxz, s_ey_x = dzx, s_ez_x = dyx;
         if (sx >= 0) {
-            const float ax_ = axv(X, i, AX_AE), bx_ = axv(X, i, AX_BE), kx_ = axv(X, i, AX_KE);
+            const float ax_ = xr[AX_AE], bx_ = xr[AX_BE], kx_ = xr[AX_KE];
             s_ey_x = stretch_v(psx0, dzx, ax_, bx_, kx_);
             s_ez_x = stretch_v(psx1, dyx, ax_, bx_, kx_);
             T::st(rP, psx0, op, sx0);
@@ -623,7 +637,7 @@ __device__ __forceinline__ void hcolumn_load(HLoads<F> &v, const HColumn<F> &c, 
 template <typename F>
 __device__ __forceinline__ void hcolumn_do(HColumn<F> &c, const HLoads<F> &v, const ColIO &io,
                                            const GridDev &g, const PsiDev &ps, int i,
-                                           const float *xr, float ch, bool store, F &hx, F &hy,
+                                           ctab_t xr, float ch, bool store, F &hx, F &hy,
                                            F &hz) {
     typedef FT<F> T;
     const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
@@ -916,7 +930,7 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
             HLoads<F> vr, vc;
             hcolumn_load<F>(vr, cr, io, g, i, in_);
             hcolumn_load<F>(vc, cc, io, g, i, in_);
-            const float *xr = X.tab + (size_t)i * AX_W;
+            const ctab_t xr = ctab_row(X, i);
             hcolumn_do<F>(cr, vr, io, g, ps, i, xr, ch, false, hx, hy, hz);
             s_row[buf][0][0][lane] = hx;
             s_row[buf][2][0][lane] = hz;
@@ -977,7 +991,7 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
         HLoads<F> vp;
         hcolumn_load<F>(vp, cp, io, g, xp0, next_of(xp0, nx, xn_last));
         F hx_;
-        hcolumn_do<F>(cp, vp, io, g, ps, xp0, X.tab + (size_t)xp0 * AX_W, ch, false, hx_, hy_m,
+        hcolumn_do<F>(cp, vp, io, g, ps, xp0, ctab_row(X, xp0), ch, false, hx_, hy_m,
                       hz_m);
     }
     hcolumn_load_e<F>(ca, io, g, i0);
@@ -1035,7 +1049,7 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
         }
         const F ey0 = ca.ey, ez0 = ca.ez;
         F hx, hy, hz;
-        const float *xr = X.tab + (size_t)i * AX_W;
+        const ctab_t xr = ctab_row(X, i);
         const F ex0 = cv.ex;
         hcolumn_do<F>(ca, cv, io, g, ps, i, xr, ch, own, hx, hy, hz);
         s_row[buf][0][w + 1][lane] = hx;
