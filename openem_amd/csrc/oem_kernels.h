@@ -548,6 +548,9 @@ __device__ __forceinline__ void pw_h_inj(const PlaneSrcDev &pw, int k, const int
 #ifndef OEM_CR
 #define OEM_CR 7            // owned rows (waves) per work-group; one halo wave more
 #endif
+// s_waitcnt vmcnt(0) with the other counters left alone (gfx9 encoding: vmcnt bits 3:0 and
+// 15:14, expcnt 6:4, lgkmcnt 11:8)
+#define OEM_WAIT_VM0 0x0F70
 
 // Buffer descriptors of the fused step: a field set is E then H in one allocation, a psi set
 // the H side then the E side; one descriptor each keeps the SGPRs the march needs low.
@@ -610,15 +613,42 @@ __device__ __forceinline__ void hcolumn_load_e(HColumn<F> &c, const ColIO &io, c
 template <typename F>
 struct HLoads {
     F ex, ex_j, ex_k, ey_k, ez_j, ey_n, ez_n, h0, h1, h2;
+    F psx0, psx1, psy0, psy1, psz0, psz1;      // H-side psi of the cell (zero outside slabs)
 };
+
+// Byte offsets of the H-side psi of plane i in a psi set (x slab, y slab, z slab; two each).
+struct PsiOffH {
+    uint32_t x0, x1, y0, y1, z0, z1;
+    int sx;
+};
+template <typename F>
+__device__ __forceinline__ PsiOffH psi_off_h(const GridDev &g, const PsiDev &ps, int i) {
+    typedef FT<F> T;
+    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
+    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
+    PsiOffH o;
+    o.sx = slab_of(i, g.nx, X.lo, X.hi);
+    o.x0 = (ps.off[0] + (uint32_t)max(o.sx, 0) * (uint32_t)g.plane) * T::B;
+    o.x1 = (ps.off[1] + (uint32_t)max(o.sx, 0) * (uint32_t)g.plane) * T::B;
+    o.y0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
+    o.y1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
+    o.z0 = (ps.off[4] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
+    o.z1 = (ps.off[5] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
+    return o;
+}
 
 template <typename F>
 __device__ __forceinline__ void hcolumn_load(HLoads<F> &v, const HColumn<F> &c, const ColIO &io,
-                                             const GridDev &g, int i, int in_) {
+                                             const GridDev &g, const PsiDev &ps, int i, int in_) {
     typedef FT<F> T;
     const uint32_t planeB = (uint32_t)g.plane * T::B;
     const uint32_t so = (uint32_t)i * planeB, sn = (uint32_t)in_ * planeB, sh = so + io.hof;
     const uint32_t c1 = io.cb, c2 = 2u * io.cb;
+    const PsiOffH po = psi_off_h<F>(g, ps, i);
+    v.psx0 = v.psx1 = v.psy0 = v.psy1 = v.psz0 = v.psz1 = T::zero();
+    if (po.sx >= 0) { v.psx0 = T::ld(io.po, c.op, po.x0); v.psx1 = T::ld(io.po, c.op, po.x1); }
+    if (c.sy >= 0) { v.psy0 = T::ld(io.po, c.oy, po.y0); v.psy1 = T::ld(io.po, c.oy, po.y1); }
+    if (c.sz >= 0) { v.psz0 = T::ld(io.po, c.oz, po.z0); v.psz1 = T::ld(io.po, c.oz, po.z1); }
     v.ex = T::ld(io.fo, c.op, so);
     v.ex_j = T::ld(io.fo, c.oj, so);
     v.ex_k = T::ld(io.fo, c.ok, so);
@@ -640,22 +670,12 @@ __device__ __forceinline__ void hcolumn_do(HColumn<F> &c, const HLoads<F> &v, co
                                            ctab_t xr, float ch, bool store, F &hx, F &hy,
                                            F &hz) {
     typedef FT<F> T;
-    const AxisDev &X = g.ax[0], &Y = g.ax[1], &Z = g.ax[2];
     const uint32_t so = (uint32_t)i * ((uint32_t)g.plane * T::B), sh = so + io.hof;
     const uint32_t c1 = io.cb, c2 = 2u * io.cb;
-    const int sx = slab_of(i, g.nx, X.lo, X.hi);
-    const int wy = Y.lo + Y.hi, wz = Z.lo + Z.hi;
-    const uint32_t sx0 = (ps.off[0] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
-    const uint32_t sx1 = (ps.off[1] + (uint32_t)max(sx, 0) * (uint32_t)g.plane) * T::B;
-    const uint32_t sy0 = (ps.off[2] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
-    const uint32_t sy1 = (ps.off[3] + (uint32_t)i * (uint32_t)(wy * g.nz)) * T::B;
-    const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
-    const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(g.ny * wz)) * T::B;
-    F psx0 = T::zero(), psx1 = T::zero(), psy0 = T::zero(), psy1 = T::zero(),
-      psz0 = T::zero(), psz1 = T::zero();
-    if (sx >= 0) { psx0 = T::ld(io.po, c.op, sx0); psx1 = T::ld(io.po, c.op, sx1); }
-    if (c.sy >= 0) { psy0 = T::ld(io.po, c.oy, sy0); psy1 = T::ld(io.po, c.oy, sy1); }
-    if (c.sz >= 0) { psz0 = T::ld(io.po, c.oz, sz0); psz1 = T::ld(io.po, c.oz, sz1); }
+    const PsiOffH po = psi_off_h<F>(g, ps, i);
+    const int sx = po.sx;
+    const uint32_t sx0 = po.x0, sx1 = po.x1, sy0 = po.y0, sy1 = po.y1, sz0 = po.z0, sz1 = po.z1;
+    F psx0 = v.psx0, psx1 = v.psx1, psy0 = v.psy0, psy1 = v.psy1, psz0 = v.psz0, psz1 = v.psz1;
 
     const float mi = xr[AX_MNX], mii = xr[AX_MNXI], idx = xr[AX_IDLP];
     const F dzy = (T::mmul(v.ez_j, c.mj, c.mji) - c.ez) * c.idy;
@@ -823,7 +843,10 @@ __device__ __forceinline__ void snap_flush(const SnapPlan<F> &p, const F *stage,
     for (int c = 0; c < nc; ++c) p.row[c * np + q] = stage[c * 64 + lane];
 }
 
-template <typename F>
+// MK: how the march reads the material coefficients (a kernel of its own each, so the wait
+// analysis of one form is not merged with the loads of another): 0 the class LUT staged in
+// LDS, 1 the class LUT in global memory (more classes than LDS holds), 2 dense per-cell arrays.
+template <typename F, int MK>
 __global__ void __launch_bounds__(64 * (OEM_CR + 1))
 k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ En,
                 F *__restrict__ Hn, PsiDev ps, PsiDev pn, GridDev g, MatDev mat, float ch,
@@ -860,8 +883,8 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
     const int j0 = __builtin_amdgcn_readfirstlane((tile / nkt) * OEM_CR);
     const int k0 = __builtin_amdgcn_readfirstlane((tile % nkt) * 64);
     const int i0 = chunk * g.chunk, i1 = min(i0 + g.chunk, nx);
-    const bool lds_lut = lut_in_lds(mat);
-    if (lds_lut) {
+    constexpr bool lds_lut = MK == 0;
+    if constexpr (lds_lut) {
         const int tid = threadIdx.y * 64 + threadIdx.x;
         const float4 *src = reinterpret_cast<const float4 *>(mat.lut);
         for (int q = tid; q < 2 * mat.n_classes; q += 64 * (OEM_CR + 1)) s_lut[q] = src[q];
@@ -928,8 +951,8 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
             const int in_ = next_of(i, nx, xn_last);
             F hx, hy, hz;
             HLoads<F> vr, vc;
-            hcolumn_load<F>(vr, cr, io, g, i, in_);
-            hcolumn_load<F>(vc, cc, io, g, i, in_);
+            hcolumn_load<F>(vr, cr, io, g, ps, i, in_);
+            hcolumn_load<F>(vc, cc, io, g, ps, i, in_);
             const ctab_t xr = ctab_row(X, i);
             hcolumn_do<F>(cr, vr, io, g, ps, i, xr, ch, false, hx, hy, hz);
             s_row[buf][0][0][lane] = hx;
@@ -989,12 +1012,13 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
         HColumn<F> cp = ca;
         hcolumn_load_e<F>(cp, io, g, xp0);
         HLoads<F> vp;
-        hcolumn_load<F>(vp, cp, io, g, xp0, next_of(xp0, nx, xn_last));
+        hcolumn_load<F>(vp, cp, io, g, ps, xp0, next_of(xp0, nx, xn_last));
         F hx_;
         hcolumn_do<F>(cp, vp, io, g, ps, xp0, ctab_row(X, xp0), ch, false, hx_, hy_m,
                       hz_m);
     }
     hcolumn_load_e<F>(ca, io, g, i0);
+    uint32_t cls_c = MK != 2 ? bld_u16(rC, oc, (uint32_t)i0 * plane2) : 0u;
     if (fplan) fcnt = snap_rank(own && fp.qjk >= 0, w, lane, fp.qjk, s_fcnt, s_fq, fslot);
 
 #pragma unroll 1
@@ -1011,25 +1035,17 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
         const uint32_t sz0 = (ps.off[4] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
         const uint32_t sz1 = (ps.off[5] + (uint32_t)i * (uint32_t)(ny * wz)) * T::B;
         const bool hin = hrow && i >= mat.hb[0] && i < mat.hb[1];
-        // the E-side inputs first, so their loads overlap the H arithmetic
-        const uint32_t cls = mat.mode == 0 ? bld_u16(rC, oc, (uint32_t)i * plane2) : 0u;
-        F pe0 = T::zero(), pe1 = T::zero(), pe2 = T::zero(), pe3 = T::zero(), pe4 = T::zero(),
-          pe5 = T::zero();
-        if (sx >= 0) { pe0 = T::ld(io.po, ca.op, sx0 + io.pef); pe1 = T::ld(io.po, ca.op, sx1 + io.pef); }
-        if (ca.sy >= 0) { pe2 = T::ld(io.po, ca.oy, sy0 + io.pef); pe3 = T::ld(io.po, ca.oy, sy1 + io.pef); }
-        if (ca.sz >= 0) { pe4 = T::ld(io.po, ca.oz, sz0 + io.pef); pe5 = T::ld(io.po, ca.oz, sz1 + io.pef); }
-        HLoads<F> cv;
-        hcolumn_load<F>(cv, ca, io, g, i, in_);
-        // the update coefficients of the cell's class; the ADE history is read only where the
-        // class flags a dispersive component (the box around the dispersive cells is mostly
-        // other material)
+        // the update coefficients of the cell's class (its index loaded a plane ahead); the ADE
+        // history is read only where the class flags a dispersive component (the box around
+        // the dispersive cells is mostly other material)
+        const uint32_t cls = cls_c;
         float cax, cbx, cay, cby, caz, cbz;
         uint32_t fl;
-        if (lds_lut) {
+        if constexpr (MK == 0) {
             const float4 r0 = s_lut[2 * cls], r1 = s_lut[2 * cls + 1];
             cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
             fl = __float_as_uint(r1.z);
-        } else if (mat.mode == 0) {
+        } else if constexpr (MK == 1) {
             const float4 *rw = reinterpret_cast<const float4 *>(mat.lut + 8u * cls);
             const float4 r0 = rw[0], r1 = rw[1];
             cax = r0.x; cbx = r0.y; cay = r0.z; cby = r0.w; caz = r1.x; cbz = r1.y;
@@ -1047,6 +1063,18 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
             if (fl & 2u) phy = T::ld(rHist, oh, sh + hcomp * T::B);
             if (fl & 4u) phz = T::ld(rHist, oh, sh + 2u * hcomp * T::B);
         }
+        F pe0 = T::zero(), pe1 = T::zero(), pe2 = T::zero(), pe3 = T::zero(), pe4 = T::zero(),
+          pe5 = T::zero();
+        if (sx >= 0) { pe0 = T::ld(io.po, ca.op, sx0 + io.pef); pe1 = T::ld(io.po, ca.op, sx1 + io.pef); }
+        if (ca.sy >= 0) { pe2 = T::ld(io.po, ca.oy, sy0 + io.pef); pe3 = T::ld(io.po, ca.oy, sy1 + io.pef); }
+        if (ca.sz >= 0) { pe4 = T::ld(io.po, ca.oz, sz0 + io.pef); pe5 = T::ld(io.po, ca.oz, sz1 + io.pef); }
+        HLoads<F> cv;
+        hcolumn_load<F>(cv, ca, io, g, ps, i, in_);
+        if (MK != 2 && i + 1 < i1) cls_c = bld_u16(rC, oc, (uint32_t)(i + 1) * plane2);
+        // every input of the plane has arrived before its first store: the E half after the
+        // barrier then waits on no vector load (loads and stores share one in-order counter,
+        // so such a wait would also wait for this plane's H stores)
+        __builtin_amdgcn_s_waitcnt(OEM_WAIT_VM0);
         const F ey0 = ca.ey, ez0 = ca.ez;
         F hx, hy, hz;
         const ctab_t xr = ctab_row(X, i);

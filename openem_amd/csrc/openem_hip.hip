@@ -945,10 +945,20 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
                 TRY(mark(points_kid(pt), false));
             }
         TRY(mark(OEM_K_UPDATE_HE, true));
-        hipLaunchKernelGGL((k_update_he<F>), dim3(c->fg.nblocks), dim3(64, OEM_CR + 1), 0,
-                           c->stream, (const F *)Eo, (const F *)Ho, E, H, po, pn, c->fg, c->mat,
-                           c->dims.ch, c->dims.cw, c->pw, c->d_step,
-                           c->mons_off ? FuseSnap{} : c->fsnap);
+        const FuseSnap fsn = c->mons_off ? FuseSnap{} : c->fsnap;
+        const dim3 hg(c->fg.nblocks), hb(64, OEM_CR + 1);
+        if (c->mat.mode != 0)
+            hipLaunchKernelGGL((k_update_he<F, 2>), hg, hb, 0, c->stream, (const F *)Eo,
+                               (const F *)Ho, E, H, po, pn, c->fg, c->mat, c->dims.ch,
+                               c->dims.cw, c->pw, c->d_step, fsn);
+        else if (c->mat.n_classes <= OEM_LDS_CLASSES)
+            hipLaunchKernelGGL((k_update_he<F, 0>), hg, hb, 0, c->stream, (const F *)Eo,
+                               (const F *)Ho, E, H, po, pn, c->fg, c->mat, c->dims.ch,
+                               c->dims.cw, c->pw, c->d_step, fsn);
+        else
+            hipLaunchKernelGGL((k_update_he<F, 1>), hg, hb, 0, c->stream, (const F *)Eo,
+                               (const F *)Ho, E, H, po, pn, c->fg, c->mat, c->dims.ch,
+                               c->dims.cw, c->pw, c->d_step, fsn);
         TRY(mark(OEM_K_UPDATE_HE, false));
     } else {
     TRY(mark(OEM_K_UPDATE_H, true));
