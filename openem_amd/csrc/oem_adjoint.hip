@@ -205,18 +205,54 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         const AxAt xc = ax_at_row(X, nx, i, AX_AE, AX_KE, AX_IDLD, AX_MPV);
         const uint32_t sx = (uint32_t)max(xc.s, 0) * plane * 4u;
         const uint32_t cls = cls_at(vo, sp);
-        float bx, by, bz;
-        cbs(cls, id, bx, by, bz);
         const float l0 = FT<float>::ld(rE, vo, sp), l1 = FT<float>::ld(rE, vo, sp + cB),
                     l2 = FT<float>::ld(rE, vo, sp + 2u * cB);
-        // nu_c = cb_c pec_c lE_c
-        const float nu0 = bx * (yc.pec * zc.pec) * l0;
-        const float nu1 = by * (zc.pec * xc.pec) * l1;
-        const float nu2 = bz * (xc.pec * yc.pec) * l2;
         float py0 = 0.f, py1 = 0.f, pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f;
         if (yc.s >= 0) { py0 = FT<float>::ld(rP, vpy, off[2] * 4u + sy); py1 = FT<float>::ld(rP, vpy, off[3] * 4u + sy); }
         if (zc.s >= 0) { pz0 = FT<float>::ld(rP, vpz, off[4] * 4u + sz); pz1 = FT<float>::ld(rP, vpz, off[5] * 4u + sz); }
         if (xc.s >= 0) { px0 = FT<float>::ld(rP, vo, off[0] * 4u + sx); px1 = FT<float>::ld(rP, vo, off[1] * 4u + sx); }
+        // every global read of the plane goes out before the barrier (the cell's lambda_H, and
+        // the neighbours' terms recomputed from their cells where the neighbour's thread is not
+        // in the block), so nothing after it waits on a load: a wait there would also wait for
+        // the stores in flight (loads and stores share one in-order counter)
+        float hl0 = 0.f, hl1 = 0.f, hl2 = 0.f;
+        if (act) {
+            hl0 = FT<float>::ld(rH, vo, sp);
+            hl1 = FT<float>::ld(rH, vo, sp + cB);
+            hl2 = FT<float>::ld(rH, vo, sp + 2u * cB);
+        }
+        // neighbours' terms: y (t0, t5) at Y.nxt[j], z (t1, t2) at Z.nxt[k], x carried; from
+        // the neighbour's thread when it is in the block (after the barrier), else recomputed
+        // here with the same expressions
+        float q0n = 0.f, q5n = 0.f, q1n = 0.f, q2n = 0.f;
+        if (!y_lds && y_nb) {
+            float b0, b1, b2;
+            cbs(cls_at(vyn, sp), (uint32_t)i * plane + (uint32_t)(jn * nz + k), b0, b1, b2);
+            const float n0 = b0 * (yn.pec * zc.pec) * FT<float>::ld(rE, vyn, sp);
+            const float n2 = b2 * (xc.pec * yn.pec) * FT<float>::ld(rE, vyn, sp + 2u * cB);
+            float a0 = 0.f, a1 = 0.f;
+            if (yn.s >= 0) { a0 = FT<float>::ld(rP, vpyn, off[2] * 4u + sy); a1 = FT<float>::ld(rP, vpyn, off[3] * 4u + sy); }
+            q0n = q_of(1.f * n0, yn, 1.f, n0, a0);
+            q5n = q_of(-1.f * n2, yn, -1.f, n2, a1);
+            (void)b1;
+        }
+        if (!z_lds && z_nb) {
+            float b0, b1, b2;
+            cbs(cls_at(vzn, sp), (uint32_t)i * plane + (uint32_t)(j * nz + kn), b0, b1, b2);
+            const float n0 = b0 * (yc.pec * zn.pec) * FT<float>::ld(rE, vzn, sp);
+            const float n1 = b1 * (zn.pec * xc.pec) * FT<float>::ld(rE, vzn, sp + cB);
+            float a0 = 0.f, a1 = 0.f;
+            if (zn.s >= 0) { a0 = FT<float>::ld(rP, vpzn, off[4] * 4u + sz); a1 = FT<float>::ld(rP, vpzn, off[5] * 4u + sz); }
+            q1n = q_of(-1.f * n0, zn, -1.f, n0, a0);
+            q2n = q_of(1.f * n1, zn, 1.f, n1, a1);
+            (void)b2;
+        }
+        float bx, by, bz;
+        cbs(cls, id, bx, by, bz);
+        // nu_c = cb_c pec_c lE_c
+        const float nu0 = bx * (yc.pec * zc.pec) * l0;
+        const float nu1 = by * (zc.pec * xc.pec) * l1;
+        const float nu2 = bz * (xc.pec * yc.pec) * l2;
         const float q0 = q_of(1.f * nu0, yc, 1.f, nu0, py0);      // t0: c0, y, slot 2
         const float q5 = q_of(-1.f * nu2, yc, -1.f, nu2, py1);    // t5: c2, y, slot 3
         const float q1 = q_of(-1.f * nu0, zc, -1.f, nu0, pz0);    // t1: c0, z, slot 4
@@ -228,37 +264,13 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         s_q[buf][2][tid] = q1;
         s_q[buf][3][tid] = q2;
         __syncthreads();
-        // neighbours' terms: y (t0, t5) at Y.nxt[j], z (t1, t2) at Z.nxt[k], x carried; from
-        // the neighbour's thread when it is in the block, else recomputed with the same
-        // expressions
-        float q0n = 0.f, q5n = 0.f, q1n = 0.f, q2n = 0.f;
         if (y_lds) {
             q0n = s_q[buf][0][ly];
             q5n = s_q[buf][1][ly];
-        } else if (y_nb) {
-            float b0, b1, b2;
-            cbs(cls_at(vyn, sp), (uint32_t)i * plane + (uint32_t)(jn * nz + k), b0, b1, b2);
-            const float n0 = b0 * (yn.pec * zc.pec) * FT<float>::ld(rE, vyn, sp);
-            const float n2 = b2 * (xc.pec * yn.pec) * FT<float>::ld(rE, vyn, sp + 2u * cB);
-            float a0 = 0.f, a1 = 0.f;
-            if (yn.s >= 0) { a0 = FT<float>::ld(rP, vpyn, off[2] * 4u + sy); a1 = FT<float>::ld(rP, vpyn, off[3] * 4u + sy); }
-            q0n = q_of(1.f * n0, yn, 1.f, n0, a0);
-            q5n = q_of(-1.f * n2, yn, -1.f, n2, a1);
-            (void)b1;
         }
         if (z_lds) {
             q1n = s_q[buf][2][lz];
             q2n = s_q[buf][3][lz];
-        } else if (z_nb) {
-            float b0, b1, b2;
-            cbs(cls_at(vzn, sp), (uint32_t)i * plane + (uint32_t)(j * nz + kn), b0, b1, b2);
-            const float n0 = b0 * (yc.pec * zn.pec) * FT<float>::ld(rE, vzn, sp);
-            const float n1 = b1 * (zn.pec * xc.pec) * FT<float>::ld(rE, vzn, sp + cB);
-            float a0 = 0.f, a1 = 0.f;
-            if (zn.s >= 0) { a0 = FT<float>::ld(rP, vpzn, off[4] * 4u + sz); a1 = FT<float>::ld(rP, vpzn, off[5] * 4u + sz); }
-            q1n = q_of(-1.f * n0, zn, -1.f, n0, a0);
-            q2n = q_of(1.f * n1, zn, 1.f, n1, a1);
-            (void)b2;
         }
         const bool x_lo = cidx_at(X.prv, i) == i;
         const bool x_nb = i + 1 < i1 || x_nb_top;
@@ -283,9 +295,9 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_e1(AdjDev d)
         if (!act) continue;
         float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
         acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
-        FT<float>::st(rH, FT<float>::ld(rH, vo, sp) + acc0, vo, sp);
-        FT<float>::st(rH, FT<float>::ld(rH, vo, sp + cB) + acc1, vo, sp + cB);
-        FT<float>::st(rH, FT<float>::ld(rH, vo, sp + 2u * cB) + acc2, vo, sp + 2u * cB);
+        FT<float>::st(rH, hl0 + acc0, vo, sp);
+        FT<float>::st(rH, hl1 + acc1, vo, sp + cB);
+        FT<float>::st(rH, hl2 + acc2, vo, sp + 2u * cB);
         // the cell-local part of the psi adjoints, into the other buffer (neighbours read the
         // old values): lpe <- b (sgn nu + lpe) per slab term.  The cell-local part of lambda_E
         // (ca pec lambda_E) is left to the H transpose, which rewrites lambda_E in place
@@ -419,6 +431,30 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         auto qh = [](const AxAt &t, float f, float psi) {
             return t.s >= 0 ? (t.ik + t.a) * f + t.a * psi : f;
         };
+        // every global read of the plane goes out before the barrier (the cell's class and
+        // lambda_E for the in-place update, and the neighbours' terms recomputed from their
+        // cells where the neighbour's thread is not in the block), so nothing after it waits
+        // on a load
+        float ax_ = 0.f, ay_ = 0.f, az_ = 0.f, el0 = 0.f, el1 = 0.f, el2 = 0.f;
+        if (act) {
+            cas(vo, sp, (uint32_t)i * plane + (uint32_t)(j * nz + k), ax_, ay_, az_);
+            el0 = FT<float>::ld(rE, vo, sp);
+            el1 = FT<float>::ld(rE, vo, sp + cB);
+            el2 = FT<float>::ld(rE, vo, sp + 2u * cB);
+        }
+        float q0n = 0.f, q5n = 0.f, q1n = 0.f, q2n = 0.f;
+        if (!y_lds && y_nb) {
+            float a0 = 0.f, a1 = 0.f;
+            if (yp.s >= 0) { a0 = FT<float>::ld(rP, vpyp, off[2] * 4u + sy); a1 = FT<float>::ld(rP, vpyp, off[3] * 4u + sy); }
+            q0n = qh(yp, mch * 1.f * FT<float>::ld(rH, vyp, sp), a0);
+            q5n = qh(yp, mch * -1.f * FT<float>::ld(rH, vyp, sp + 2u * cB), a1);
+        }
+        if (!z_lds && z_nb) {
+            float a0 = 0.f, a1 = 0.f;
+            if (zp.s >= 0) { a0 = FT<float>::ld(rP, vpzp, off[4] * 4u + sz); a1 = FT<float>::ld(rP, vpzp, off[5] * 4u + sz); }
+            q1n = qh(zp, mch * -1.f * FT<float>::ld(rH, vzp, sp), a0);
+            q2n = qh(zp, mch * 1.f * FT<float>::ld(rH, vzp, sp + cB), a1);
+        }
         const float q0 = qh(yc, f0p, py0);      // t0: c0 sgn +1, y, slot 2
         const float q5 = qh(yc, f2m, py1);      // t5: c2 sgn -1, y, slot 3
         const float q1 = qh(zc, f0m, pz0);      // t1: c0 sgn -1, z, slot 4
@@ -430,24 +466,13 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         s_q[buf][2][tid] = q1;
         s_q[buf][3][tid] = q2;
         __syncthreads();
-        float q0n = 0.f, q5n = 0.f, q1n = 0.f, q2n = 0.f;
         if (y_lds) {
             q0n = s_q[buf][0][ly];
             q5n = s_q[buf][1][ly];
-        } else if (y_nb) {
-            float a0 = 0.f, a1 = 0.f;
-            if (yp.s >= 0) { a0 = FT<float>::ld(rP, vpyp, off[2] * 4u + sy); a1 = FT<float>::ld(rP, vpyp, off[3] * 4u + sy); }
-            q0n = qh(yp, mch * 1.f * FT<float>::ld(rH, vyp, sp), a0);
-            q5n = qh(yp, mch * -1.f * FT<float>::ld(rH, vyp, sp + 2u * cB), a1);
         }
         if (z_lds) {
             q1n = s_q[buf][2][lz];
             q2n = s_q[buf][3][lz];
-        } else if (z_nb) {
-            float a0 = 0.f, a1 = 0.f;
-            if (zp.s >= 0) { a0 = FT<float>::ld(rP, vpzp, off[4] * 4u + sz); a1 = FT<float>::ld(rP, vpzp, off[5] * 4u + sz); }
-            q1n = qh(zp, mch * -1.f * FT<float>::ld(rH, vzp, sp), a0);
-            q2n = qh(zp, mch * 1.f * FT<float>::ld(rH, vzp, sp + cB), a1);
         }
         const bool x_hi = cidx_at(X.nxt, i) == i;
         const bool x_nb = i > i0 || x_nb_bot;
@@ -473,11 +498,9 @@ __global__ void __launch_bounds__(OEM_ADJ_TK * OEM_ADJ_TJ) k_adj_h1(AdjDev d)
         acc2 += v0; acc1 += v1; acc0 += v2; acc2 += v3; acc1 += v4; acc0 += v5;
         // lambda_E <- ca pec lambda_E (the E update's cell-local part, transposed) + the sum,
         // in place: no thread reads another cell's lambda_E here
-        float ax_, ay_, az_;
-        cas(vo, sp, (uint32_t)i * plane + (uint32_t)(j * nz + k), ax_, ay_, az_);
-        const float e0 = ax_ * (yc.pec * zc.pec) * FT<float>::ld(rE, vo, sp);
-        const float e1 = ay_ * (zc.pec * xc.pec) * FT<float>::ld(rE, vo, sp + cB);
-        const float e2 = az_ * (xc.pec * yc.pec) * FT<float>::ld(rE, vo, sp + 2u * cB);
+        const float e0 = ax_ * (yc.pec * zc.pec) * el0;
+        const float e1 = ay_ * (zc.pec * xc.pec) * el1;
+        const float e2 = az_ * (xc.pec * yc.pec) * el2;
         FT<float>::st(rE, e0 + acc0, vo, sp);
         FT<float>::st(rE, e1 + acc1, vo, sp + cB);
         FT<float>::st(rE, e2 + acc2, vo, sp + 2u * cB);
