@@ -299,10 +299,10 @@ def _mode_power(sc) -> float:
 
 def _serving_estimate_s(specs, best_rate: float) -> float:
     """Wall seconds the serving leg (batched, then serial) is expected to take: the stream's
-    cell-steps at 0.63 and 0.53 of the single-scene rate, host work included (the whole stream
-    measured 0.64-0.66 and 0.54-0.55)."""
+    cell-steps at 0.60 and 0.51 of the single-scene rate, host work included (the whole stream
+    measured 0.60-0.66 and 0.51-0.55)."""
     cs = sum(sp["side"] ** 3 * sp["n_steps"] for sp in specs)
-    return cs / (best_rate * 1e6) * (1 / 0.63 + 1 / 0.53)
+    return cs / (best_rate * 1e6) * (1 / 0.60 + 1 / 0.51)
 
 
 def main() -> int:
@@ -426,7 +426,7 @@ def main() -> int:
         specs = scenes.serving_requests(256, rate=20.0, seed=0)[:args.serve_requests]
         # keep room for the smallest gradient leg (4 designs and the central differences)
         left = (args.budget_s - (time.perf_counter() - t_start)
-                - (4 * 2.5 + 2.0 * args.adjoint_check + 10 if args.adjoint_designs > 0 else 0))
+                - (4 * 2.2 + 2.0 * args.adjoint_check + 4 if args.adjoint_designs > 0 else 0))
         n_req = len(specs)
         while n_req > 16 and _serving_estimate_s(specs[:n_req], value) > left:
             n_req -= 16
@@ -441,8 +441,9 @@ def main() -> int:
     if args.adjoint_designs > 0:
         left = args.budget_s - (time.perf_counter() - t_start)
         designs = args.adjoint_designs
-        per, fd = 2.5, 2.0 * args.adjoint_check        # seconds per design / per pixel pair
-        while designs > 4 and designs * per + fd + 10 > left:
+        # seconds per design with two in flight (2.18-2.23 measured) / per pixel's pair of runs
+        per, fd = 2.2, 2.0 * args.adjoint_check
+        while designs > 4 and designs * per + fd + 4 > left:
             designs -= 4
         if designs < args.adjoint_designs:
             print(f"[bench] gradients shortened to {designs} designs by the "
