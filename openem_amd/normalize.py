@@ -9,10 +9,14 @@ source:
   ``|D|^2 n A / (2 eta0) cos(k dz_h / 2)`` (unit power through the source cross-section, in
   the discrete Poynting flux a Yee grid conserves);
 * mode source: ``D`` of ``amp_e / 2`` (unit-power modes, no unit conversion);
-* point dipole: ``D 1e6`` of the whole-step amplitude table.
-
-The TF/SF branch of the reference divides by constants fitted against tidy3d runs (sqrt(2 eta0)
-and a 3.5-step phase); with no such run available here it is not built and fails closed.
+* point dipole: ``D 1e6`` of the whole-step amplitude table;
+* TF/SF box (takes priority, as in the reference): ``D / sqrt(2 eta0)`` with ``D`` the spectrum
+  of the incident E on the box's injection face.  tidy3d normalises a TF/SF source to unit
+  intensity, 1 W/um^2, whose plane wave has ``|E| = sqrt(2 eta0)`` V/um; the reference divides
+  by the same ``sqrt(2 eta0)`` (fitted there against tidy3d runs) and by a 3.5-step phase that
+  aligns its injection table with its monitor DFT (``openem/normalize.py:123-135``).  Here ``D``
+  is taken from the line that drives the box, on the injection face itself, so no time shift
+  is needed: the normalised incident field on that face is ``sqrt(2 eta0)`` at phase 0.
 """
 
 from __future__ import annotations
@@ -20,11 +24,14 @@ from __future__ import annotations
 import numpy as np
 
 from . import flux as flux_mod
+from . import tfsf as tfsf_mod
 from .grid import transverse_axes
 
 UNIT_E = 1.0e6                  # V/m -> V/um
 ETA0 = 376.730313668
 C_0 = 299792458.0
+#: |E| in V/um of a plane wave of unit intensity (1 W/um^2)
+SQRT_2ETA0 = float(np.sqrt(2.0 * ETA0))
 
 
 def background_index(sc) -> float:
@@ -40,11 +47,15 @@ def background_index(sc) -> float:
 
 def source_table(sc) -> np.ndarray:
     """Injection table of the normalisation source (reference ``openem/normalize.py:92-108``):
-    the first plane wave's incident E, else the first mode source's ``amp_e / 2``, else the
-    first dipole's whole-step amplitudes."""
+    the first TF/SF box's incident E on its injection face along the driving current's
+    direction, else the first plane wave's incident E, else the first mode source's
+    ``amp_e / 2``, else the first dipole's whole-step amplitudes."""
     if sc.tfsf_sources:
-        raise NotImplementedError("TF/SF normalisation constants are fitted to tidy3d runs; "
-                                  "not available")
+        src = sc.tfsf_sources[0]
+        e = tfsf_mod.incident_at_face(sc, src, int(sc.num_time_steps))
+        p = np.asarray(src.jxy, dtype=np.float64)
+        p = p / np.linalg.norm(p)
+        return e[:, 0] * p[0] + e[:, 1] * p[1]
     if sc.sources:
         return np.asarray(sc.sources[0].ex_inc, dtype=np.float64)
     if sc.mode_sources:
@@ -92,8 +103,26 @@ def _source_area(sc) -> float:
     return float(sc.grid.axes[t1].dl.sum() * sc.grid.axes[t2].dl.sum())
 
 
+def tfsf_spectrum(sc, freqs) -> np.ndarray:
+    """Spectrum of the TF/SF box's incident plane wave on its injection face: the plain DFT
+    (t = n dt) of the incident E along the driving current, with the magnitude of the whole
+    vector (an oblique wave also has E normal to the face)."""
+    f = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+    src = sc.tfsf_sources[0]
+    e = tfsf_mod.incident_at_face(sc, src, int(sc.num_time_steps))
+    t = np.arange(e.shape[0], dtype=np.float64) * sc.dt
+    D = np.exp(2j * np.pi * f[:, None] * t[None, :]) @ e            # (nf, 3)
+    p = np.asarray(src.jxy, dtype=np.float64)
+    p = p / np.linalg.norm(p)
+    dt_ = D[:, 0] * p[0] + D[:, 1] * p[1]
+    mag = np.sqrt(np.sum(np.abs(D) ** 2, axis=1))
+    return mag * np.exp(1j * np.angle(dt_))
+
+
 def field_norm(sc, freqs) -> np.ndarray:
     """Divisor of field phasors per frequency (reference ``openem/normalize.py:127-194``)."""
+    if sc.tfsf_sources:
+        return tfsf_spectrum(sc, freqs) / SQRT_2ETA0
     D = norm_spectrum(sc, freqs)
     if sc.sources:
         return (D * UNIT_E * np.sqrt(background_index(sc) * _source_area(sc) / (2.0 * ETA0))
@@ -132,7 +161,7 @@ def normalized_flux(sc, result, name: str) -> np.ndarray:
     ``normal_dir``, per unit source power (reference ``openem/nb/backend.py:963-982``)."""
     m = next(mm for mm in sc.flux_monitors if mm.name == name)
     vals = flux_mod.monitor_flux(result.phasors[name].data, sc.grid, m) * float(m.normal_dir)
-    if sc.sources:
+    if sc.sources and not sc.tfsf_sources:
         return vals / plane_flux_divisor(sc, m, m.freqs)
     return vals / flux_norm(sc, m.freqs)
 

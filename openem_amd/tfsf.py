@@ -208,6 +208,38 @@ def build_points(grid, src, shape):
     return h_pts, e_pts, cols
 
 
+def _line_setup(sc, src):
+    """The line's background column, source edge and transverse wave vector of ``src``."""
+    i0 = src.box_lo[0] if 0 not in src.open_axes else 0
+    j0 = src.box_lo[1] if 1 not in src.open_axes else 0
+    eps_edge = np.asarray(sc.eps_ex[i0, j0, :], dtype=np.float64)
+    eps_ctr = np.asarray(sc.eps_ez[i0, j0, :], dtype=np.float64)
+    if src.eps_line is not None:
+        eps_edge, eps_ctr = src.eps_line
+    k_src = src.box_lo[2] - 3 if src.direction > 0 else src.box_hi[2] + 3
+    return eps_edge, eps_ctr, k_src
+
+
+def incident_at_face(sc, src, n_steps: int) -> np.ndarray:
+    """``(n_steps + 1, 3)`` complex: the incident E (x, y, z) at t = n dt on the box's injection
+    face (the lower z face for +z travel, the upper for -z), from the same line that drives the
+    box, at the transverse origin.  Ex, Ey sit on the face's edge plane; Ez, which the line
+    keeps at cell centres, is taken at the centre just inside the box (for a plane wave only
+    its magnitude enters the normalisation)."""
+    grid = sc.grid
+    eps_edge, eps_ctr, k_src = _line_setup(sc, src)
+    zf = src.box_lo[2] if src.direction > 0 else src.box_hi[2]
+    zc = zf if src.direction > 0 else zf - 1
+    cols = [(0, zf), (1, zf), (2, zc)]
+    amp = src.waveform.amp_half
+    if amp.size < n_steps + 1:
+        raise ValueError("TF/SF waveform table shorter than the run")
+    tab_h, _ = run_line(grid.z.dl, eps_edge, eps_ctr, sc.dt, src.kxy[0], src.kxy[1],
+                        float(grid.x.dl[0]), float(grid.y.dl[0]), k_src, src.jxy,
+                        amp * src.amp_scale, n_steps, cols)
+    return tab_h
+
+
 def tables(sc, src, n_total: int) -> TFSFTables:
     """Build the point lists and run the line for one :class:`~openem_amd.model.TFSFSource`."""
     grid = sc.grid
@@ -236,13 +268,7 @@ def tables(sc, src, n_total: int) -> TFSFTables:
                 raise NotImplementedError("oblique TF/SF needs a uniform x/y grid")
     h_pts, e_pts, cols = build_points(grid, src, shape)
     # background column at the box corner (the line's medium)
-    i0 = src.box_lo[0] if 0 not in src.open_axes else 0
-    j0 = src.box_lo[1] if 1 not in src.open_axes else 0
-    eps_edge = np.asarray(sc.eps_ex[i0, j0, :], dtype=np.float64)
-    eps_ctr = np.asarray(sc.eps_ez[i0, j0, :], dtype=np.float64)
-    if src.eps_line is not None:
-        eps_edge, eps_ctr = src.eps_line
-    k_src = src.box_lo[2] - 3 if src.direction > 0 else src.box_hi[2] + 3
+    eps_edge, eps_ctr, k_src = _line_setup(sc, src)
     amp = src.waveform.amp_half
     if amp.size < n_total + 1:
         raise ValueError("TF/SF waveform table shorter than the run")
