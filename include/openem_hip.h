@@ -201,7 +201,7 @@ int oem_read_flux_time_monitor(oem_ctx *ctx, int32_t mon_id, double *out, size_t
  * `openem/knobs.py:64`); set by the launch autotuner (openem_amd/autotune.py). */
 int oem_set_chunk(oem_ctx *ctx, int32_t chunk);
 
-/* Step form: 0 = the library's choice (the fused H+E march from 12 M cells, else the two
+/* Step form: 0 = the library's choice (the fused H+E march from 10 M cells, else the two
  * sweeps; OEM_FUSE=0/1 forces one), 1 = the two sweeps, 2 = the fused march (OEM_ENOTSUP when a
  * field set exceeds its 32-bit offsets or the context is pooled).  Before oem_prepare.  No
  * counterpart in the reference (its H+E fusion is a global knob, `openem/knobs.py:61`); set by

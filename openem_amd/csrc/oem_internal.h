@@ -54,7 +54,7 @@ struct FluxTimeMonitor {
 enum { PT_DIPOLE = 0, PT_TFSF = 1, PT_MODE = 2 };
 
 // Smallest scene (cells) that takes the fused H+E step by default.
-#define OEM_FUSE_MIN_CELLS (12u << 20)
+#define OEM_FUSE_MIN_CELLS (10u << 20)
 
 struct Points {
     PointsDev d;

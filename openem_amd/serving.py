@@ -12,7 +12,7 @@ with the same launches (batched kernels over every resident scene in a hipGraph 
   of blocks in flight and hands finished scenes to the readout, which reads the phasors,
   computes the fluxes and releases the scene's memory;
 * scenes of at least ``solo_cells`` cells nearly fill the GPU by themselves: they run as
-  standalone contexts (hipGraph blocks; the fused H+E step from 12 M cells) in ``solo_lanes``
+  standalone contexts (hipGraph blocks; the fused H+E step from 10 M cells) in ``solo_lanes``
   lanes beside the batch, each lane one scene at a time on its own stream, so one large scene's
   setup, readout and release overlap another's stepping, and two large scenes together fill the
   GPU better than one.  A free lane takes the waiting large scene with the most cell-steps.

@@ -3,7 +3,7 @@
 The fused step ping-pongs between two field sets (after s steps the state is in set s % 2), so
 everything that reads the state (time monitors, shutoff reductions, returned fields) or restores
 it (adjoint checkpoints) must follow the parity.  Each test forces the fused step on a small
-scene (OEM_FUSE=1; by default it starts at 12 M cells) and compares with the two sweeps
+scene (OEM_FUSE=1; by default it starts at 10 M cells) and compares with the two sweeps
 (OEM_FUSE=0): bit for bit where no point list corrects H, to float32 rounding where one does
 (the waveguide's mode source).
 """

@@ -242,7 +242,7 @@ FUSED_CHUNKS = {"default": None, "chunk3": "3"}
 @pytest.mark.parametrize("chunk", sorted(FUSED_CHUNKS))
 @pytest.mark.parametrize("name", sorted(FUSED_CASES) + sorted(FUSED_COMPLEX))
 def test_fused_step_equals_the_two_sweeps(gpu_lib, monkeypatch, name, chunk):
-    """The fused H+E step (OEM_FUSE=1, the default from 12 M cells; ping-pong buffers, halo
+    """The fused H+E step (OEM_FUSE=1, the default from 10 M cells; ping-pong buffers, halo
     recompute) gives the fields and phasors of the separate H and E sweeps (OEM_FUSE=0) bit for
     bit: every cell is updated from the same inputs by the same expressions (no multiply-add
     contraction)."""
