@@ -23,7 +23,9 @@
 // the transposed sweeps: a (j, k) column per thread from a block of 256, marching `chunk` x
 // planes per block (grid z; OEM_ADJ_CHUNK by default, the environment's OEM_ADJ_CHUNK if set)
 #define OEM_ADJ_TK 64
+#ifndef OEM_ADJ_TJ
 #define OEM_ADJ_TJ 4
+#endif
 // 8 planes measured best on the 400 x 400 x 48 bend (s per design: 4 3.04-3.06, 6 3.01, 8
 // 3.02-3.09, 12 3.08, 16 3.11-3.18, 24 3.10, 32 3.17)
 #define OEM_ADJ_CHUNK 8
