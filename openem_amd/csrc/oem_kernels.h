@@ -57,6 +57,15 @@ __device__ __forceinline__ uint32_t bld_u16(rsrc_t r, uint32_t voff, uint32_t so
 // __builtin_bit_cast.
 typedef unsigned int oem_u32x2 __attribute__((vector_size(8)));
 
+// Cache policy of every buffer store: non-temporal (cache-policy bit 1).  Nothing a kernel
+// stores is read again in the same launch, so the stored lines need not displace, in L2, the
+// lines that neighbouring work-groups read again (the fused march's halo and neighbour lines):
+// on the 512^3 step its reads fell from 2.29 to 2.17 M KiB per launch and the march from 1.729
+// to 1.703 ms; the Bloch step from 1.105 to 1.053 ms.
+#ifndef OEM_ST_AUX
+#define OEM_ST_AUX 2
+#endif
+
 template <typename F> struct FT;
 template <> struct FT<float> {
     static constexpr uint32_t B = 4;
@@ -64,7 +73,7 @@ template <> struct FT<float> {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
     }
     static __device__ __forceinline__ void st(rsrc_t r, float x, uint32_t v, uint32_t s) {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, x), r, v, s, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, x), r, v, s, OEM_ST_AUX);
     }
     static __device__ __forceinline__ float mmul(float x, float mr, float) { return x * mr; }
     static __device__ __forceinline__ float zero() { return 0.f; }
@@ -81,7 +90,7 @@ template <> struct FT<cf2> {
         return __builtin_bit_cast(cf2, q);
     }
     static __device__ __forceinline__ void st(rsrc_t r, cf2 x, uint32_t v, uint32_t s) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(oem_u32x2, x), r, v, s, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(oem_u32x2, x), r, v, s, OEM_ST_AUX);
     }
     static __device__ __forceinline__ cf2 mmul(cf2 x, float mr, float mi) {
         return {x.r * mr - x.i * mi, x.r * mi + x.i * mr};
