@@ -142,6 +142,7 @@ struct oem_ctx {
     GridDev g{};
     GridDev fg{};               // launch geometry of the fused H+E step
     FuseSnap fsnap{};           // the DFT monitors the fused step snaps
+    bool fsnap_multi = false;   // some work-group meets more than one of them
     bool axis_set[3] = {false, false, false};
     MatDev mat{};
     bool mat_set = false;

@@ -724,7 +724,7 @@ __device__ __forceinline__ void hcolumn_do(HColumn<F> &c, const HLoads<F> &v, co
 
 // DFT monitors whose snapshot the fused step writes as it stores the new fields, in the layout
 // of k_dft_snap (a flux plane: E_t1, E_t2 and the H averages over the plane and the one below;
-// a field box: E and H); no work-group meets two of them.  Cells that an E-side point list
+// a field box: E and H); no work-group meets more than three of them.  Cells that an E-side point list
 // corrects after the step are snapped again afterwards (k_dft_snap_list).
 #define OEM_FUSE_MONS 4
 struct FuseMon {
@@ -855,7 +855,9 @@ __device__ __forceinline__ void snap_flush(const SnapPlan<F> &p, const F *stage,
 // MK: how the march reads the material coefficients (a kernel of its own each, so the wait
 // analysis of one form is not merged with the loads of another): 0 the class LUT staged in
 // LDS, 1 the class LUT in global memory (more classes than LDS holds), 2 dense per-cell arrays.
-template <typename F, int MK>
+// MULTI: up to three fused monitors meet a work-group (else at most one, and the kernel holds
+// no registers for the others).
+template <typename F, int MK, bool MULTI>
 __global__ void __launch_bounds__(64 * (OEM_CR + 1))
 k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ En,
                 F *__restrict__ Hn, PsiDev ps, PsiDev pn, GridDev g, MatDev mat, float ch,
@@ -915,30 +917,45 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
     // reads no descriptor.  Up to 64 points go through an LDS stage that the halo wave stores
     // with full-width stores after the next barrier (a row wave holds only a few of them: 2
     // lanes of a 2-layer box, 1 of a z plane)
+    // A second and third monitor meeting the work-group (the host lets at most three meet any
+    // one) have their points stored directly by the row waves (fp2, fp3).
     int fcnt = -1, fslot = -1;
     bool fplan = false;
-    SnapPlan<F> fp{};
+    SnapPlan<F> fp{}, fp2{}, fp3{};
     fp.ilo = fp.ihi = 0;
+    fp2.ilo = fp2.ihi = 0;
+    fp3.ilo = fp3.ihi = 0;
     if (fs.n) {
-        const uint32_t fmask = __builtin_amdgcn_readfirstlane(
+        uint32_t fmask = __builtin_amdgcn_readfirstlane(
             fuse_snap_mask(fs, nx, ny, nz, i0, i1, j0, j0 + OEM_CR, k0, k0 + 64));
         if (fmask) {
             const int fstep = __builtin_amdgcn_readfirstlane(step_now(stp));
             const int jr = min(j0 + min(w, OEM_CR - 1), ny - 1);
-            fp = snap_plan<F>(fs.m[__builtin_ctz(fmask)], fstep, nx, ny, nz, jr, kl);
-            {
-                // all but qjk are the same across the work-group
-                const uint64_t rp = reinterpret_cast<uint64_t>(fp.row);
+            // all but qjk are the same across the work-group
+            auto uniform = [](SnapPlan<F> &p) {
+                const uint64_t rp = reinterpret_cast<uint64_t>(p.row);
                 const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)rp);
                 const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(rp >> 32));
-                fp.row = reinterpret_cast<F *>(((uint64_t)hi << 32) | lo);
-                fp.np = __builtin_amdgcn_readfirstlane(fp.np);
-                fp.sel = __builtin_amdgcn_readfirstlane(fp.sel);
-                fp.ilo = __builtin_amdgcn_readfirstlane(fp.ilo);
-                fp.ihi = __builtin_amdgcn_readfirstlane(fp.ihi);
-                fp.si = __builtin_amdgcn_readfirstlane(fp.si);
-            }
+                p.row = reinterpret_cast<F *>(((uint64_t)hi << 32) | lo);
+                p.np = __builtin_amdgcn_readfirstlane(p.np);
+                p.sel = __builtin_amdgcn_readfirstlane(p.sel);
+                p.ilo = __builtin_amdgcn_readfirstlane(p.ilo);
+                p.ihi = __builtin_amdgcn_readfirstlane(p.ihi);
+                p.si = __builtin_amdgcn_readfirstlane(p.si);
+            };
+            fp = snap_plan<F>(fs.m[__builtin_ctz(fmask)], fstep, nx, ny, nz, jr, kl);
+            uniform(fp);
             fplan = true;
+            fmask &= fmask - 1u;
+            if (MULTI && fmask) {
+                fp2 = snap_plan<F>(fs.m[__builtin_ctz(fmask)], fstep, nx, ny, nz, jr, kl);
+                uniform(fp2);
+                fmask &= fmask - 1u;
+                if (fmask) {
+                    fp3 = snap_plan<F>(fs.m[__builtin_ctz(fmask)], fstep, nx, ny, nz, jr, kl);
+                    uniform(fp3);
+                }
+            }
         }
     }
     if (w == OEM_CR) {
@@ -1134,6 +1151,14 @@ k_update_he(const F *__restrict__ Eo, const F *__restrict__ Ho, F *__restrict__ 
             if (i >= fp.ilo && i < fp.ihi && fp.qjk >= 0)
                 snap_planned<F>(fp, i, fcnt >= 0 ? s_fsn[buf] : nullptr, fslot, nex, ney, nez, hx,
                                 hy, hz, hy_m, hz_m, hx_j, hz_j, hx_k, hy_k);
+            if constexpr (MULTI) {
+                if (i >= fp2.ilo && i < fp2.ihi && fp2.qjk >= 0)
+                    snap_planned<F>(fp2, i, nullptr, 0, nex, ney, nez, hx, hy, hz, hy_m, hz_m,
+                                    hx_j, hz_j, hx_k, hy_k);
+                if (i >= fp3.ilo && i < fp3.ihi && fp3.qjk >= 0)
+                    snap_planned<F>(fp3, i, nullptr, 0, nex, ney, nez, hx, hy, hz, hy_m, hz_m,
+                                    hx_j, hz_j, hx_k, hy_k);
+            }
         }
         hy_m = hy;
         hz_m = hz;

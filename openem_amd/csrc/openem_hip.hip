@@ -675,9 +675,10 @@ extern "C" int oem_read_flux_time_monitor(oem_ctx *ctx, int32_t mon_id, double *
     return OEM_OK;
 }
 
-// The DFT monitors whose snapshots the fused step writes: up to OEM_FUSE_MONS, in order, whose
-// footprints in work-groups (x chunks x row tiles x k tiles) are disjoint, so that a work-group
-// meets at most one of them; the others keep k_dft_snap.  Each comes with the list of its
+// The DFT monitors whose snapshots the fused step writes: up to OEM_FUSE_MONS, in order, as long
+// as no work-group (footprints in x chunks x row tiles x k tiles) meets more than three of them
+// (real fields) or more than one (complex fields: the complex march's extra stores cost more
+// than the snapshot kernel they replace); the others keep k_dft_snap.  Each comes with the list of its
 // points at cells that an E-side point list corrects after the step (those are snapped again
 // after the corrections, so every snapshot holds the final fields).
 static int plan_fused_snaps(oem_ctx *ctx) {
@@ -702,15 +703,39 @@ static int plan_fused_snaps(oem_ctx *ctx) {
     std::sort(ecells.begin(), ecells.end());
     ecells.erase(std::unique(ecells.begin(), ecells.end()), ecells.end());
     ctx->fsnap.n = 0;
+    ctx->fsnap_multi = false;
     for (auto &mon : ctx->mons) {
         if (ctx->fsnap.n == OEM_FUSE_MONS) break;
         const MonDev &m = mon.d;
         const std::array<int, 6> fp = footprint(m);
-        bool meets = false;
-        for (const auto &t : taken)
-            meets |= fp[0] <= t[1] && t[0] <= fp[1] && fp[2] <= t[3] && t[2] <= fp[3] &&
-                     fp[4] <= t[5] && t[4] <= fp[5];
-        if (meets) continue;
+        // the footprint shared by two boxes of tiles (first > last on an axis: none)
+        auto meet = [](const std::array<int, 6> &a, const std::array<int, 6> &b) {
+            std::array<int, 6> c{};
+            for (int q = 0; q < 3; ++q) {
+                c[2 * q] = std::max(a[2 * q], b[2 * q]);
+                c[2 * q + 1] = std::min(a[2 * q + 1], b[2 * q + 1]);
+            }
+            return c;
+        };
+        auto empty = [](const std::array<int, 6> &c) {
+            return c[0] > c[1] || c[2] > c[3] || c[4] > c[5];
+        };
+        // a second (complex) or fourth (real) in any work-group: the candidate and one or three
+        // taken footprints share a tile
+        bool shared = false, fourth = false;
+        for (size_t a = 0; a < taken.size() && !fourth; ++a) {
+            const std::array<int, 6> fa = meet(fp, taken[a]);
+            if (empty(fa)) continue;
+            shared = true;
+            for (size_t b = a + 1; b < taken.size() && !fourth; ++b) {
+                const std::array<int, 6> fb = meet(fa, taken[b]);
+                if (empty(fb)) continue;
+                for (size_t c = b + 1; c < taken.size() && !fourth; ++c)
+                    fourth = !empty(meet(fb, taken[c]));
+            }
+        }
+        if (fourth || (shared && ctx->cplx)) continue;
+        ctx->fsnap_multi |= shared;
         taken.push_back(fp);
         std::vector<int> qs;
         for (const int id : ecells) {
@@ -947,18 +972,21 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
         TRY(mark(OEM_K_UPDATE_HE, true));
         const FuseSnap fsn = c->mons_off ? FuseSnap{} : c->fsnap;
         const dim3 hg(c->fg.nblocks), hb(64, OEM_CR + 1);
-        if (c->mat.mode != 0)
-            hipLaunchKernelGGL((k_update_he<F, 2>), hg, hb, 0, c->stream, (const F *)Eo,
-                               (const F *)Ho, E, H, po, pn, c->fg, c->mat, c->dims.ch,
-                               c->dims.cw, c->pw, c->d_step, fsn);
-        else if (c->mat.n_classes <= OEM_LDS_CLASSES)
-            hipLaunchKernelGGL((k_update_he<F, 0>), hg, hb, 0, c->stream, (const F *)Eo,
-                               (const F *)Ho, E, H, po, pn, c->fg, c->mat, c->dims.ch,
-                               c->dims.cw, c->pw, c->d_step, fsn);
-        else
-            hipLaunchKernelGGL((k_update_he<F, 1>), hg, hb, 0, c->stream, (const F *)Eo,
-                               (const F *)Ho, E, H, po, pn, c->fg, c->mat, c->dims.ch,
-                               c->dims.cw, c->pw, c->d_step, fsn);
+        const int mk = c->mat.mode != 0 ? 2 : c->mat.n_classes <= OEM_LDS_CLASSES ? 0 : 1;
+#define OEM_HE_LAUNCH(MKV, MULTIV)                                                        \
+        hipLaunchKernelGGL((k_update_he<F, MKV, MULTIV>), hg, hb, 0, c->stream,            \
+                           (const F *)Eo, (const F *)Ho, E, H, po, pn, c->fg, c->mat,       \
+                           c->dims.ch, c->dims.cw, c->pw, c->d_step, fsn)
+        if (!c->mons_off && c->fsnap_multi) {
+            if (mk == 0) OEM_HE_LAUNCH(0, true);
+            else if (mk == 1) OEM_HE_LAUNCH(1, true);
+            else OEM_HE_LAUNCH(2, true);
+        } else {
+            if (mk == 0) OEM_HE_LAUNCH(0, false);
+            else if (mk == 1) OEM_HE_LAUNCH(1, false);
+            else OEM_HE_LAUNCH(2, false);
+        }
+#undef OEM_HE_LAUNCH
         TRY(mark(OEM_K_UPDATE_HE, false));
     } else {
     TRY(mark(OEM_K_UPDATE_H, true));

@@ -96,8 +96,10 @@ def test_adjoint_checkpoints_restore_either_set(gpu_lib, monkeypatch):
 
 
 # Monitor sets on the 24 x 22 x 150 box soup (dipole at the centre cell (12, 11, 75)).  The
-# fused step snaps monitors whose work-groups (x chunks x 7-row tiles x 64-wide k tiles) are
-# disjoint, the others keep the snapshot kernel; each set names what it covers.  Flux planes at
+# fused step snaps monitors as long as no work-group (x chunks x 7-row tiles x 64-wide k tiles)
+# meets more than three of them (the first through the LDS stage or its own stores, the second
+# and third always by direct stores); the others keep the snapshot kernel; each set names what
+# it covers.  Flux planes at
 # the first plane of an x chunk (3-plane chunks: 6, 15; 16-plane chunks: 16), of a row tile (7,
 # 14) and of a k tile (64, 128) take the H below from the prologue, the halo row and the halo
 # column; boxes and planes through the dipole are snapped again after its correction; a
@@ -110,10 +112,24 @@ SNAP_SETS = {
     "y7_y14": (None, lambda f: [FluxMonitor("y7", 1, 7, -1, f[1:3]),
                                 FluxMonitor("y14", 1, 14, +1, f[:2])], lambda f: []),
     # z planes at k-tile starts (7 points per work-group, staged) and z75 sharing z64's tiles
-    # (snapshot kernel)
+    # (the second monitor of those work-groups: direct stores)
     "z64_z128_z75": (None, lambda f: [FluxMonitor("z64", 2, 64, +1, f[:3]),
                                       FluxMonitor("z128", 2, 128, +1, f[:2]),
                                       FluxMonitor("z75", 2, 75, -1, f[:2])], lambda f: []),
+    # four z planes in one k tile: three in every work-group of the tile, the fourth keeps the
+    # snapshot kernel
+    "z64_z70_z75_z80": (None, lambda f: [FluxMonitor("z64", 2, 64, +1, f[:3]),
+                                         FluxMonitor("z70", 2, 70, -1, f[1:3]),
+                                         FluxMonitor("z75", 2, 75, +1, f[:2]),
+                                         FluxMonitor("z80", 2, 80, -1, f[:2])], lambda f: []),
+    # x, z and y planes crossing (one work-group holds all three)
+    "x6_z75_y7": (None, lambda f: [FluxMonitor("x6", 0, 6, +1, f[:2]),
+                                   FluxMonitor("z75", 2, 75, -1, f[1:3]),
+                                   FluxMonitor("y7", 1, 7, +1, f[:2])], lambda f: []),
+    # a z plane through the dipole and a box around it (both snapped again after the dipole's
+    # correction; the box is the second monitor of their shared work-groups)
+    "z75_box_dipole": (None, lambda f: [FluxMonitor("z75", 2, 75, -1, f[:2])],
+                       lambda f: [FieldMonitor("dip", f[:2], (10, 9, 70), (5, 6, 10))]),
     # a z plane through the dipole (corrected cells snapped again)
     "z75_dipole": (None, lambda f: [FluxMonitor("z75", 2, 75, -1, f[:2])], lambda f: []),
     # a box around the dipole (staged, corrected cells snapped again) and an x plane at the
