@@ -183,7 +183,7 @@ def serving_bench(n_req: int, arena_gib: int, workers: int, solo_lanes: int = 2,
 
 
 def adjoint_bench(n_designs: int, steps: int, check_pixels: int, n: int = 400,
-                  concurrency: int = 2, record_gib: float = 24.0) -> dict:
+                  concurrency: int = 3, record_gib: float = 24.0) -> dict:
     """Inverse-design gradients of the 90-degree bend: 400 x 400 x 48 cells, a 160 x 160-pixel
     design region, the fundamental mode in and the output mode power as the objective, 6000
     steps.  Each design runs forward + device adjoint (the forward run records E at the design
@@ -331,7 +331,7 @@ def main() -> int:
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--adjoint-designs", type=int, default=32,
                     help="designs of the gradient leg (0: skip)")
-    ap.add_argument("--adjoint-concurrency", type=int, default=2,
+    ap.add_argument("--adjoint-concurrency", type=int, default=3,
                     help="designs in flight at once, each on its own context and stream")
     ap.add_argument("--adjoint-steps", type=int, default=6000)
     ap.add_argument("--adjoint-check", type=int, default=16)
@@ -441,7 +441,7 @@ def main() -> int:
     if args.adjoint_designs > 0:
         left = args.budget_s - (time.perf_counter() - t_start)
         designs = args.adjoint_designs
-        # seconds per design with two in flight (2.18-2.23 measured) / per pixel's pair of runs
+        # seconds per design with three in flight (2.1-2.2 measured) / per pixel's pair of runs
         per, fd = 2.2, 2.0 * args.adjoint_check
         while designs > 4 and designs * per + fd + 4 > left:
             designs -= 4
