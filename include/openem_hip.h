@@ -281,7 +281,7 @@ int oem_set_overlap_weights(oem_ctx *ctx, int32_t mon_id, int32_t n_sets, const 
  * (`openem/nb/backend.py:1422`), done there on the host. */
 int oem_monitor_overlap(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
 
-/* ---- Discrete adjoint (real, non-dispersive scenes) ---------------------------------
+/* ---- Discrete adjoint (real-field scenes, ADE poles included) ------------------------
  * The reference hands differentiation to tidy3d's adjoint (`openem/nb/autograd_hook.py:1-28`,
  * gradients assembled in `autograd_hook._vjp`); here the exact transpose of the time step runs
  * on the device.  Use: checkpoint the forward run every K steps, then per segment from the last:
@@ -289,7 +289,7 @@ int oem_monitor_overlap(oem_ctx *ctx, int32_t mon_id, double *out, size_t count)
  * oem_adjoint_back (the transposed steps; the objective's dJ/dfield and the eps gradient are
  * applied on the way). */
 int oem_checkpoint_alloc(oem_ctx *ctx, int32_t n_slots);
-/* E, H, psi and the step index (between DFT batches, i.e. after an oem_run call). */
+/* E, H, psi, the ADE pole state and the step index (between runs: after an oem_run call). */
 int oem_checkpoint_save(oem_ctx *ctx, int32_t slot);
 int oem_checkpoint_load(oem_ctx *ctx, int32_t slot);
 /* Design entries (component 0..2, flat cell) with d ca/d eps, d cb/d eps, ca, cb (float32 each)

@@ -1,4 +1,4 @@
-"""Gradients by the discrete adjoint on the device (real, non-dispersive scenes).
+"""Gradients by the discrete adjoint on the device (real-field scenes, ADE poles included).
 
 The reference computes inverse-design gradients through tidy3d's adjoint
 (``openem/nb/autograd_hook.py:1-28``).  Here the exact transpose of the time step runs on the
@@ -38,12 +38,33 @@ class Gradient:
     segments: int
 
 
+def _design_g(sc, comp: np.ndarray, cell: np.ndarray) -> np.ndarray:
+    """The ADE coupling ``G = sum 2 Re B`` of each design entry's (component, cell), 0 where it
+    has no poles (reference ``openem/coeffs.py:98-136`` folds it into ca and cb)."""
+    out = np.zeros(comp.size)
+    if not sc.any_dispersion:
+        return out
+    d = sc.dispersion
+    ncell = int(np.prod(sc.shape))
+    key = d.comp.astype(np.int64) * ncell + d.cell
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    want = comp.astype(np.int64) * ncell + cell
+    pos = np.minimum(np.searchsorted(ks, want), ks.size - 1)
+    hit = ks[pos] == want
+    out[hit] = d.g[order[pos[hit]]]
+    return out
+
+
 def design_coefficients(tabs, comp: np.ndarray, cell: np.ndarray):
-    """``(dca, dcb, ca, cb)`` float32 per design entry: the E-update coefficients and their eps
-    derivatives, ``ca = (eps - s)/(eps + s)``, ``cb = (dt/eps0)/(eps + s)``."""
+    """``(dca, dcb, ca, cb)`` float32 per design entry: the E-update coefficients
+    ``ca = (eps - s)/(eps + s + G)``, ``cb = (dt/eps0)/(eps + s + G)`` and their eps derivatives
+    ``(2 s + G)/den^2``, ``-(dt/eps0)/den^2`` (G, the ADE coupling of an entry with poles, does
+    not depend on eps)."""
     sc = tabs.scene
     eps = (sc.eps_ex, sc.eps_ey, sc.eps_ez)
     sig = (sc.sigma_ex, sc.sigma_ey, sc.sigma_ez)
+    gd = _design_g(sc, comp, cell)
     out = [np.zeros(comp.size) for _ in range(4)]
     for c in range(3):
         sel = comp == c
@@ -53,8 +74,8 @@ def design_coefficients(tabs, comp: np.ndarray, cell: np.ndarray):
         s = (0.0 if sig[c] is None else
              np.broadcast_to(np.asarray(sig[c]), sc.shape).reshape(-1)[cell[sel]]
              * sc.dt / (2.0 * cpml.EPSILON_0))
-        den = e + s
-        out[0][sel] = 2.0 * s / den ** 2
+        den = e + s + gd[sel]
+        out[0][sel] = (2.0 * s + gd[sel]) / den ** 2
         out[1][sel] = -(sc.dt / cpml.EPSILON_0) / den ** 2
         out[2][sel] = tabs.ca[c].reshape(-1)[cell[sel]]
         out[3][sel] = tabs.cb[c].reshape(-1)[cell[sel]]

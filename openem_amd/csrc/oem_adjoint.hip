@@ -1,5 +1,5 @@
-// Discrete adjoint of the time step (real fields, no dispersion): checkpoints, recomputation of
-// a segment with E recorded at the design cells, and the backward sweep.
+// Discrete adjoint of the time step (real fields, with or without ADE poles): checkpoints,
+// recomputation of a segment with E recorded at the design cells, and the backward sweep.
 //
 // The forward step is linear in (E, H, psi_H, psi_E) plus source terms; with
 //   H' = H - ch (s1 - s2),  s = (1/kappa + a) D_fwd E + b psi_H,  psi_H' = b psi_H + a D_fwd E
@@ -11,6 +11,15 @@
 // Each transpose is one kernel that reads the old adjoint fields and writes the new cell-local
 // ones (lambda_E, the psi adjoints) to a second buffer, so no thread reads a value another
 // overwrites; the buffers swap after each kernel.
+//
+// ADE poles (per entry: component c at a cell, complex pole states P_p driven by the real E)
+// add two cell-local transposes per step, one kernel over the entries before the eps gradient
+// (k_adj_disp): the post update P^{n+1} = P_mid + B E^{n+1} gives lambda_E^{n+1} the term
+// sum_p Re(conj(B_p) lambda_P_p); the pre update dP = (A-1) P + B E^n, hist = sum 2 Re dP, fed
+// into E^{n+1} as -cw cb pec hist, gives lambda_dP = lambda_P + 2 (-cw cb pec lambda_E^{n+1}),
+// lambda_P <- lambda_P + conj(A-1) lambda_dP and lambda_E^n the term sum_p Re(conj(B_p)
+// lambda_dP_p).  That last term is kept per entry until the H transpose has written
+// lambda_E^n, and the next step's k_adj_disp adds it first.
 //
 // Each term of an update: (target component c, source component b, derivative axis a, sign,
 // psi slot).  psi slot = off[2 a + m] of the slab layout (x: Hy/Ey 0, Hz/Ez 1; y: Hx/Ex 0,
@@ -570,6 +579,80 @@ k_adj_gather(const float *E, const int *comp, const int *cell, int n, size_t nce
     if (q < n) rec[q] = E[(size_t)comp[q] * ncell + cell[q]];
 }
 
+// Pole entry e of either ADE layout: its component, cell and pole range [t0, t1), and pole t's
+// coefficients as (A-1 re, A-1 im, B re, B im).
+__device__ __forceinline__ void adj_entry(const DispDev &d, int e, uint32_t &c, uint32_t &id,
+                                          int &t0, int &t1) {
+    if (d.np) {
+        const uint2 mt = d.meta[e];
+        c = mt.x >> 30;
+        id = mt.x & 0x3fffffffu;
+        t0 = e * d.np;
+        t1 = t0 + d.np;
+    } else {
+        c = (uint32_t)d.comp[e];
+        id = (uint32_t)d.cell[e];
+        t0 = d.ofs[e];
+        t1 = d.ofs[e + 1];
+    }
+}
+__device__ __forceinline__ float4 adj_coef(const DispDev &d, int t) {
+    if (d.np) return d.coef[d.c16[t]];
+    const int ci = d.cidx[t];
+    return make_float4(d.am1_re[ci], d.am1_im[ci], d.b_re[ci], d.b_im[ci]);
+}
+
+// wnu[e] = -cw cb pec of entry e's E component at its cell: lambda_hist = wnu lambda_E
+__global__ void __launch_bounds__(256)
+k_adj_disp_setup(DispDev d, MatDev mat, GridDev g, float cw, size_t ncell, float *wnu)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.n_entry) return;
+    uint32_t c, id;
+    int t0, t1;
+    adj_entry(d, e, c, id, t0, t1);
+    const int pl = g.ny * g.nz;
+    const int i = (int)id / pl, r = (int)id - i * pl, j = r / g.nz, k = r - j * g.nz;
+    const float cb = mat.mode == 0 ? mat.lut[8 * (size_t)mat.cls[id] + 2 * c + 1]
+                                   : mat.cb[(size_t)c * ncell + id];
+    const float px = axv(g.ax[0], i, AX_PEC), py = axv(g.ax[1], j, AX_PEC),
+                pz = axv(g.ax[2], k, AX_PEC);
+    const float pec = c == 0 ? py * pz : c == 1 ? pz * px : px * py;
+    wnu[e] = -cw * cb * pec;
+}
+
+// The ADE transposes of one backward step, one thread per pole entry: lambda_E^{n+1} gains the
+// pending term of lambda_E (from the step after) and the post update's term; then the pre
+// update's transpose moves lambda_P and leaves lambda_E^n's term pending.
+__global__ void __launch_bounds__(256)
+k_adj_disp(float *lE, DispDev d, float2 *lP, float *pend, const float *wnu, size_t ncell)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.n_entry) return;
+    uint32_t c, id;
+    int t0, t1;
+    adj_entry(d, e, c, id, t0, t1);
+    float *le = lE + (size_t)c * ncell + id;
+    float l = *le + pend[e];
+    for (int t = t0; t < t1; ++t) {
+        const float4 cf = adj_coef(d, t);
+        const float2 p = lP[t];
+        l += cf.z * p.x + cf.w * p.y;                 // Re(conj(B) lambda_P)
+    }
+    *le = l;
+    const float lh2 = 2.f * wnu[e] * l;               // 2 lambda_hist, on the real part
+    float r = 0.f;
+    for (int t = t0; t < t1; ++t) {
+        const float4 cf = adj_coef(d, t);
+        const float2 p = lP[t];
+        const float dr = p.x + lh2, di = p.y;         // lambda_dP
+        // lambda_P + conj(A-1) lambda_dP
+        lP[t] = make_float2(p.x + cf.x * dr + cf.y * di, p.y + cf.x * di - cf.y * dr);
+        r += cf.z * dr + cf.w * di;                   // Re(conj(B) lambda_dP)
+    }
+    pend[e] = r;
+}
+
 }  // namespace
 
 struct oem_adjoint_state {
@@ -593,6 +676,9 @@ struct oem_adjoint_state {
     std::vector<Points> obj;
     std::unordered_set<uint64_t> obj_keys;
     bool obj_disjoint = true;
+    // ADE poles: lambda_P per pole, the pending lambda_E^n term and -cw cb pec per entry
+    float2 *lP = nullptr;
+    float *pend = nullptr, *wnu = nullptr;
 };
 
 namespace oemi {
@@ -608,8 +694,7 @@ static int adj_chunk() {
 
 static int adj_check(oem_ctx *ctx) {
     if (!ctx->prepared) return fail(OEM_ESTATE, "oem_prepare not called");
-    if (ctx->cplx || ctx->has_disp)
-        return fail(OEM_ENOTSUP, "the adjoint covers real, non-dispersive scenes");
+    if (ctx->cplx) return fail(OEM_ENOTSUP, "the adjoint covers real-field scenes");
     if (ctx->pool) return fail(OEM_ENOTSUP, "pooled contexts have no adjoint");
     if ((ctx->dims.nx + adj_chunk() - 1) / adj_chunk() > 65535)
         return fail(OEM_ENOTSUP, "the adjoint launch covers nx <= %d", 65535 * adj_chunk());
@@ -621,7 +706,12 @@ static int adj_check(oem_ctx *ctx) {
     return OEM_OK;
 }
 
-static size_t state_floats(oem_ctx *c) { return 6 * c->ncell + 2 * (size_t)c->psi.total; }
+// a checkpoint: E, H, both psi sets and the ADE pole state ((re, im) per pole; between runs it
+// holds P^n, the last step's post not fused with a next pre)
+static size_t pole_floats(oem_ctx *c) { return c->has_disp ? 2 * (size_t)c->disp_n_pole : 0; }
+static size_t state_floats(oem_ctx *c) {
+    return 6 * c->ncell + 2 * (size_t)c->psi.total + pole_floats(c);
+}
 
 extern "C" int oem_checkpoint_alloc(oem_ctx *ctx, int32_t n_slots) {
     CHECK_CTX(ctx);
@@ -639,10 +729,16 @@ static int copy_state(oem_ctx *c, float *dst_base, bool save) {
     const bool odd = c->fused && (c->step & 1);
     float *E = odd ? c->E2 : c->E, *H = odd ? c->H2 : c->H;
     const PsiDev &ps = odd ? c->psi2 : c->psi;
-    float *parts[4] = {E, H, static_cast<float *>(ps.h), static_cast<float *>(ps.e)};
-    const size_t n[4] = {3 * nc, 3 * nc, pt, pt};
+    const size_t np = (size_t)c->disp_n_pole;
+    const bool compact = c->has_disp && c->disp.np;
+    float *parts[6] = {E, H, static_cast<float *>(ps.h), static_cast<float *>(ps.e),
+                       compact ? reinterpret_cast<float *>(c->disp.sp) : c->disp.p_re,
+                       compact ? nullptr : c->disp.p_im};
+    const size_t n[6] = {3 * nc, 3 * nc, pt, pt, c->has_disp ? (compact ? 2 * np : np) : 0,
+                         c->has_disp && !compact ? np : 0};
     size_t o = 0;
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 6; ++q) {
+        if (n[q] == 0) continue;
         void *dst = save ? (void *)(dst_base + o) : (void *)parts[q];
         const void *src = save ? (const void *)parts[q] : (const void *)(dst_base + o);
         HIP_OK(hipMemcpyAsync(dst, src, n[q] * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
@@ -657,6 +753,7 @@ extern "C" int oem_checkpoint_save(oem_ctx *ctx, int32_t slot) {
     oem_adjoint_state *a = ctx->adj;
     if (slot < 0 || slot >= a->n_slots) return fail(OEM_EINVAL, "checkpoint slot %d", slot);
     if (ctx->pending) return fail(OEM_ESTATE, "checkpoint between DFT batches only");
+    if (ctx->pre_ready) return fail(OEM_ESTATE, "checkpoint between runs only");
     TRY(copy_state(ctx, a->ckpt + (size_t)slot * state_floats(ctx), true));
     a->ckpt_step[slot] = ctx->step;
     HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -675,6 +772,7 @@ extern "C" int oem_checkpoint_load(oem_ctx *ctx, int32_t slot) {
     HIP_OK(hipMemcpyAsync(ctx->d_step, st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
     ctx->pending = 0;
+    ctx->pre_ready = false;
     return OEM_OK;
 }
 
@@ -707,6 +805,15 @@ extern "C" int oem_adjoint_setup(oem_ctx *ctx, int32_t n_design, const int32_t *
     TRY(ctx->upload(&a->cb, cb, (size_t)n_design));
     TRY(ctx->alloc(&a->grad, (size_t)n_design, true));
     TRY(ctx->alloc(&a->rec, (size_t)(max_seg + 1) * n_design, false));
+    if (ctx->has_disp) {
+        const int ne = ctx->disp.n_entry;
+        TRY(ctx->alloc(&a->lP, std::max<size_t>((size_t)ctx->disp_n_pole, 1), true));
+        TRY(ctx->alloc(&a->pend, (size_t)ne, true));
+        TRY(ctx->alloc(&a->wnu, (size_t)ne, true));
+        hipLaunchKernelGGL(k_adj_disp_setup, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream,
+                           ctx->disp, ctx->mat, ctx->g, ctx->dims.cw, ctx->ncell, a->wnu);
+        HIP_OK(hipGetLastError());
+    }
     a->n_design = n_design;
     a->max_seg = max_seg;
     HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -754,14 +861,15 @@ extern "C" int oem_adjoint_record(oem_ctx *ctx, int64_t n_steps) {
     hipLaunchKernelGGL(k_adj_gather, g, b, 0, ctx->stream, ctx->cur_E(), a->d_comp, a->d_cell,
                        a->n_design, ctx->ncell, a->rec);
     int rc = OEM_OK;
+    // the last step's ADE post is not fused with a next pre: the pole state ends as P^n, as a
+    // run leaves it
     for (int64_t s = 0; s < n_steps && !rc; ++s) {
-        rc = oemi::launch_one_step(ctx, false);
+        rc = oemi::launch_one_step(ctx, s == n_steps - 1);
         if (!rc)
             hipLaunchKernelGGL(k_adj_gather, g, b, 0, ctx->stream, ctx->cur_E(), a->d_comp,
                                a->d_cell, a->n_design, ctx->ncell, a->rec + (size_t)(s + 1) * a->n_design);
     }
     ctx->mons_off = false;
-    ctx->pre_ready = false;
     TRY(rc);
     HIP_OK(hipGetLastError());
     a->rec_n = (int)n_steps;
@@ -845,6 +953,9 @@ extern "C" int oem_adjoint_back(oem_ctx *ctx, int64_t n_steps) {
                                    ctx->stream, P.magnetic ? a->lH : a->lE, P.d, ctx->mat,
                                    ctx->ncell, n);
         }
+        if (ctx->has_disp)
+            hipLaunchKernelGGL(k_adj_disp, dim3((ctx->disp.n_entry + 255) / 256), dim3(256), 0,
+                               ctx->stream, a->lE, ctx->disp, a->lP, a->pend, a->wnu, ctx->ncell);
         hipLaunchKernelGGL(k_adj_grad, gg, gb, 0, ctx->stream, a->lE, a->d_comp, a->d_cell,
                            a->dca, a->dcb, a->ca, a->cb, a->rec + (size_t)s * a->n_design,
                            a->rec + (size_t)(s + 1) * a->n_design, a->n_design, ctx->ncell,

@@ -149,6 +149,7 @@ struct oem_ctx {
     std::vector<char> class_flagged;   // LUT classes that read the ADE history term
     DispDev disp{};
     bool has_disp = false;
+    int64_t disp_n_pole = 0;           // poles over all entries (the pole state's length)
     PlaneSrcDev pw{};
     std::vector<Points> points;
     std::vector<Monitor> mons;

@@ -335,6 +335,7 @@ extern "C" int oem_set_dispersion(oem_ctx *ctx, int32_t n_entry, const int32_t *
     for (int q = 0; q < 6; ++q) { D.hb[q] = hb[q]; ctx->mat.hb[q] = hb[q]; }
     ctx->mat.hist = D.hist;
     ctx->has_disp = true;
+    ctx->disp_n_pole = n_pole;
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return OEM_OK;
 }

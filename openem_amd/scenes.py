@@ -637,3 +637,29 @@ def bend_scene(design: np.ndarray | None = None, n: int = 400, nz: int = 48, dl:
     src = modes_mod.mode_source(base, m_src, pulse, +1, num_steps)
     mon = ModeMonitor("out", 1, km, np.array([f0]))
     return dataclasses.replace(base, mode_sources=[src], mode_monitors=[mon]), cells
+
+
+def with_pole_block(sc: Scene, lo, hi, materials) -> Scene:
+    """``sc`` with ADE poles on all three components of the cells ``lo <= (i, j, k) < hi``
+    (eps_inf stays the scene's eps): ``materials`` is a list of pole lists, material
+    ``(i + j + k) % len(materials)`` at cell (i, j, k).  One material gives every entry the same
+    pole count; materials of different counts give the general (CSR) pole layout."""
+    import dataclasses
+    mat = np.full(sc.shape, -1, dtype=np.int64)
+    ii, jj, kk = np.meshgrid(*(np.arange(a, b) for a, b in zip(lo, hi)), indexing="ij")
+    mat[ii, jj, kk] = (ii + jj + kk) % len(materials)
+    return dataclasses.replace(sc, dispersion=_staircase_dispersion(mat, materials, sc.dt))
+
+
+def pole_guide_scene(num_steps: int = 260, two_materials: bool = False) -> Scene:
+    """A 40 x 24 x 22 strip guide (40 nm cells, source plane x = 12, forward mode monitor
+    x = 26) with a block of Lorentz poles on and beside the core's upper half for x in [15, 24):
+    one 2-pole material (every entry the same pole count) or, ``two_materials``, a 1-pole one on
+    alternate cells as well (the general pole layout)."""
+    from .model import lorentz_poles
+    sc = waveguide_scene(shape=(40, 24, 22), n_pml=6, width_cells=6, thick_cells=4,
+                         fwidth_rel=0.3, num_steps=num_steps, nf=1)
+    mats = [lorentz_poles(1.5, 2.5e15, 2e14) + lorentz_poles(0.7, 1.2e15, 5e13)]
+    if two_materials:
+        mats.append(lorentz_poles(2.0, 1.8e15, 1e14))
+    return with_pole_block(sc, (15, 8, 11), (24, 16, 16), mats)

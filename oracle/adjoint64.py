@@ -2,21 +2,29 @@
 
 Only ``tests/`` may import this module (it checks the device adjoint).
 
-The forward step of :class:`oracle.yee64.Oracle64` on a real, non-dispersive scene is linear in
-the state ``(E, H, psi_H, psi_E)`` plus additive source terms, with the E update
-``E' = pec (ca E + cb S)`` depending on eps through ``ca`` and ``cb``
+The forward step of :class:`oracle.yee64.Oracle64` on a real scene is linear in the state
+``(E, H, psi_H, psi_E, P)`` plus additive source terms, with the E update
+``E' = pec (ca E + cb (S - cw hist))`` depending on eps through ``ca`` and ``cb``
 (reference ``openem/kernels/yee.cu:154-240``, coefficients ``openem/coeffs.py:98-136``).  For an
 objective ``J`` of the DFT phasors the exact gradient of the discrete run is
 
-    dJ/deps = sum_n lambda_E^{n+1} . (d ca/d eps E^n + d cb/d eps S^n),
-    S^n = (E^{n+1} - ca E^n) / cb,
+    dJ/deps = sum_n lambda_E^{n+1} . (d ca/d eps E^n + d cb/d eps X^n),
+    X^n = (E^{n+1} - ca E^n) / cb,
 
 where ``lambda`` runs backwards through the transposed step: the objective's gradient with
-respect to E^{n+1} and H^{n+1/2} is added, then the transpose of the E update (its backward
-differences become forward differences of ``cb pec lambda_E``, the CPML recursion transposes
-onto the psi adjoints), then the transpose of the H update.  This is the "discrete adjoint"
-that tidy3d's ``grad`` stands for in the reference (``openem/nb/autograd_hook.py:1-28``); the
+respect to E^{n+1} and H^{n+1/2} is added, then the transpose of the ADE post update, then the
+transpose of the E update (its backward differences become forward differences of
+``cb pec lambda_E``, the CPML recursion transposes onto the psi adjoints), then the transpose of
+the ADE pre update, then the transpose of the H update.  This is the "discrete adjoint" that
+tidy3d's ``grad`` stands for in the reference (``openem/nb/autograd_hook.py:1-28``); the
 reference hands it to tidy3d, here it is written out.
+
+ADE poles (``openem/kernels/dispersion.cu:68-147``), per pole entry of component c at a cell:
+pre ``dP = (A-1) P + B E^n``, ``P_mid = P + dP``, ``hist = sum 2 Re dP``; post
+``P^{n+1} = P_mid + B E^{n+1}``.  P is complex and E real, so with the adjoint of P written as
+one complex number ``lP = dJ/d Re P + i dJ/d Im P``: a product ``a z`` transposes to
+``conj(a) lz``, ``B e`` to ``Re(conj(B) lz)`` and ``2 Re dP`` to ``2 l`` on the real part.
+Bloch (complex-field) scenes are not covered.
 """
 
 from __future__ import annotations
@@ -36,11 +44,11 @@ _H_TERMS = _E_TERMS = _TERMS
 
 
 class Adjoint64:
-    """Backward sweep over an :class:`Oracle64` run (real fields, no dispersion)."""
+    """Backward sweep over an :class:`Oracle64` run (real fields, with or without ADE poles)."""
 
     def __init__(self, tabs):
-        if tabs.bloch or tabs.disp is not None:
-            raise NotImplementedError("the adjoint covers real, non-dispersive scenes")
+        if tabs.bloch:
+            raise NotImplementedError("the adjoint covers real-field scenes")
         self.t = tabs
         self.o = Oracle64(tabs)                      # for the difference operators and masks
         shp = tabs.shape
@@ -48,6 +56,11 @@ class Adjoint64:
         self.lH = np.zeros((3,) + shp)
         self.lpe = {k: np.zeros(shp) for k in _PSI}
         self.lph = {k: np.zeros(shp) for k in _PSI}
+        d = tabs.disp
+        if d is not None:
+            self.lP = np.zeros(d.am1.size, dtype=np.complex128)
+            self._entry = self.o._pole_entry          # pole -> entry
+            self._idx4 = self.o._idx4                 # entry -> (comp, i, j, k)
 
     @staticmethod
     def _scatter_add(acc, idx, src):
@@ -84,15 +97,34 @@ class Adjoint64:
         m = self.o._m[a]
         return m["a" + side], m["b" + side], m["k" + side]
 
+    def _entry_sum(self, v):
+        """Per entry sum of the real per-pole values ``v``."""
+        return np.bincount(self._entry, weights=v, minlength=self.t.disp.comp.size)
+
+    def absorb_poles(self):
+        """Transpose of the ADE post update ``P^{n+1} = P_mid + B E^{n+1}``: E^{n+1} also feeds
+        the pole state, so lambda_E^{n+1} gains ``sum_p Re(conj(B_p) lP_p)`` at every pole entry
+        (``lP`` passes to P_mid unchanged).  After it, lambda_E is the full dJ/dE^{n+1} the eps
+        gradient takes.  No-op without poles."""
+        d = self.t.disp
+        if d is not None:
+            self.lE[self._idx4] += self._entry_sum(np.real(np.conj(d.b) * self.lP))
+
     def step_back(self, gE=None, gH=None):
         """Adjoint of one forward step: ``gE``/``gH`` are dJ/dE^{n+1}, dJ/dH^{n+1/2} of this
         step's samples (added first); leaves lambda for the state before the step."""
+        if gE is not None:
+            self.lE += gE
+        if gH is not None:
+            self.lH += gH
+        self.absorb_poles()
+        self.transpose()
+
+    def transpose(self):
+        """The E update, ADE pre and H update transposed (lambda_E already absorbed the ADE
+        post, :meth:`absorb_poles`)."""
         t, m = self.t, self.o._m
         lE, lH = self.lE, self.lH
-        if gE is not None:
-            lE += gE
-        if gH is not None:
-            lH += gH
         # E update transpose
         pec = [m[1]["pec"] * m[2]["pec"], m[2]["pec"] * m[0]["pec"], m[0]["pec"] * m[1]["pec"]]
         nu = [t.cb[c] * pec[c] * lE[c] for c in range(3)]
@@ -108,6 +140,14 @@ class Adjoint64:
         for c in range(3):
             lE[c] = t.ca[c] * pec[c] * lE[c]
         self.lpe = new_pe
+        d = t.disp
+        if d is not None:
+            # ADE pre transposed: hist enters E^{n+1} as -cb cw pec hist, so lhist = -cw nu;
+            # dP feeds P_mid (identity) and hist (2 Re); P^n feeds P_mid and dP, E^n feeds dP
+            lhist = -t.cw * np.stack(nu)[self._idx4]
+            ldp = self.lP + 2.0 * lhist[self._entry]
+            self.lP = self.lP + np.conj(d.am1) * ldp
+            lE[self._idx4] += self._entry_sum(np.real(np.conj(d.b) * ldp))
         lH += dH
         # H update transpose: H_c' = H_c - ch (s1 - s2)
         dE = np.zeros_like(lE)
@@ -125,16 +165,19 @@ class Adjoint64:
 
 def eps_derivatives(tabs, sc):
     """Per component (d ca/d eps, d cb/d eps) of ``ca = (eps - s)/(eps + s + G)``,
-    ``cb = (dt/eps0)/(eps + s + G)``."""
+    ``cb = (dt/eps0)/(eps + s + G)``: ``((2 s + G)/den^2, -(dt/eps0)/den^2)`` (G, the ADE
+    coupling ``sum 2 Re B`` of a pole entry, does not depend on eps)."""
     from openem_amd import cpml
     out = []
     sig = (sc.sigma_ex, sc.sigma_ey, sc.sigma_ez)
     eps = (sc.eps_ex, sc.eps_ey, sc.eps_ez)
+    gg = sc.dispersion.g_dense(sc.shape) if sc.any_dispersion else (0.0, 0.0, 0.0)
     for c in range(3):
         e = np.broadcast_to(np.asarray(eps[c], dtype=np.float64), sc.shape)
         s = 0.0 if sig[c] is None else np.asarray(sig[c]) * sc.dt / (2 * cpml.EPSILON_0)
-        den = e + s
-        out.append(((2 * s) / den ** 2 + 0 * e, -(sc.dt / cpml.EPSILON_0) / den ** 2 + 0 * e))
+        den = e + s + gg[c]
+        out.append(((2 * s + gg[c]) / den ** 2 + 0 * e,
+                    -(sc.dt / cpml.EPSILON_0) / den ** 2 + 0 * e))
     return out
 
 
@@ -192,7 +235,8 @@ def gradient_mode_power(sc, tabs, mon_index: int, design: np.ndarray, f_index: i
             v = 2.0 * np.real(np.conj(a) * K[slot] * wh) * 0.5
             adj.lH[comp][plane] += v
             adj.lH[comp][below] += v
+        adj.absorb_poles()
         e0, e1 = traj[n], traj[n + 1]
         grad += adj.lE.reshape(-1)[flat] * (dca * e0 + dcb * (e1 - ca * e0) / cb)
-        adj.step_back()
+        adj.transpose()
     return J, grad
