@@ -809,9 +809,13 @@ extern "C" int oem_prepare(oem_ctx *ctx) {
         chunk = std::atoi(env);
     } else if (ctx->pool) {
         // a pool work-group marches a fixed number of planes: the batch of scenes supplies the
-        // parallelism
+        // parallelism.  Pool throughput of k box soups of side n (Mcell-steps/s, chunks 4 / 6 /
+        // 8 / 12 / 16 / 32 / 64): 48 x 64^3 26.9 / 27.0 / 26.8-27.0 / 26.3 / 25.9 / 26.3 / 23.6
+        // G, 24 x 96^3 34.8 / 34.8 / 34.8-34.9 / 34.3 / 33.9 / 32.3 / 31.1 G, 12 x 128^3 37.0 /
+        // 37.4 / 37.7 / 37.5 / 36.7 / 35.5 / 33.9 G, 6 x 160^3 42.3 / 43.2 / 43.1 / 42.7 / 42.5 /
+        // 40.4 / 34.5 G, 3 x 180^3 42.3 / 42.5 / 42.7-42.8 / 41.2 / 40.0 / 36.6 / 32.0 G
         const char *pe = std::getenv("OEM_POOL_CHUNK");
-        chunk = (pe && std::atoi(pe) > 0) ? std::atoi(pe) : 32;
+        chunk = (pe && std::atoi(pe) > 0) ? std::atoi(pe) : 8;
     } else {
         // 64 x 8 tiles marching 32 planes at 512^3: measured best of tiles 4/8/16 rows and
         // chunks 32/64/128 planes
