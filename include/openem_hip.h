@@ -281,7 +281,7 @@ int oem_set_overlap_weights(oem_ctx *ctx, int32_t mon_id, int32_t n_sets, const 
  * (`openem/nb/backend.py:1422`), done there on the host. */
 int oem_monitor_overlap(oem_ctx *ctx, int32_t mon_id, double *out, size_t count);
 
-/* ---- Discrete adjoint (real-field scenes, ADE poles included) ------------------------
+/* ---- Discrete adjoint (real and complex Bloch fields, ADE poles included) ------------
  * The reference hands differentiation to tidy3d's adjoint (`openem/nb/autograd_hook.py:1-28`,
  * gradients assembled in `autograd_hook._vjp`); here the exact transpose of the time step runs
  * on the device.  Use: checkpoint the forward run every K steps, then per segment from the last:
@@ -298,15 +298,17 @@ int oem_adjoint_setup(oem_ctx *ctx, int32_t n_design, const int32_t *comp, const
                       const float *dca, const float *dcb, const float *ca, const float *cb,
                       int32_t max_seg);
 /* The objective's gradient with respect to the field samples of forward step n:
- * lambda[comp][cell] += Re(z * tab[n]) (z: (re, im) floats per point, tab: n_steps complex
- * doubles); magnetic = 1 for H^{n+1/2}, 0 for E^{n+1}. */
+ * lambda[comp][cell] += z * tab[n] (z: (re, im) floats per point, tab: n_steps complex
+ * doubles; the real part on real fields, dJ/d Re + i dJ/d Im on complex fields); magnetic = 1
+ * for H^{n+1/2}, 0 for E^{n+1}. */
 int oem_adjoint_objective(oem_ctx *ctx, int32_t magnetic, int32_t n_pts, const int32_t *comp,
                           const int32_t *cell, const float *z, const double *tab);
 /* Recompute n_steps forward from the loaded checkpoint (no DFT), recording E at the design. */
 int oem_adjoint_record(oem_ctx *ctx, int64_t n_steps);
 /* The forward run itself, n_steps with the monitors (as oem_run), recording E at the design
  * after every step, so no checkpoint or recomputation is needed when the record ((n_steps + 1)
- * x n_design floats, allocated by oem_adjoint_setup's max_seg) fits in memory.  The
+ * x n_design values of the field type, allocated by oem_adjoint_setup's max_seg) fits in
+ * memory.  The
  * reference's counterpart is the same: tidy3d's forward run feeds its adjoint
  * (`openem/nb/autograd_hook.py:1-28`). */
 int oem_adjoint_forward(oem_ctx *ctx, int64_t n_steps);

@@ -1,14 +1,17 @@
-"""Gradients by the discrete adjoint on the device (real-field scenes, ADE poles included).
+"""Gradients by the discrete adjoint on the device (real and complex Bloch fields, ADE poles
+included).
 
 The reference computes inverse-design gradients through tidy3d's adjoint
 (``openem/nb/autograd_hook.py:1-28``).  Here the exact transpose of the time step runs on the
 MI355X (``csrc/oem_adjoint.hip``) with segmented checkpoint recomputation:
 
 1. forward run, the state checkpointed every ``checkpoint_every`` steps;
-2. the objective from the device mode overlap: ``J = |a+|^2`` of a mode monitor;
+2. the objective: ``J = |a+|^2`` of a mode monitor (the device mode overlap), or ``J = |a|^2``
+   of a fixed projection ``a = sum K . phasors`` of any plane monitor;
 3. for each segment from the last: reload its checkpoint, recompute it with E recorded at the
    design cells, and step the adjoint back over it, accumulating
-   ``dJ/d eps = sum_n lambda_E^{n+1} (ca' E^n + cb' S^n)``.
+   ``dJ/d eps = sum_n lambda_E^{n+1} (ca' E^n + cb' S^n)`` (``Re(conj(lambda_E^{n+1}) (...))``
+   on complex fields, whose step transposes to its conjugate transpose).
 
 When E at the design cells for every step fits in ``record_bytes`` of device memory, the forward
 run records it directly and the adjoint steps back over the whole run at once: no checkpoints
@@ -104,37 +107,78 @@ def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
     m = tabs.monitors[mi]
     if not m.modes:
         raise ValueError(f"{monitor} is not a mode monitor")
+    # a+ = sum K . phasors with K the summed overlap weights
+    g = modes_mod.overlap_weight_sets(m.modes, sc.grid)
+    K = 0.25 * modes_mod.orientation(m.axis) * (g[0, :, f_index] + g[1, :, f_index])
+    return _overlap_gradient(sc, tabs, mi, K, True, comp, cell, f_index, checkpoint_every,
+                             device, record_bytes)
+
+
+def overlap_power_gradient(sc, monitor: str, weights, comp, cell, f_index: int = 0,
+                           checkpoint_every: int = 500, n_total: int | None = None,
+                           device: int = 0, record_bytes: int = 24 << 30) -> Gradient:
+    """``J = |a|^2`` with ``a = sum weights . phasors`` of plane monitor ``monitor`` at frequency
+    ``f_index`` (``weights`` (4, n1, n2) complex over its E t1, E t2, H t1, H t2 slots: a
+    projection on a diffraction order or any fixed field profile), and ``dJ/d eps`` at the design
+    entries, on real or complex (Bloch) fields."""
+    tabs = tables_mod.build(sc, n_total)
+    names = [m.name for m in tabs.monitors]
+    mi = names.index(monitor)
+    m = tabs.monitors[mi]
+    if m.kind != "plane":
+        raise ValueError(f"{monitor} is not a plane monitor")
+    K = np.asarray(weights, dtype=np.complex128)
+    if K.shape != (4, m.n1, m.n2):
+        raise ValueError(f"weights of shape {K.shape}, the monitor needs {(4, m.n1, m.n2)}")
+    return _overlap_gradient(sc, tabs, mi, K, False, comp, cell, f_index, checkpoint_every,
+                             device, record_bytes)
+
+
+def _overlap_gradient(sc, tabs, mi: int, K: np.ndarray, mode: bool, comp, cell, f_index: int,
+                      checkpoint_every: int, device: int, record_bytes: int) -> Gradient:
+    """The forward run (recorded, or checkpointed), ``a = sum K . phasors`` of plane monitor
+    ``mi`` (the device mode overlap when ``mode``), the objective's point lists and the backward
+    sweep.  A sample x of slot c at step n enters ``a`` as ``K_c w_n x``, so dJ/dx is
+    ``2 Re(conj(a) K_c w_n)`` on real fields and ``2 a conj(K_c w_n)`` (``dJ/d Re x + i dJ/d Im
+    x``) on complex ones: the device adds ``z tab[n]`` (its real part on real fields), so complex
+    fields take ``z`` and the phase table conjugated."""
+    m = tabs.monitors[mi]
+    cplx = bool(tabs.bloch)
     comp = np.ascontiguousarray(comp, dtype=np.int32)
     cell = np.ascontiguousarray(cell, dtype=np.int32)
     n = tabs.n_total
-    one_pass = (n + 1) * comp.size * 4 <= record_bytes
-    K = n if one_pass else max(1, int(checkpoint_every))
-    n_seg = (n + K - 1) // K
+    one_pass = (n + 1) * comp.size * 4 * (2 if cplx else 1) <= record_bytes
+    Kseg = n if one_pass else max(1, int(checkpoint_every))
+    n_seg = (n + Kseg - 1) // Kseg
     dca, dcb, ca, cb = design_coefficients(tabs, comp, cell)
     st = Stepper(tabs, device=device)
     lib, ctx = st.lib, st.ctx
     try:
         _lib.check(lib, lib.oem_adjoint_setup(ctx, comp.size, _lib.ptr(comp), _lib.ptr(cell),
                                               _lib.ptr(dca), _lib.ptr(dcb), _lib.ptr(ca),
-                                              _lib.ptr(cb), K))
+                                              _lib.ptr(cb), Kseg))
         if one_pass:
             _lib.check(lib, lib.oem_adjoint_forward(ctx, n))
         else:
             _lib.check(lib, lib.oem_checkpoint_alloc(ctx, n_seg))
             for s in range(n_seg):
                 _lib.check(lib, lib.oem_checkpoint_save(ctx, s))
-                st.run(min(K, n - s * K))
-        st.setup_modes()
-        ap, _ = st.mode_amplitudes(mi)
-        a = complex(ap[f_index])
-        # dJ/d(sample) = 2 Re(conj(a) K_c e^{i w t}); K the summed overlap weights
-        g = modes_mod.overlap_weight_sets(m.modes, sc.grid)
-        Kw = 0.25 * modes_mod.orientation(m.axis) * (g[0, :, f_index] + g[1, :, f_index])
+                st.run(min(Kseg, n - s * Kseg))
+        if mode:
+            st.setup_modes()
+            ap, _ = st.mode_amplitudes(mi)
+            a = complex(ap[f_index])
+        else:
+            a = complex(np.sum(K * st.read_monitor(mi)[:, f_index]))
         t1, t2 = transverse_axes(m.axis)
         ph = m.phase[:, f_index]
-        tab_e = np.ascontiguousarray(ph[:, 0] + 1j * ph[:, 1], dtype=np.complex128).view(np.float64)
-        tab_h = np.ascontiguousarray(ph[:, 2] + 1j * ph[:, 3], dtype=np.complex128).view(np.float64)
-        z = 2.0 * np.conj(a) * Kw                         # (4, n1, n2)
+        we = ph[:, 0] + 1j * ph[:, 1]
+        wh = ph[:, 2] + 1j * ph[:, 3]
+        z = 2.0 * np.conj(a) * K                          # (4, n1, n2)
+        if cplx:
+            z, we, wh = np.conj(z), np.conj(we), np.conj(wh)
+        tab_e = np.ascontiguousarray(we, dtype=np.complex128).view(np.float64)
+        tab_h = np.ascontiguousarray(wh, dtype=np.complex128).view(np.float64)
         cells_k = _plane_cells(tabs.shape, m.axis, m.km)
         cells_km = _plane_cells(tabs.shape, m.axis, m.km - 1)
         for magnetic, slot, fcomp, planes, tab, w in (
@@ -155,7 +199,7 @@ def mode_power_gradient(sc, monitor: str, comp, cell, f_index: int = 0,
             _lib.check(lib, lib.oem_adjoint_back(ctx, n))
         else:
             for s in range(n_seg - 1, -1, -1):
-                ln = min(K, n - s * K)
+                ln = min(Kseg, n - s * Kseg)
                 _lib.check(lib, lib.oem_checkpoint_load(ctx, s))
                 _lib.check(lib, lib.oem_adjoint_record(ctx, ln))
                 _lib.check(lib, lib.oem_adjoint_back(ctx, ln))

@@ -169,7 +169,7 @@ struct oem_ctx {
         bool on = false;
         const int *comp = nullptr, *cell = nullptr;
         int n = 0;
-        float *rec = nullptr;
+        float *rec = nullptr;       // of the field type (two floats per value on complex fields)
         int64_t base = 0;
     } rec_hook;
     struct oem_adjoint_state *adj = nullptr;

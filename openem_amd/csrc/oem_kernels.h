@@ -37,6 +37,9 @@ __device__ __forceinline__ cf2 operator+(cf2 a, cf2 b) { return {a.r + b.r, a.i 
 __device__ __forceinline__ cf2 operator-(cf2 a, cf2 b) { return {a.r - b.r, a.i - b.i}; }
 __device__ __forceinline__ cf2 operator*(float s, cf2 a) { return {s * a.r, s * a.i}; }
 __device__ __forceinline__ cf2 operator*(cf2 a, float s) { return {a.r * s, a.i * s}; }
+__device__ __forceinline__ cf2 operator-(cf2 a) { return {-a.r, -a.i}; }
+__device__ __forceinline__ cf2 &operator+=(cf2 &a, cf2 b) { a.r += b.r; a.i += b.i; return a; }
+__device__ __forceinline__ cf2 &operator-=(cf2 &a, cf2 b) { a.r -= b.r; a.i -= b.i; return a; }
 
 // ---------------------------------------------------------------- buffer resources ----------
 // An SGPR descriptor + a fixed 32-bit per-lane VGPR offset + a per-iteration SGPR offset keeps

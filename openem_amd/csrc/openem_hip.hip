@@ -1050,12 +1050,11 @@ int launch_step_t(oem_ctx *c, int flush_after, Prof *pr, bool last) {
                 TRY(mark(points_kid(pt), false));
             }
         TRY(disp_post());
-        if constexpr (sizeof(F) == 4) {
-            if (c->rec_hook.on)
-                hipLaunchKernelGGL((k_rec_gather<float>), dim3(blocks_for(c->rec_hook.n, 256)), dim3(256), 0,
-                                   c->stream, (const float *)E, c->rec_hook.comp, c->rec_hook.cell,
-                                   c->rec_hook.n, nc, c->rec_hook.rec, c->d_step, c->rec_hook.base);
-        }
+        if (c->rec_hook.on)
+            hipLaunchKernelGGL((k_rec_gather<F>), dim3(blocks_for(c->rec_hook.n, 256)), dim3(256), 0,
+                               c->stream, (const F *)E, c->rec_hook.comp, c->rec_hook.cell,
+                               c->rec_hook.n, nc, reinterpret_cast<F *>(c->rec_hook.rec), c->d_step,
+                               c->rec_hook.base);
         for (auto &tm : c->tmons) {
             if (c->mons_off) break;
             for (int shot = 0; shot < 2; ++shot) {

@@ -24,7 +24,14 @@ pre ``dP = (A-1) P + B E^n``, ``P_mid = P + dP``, ``hist = sum 2 Re dP``; post
 ``P^{n+1} = P_mid + B E^{n+1}``.  P is complex and E real, so with the adjoint of P written as
 one complex number ``lP = dJ/d Re P + i dJ/d Im P``: a product ``a z`` transposes to
 ``conj(a) lz``, ``B e`` to ``Re(conj(B) lz)`` and ``2 Re dP`` to ``2 l`` on the real part.
-Bloch (complex-field) scenes are not covered.
+
+Bloch (complex-field) scenes: every coefficient of the step is real except the wrap masks of
+the Bloch axes (the phase ``exp(i k L)`` the neighbour across the boundary carries), so the step
+is complex-linear and its adjoint, in the same complex convention ``lambda = dJ/d Re + i dJ/d Im``,
+is the conjugate transpose: the transposes above with the masks conjugated.  The pole states
+driven by Re E (P) and by Im E (Q) each have their adjoint; hist = sum 2 Re dP + i sum 2 Re dQ
+hands the real part of lambda_hist to P and the imaginary part to Q.  The eps gradient is then
+``Re(conj(lambda_E^{n+1}) (ca' E^n + cb' X^n))``.
 """
 
 from __future__ import annotations
@@ -47,18 +54,18 @@ class Adjoint64:
     """Backward sweep over an :class:`Oracle64` run (real fields, with or without ADE poles)."""
 
     def __init__(self, tabs):
-        if tabs.bloch:
-            raise NotImplementedError("the adjoint covers real-field scenes")
         self.t = tabs
         self.o = Oracle64(tabs)                      # for the difference operators and masks
         shp = tabs.shape
-        self.lE = np.zeros((3,) + shp)
-        self.lH = np.zeros((3,) + shp)
-        self.lpe = {k: np.zeros(shp) for k in _PSI}
-        self.lph = {k: np.zeros(shp) for k in _PSI}
+        dt = np.complex128 if tabs.bloch else np.float64
+        self.lE = np.zeros((3,) + shp, dtype=dt)
+        self.lH = np.zeros((3,) + shp, dtype=dt)
+        self.lpe = {k: np.zeros(shp, dtype=dt) for k in _PSI}
+        self.lph = {k: np.zeros(shp, dtype=dt) for k in _PSI}
         d = tabs.disp
         if d is not None:
-            self.lP = np.zeros(d.am1.size, dtype=np.complex128)
+            self.lP = np.zeros(d.am1.size, dtype=np.complex128)    # driven by Re E
+            self.lQ = np.zeros(d.am1.size, dtype=np.complex128)    # driven by Im E (Bloch)
             self._entry = self.o._pole_entry          # pole -> entry
             self._idx4 = self.o._idx4                 # entry -> (comp, i, j, k)
 
@@ -78,7 +85,7 @@ class Adjoint64:
         idl = _bc(self.t.axes[a].idl_p, a)
         w = g * idl
         out = -w
-        src = np.moveaxis(w * _bc(m["mnx"], a), a, 0)
+        src = np.moveaxis(w * np.conj(_bc(m["mnx"], a)), a, 0)
         acc = np.moveaxis(out, a, 0)
         self._scatter_add(acc, m["nxt"], src)
         return np.moveaxis(acc, 0, a)
@@ -88,7 +95,7 @@ class Adjoint64:
         idl = _bc(self.t.axes[a].idl_d, a)
         w = g * idl
         out = w.copy()
-        src = np.moveaxis(-w * _bc(m["mpv"], a), a, 0)
+        src = np.moveaxis(-w * np.conj(_bc(m["mpv"], a)), a, 0)
         acc = np.moveaxis(out, a, 0)
         self._scatter_add(acc, m["prv"], src)
         return np.moveaxis(acc, 0, a)
@@ -101,6 +108,13 @@ class Adjoint64:
         """Per entry sum of the real per-pole values ``v``."""
         return np.bincount(self._entry, weights=v, minlength=self.t.disp.comp.size)
 
+    def _pole_terms(self, lp, lq):
+        """``sum_p Re(conj(B_p) l_p)`` per entry for the Re E driven state (+ i times that of
+        the Im E driven one on Bloch scenes)."""
+        b = np.conj(self.t.disp.b)
+        r = self._entry_sum(np.real(b * lp))
+        return r + 1j * self._entry_sum(np.real(b * lq)) if self.t.bloch else r
+
     def absorb_poles(self):
         """Transpose of the ADE post update ``P^{n+1} = P_mid + B E^{n+1}``: E^{n+1} also feeds
         the pole state, so lambda_E^{n+1} gains ``sum_p Re(conj(B_p) lP_p)`` at every pole entry
@@ -108,7 +122,7 @@ class Adjoint64:
         gradient takes.  No-op without poles."""
         d = self.t.disp
         if d is not None:
-            self.lE[self._idx4] += self._entry_sum(np.real(np.conj(d.b) * self.lP))
+            self.lE[self._idx4] += self._pole_terms(self.lP, self.lQ)
 
     def step_back(self, gE=None, gH=None):
         """Adjoint of one forward step: ``gE``/``gH`` are dJ/dE^{n+1}, dJ/dH^{n+1/2} of this
@@ -145,9 +159,12 @@ class Adjoint64:
             # ADE pre transposed: hist enters E^{n+1} as -cb cw pec hist, so lhist = -cw nu;
             # dP feeds P_mid (identity) and hist (2 Re); P^n feeds P_mid and dP, E^n feeds dP
             lhist = -t.cw * np.stack(nu)[self._idx4]
-            ldp = self.lP + 2.0 * lhist[self._entry]
+            ldp = self.lP + 2.0 * np.real(lhist)[self._entry]
+            ldq = self.lQ + 2.0 * np.imag(lhist)[self._entry]
             self.lP = self.lP + np.conj(d.am1) * ldp
-            lE[self._idx4] += self._entry_sum(np.real(np.conj(d.b) * ldp))
+            if t.bloch:
+                self.lQ = self.lQ + np.conj(d.am1) * ldq
+            lE[self._idx4] += self._pole_terms(ldp, ldq)
         lH += dH
         # H update transpose: H_c' = H_c - ch (s1 - s2)
         dE = np.zeros_like(lE)
@@ -191,23 +208,27 @@ def mode_objective_weights(tabs, mon_index: int, f_index: int = 0):
     return 0.25 * sign * (g[0, :, f_index] + g[1, :, f_index])
 
 
-def gradient_mode_power(sc, tabs, mon_index: int, design: np.ndarray, f_index: int = 0):
-    """J = |a+|^2 of a mode monitor after ``tabs.n_total`` steps and dJ/d eps at ``design``
-    ((n, 2) int: component, flat cell), by a float64 adjoint over the stored trajectory of E at
-    the design entries (E^n for n = 0 .. n_total, the only forward values the gradient needs)."""
+def gradient_overlap_power(sc, tabs, mon_index: int, K: np.ndarray, design: np.ndarray,
+                           f_index: int = 0):
+    """J = |a|^2, ``a = sum K . phasors`` of plane monitor ``mon_index`` at frequency ``f_index``
+    (``K`` (4, n1, n2) complex over its E t1, E t2, H t1, H t2 slots) after ``tabs.n_total``
+    steps, and dJ/d eps at ``design`` ((n, 2) int: component, flat cell), by a float64 adjoint
+    over the stored trajectory of E at the design entries (E^n for n = 0 .. n_total, the only
+    forward values the gradient needs).  A sample x of slot c at step n enters ``a`` as
+    ``K_c w_n x`` (w_n the slot's DFT weight), so dJ/dx = ``2 a conj(K_c w_n)`` in the complex
+    convention (its real part for real fields); the H slots average the planes km and km - 1."""
     from openem_amd.grid import transverse_axes
     o = Oracle64(tabs)
     n_total = tabs.n_total
     dc = np.asarray(design[:, 0], dtype=np.int64)
     dcell = np.asarray(design[:, 1], dtype=np.int64)
     flat = dc * int(np.prod(tabs.shape)) + dcell          # into E.reshape(-1)
-    traj = np.empty((n_total + 1, len(design)))
+    traj = np.empty((n_total + 1, len(design)), dtype=o.E.dtype)
     traj[0] = o.E.reshape(-1)[flat]
     for n in range(n_total):
         o.step()
         traj[n + 1] = o.E.reshape(-1)[flat]
     m = tabs.monitors[mon_index]
-    K = mode_objective_weights(tabs, mon_index, f_index)
     ph = m.phase[:, f_index]                              # (n_total, 4): cos/sin at t_e, t_h
     data = o.acc[mon_index][:, f_index]
     a = np.sum(K * data)
@@ -224,19 +245,27 @@ def gradient_mode_power(sc, tabs, mon_index: int, design: np.ndarray, f_index: i
     below = [slice(None)] * 3
     plane[m.axis], below[m.axis] = m.km, m.km - 1
     plane, below = tuple(plane), tuple(below)
+    part = (lambda v: v) if tabs.bloch else np.real
     for n in range(n_total - 1, -1, -1):
         we = ph[n, 0] + 1j * ph[n, 1]
         wh = ph[n, 2] + 1j * ph[n, 3]
         # dJ/dE^{n+1} on the monitor plane, dJ/dH^{n+1/2} split over the two planes the
         # colocated H sample averages
         for slot, comp in ((0, t1), (1, t2)):
-            adj.lE[comp][plane] += 2.0 * np.real(np.conj(a) * K[slot] * we)
+            adj.lE[comp][plane] += part(2.0 * a * np.conj(K[slot] * we))
         for slot, comp in ((2, t1), (3, t2)):
-            v = 2.0 * np.real(np.conj(a) * K[slot] * wh) * 0.5
+            v = part(2.0 * a * np.conj(K[slot] * wh)) * 0.5
             adj.lH[comp][plane] += v
             adj.lH[comp][below] += v
         adj.absorb_poles()
         e0, e1 = traj[n], traj[n + 1]
-        grad += adj.lE.reshape(-1)[flat] * (dca * e0 + dcb * (e1 - ca * e0) / cb)
+        grad += np.real(np.conj(adj.lE.reshape(-1)[flat]) * (dca * e0 + dcb * (e1 - ca * e0) / cb))
         adj.transpose()
     return J, grad
+
+
+def gradient_mode_power(sc, tabs, mon_index: int, design: np.ndarray, f_index: int = 0):
+    """J = |a+|^2 of a mode monitor (the overlap with its mode's weights) and dJ/d eps at
+    ``design``: :func:`gradient_overlap_power` with ``K`` from :func:`mode_objective_weights`."""
+    K = mode_objective_weights(tabs, mon_index, f_index)
+    return gradient_overlap_power(sc, tabs, mon_index, K, design, f_index)

@@ -177,3 +177,99 @@ def test_design_coefficients_with_poles_match_the_oracle_derivatives():
     g = adjoint._design_g(sc, comp[:12], cell[:12])
     assert np.all(g != 0) and np.all(adjoint._design_g(sc, comp[12:], cell[12:])[
         ~np.isin(comp[12:] * np.prod(sc.shape) + cell[12:], d.comp * np.prod(sc.shape) + d.cell)] == 0)
+
+
+def _bloch_cell(n_steps=260):
+    """12 x 12 x 48 Bloch cell (30 deg oblique incidence, CPML z, oblique TF/SF box over z 12-31,
+    one Drude + Lorentz metal cylinder on the substrate)."""
+    return scenes.bloch_cell_scene(nxy=12, nz=48, n_pml=8, num_steps=n_steps, with_metal=True,
+                                   nf=3, n_cyl=1)
+
+
+def _bloch_objective(t):
+    """Seeded complex weights over the transmission plane's four phasor slots."""
+    mon = [m.name for m in t.monitors].index("T")
+    m = t.monitors[mon]
+    rng = np.random.default_rng(2)
+    K = rng.standard_normal((4, m.n1, m.n2)) + 1j * rng.standard_normal((4, m.n1, m.n2))
+    return mon, K
+
+
+def _bloch_design(sc):
+    """Two metal cells (poles and conductivity), cells on the x wrap planes (i = 0, i = nx - 1)
+    and the y wrap plane (j = 0), one interior cell: all inside the TF/SF box, off its faces."""
+    nx, ny, nz = sc.shape
+    d = sc.dispersion
+    metal = [(int(c), int(x)) for c, x in zip(d.comp, d.cell)]
+    ijk = [(2, 0, 5, 20), (1, nx - 1, 0, 25), (0, 6, 0, 14), (2, 4, 9, 28)]
+    return np.array([metal[7], metal[len(metal) // 2]]
+                    + [(c, (i * ny + j) * nz + k) for c, i, j, k in ijk])
+
+
+def _bloch_fd(sc, mon, K, design, h=1e-3):
+    def J(s):
+        tt = tables.build(s)
+        o = Oracle64(tt).run(tt.n_total)
+        return abs(np.sum(K * o.acc[mon][:, 0])) ** 2
+    out = []
+    for c, cell in design:
+        vals = []
+        for s in (+1, -1):
+            eps = [np.array(np.broadcast_to(e, sc.shape), dtype=np.float64)
+                   for e in (sc.eps_ex, sc.eps_ey, sc.eps_ez)]
+            eps[c].reshape(-1)[cell] += s * h
+            vals.append(J(dataclasses.replace(sc, eps_ex=eps[0], eps_ey=eps[1], eps_ez=eps[2])))
+        out.append((vals[0] - vals[1]) / (2 * h))
+    return np.array(out)
+
+
+def test_bloch_adjoint_matches_central_differences():
+    """Complex fields: the conjugate-transposed step (wrap masks conjugated, pole adjoints of the
+    Re E and Im E driven states) gives the gradient of J = |sum K . phasors|^2 on the Bloch cell
+    to 1e-5 of central differences (measured 1e-10 - 1.1e-6), at metal and wrap-plane cells."""
+    from oracle.adjoint64 import gradient_overlap_power
+    sc = _bloch_cell()
+    t = tables.build(sc)
+    assert t.bloch
+    mon, K = _bloch_objective(t)
+    design = _bloch_design(sc)
+    _, grad = gradient_overlap_power(sc, t, mon, K, design)
+    fd = _bloch_fd(sc, mon, K, design)
+    rel = np.abs(grad - fd) / np.abs(fd)
+    print("relative error per entry:", rel)
+    assert rel.max() < 1e-5
+
+
+def test_planted_bloch_transpose_mistakes_fail_the_central_differences(monkeypatch):
+    """Mistakes planted in a copy of the oracle's complex transposes: the wrap masks not
+    conjugated (misses by 3-700%), the Im E driven pole state's terms in lambda_E dropped (the
+    transposed step then amplifies lambda by ~20x per step and overflows).  Neither stays within
+    1% of the central differences on every entry."""
+    import oracle.adjoint64 as A
+    sc = _bloch_cell()
+    t = tables.build(sc)
+    mon, K = _bloch_objective(t)
+    design = _bloch_design(sc)[[0, 2, 3]]
+    fd = _bloch_fd(sc, mon, K, design)
+    orig = A.Adjoint64
+
+    def masks_not_conjugated(self):
+        for m in self.o._m:
+            m["mnx"], m["mpv"] = np.conj(m["mnx"]), np.conj(m["mpv"])
+
+    def im_pole_adjoint_dropped(self):
+        b = np.conj(self.t.disp.b)
+        self._pole_terms = lambda lp, lq: self._entry_sum(np.real(b * lp))
+
+    for name, edit in (("masks not conjugated", masks_not_conjugated),
+                       ("Im E pole adjoint dropped", im_pole_adjoint_dropped)):
+        class Planted(orig):
+            def __init__(self, tabs, _edit=edit):
+                super().__init__(tabs)
+                _edit(self)
+        monkeypatch.setattr(A, "Adjoint64", Planted)
+        with np.errstate(over="ignore", invalid="ignore"):
+            _, g = A.gradient_overlap_power(sc, t, mon, K, design)
+            rel = np.abs(g - fd) / np.abs(fd)
+        print(name, rel)
+        assert not np.all(rel <= 1e-2), name

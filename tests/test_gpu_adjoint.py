@@ -158,3 +158,59 @@ def test_dispersive_recording_equals_checkpoint_recomputation(gpu_lib):
     assert np.abs(rec.grad).max() > 0
     assert rec.J == seg.J
     assert np.array_equal(rec.grad, seg.grad)
+
+
+def _bloch_case():
+    """The 12 x 12 x 48 Bloch cell of the CPU adjoint tests (oblique TF/SF, Drude + Lorentz
+    metal cylinder), seeded weights over its transmission plane, and design entries at metal
+    cells, on the x and y wrap planes and inside."""
+    sc = scenes.bloch_cell_scene(nxy=12, nz=48, n_pml=8, num_steps=260, with_metal=True, nf=3,
+                                 n_cyl=1)
+    t = tables.build(sc)
+    mon = [m.name for m in t.monitors].index("T")
+    m = t.monitors[mon]
+    rng = np.random.default_rng(2)
+    K = rng.standard_normal((4, m.n1, m.n2)) + 1j * rng.standard_normal((4, m.n1, m.n2))
+    nx, ny, nz = sc.shape
+    d = sc.dispersion
+    metal = [(int(c), int(x)) for c, x in zip(d.comp, d.cell)]
+    ijk = [(2, 0, 5, 20), (1, nx - 1, 0, 25), (0, 6, 0, 14), (2, 4, 9, 28), (0, 11, 11, 16),
+           (1, 3, 7, 22)]
+    design = np.array([metal[7], metal[len(metal) // 2], metal[-3]]
+                      + [(c, (i * ny + j) * nz + k) for c, i, j, k in ijk])
+    return sc, t, mon, K, design
+
+
+@pytest.mark.parametrize("form", ["sweeps", "fused"])
+def test_bloch_adjoint_matches_the_oracle_adjoint(gpu_lib, monkeypatch, form):
+    """Complex (Bloch) fields under the adjoint: the conjugate-transposed step on the device
+    (wrap masks conjugated, the Im E driven pole state's adjoint, complex records and objective)
+    against the float64 oracle adjoint, through checkpoint segments of 64 steps, 1e-3 of the
+    largest |gradient|; on the two sweeps and on the fused step."""
+    from oracle.adjoint64 import gradient_overlap_power
+    sc, t, mon, K, design = _bloch_case()
+    monkeypatch.setenv("OEM_FUSE", "1" if form == "fused" else "0")
+    print(f"Bloch adjoint ({form}): shape {sc.shape}, {sc.dispersion.n_entry} pole entries, "
+          f"{t.n_total} steps, {len(design)} design entries", flush=True)
+    g = adjoint.overlap_power_gradient(sc, "T", K, design[:, 0], design[:, 1],
+                                       checkpoint_every=64, record_bytes=0)
+    assert g.segments == 5
+    J0, ref = gradient_overlap_power(sc, t, mon, K, design)
+    assert abs(g.J - J0) <= 1e-4 * J0, (g.J, J0)
+    scale = np.abs(ref).max()
+    err = np.abs(g.grad - ref).max()
+    print(f"max |grad| {scale:.4e}, max err {err:.3e} ({err / scale:.2e} of it)", flush=True)
+    assert err <= 1e-3 * scale, (g.grad, ref)
+
+
+def test_bloch_recording_equals_checkpoint_recomputation(gpu_lib):
+    """Complex fields: the recording forward run and checkpointed segments recomputed one by one
+    give the same gradient bit for bit (checkpoints restore both pole states)."""
+    sc, t, mon, K, design = _bloch_case()
+    rec = adjoint.overlap_power_gradient(sc, "T", K, design[:, 0], design[:, 1])
+    seg = adjoint.overlap_power_gradient(sc, "T", K, design[:, 0], design[:, 1],
+                                         checkpoint_every=48, record_bytes=0)
+    assert rec.segments == 1 and seg.segments == 6
+    assert np.abs(rec.grad).max() > 0
+    assert rec.J == seg.J
+    assert np.array_equal(rec.grad, seg.grad)
