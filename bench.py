@@ -297,6 +297,43 @@ def _mode_power(sc) -> float:
     return float(abs(r.mode_amps["out"]["a_plus"][0]) ** 2)
 
 
+def pool_bench(k: int, n: int, blocks: int = 8, dft_batch: int = 16) -> dict:
+    """The serving pool's kernel throughput: ``k`` box soups of side ``n`` resident and stepped
+    together (one graph of ``dft_batch`` steps per block over every scene), ``blocks`` blocks
+    timed after one warm-up block, against one of the scenes stepped alone."""
+    from openem_amd import scenes, serving, tables
+    from openem_amd.solver import Stepper
+    steps = (blocks + 1) * dft_batch
+    tabs = tables.build(scenes.box_soup_scene(n=n, num_steps=steps + 32, n_boxes=8, seed=3),
+                        steps + 32)
+    pool = serving.Pool(int(serving.device_bytes(tabs, dft_batch) * k * 1.3) + (64 << 20),
+                        dft_batch)
+    sts = [Stepper(tabs, dft_batch=dft_batch, pool=pool) for _ in range(k)]
+    try:
+        for st in sts:
+            pool.attach(st)
+        pool.launch(1)
+        pool.wait(0)
+        t0 = time.perf_counter()
+        pool.launch(blocks)
+        pool.wait(0)
+        dt = time.perf_counter() - t0
+    finally:
+        for st in sts:
+            st.close()
+        pool.close()
+    alone = Stepper(tabs, dft_batch=dft_batch)
+    try:
+        alone.run(32)
+        ms = alone.timed_run(blocks * dft_batch)
+    finally:
+        alone.close()
+    cs = k * n ** 3 * blocks * dft_batch
+    return {"scenes": k, "side": n, "steps": blocks * dft_batch,
+            "pool_mcell_steps_per_s": round(cs / dt / 1e6, 1),
+            "alone_mcell_steps_per_s": round(n ** 3 * blocks * dft_batch / ms / 1e3, 1)}
+
+
 def _serving_estimate_s(specs, best_rate: float) -> float:
     """Wall seconds the serving leg (batched, then serial) is expected to take: the stream's
     cell-steps at 0.60 and 0.51 of the single-scene rate, host work included (the whole stream
@@ -342,6 +379,9 @@ def main() -> int:
                     help="steps of the full-size Bloch unit cell to time (0: skip)")
     ap.add_argument("--bloch-only", action="store_true",
                     help="only the Bloch unit cell leg")
+    ap.add_argument("--pool-only", type=int, nargs=2, metavar=("K", "N"),
+                    help="only the serving pool's throughput: K box soups of side N stepped "
+                         "together")
     ap.add_argument("--budget-s", type=float, default=540.0,
                     help="wall-clock budget of the whole run: a leg that would overrun it is "
                          "shortened (and says so in its record)")
@@ -365,6 +405,10 @@ def main() -> int:
         return 0
     if args.bloch_only:
         print(json.dumps({"metric": METRIC, "bloch": bloch_bench(args.bloch_steps)}), flush=True)
+        return 0
+    if args.pool_only:
+        print(json.dumps({"metric": "serving pool throughput (Mcell-steps/s)",
+                          "pool": pool_bench(*args.pool_only)}), flush=True)
         return 0
     if args.serve_only:
         print(json.dumps({"metric": METRIC, "serving": serving_bench(
