@@ -1,5 +1,6 @@
-// Discrete adjoint of the time step (real fields, with or without ADE poles): checkpoints,
-// recomputation of a segment with E recorded at the design cells, and the backward sweep.
+// Discrete adjoint of the time step (real or complex Bloch fields, with or without ADE poles):
+// checkpoints, recomputation of a segment with E recorded at the design cells, and the backward
+// sweep.
 //
 // The forward step is linear in (E, H, psi_H, psi_E) plus source terms; with
 //   H' = H - ch (s1 - s2),  s = (1/kappa + a) D_fwd E + b psi_H,  psi_H' = b psi_H + a D_fwd E
@@ -20,6 +21,13 @@
 // lambda_P <- lambda_P + conj(A-1) lambda_dP and lambda_E^n the term sum_p Re(conj(B_p)
 // lambda_dP_p).  That last term is kept per entry until the H transpose has written
 // lambda_E^n, and the next step's k_adj_disp adds it first.
+//
+// Complex (Bloch) fields: every coefficient is real except the wrap masks of the Bloch axes
+// (the phase a neighbour across the boundary carries), so the step is complex-linear and its
+// adjoint, with lambda = dJ/d Re + i dJ/d Im, is the conjugate transpose: the same kernels on
+// complex values with the masks conjugated.  The pole state driven by Im E has its own adjoint
+// (lQ), fed by the imaginary parts as lP is by the real ones; the eps gradient is
+// Re(conj(lambda_E) (ca' E^n + cb' S^n)).
 //
 // Each term of an update: (target component c, source component b, derivative axis a, sign,
 // psi slot).  psi slot = off[2 a + m] of the slab layout (x: Hy/Ey 0, Hz/Ez 1; y: Hx/Ex 0,
