@@ -291,6 +291,43 @@ def bloch_bench(steps: int, warmup: int = 10, profile_steps: int = 10) -> dict:
                          "avg_launch_ms": round(avg, 5)}}
 
 
+def bloch_gradient_bench(steps: int, n_design: int = 4096, seed: int = 0) -> dict:
+    """One gradient on the full-size Bloch cell (complex fields, the cell of
+    :func:`bloch_bench` over ``steps`` steps): J = |sum K . phasors|^2 of the transmission plane
+    with seeded weights K, dJ/d eps at ``n_design`` seeded (component, cell) entries in the
+    cylinders' layer, the forward run recording the design (no checkpoints); beside it the
+    forward run alone.  The gradient's wall time includes its table build."""
+    from openem_amd import adjoint, scenes, tables
+    from openem_amd.solver import Stepper
+    sc = scenes.bloch_cell_scene(num_steps=steps)
+    tabs = tables.build(sc)
+    m = tabs.monitors[[q.name for q in tabs.monitors].index("T")]
+    rng = np.random.default_rng(seed)
+    K = rng.standard_normal((4, m.n1, m.n2)) + 1j * rng.standard_normal((4, m.n1, m.n2))
+    nx, ny, nz = sc.shape
+    d = sc.dispersion
+    kd = d.cell % nz
+    k_lo, k_hi = int(kd.min()), int(kd.max()) + 1                     # the cylinders' layer
+    comp = rng.integers(0, 3, n_design)
+    cell = (rng.integers(0, nx, n_design) * ny + rng.integers(0, ny, n_design)) * nz \
+        + rng.integers(k_lo, k_hi, n_design)
+    t0 = time.perf_counter()
+    g = adjoint.overlap_power_gradient(sc, "T", K, comp, cell)
+    wall = time.perf_counter() - t0
+    st = Stepper(tabs)
+    try:
+        st.run(16)
+        fwd = st.timed_run(steps - 16) / (steps - 16)
+    finally:
+        st.close()
+    print(f"[bench] bloch gradient: {wall:.2f} s for {steps} steps ({fwd:.4f} ms per forward "
+          f"step), J {g.J:.6e}", file=sys.stderr, flush=True)
+    return {"workload": "Bloch unit cell 256x256x600 complex, J = |K . T-plane phasors|^2",
+            "cells": int(np.prod(sc.shape)), "steps": steps, "design_entries": n_design,
+            "s_per_gradient": round(wall, 3), "forward_ms_per_step": round(fwd, 4),
+            "max_abs_grad": float(np.abs(g.grad).max())}
+
+
 def _mode_power(sc) -> float:
     from openem_amd import solver
     r = solver.run(sc, use_shutoff=False, verbose=False)
@@ -379,6 +416,8 @@ def main() -> int:
                     help="steps of the full-size Bloch unit cell to time (0: skip)")
     ap.add_argument("--bloch-only", action="store_true",
                     help="only the Bloch unit cell leg")
+    ap.add_argument("--bloch-gradient-only", type=int, default=0, metavar="STEPS",
+                    help="only one gradient on the full-size Bloch cell over STEPS steps")
     ap.add_argument("--pool-only", type=int, nargs=2, metavar=("K", "N"),
                     help="only the serving pool's throughput: K box soups of side N stepped "
                          "together")
@@ -405,6 +444,11 @@ def main() -> int:
         return 0
     if args.bloch_only:
         print(json.dumps({"metric": METRIC, "bloch": bloch_bench(args.bloch_steps)}), flush=True)
+        return 0
+    if args.bloch_gradient_only:
+        print(json.dumps({"metric": "Bloch-cell gradient (forward + device adjoint)",
+                          "bloch_gradient": bloch_gradient_bench(args.bloch_gradient_only)}),
+              flush=True)
         return 0
     if args.pool_only:
         print(json.dumps({"metric": "serving pool throughput (Mcell-steps/s)",
