@@ -216,10 +216,20 @@ def mode_power_gradients(jobs, concurrency: int = 4, on_done=None, **kw) -> list
     checkpoints and adjoint state), so one design's host work and launch gaps overlap another's
     kernels (``concurrency`` 1: one after another on the calling thread).  ``on_done(index,
     gradient)`` is called as each design finishes.  Results are in job order."""
+    return _in_flight(mode_power_gradient, jobs, concurrency, on_done, kw)
+
+
+def overlap_power_gradients(jobs, concurrency: int = 4, on_done=None, **kw) -> list:
+    """:func:`overlap_power_gradient` for many ``(scene, monitor, weights, comp, cell)`` jobs,
+    ``concurrency`` in flight at once, as :func:`mode_power_gradients`."""
+    return _in_flight(overlap_power_gradient, jobs, concurrency, on_done, kw)
+
+
+def _in_flight(fn, jobs, concurrency: int, on_done, kw: dict) -> list:
     from concurrent.futures import ThreadPoolExecutor
 
     def one(ij):
-        g = mode_power_gradient(*ij[1], **kw)
+        g = fn(*ij[1], **kw)
         if on_done is not None:
             on_done(ij[0], g)
         return g

@@ -27,7 +27,9 @@ SceneBuilder = Callable[[np.ndarray], tuple]
 
 
 def _pixel_gradients(designs: np.ndarray, build: SceneBuilder, monitor: str, concurrency: int,
-                     kw: dict):
+                     kw: dict, weights=None):
+    """J and dJ/d pixel per design: the mode power of ``monitor`` (``weights`` None) or the power
+    of its projection on ``weights`` (:func:`openem_amd.adjoint.overlap_power_gradient`)."""
     jobs, shapes = [], []
     for d in designs:
         sc, cells = build(d)
@@ -35,9 +37,14 @@ def _pixel_gradients(designs: np.ndarray, build: SceneBuilder, monitor: str, con
         if cells.shape[:d.ndim] != d.shape:
             raise ValueError(f"design cells {cells.shape} do not start with the design shape "
                              f"{d.shape}")
-        jobs.append((sc, monitor, np.repeat(np.arange(3), cells.size), np.tile(cells.ravel(), 3)))
+        comp, cell = np.repeat(np.arange(3), cells.size), np.tile(cells.ravel(), 3)
+        jobs.append((sc, monitor, comp, cell) if weights is None
+                    else (sc, monitor, weights, comp, cell))
         shapes.append(cells.shape)
-    res = adjoint.mode_power_gradients(jobs, concurrency=concurrency, **kw)
+    if weights is None:
+        res = adjoint.mode_power_gradients(jobs, concurrency=concurrency, **kw)
+    else:
+        res = adjoint.overlap_power_gradients(jobs, concurrency=concurrency, **kw)
     J = np.array([g.J for g in res])
     grads = []
     for g, shp, d in zip(res, shapes, designs):
@@ -48,15 +55,16 @@ def _pixel_gradients(designs: np.ndarray, build: SceneBuilder, monitor: str, con
 
 
 class ModePower(torch.autograd.Function):
-    """``J = |a+|^2`` of mode monitor ``monitor`` for each design of a batch ``(B, *pixels)``;
-    the vector-Jacobian product is the device adjoint's ``dJ/d eps`` per pixel."""
+    """``J = |a+|^2`` of mode monitor ``monitor`` (or, with ``weights``, ``J = |sum weights .
+    phasors|^2`` of plane monitor ``monitor``) for each design of a batch ``(B, *pixels)``; the
+    vector-Jacobian product is the device adjoint's ``dJ/d eps`` per pixel."""
 
     @staticmethod
-    def forward(ctx, designs, build, monitor, concurrency, kw):
+    def forward(ctx, designs, build, monitor, concurrency, kw, weights=None):
         if designs.dim() < 2:
             raise ValueError("designs must be a batch (B, *pixels)")
         host = designs.detach().to("cpu", torch.float64).numpy()
-        J, g = _pixel_gradients(host, build, monitor, concurrency, kw)
+        J, g = _pixel_gradients(host, build, monitor, concurrency, kw, weights)
         ctx.save_for_backward(torch.from_numpy(g).to(designs.device, designs.dtype))
         return torch.from_numpy(J).to(designs.device, designs.dtype)
 
@@ -64,7 +72,7 @@ class ModePower(torch.autograd.Function):
     def backward(ctx, grad_out):
         (g,) = ctx.saved_tensors
         shape = (-1,) + (1,) * (g.dim() - 1)
-        return grad_out.reshape(shape) * g, None, None, None, None
+        return grad_out.reshape(shape) * g, None, None, None, None, None
 
 
 def mode_power(designs: torch.Tensor, build: SceneBuilder, monitor: str = "out",
@@ -77,3 +85,15 @@ def mode_power(designs: torch.Tensor, build: SceneBuilder, monitor: str = "out",
     if not batched:
         return ModePower.apply(designs.unsqueeze(0), build, monitor, concurrency, kw)[0]
     return ModePower.apply(designs, build, monitor, concurrency, kw)
+
+
+def overlap_power(designs: torch.Tensor, build: SceneBuilder, monitor: str, weights,
+                  batched: bool = True, concurrency: int = 2, **kw) -> torch.Tensor:
+    """Differentiable ``J = |sum weights . phasors|^2`` of plane monitor ``monitor`` per design
+    (``weights`` (4, n1, n2) complex over its E t1, E t2, H t1, H t2 slots: a diffraction
+    order's projection on a Bloch cell, for one; real or complex fields), otherwise as
+    :func:`mode_power`; ``kw`` goes to :func:`openem_amd.adjoint.overlap_power_gradient`."""
+    w = np.asarray(weights, dtype=np.complex128)
+    if not batched:
+        return ModePower.apply(designs.unsqueeze(0), build, monitor, concurrency, kw, w)[0]
+    return ModePower.apply(designs, build, monitor, concurrency, kw, w)

@@ -61,3 +61,30 @@ def test_cells_must_follow_the_design_shape(monkeypatch):
     d = torch.ones(1, 2, 2, dtype=torch.float64)
     with pytest.raises(ValueError):
         autograd.mode_power(d, lambda x: (None, np.arange(6).reshape(3, 2)))
+
+
+def test_overlap_power_passes_its_weights_and_scales_backward(monkeypatch):
+    """The plane-projection objective: each job carries the weights, J per design comes back,
+    backward is the incoming gradient times dJ/d eps summed per pixel."""
+    cells = np.arange(2 * 2 * 3).reshape(2, 2, 3) + 40
+    K = np.full((4, 5, 6), 0.5 + 0.25j)
+    seen = []
+
+    def grads(jobs, concurrency, **kw):
+        out = []
+        for (sc, mon, w, comp, cell) in jobs:
+            assert mon == "T" and w is not None and np.array_equal(w, K)
+            assert np.array_equal(cell, np.tile(cells.ravel(), 3))
+            seen.append(kw)
+            out.append(adjoint.Gradient(J=2.0 * sc, a_plus=0j, grad=np.arange(comp.size) * 1.0,
+                                        steps=1, segments=1))
+        return out
+
+    monkeypatch.setattr(adjoint, "overlap_power_gradients", grads)
+    d = torch.ones(3, 2, 2, dtype=torch.float64, requires_grad=True)
+    J = autograd.overlap_power(d, lambda x: (float(x.sum()), cells), "T", K, f_index=1)
+    assert np.allclose(J.detach().numpy(), 8.0)
+    assert all(k == {"f_index": 1} for k in seen)
+    J.sum().backward()
+    per = np.arange(36.0).reshape(3, 2, 2, 3).sum(axis=(0, 3))
+    assert np.allclose(d.grad.numpy(), np.broadcast_to(per, (3, 2, 2)))
