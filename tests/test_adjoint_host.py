@@ -273,3 +273,18 @@ def test_planted_bloch_transpose_mistakes_fail_the_central_differences(monkeypat
             rel = np.abs(g - fd) / np.abs(fd)
         print(name, rel)
         assert not np.all(rel <= 1e-2), name
+
+
+def test_overlap_gradient_checks_its_arguments():
+    """The plane-projection gradient refuses weights of the wrong shape and a monitor that is
+    not a plane before it touches the device."""
+    from openem_amd import adjoint
+    sc = _small_guide(40)
+    t = tables.build(sc)
+    m = t.monitors[[q.name for q in t.monitors].index("fwd_flux")]
+    with pytest.raises(ValueError, match="weights of shape"):
+        adjoint.overlap_power_gradient(sc, "fwd_flux", np.ones((4, m.n1, m.n2 + 1)), [0], [0])
+    box = scenes.mini_scene()
+    name = box.field_monitors[0].name
+    with pytest.raises(ValueError, match="not a plane monitor"):
+        adjoint.overlap_power_gradient(box, name, np.ones((4, 1, 1)), [0], [0])
